@@ -1,0 +1,306 @@
+"""Synthetic hsolve populations: Python view of libhsolve_model.
+
+The setup compiler (``csrc/model/hsm_model.cpp``) emits the arrays GENESIS's
+hsolve builds at SETUP/RESET (``genesis/src/hines/hines_init.c:40-1273``,
+``hines_chip_init.c:465-1552``).  This module wraps it with ctypes and returns
+numpy copies, so tests, the benchmark and the device library all consume the
+same arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._paths import lib_path
+
+BEULER = 10
+CRANK = 11
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class GateSpec(C.Structure):
+    _fields_ = [("power", C.c_int32), ("instant", C.c_int32), ("mode", C.c_int32),
+                ("pad", C.c_int32), ("args", C.c_double * 10)]
+
+
+class ChanType(C.Structure):
+    _fields_ = [("Ek", C.c_double), ("gate", GateSpec * 3)]
+
+
+class SynType(C.Structure):
+    _fields_ = [("Ek", C.c_double), ("tau1", C.c_double), ("tau2", C.c_double)]
+
+
+class Spec(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("method", C.c_int32), ("chanmode", C.c_int32),
+        ("xdivs", C.c_int32), ("float_fields", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
+        ("ncomp", C.c_int32), ("pad1", C.c_int32), ("parent", _i32p),
+        ("Cm", _f64p), ("Rm", _f64p), ("Ra", _f64p), ("Em", _f64p), ("initVm", _f64p),
+        ("inject", _f64p),
+        ("nchantypes", C.c_int32), ("nchan", C.c_int32), ("chantypes", C.POINTER(ChanType)),
+        ("chan_comp", _i32p), ("chan_type", _i32p), ("chan_gbar", _f64p),
+        ("nspike", C.c_int32), ("pad2", C.c_int32), ("spike_comp", _i32p),
+        ("spike_thresh", _f64p), ("spike_refract", _f64p),
+        ("nsyntypes", C.c_int32), ("nsyn", C.c_int32), ("syntypes", C.POINTER(SynType)),
+        ("syn_comp", _i32p), ("syn_type", _i32p), ("syn_gmax", _f64p),
+    ]
+
+
+class Out(C.Structure):
+    _fields_ = [
+        ("ncompts", C.c_int32), ("nchips", C.c_int32), ("nops", C.c_int32),
+        ("nfuncs", C.c_int32), ("nravals", C.c_int32), ("ncols", C.c_int32),
+        ("xdivs", C.c_int32), ("ngivals", C.c_int32), ("nstab", C.c_int32),
+        ("method", C.c_int32), ("chanmode", C.c_int32), ("nchildren", C.c_int32),
+        ("xmin", C.c_double), ("invdx", C.c_double), ("dt", C.c_double),
+        ("ops", _i32p), ("chip", _f64p), ("tablist", _f64p), ("xvals", _f64p),
+        ("funcs", _i32p), ("ravals", _f64p), ("vm", _f64p), ("givals", _f64p),
+        ("stablist", _f64p), ("compops", _i32p), ("compchips", _i32p),
+        ("elmnum", _i32p), ("hnum", _i32p), ("parent_row", _i32p),
+        ("spike_row", _i32p), ("spike_op", _i32p), ("syn_op", _i32p), ("syn_chip", _i32p),
+    ]
+
+
+class Pop(C.Structure):
+    _fields_ = [
+        ("n_neurons", C.c_int32), ("nsizes", C.c_int32), ("sizes", _i32p), ("probs", _f64p),
+        ("seed", C.c_uint64), ("dt", C.c_double), ("method", C.c_int32), ("max_kids", C.c_int32),
+        ("soma_len", C.c_double), ("soma_dia", C.c_double),
+        ("dend_len_lo", C.c_double), ("dend_len_hi", C.c_double),
+        ("dend_dia_lo", C.c_double), ("dend_dia_hi", C.c_double),
+        ("CM", C.c_double), ("RM", C.c_double), ("RA", C.c_double), ("Em", C.c_double),
+        ("gNa", C.c_double), ("gK", C.c_double), ("gKdr", C.c_double),
+        ("inject_lo", C.c_double), ("inject_hi", C.c_double), ("erest_act", C.c_double),
+        ("float_fields", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        p = lib_path("libhsolve_model.so")
+        _lib = C.CDLL(p)
+        _lib.hsm_build.argtypes = [C.POINTER(Spec), C.POINTER(C.POINTER(Out))]
+        _lib.hsm_build.restype = C.c_int32
+        _lib.hsm_build_population.argtypes = [C.POINTER(Pop), C.POINTER(C.POINTER(Out))]
+        _lib.hsm_build_population.restype = C.c_int32
+        _lib.hsm_free.argtypes = [C.POINTER(Out)]
+        _lib.hsm_last_error.restype = C.c_char_p
+        _lib.hsm_tabinterp.argtypes = [_f64p, C.c_int32, C.c_double, C.c_double, C.c_double]
+        _lib.hsm_tabinterp.restype = C.c_double
+    return _lib
+
+
+@dataclass
+class Model:
+    """Hsolve-format arrays of one solver (row index = Hines number)."""
+    ncompts: int
+    dt: float
+    method: int
+    chanmode: int
+    xdivs: int
+    xmin: float
+    invdx: float
+    ncols: int
+    ops: np.ndarray
+    chip: np.ndarray
+    tablist: np.ndarray
+    xvals: np.ndarray
+    funcs: np.ndarray
+    ravals: np.ndarray
+    vm: np.ndarray
+    givals: np.ndarray
+    stablist: np.ndarray
+    compops: np.ndarray
+    compchips: np.ndarray
+    elmnum: np.ndarray
+    hnum: np.ndarray
+    parent_row: np.ndarray
+    spike_row: np.ndarray
+    spike_op: np.ndarray
+    syn_op: np.ndarray
+    syn_chip: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def nchips(self):
+        return int(self.chip.size)
+
+    @property
+    def nops(self):
+        return int(self.ops.size)
+
+    @property
+    def nstab(self):
+        return int(self.stablist.size // 4)
+
+
+def _arr(ptr, n, dt):
+    if n <= 0:
+        return np.zeros(0, dtype=dt)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dt, copy=True)
+
+
+def _take(outp, nspike, nsyn) -> Model:
+    o = outp.contents
+    n = o.ncompts
+    m = Model(
+        ncompts=n, dt=o.dt, method=o.method, chanmode=o.chanmode, xdivs=o.xdivs,
+        xmin=o.xmin, invdx=o.invdx, ncols=o.ncols,
+        ops=_arr(o.ops, o.nops, np.int32), chip=_arr(o.chip, o.nchips, np.float64),
+        tablist=_arr(o.tablist, (o.xdivs + 2) * o.ncols if o.ncols else 0, np.float64),
+        xvals=_arr(o.xvals, o.xdivs + 2 if o.ncols else 0, np.float64),
+        funcs=_arr(o.funcs, o.nfuncs, np.int32), ravals=_arr(o.ravals, o.nravals, np.float64),
+        vm=_arr(o.vm, n, np.float64), givals=_arr(o.givals, o.ngivals, np.float64),
+        stablist=_arr(o.stablist, 4 * o.nstab, np.float64),
+        compops=_arr(o.compops, n, np.int32), compchips=_arr(o.compchips, n, np.int32),
+        elmnum=_arr(o.elmnum, n, np.int32), hnum=_arr(o.hnum, n, np.int32),
+        parent_row=_arr(o.parent_row, n, np.int32),
+        spike_row=_arr(o.spike_row, nspike, np.int32), spike_op=_arr(o.spike_op, nspike, np.int32),
+        syn_op=_arr(o.syn_op, nsyn, np.int32), syn_chip=_arr(o.syn_chip, nsyn, np.int32),
+    )
+    lib().hsm_free(outp)
+    return m
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def hh_chantypes(erest: float, ena: float | None = None, ek: float | None = None):
+    """Na (X^3 Y) and K (X^4) HH squid-axon gate specs as in
+    genesis/Scripts/benchmark/hh1952_ap_verify.g:49-62."""
+    E = erest
+    ena = E + 0.115 if ena is None else ena
+    ek = E - 0.012 if ek is None else ek
+    na = dict(Ek=ena, gates=[
+        (3, 0, 0, [100000.0 * (0.025 + E), -100000.0, -1.0, -1.0 * (0.025 + E), -0.01,
+                   4000.0, 0.0, 0.0, 0.0 - E, 0.018]),
+        (1, 0, 0, [70.0, 0.0, 0.0, 0.0 - E, 0.02, 1000.0, 0.0, 1.0, -0.03 - E, -0.01]),
+    ])
+    k = dict(Ek=ek, gates=[
+        (4, 0, 0, [10000.0 * (0.010 + E), -10000.0, -1.0, -1.0 * (0.010 + E), -0.01,
+                   125.0, 0.0, 0.0, 0.0 - E, 0.08]),
+    ])
+    return na, k
+
+
+def build(comps: dict, chantypes: list, chans: list, *, dt: float, method: int = BEULER,
+          chanmode: int = 4, xdivs: int = 3000, xmin: float = -0.1, xmax: float = 0.05,
+          spikes: list | None = None, syntypes: list | None = None,
+          syns: list | None = None, float_fields: bool = True) -> Model:
+    """Build a solver from element-level descriptions.
+
+    comps: dict of equal-length sequences parent, Cm, Rm, Ra, Em, initVm, inject
+           (element-list order; parent is an element index or -1).
+    chantypes: list of dict(Ek=..., gates=[(power, instant, mode, args10), ...]) (X, Y, Z)
+    chans: list of (comp, type, gbar) in creation order.
+    spikes: list of (comp, thresh, abs_refract).
+    syntypes: list of dict(Ek, tau1, tau2); syns: list of (comp, type, gmax).
+    """
+    spikes = spikes or []
+    syntypes = syntypes or []
+    syns = syns or []
+    n = len(comps["parent"])
+    arrs = {k: np.ascontiguousarray(comps[k], dtype=np.float64)
+            for k in ("Cm", "Rm", "Ra", "Em", "initVm", "inject")}
+    parent = np.ascontiguousarray(comps["parent"], dtype=np.int32)
+    ct = (ChanType * max(1, len(chantypes)))()
+    for i, t in enumerate(chantypes):
+        ct[i].Ek = t["Ek"]
+        for g, (pw, inst, mode, args) in enumerate(t["gates"]):
+            ct[i].gate[g].power = pw
+            ct[i].gate[g].instant = inst
+            ct[i].gate[g].mode = mode
+            for j in range(10):
+                ct[i].gate[g].args[j] = args[j]
+    st = (SynType * max(1, len(syntypes)))()
+    for i, t in enumerate(syntypes):
+        st[i].Ek, st[i].tau1, st[i].tau2 = t["Ek"], t["tau1"], t["tau2"]
+    chan_comp = np.array([c[0] for c in chans] or [0], dtype=np.int32)
+    chan_type = np.array([c[1] for c in chans] or [0], dtype=np.int32)
+    chan_gbar = np.array([c[2] for c in chans] or [0.0], dtype=np.float64)
+    sp_comp = np.array([s[0] for s in spikes] or [0], dtype=np.int32)
+    sp_th = np.array([s[1] for s in spikes] or [0.0], dtype=np.float64)
+    sp_rf = np.array([s[2] for s in spikes] or [0.0], dtype=np.float64)
+    sy_comp = np.array([s[0] for s in syns] or [0], dtype=np.int32)
+    sy_type = np.array([s[1] for s in syns] or [0], dtype=np.int32)
+    sy_g = np.array([s[2] for s in syns] or [0.0], dtype=np.float64)
+    s = Spec()
+    s.dt, s.method, s.chanmode, s.xdivs, s.xmin, s.xmax = dt, method, chanmode, xdivs, xmin, xmax
+    s.float_fields = 1 if float_fields else 0
+    s.ncomp = n
+    s.parent = _p(parent, C.c_int32)
+    for k in ("Cm", "Rm", "Ra", "Em", "initVm", "inject"):
+        setattr(s, k, _p(arrs[k], C.c_double))
+    s.nchantypes, s.nchan = len(chantypes), len(chans)
+    s.chantypes = ct
+    s.chan_comp, s.chan_type, s.chan_gbar = (_p(chan_comp, C.c_int32), _p(chan_type, C.c_int32),
+                                             _p(chan_gbar, C.c_double))
+    s.nspike = len(spikes)
+    s.spike_comp, s.spike_thresh, s.spike_refract = (_p(sp_comp, C.c_int32), _p(sp_th, C.c_double),
+                                                     _p(sp_rf, C.c_double))
+    s.nsyntypes, s.nsyn = len(syntypes), len(syns)
+    s.syntypes = st
+    s.syn_comp, s.syn_type, s.syn_gmax = (_p(sy_comp, C.c_int32), _p(sy_type, C.c_int32),
+                                          _p(sy_g, C.c_double))
+    outp = C.POINTER(Out)()
+    rc = lib().hsm_build(C.byref(s), C.byref(outp))
+    if rc != 0:
+        raise ValueError(lib().hsm_last_error().decode())
+    return _take(outp, len(spikes), len(syns))
+
+
+# ---------------------------------------------------------------------------
+# Synthetic workloads
+
+def population(n_neurons: int, sizes=(16,), probs=(1.0,), *, seed: int = 0, dt: float = 20e-6,
+               method: int = BEULER, soma=(30e-6, 30e-6), dend_len=(10e-6, 100e-6),
+               dend_dia=(0.5e-6, 4e-6), CM=0.01, RM=0.33, RA=1.0, Em=-0.065,
+               gNa=1200.0, gK=360.0, gKdr=50.0, inject=(0.2e-9, 1.0e-9),
+               max_kids: int = 3, float_fields: bool = True) -> Model:
+    """Random Hines trees: soma (root, Na/K) plus dendrites (leak + Kdr).
+
+    Compartment i>0 takes a parent uniform over compartments 0..i-1, resampled
+    while that parent already has ``max_kids`` children.  Sizes are drawn from
+    ``sizes`` with weights ``probs``.  All values SI.
+    """
+    sz = np.ascontiguousarray(sizes, dtype=np.int32)
+    pr = np.ascontiguousarray(probs, dtype=np.float64)
+    p = Pop()
+    p.n_neurons, p.nsizes = n_neurons, len(sz)
+    p.sizes, p.probs = _p(sz, C.c_int32), _p(pr, C.c_double)
+    p.seed, p.dt, p.method, p.max_kids = seed, dt, method, max_kids
+    p.soma_len, p.soma_dia = soma
+    p.dend_len_lo, p.dend_len_hi = dend_len
+    p.dend_dia_lo, p.dend_dia_hi = dend_dia
+    p.CM, p.RM, p.RA, p.Em = CM, RM, RA, Em
+    p.gNa, p.gK, p.gKdr = gNa, gK, gKdr
+    p.inject_lo, p.inject_hi = inject
+    p.erest_act = Em
+    p.float_fields = 1 if float_fields else 0
+    outp = C.POINTER(Out)()
+    rc = lib().hsm_build_population(C.byref(p), C.byref(outp))
+    if rc != 0:
+        raise ValueError(lib().hsm_last_error().decode())
+    return _take(outp, 0, 0)
+
+
+def isopotential_hh(n: int, *, seed: int = 0, dt: float = 20e-6, inject=(0.1e-9, 0.5e-9)) -> Model:
+    """N single-compartment HH cells: 30x30 um soma, Cm 1 uF/cm^2,
+    Rm 3.3 kOhm cm^2, Em -65 mV, Na/K 1200/360 S/m^2, constant inject."""
+    return population(n, (1,), (1.0,), seed=seed, dt=dt, inject=inject)
+
+
+def tabinterp(table: np.ndarray, xdivs: int, xmin: float, invdx: float, x: float) -> float:
+    t = np.ascontiguousarray(table, dtype=np.float64)
+    return lib().hsm_tabinterp(_p(t, C.c_double), xdivs, xmin, invdx, x)

@@ -1,0 +1,661 @@
+/*
+ * hsolve_oracle.c -- CPU parity oracle for the hsolve chanmode-4/5 step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see hsolve_oracle.h).  Restates, in the same
+ * array-of-structures order and with the same floating-point expression
+ * shapes, the reference CPU code:
+ *   - channel-opcode interpreter: genesis/src/hines/hines_chip.c:78-602
+ *     (built as do_chip_hh4_update via hines_4chip.c:19-24);
+ *   - Euler / Crank-Nicolson elimination: genesis/src/hines/hines_solve.c
+ *     :223-296 and :140-218;
+ *   - step phase order: genesis/src/hines/hines.c:222-270.
+ * Interpreter registers persist across compartments exactly as in the
+ * reference (Ek, Gk, Ik, GHK, k, concin/fluxindex, the table-row caches).
+ */
+#include "hsolve_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* opcodes: genesis/src/hines/hines_defs.h:178-222 */
+#define COMPT_OP 100
+#define FCOMPT_OP 101
+#define LCOMPT_OP 102
+#define SPIKE_OP 200
+#define ELMMPUMP_OP 500
+#define CONC_VAL_OP 1000
+#define NERNST_OP 2000
+#define GHK_OP 2010
+#define CHAN_OP 3000
+#define CHAN_EK_OP 3001
+#define SAME_GHK_OP 3002
+#define DO_TABGHK_OP 3003
+#define ADD_CURR_OP 3200
+#define FLUX_OP 3300
+#define SCALE_FLUX_OP 3301
+#define STORE_ALLI_OP 3501
+#define STORE_ALLC_OP 3502
+#define IPOL1V_OP 4001
+#define IPOL1C_OP 4002
+#define IPOL2V_OP 4003
+#define IPOL2C_OP 4004
+#define SYN2_OP 4100
+#define SYN3_OP 4101
+#define MGBLOCK_OP 4200
+#define NEWVOLT_OP 5100
+#define NEWCONC1_OP 5110
+#define NEWCONC2_OP 5111
+#define DOMAIN_OP 5112
+/* funcs codes: hines_defs.h:119-128 */
+#define FORWARD_ELIM 0
+#define BACKWARD_ELIM 0
+#define SET_DIAG 1
+#define SKIP_DIAG 2
+#define FASTSIBARRAY_ELIM 3
+#define COPY_ARRAY 4
+#define SIBARRAY_ELIM 5
+#define CALC_RESULTS 6
+#define FINISH 7
+
+#define OHS_TINY 1.0e-6   /* hines_defs.h:114 */
+#define CRANK_INT 11
+
+struct ohs {
+    ohs_arrays a;
+    int32_t *ops;
+    double *chip, *vm, *results, *givals, *conc, *flux, *codiag, *itotal;
+    double *ravals;
+    int32_t *spike_op, *spike_step;
+    int32_t nspk, capspk;
+    int32_t step;
+    ohs_syn_fn syn;
+    ohs_rand_fn rnd;
+    void *ctx;
+    int32_t trace_on;
+    uint64_t *trace;
+};
+
+static void *dupmem(const void *p, size_t n)
+{
+    void *q = calloc(n ? n : 1, 1);
+    if (p && n) memcpy(q, p, n);
+    return q;
+}
+
+ohs *ohs_create(const ohs_arrays *a)
+{
+    ohs *h = (ohs *)calloc(1, sizeof(ohs));
+    h->a = *a;
+    h->ops = (int32_t *)dupmem(a->ops, sizeof(int32_t) * a->nops);
+    h->chip = (double *)dupmem(a->chip, sizeof(double) * a->nchips);
+    h->vm = (double *)dupmem(a->vm, sizeof(double) * a->ncompts);
+    h->results = (double *)calloc(2 * (size_t)a->ncompts + 2, sizeof(double));
+    h->ravals = (double *)dupmem(a->ravals, sizeof(double) * a->nravals);
+    if (a->givals)
+        h->givals = (double *)dupmem(a->givals, sizeof(double) * a->ngivals);
+    else
+        h->givals = NULL;
+    if (a->nconcs > 0) {
+        h->conc = (double *)dupmem(a->conc, sizeof(double) * (a->nconcs + 1));
+        h->flux = (double *)calloc(a->nconcs + 1, sizeof(double));
+        h->codiag = (double *)calloc(a->nconcs + 1, sizeof(double));
+    }
+    if (a->nelm_names > 0) h->itotal = (double *)calloc(a->nelm_names, sizeof(double));
+    h->capspk = 1024;
+    h->spike_op = (int32_t *)malloc(sizeof(int32_t) * h->capspk);
+    h->spike_step = (int32_t *)malloc(sizeof(int32_t) * h->capspk);
+    h->trace = (uint64_t *)calloc(a->ncompts + 1, sizeof(uint64_t));
+    return h;
+}
+
+void ohs_destroy(ohs *h)
+{
+    if (!h) return;
+    free(h->ops); free(h->chip); free(h->vm); free(h->results); free(h->ravals);
+    free(h->givals); free(h->conc); free(h->flux); free(h->codiag); free(h->itotal);
+    free(h->spike_op); free(h->spike_step); free(h->trace);
+    free(h);
+}
+
+void ohs_set_hooks(ohs *h, ohs_syn_fn syn, ohs_rand_fn rnd, void *ctx)
+{
+    h->syn = syn; h->rnd = rnd; h->ctx = ctx;
+}
+
+void ohs_trace_enable(ohs *h, int32_t on) { h->trace_on = on; }
+void ohs_trace(ohs *h, uint64_t *per_comp)
+{
+    memcpy(per_comp, h->trace, sizeof(uint64_t) * h->a.ncompts);
+}
+
+static inline uint64_t fnv(uint64_t x, int32_t v)
+{
+    uint32_t u = (uint32_t)v;
+    for (int i = 0; i < 4; i++) { x ^= (u >> (8 * i)) & 0xffu; x *= 1099511628211ull; }
+    return x;
+}
+
+static void record_spike(ohs *h, int32_t opidx)
+{
+    if (h->nspk == h->capspk) {
+        h->capspk *= 2;
+        h->spike_op = (int32_t *)realloc(h->spike_op, sizeof(int32_t) * h->capspk);
+        h->spike_step = (int32_t *)realloc(h->spike_step, sizeof(int32_t) * h->capspk);
+    }
+    h->spike_op[h->nspk] = opidx;
+    h->spike_step[h->nspk] = h->step;
+    h->nspk++;
+}
+
+/* do_chip_hh4_update with SOLVEINTERPOL (hines_chip.c:78-602).  The op
+ * cursor `p`, the chip cursor `c` and the givals cursor `g` advance exactly as
+ * the reference's pointers do. */
+int32_t ohs_chip_update(ohs *h)
+{
+    const ohs_arrays *A = &h->a;
+    int32_t *ops = h->ops;
+    double *chip = h->chip;
+    double *results = h->results;
+    const double *vmv = h->vm;
+    const double *tablist = A->tablist, *xvals = A->xvals, *yvals = A->yvals;
+    const double *stablist = A->stablist;
+    double *conc = h->conc, *flux = h->flux, *codiag = h->codiag, *itotal = h->itotal;
+    const int ncols = A->ncols, xdivs = A->xdivs, ydivs = A->ydivs;
+    const double xmin = A->xmin, invdx = A->invdx, ymin = A->ymin, invdy = A->invdy;
+    const int chanmode5 = (A->chanmode == 5);
+    const int storeall = A->storemode;
+    const int keepg = (h->givals != NULL);
+    double *givals = keepg ? h->givals + 1 : NULL;
+    double gscratch[16];
+
+    size_t p = 0, c = 0, r = 0, vi = 0;
+    int k = 0, cfilo = 0, filo = 0, doGHK = 0;
+    double Vm, sumgchan, ichan, Gk = 0, X, Y, Ek = 0.0, GHK = 0.0, concen, Ik = 0.0;
+    double vipol = 0.0, cipol = 0.0, ipol = 0.0, Im = 0.0, gmax = 0.0;
+    double xlo, xhi, ylo = 0.0, yhi = 0.0;
+    const double *vtab = NULL, *ctab = NULL, *tab = NULL, *tabn;
+    double *concin = NULL, *fluxindex = NULL, *concout;
+    uint64_t tr = 0;
+    int comp = -1;
+
+    if (tablist) { xlo = xhi = 0.0; } else { xlo = -1e20; xhi = 1e20; }
+    if (storeall && itotal)
+        for (k = 0; k < A->nelm_names; k++) itotal[k] = 0.0;
+
+#define GOUT(v) do { if (keepg) *givals++ = (v); } while (0)
+
+    while (ops[p++] != LCOMPT_OP) {
+        /* start of a compartment (hines_chip.c:157-168) */
+        if (h->trace_on && comp >= 0) h->trace[comp] = tr;
+        comp++;
+        tr = 1469598103934665603ull;
+        Vm = vmv[vi++];
+        sumgchan = 0.0;
+        ichan = chip[c] + chip[c + 1];
+        if (keepg) {
+            Im = chip[c] - Vm / *givals++;
+            *givals++ = Im;
+        } else {
+            Im = 0.0;
+        }
+        if (storeall == 1 && itotal) itotal[0] += Im;
+        c += 2;
+        (void)gscratch;
+
+        while (ops[p] > LCOMPT_OP) {
+            int op = ops[p++];
+            if (h->trace_on) tr = fnv(tr, op);
+            switch (op) {
+            case NEWVOLT_OP:
+                vipol = (Vm - xmin) * invdx;
+                if ((xlo > Vm) || (Vm >= xhi)) {
+                    if (vipol < 0.0) vipol = 0.0;
+                    else if (vipol > xdivs) vipol = xdivs;
+                    filo = (int)vipol;
+                    vipol = vipol - (double)filo;
+                    xlo = xvals[filo];
+                    xhi = xvals[filo + 1];
+                    vtab = tablist + (size_t)filo * ncols;
+                } else {
+                    vipol = vipol - (double)filo;
+                }
+                continue;
+            case CONC_VAL_OP:
+                k = ops[p++];
+                if (h->trace_on) tr = fnv(tr, k);
+                concin = conc + k;
+                fluxindex = flux + k;
+                *fluxindex = 0.0;
+                continue;
+            case DOMAIN_OP:
+                concen = chip[c++] * Ik;
+                goto cinterpol;
+            case NEWCONC2_OP:
+                if (h->trace_on) tr = fnv(tr, ops[p]);
+                concen = conc[ops[p++]];
+                tab = ctab;
+                ipol = cipol;
+                goto cinterpol;
+            case NEWCONC1_OP:
+                concen = *concin;
+            cinterpol:
+                cipol = (concen - ymin) * invdy;
+                if ((ylo > concen) || (concen >= yhi)) {
+                    if (cipol < 0.0) cipol = 0.0;
+                    else if (cipol > ydivs) cipol = ydivs;
+                    cfilo = (int)cipol;
+                    cipol = cipol - (double)cfilo;
+                    ylo = yvals[cfilo];
+                    yhi = yvals[cfilo + 1];
+                    ctab = tablist + (size_t)cfilo * ncols;
+                } else {
+                    cipol = cipol - cfilo;
+                }
+                continue;
+            case SPIKE_OP:
+                ops[p] -= 1;
+                if (Vm > chip[c++]) {
+                    if (ops[p] <= 0) {
+                        record_spike(h, (int32_t)p - 1);
+                        ops[p] = ops[p + 1];
+                    }
+                }
+                p += 2;
+                continue;
+            case ELMMPUMP_OP:
+                X = 1.0 / (*concin + chip[c++]);
+                ichan += Ik = chip[c] * *concin * X;
+                if (chanmode5) GOUT(Ik / chip[c]);
+                else GOUT(Ik);
+                c++;
+                Im += Ik;
+                codiag[k] += chip[c++] * X;
+                continue;
+            case FLUX_OP:
+                *fluxindex += Ik;
+                continue;
+            case STORE_ALLI_OP:
+                if (h->trace_on) tr = fnv(tr, ops[p]);
+                itotal[ops[p++]] += Ik;
+                continue;
+            case STORE_ALLC_OP:
+                if (h->trace_on) tr = fnv(tr, ops[p]);
+                itotal[ops[p++]] += Gk;
+                continue;
+            case SCALE_FLUX_OP:
+                *fluxindex += chip[c++] * Ik;
+                continue;
+            case GHK_OP:
+                concout = concin + 1;
+                GHK = chip[c++];
+                X = Vm * GHK;
+                Y = exp(-X);
+                Ek = chip[c++];
+                Ik = Ek * (*concin - (Y * *concout));
+                if (fabs(X) < OHS_TINY) {
+                    Gk = 1.0 - 0.5 * X;
+                    GHK = 0.5 * GHK * Ek * (*concin + *concout);
+                    Ik = -Ik / Gk;
+                } else {
+                    Gk = 1.0 - Y;
+                    GHK = GHK * (Ik / Gk - Ek * GHK * (*concin - *concout) * Vm * Y / (Gk * Gk));
+                    Ik = -X * Ik / Gk;
+                }
+                Ek = (Ik + GHK * Vm) / GHK;
+                /* fall through */
+            case SAME_GHK_OP:
+                gmax = chip[c++];
+                Gk = gmax * GHK;
+                break;
+            case DO_TABGHK_OP:
+                doGHK = 1;
+                break;
+            case NERNST_OP:
+                Ek = log(*(concin + 1) / *concin) * chip[c++];
+                gmax = Gk = chip[c++];
+                break;
+            case CHAN_EK_OP:
+                Ek = chip[c++];
+                /* fall through */
+            case CHAN_OP:
+                gmax = Gk = chip[c++];
+                break;
+            default:
+                return -1;
+            }
+
+            /* conductance ops until the current is added (hines_chip.c:404-590) */
+            for (;;) {
+                k = ops[p++];
+                if (h->trace_on) tr = fnv(tr, k);
+                if (k == IPOL1V_OP || k == IPOL1C_OP) {
+                    double f = (k == IPOL1V_OP) ? vipol : cipol;
+                    if (h->trace_on) tr = fnv(tr, ops[p]);
+                    tab = ((k == IPOL1V_OP) ? vtab : ctab) + ops[p++];
+                    X = *tab;
+                    tabn = tab + ncols;
+                    X = X + f * (*tabn - X);
+                    tab++;
+                    tabn++;
+                    Y = *tab;
+                    Y = Y + f * (*tabn - Y);
+                    goto dopower;
+                } else if (k == IPOL2V_OP || k == IPOL2C_OP) {
+                    double f;
+                    if (h->trace_on) tr = fnv(tr, ops[p]);
+                    if (k == IPOL2V_OP) {
+                        tab = vtab + cfilo + cfilo + ops[p++];
+                        f = vipol;
+                    } else {
+                        tab += cfilo + cfilo + ops[p++];
+                        f = ipol;
+                    }
+                    tabn = tab + ncols;
+                    X = (1.0 - f) * ((1.0 - cipol) * (*tab) + cipol * (*(tab + 2))) +
+                        f * ((1.0 - cipol) * (*tabn) + cipol * (*(tabn + 2)));
+                    tab++;
+                    tabn++;
+                    Y = (1.0 - f) * ((1.0 - cipol) * (*tab) + cipol * (*(tab + 2))) +
+                        f * ((1.0 - cipol) * (*tabn) + cipol * (*(tabn + 2)));
+                dopower:
+                    k = ops[p++];
+                    if (h->trace_on) tr = fnv(tr, k);
+                    if (k == 0) {
+                        Ek = Y;
+                        if (doGHK) {
+                            GHK = X;
+                            doGHK = 0;
+                            break;
+                        } else {
+                            Gk *= X;
+                            goto doaddcurr;
+                        }
+                    }
+                    if (k > 0) {
+                        X = chip[c] = (chip[c] * (2.0 - X) + Y) / X;
+                    } else {
+                        X = chip[c] = Y / X;
+                        k = -k;
+                    }
+                    c++;
+                    if (k == 1) {
+                        Gk *= X;
+                    } else if (k == 2) {
+                        Gk *= X * X;
+                    } else if (k == 3) {
+                        Gk *= X * X * X;
+                    } else if (k == 4) {
+                        X *= X;
+                        Gk *= X * X;
+                    } else if (k == 5) {
+                        Gk *= X;
+                        X *= X;
+                        X *= X;
+                        Gk *= X;
+                    } else {
+                        X *= X;
+                        Gk *= X;
+                        X *= X;
+                        Gk *= X;
+                    }
+                } else if (k >= SYN2_OP && k <= SYN3_OP) {
+                    if (k == SYN3_OP) {
+                        double u = h->rnd ? h->rnd(h->ctx) : 1.0;
+                        if (u <= chip[c++]) chip[c] += stablist[ops[p] + 3];
+                    }
+                    k = ops[p++];
+                    if (h->trace_on) tr = fnv(tr, k);
+                    chip[c] *= stablist[k];
+                    if (ops[p] == 0) {
+                        int32_t nxt = -1;
+                        if (h->syn) nxt = h->syn(h->ctx, k, ops[p + 1], (int32_t)c, h->step);
+                        ops[p] = nxt;
+                    }
+                    ops[p] -= 1;
+                    p += 2;
+                    X = chip[c];
+                    c++;
+                    Gk *= (chip[c] = X * stablist[k + 1] + chip[c] * stablist[k + 2]);
+                    c++;
+                    if (ops[p] == MGBLOCK_OP) {
+                        if (chanmode5) {
+                            GOUT(Gk / gmax);
+                            GOUT(Gk * (Ek - Vm) / gmax);
+                        } else {
+                            GOUT(Gk);
+                            GOUT(Gk * (Ek - Vm));
+                        }
+                        GOUT(Ek);
+                        X = chip[c] * exp(Vm * chip[c + 1]);
+                        c += 2;
+                        Gk *= X / (X + chip[c++]);
+                        p++;
+                    }
+                    goto doaddcurr;
+                } else if (k == ADD_CURR_OP) {
+                doaddcurr:
+                    sumgchan += Gk;
+                    ichan += Ek * Gk;
+                    if (chanmode5) {
+                        GOUT(Gk / gmax);
+                        Ik = Gk * (Ek - Vm);
+                        GOUT(Ik / gmax);
+                    } else {
+                        GOUT(Gk);
+                        Ik = Gk * (Ek - Vm);
+                        GOUT(Ik);
+                    }
+                    GOUT(Ek);
+                    Im += Ik;
+                    break;
+                } else {
+                    return -1;
+                }
+            }
+        }
+        /* end of compartment (hines_chip.c:592-599) */
+        results[r++] = Vm + ichan * chip[c];
+        results[r++] = sumgchan * chip[c] + chip[c + 1];
+        c += 2;
+        GOUT(Im);
+    }
+    if (h->trace_on && comp >= 0) h->trace[comp] = tr;
+#undef GOUT
+    return 0;
+}
+
+/* Shared forward sweep of do_euler_hsolve / do_crank_hsolve
+ * (hines_solve.c:236-273 and :153-190).  Returns the funcs cursor after the
+ * forward FINISH and leaves the row cursor on the last row. */
+static size_t fwd_sweep(ohs *h, size_t *rowp, double *rv, double *dv, size_t *rap)
+{
+    const int32_t *funcs = h->a.funcs;
+    double *res = h->results;
+    double *raval = h->ravals;
+    size_t f = 0, ra = 0, row = 2;
+    double temp = 0.0, resultval = res[row], diaval = res[row + 1];
+    int op = funcs[f++];
+    for (;;) {
+        if (op == FORWARD_ELIM) {
+            temp = raval[ra++] / res[funcs[f] + 1];
+            diaval -= raval[ra++] * temp;
+            resultval -= res[funcs[f++]] * temp;
+        } else if (op == SET_DIAG) {
+            res[row] = resultval;
+            res[row + 1] = diaval;
+            row += 2;
+            resultval = res[row];
+            diaval = res[row + 1];
+        } else if (op == SKIP_DIAG) {
+            res[row] = resultval;
+            res[row + 1] = diaval;
+            row += 4;
+            resultval = res[row];
+            diaval = res[row + 1];
+        } else if (op == FASTSIBARRAY_ELIM) {
+            raval[funcs[f++]] -= raval[ra++] * temp;
+        } else if (op == COPY_ARRAY) {
+            raval[funcs[f++]] = raval[ra++];
+        } else if (op == SIBARRAY_ELIM) {
+            raval[funcs[f + 1]] -= raval[funcs[f]] * temp;
+            f += 2;
+        } else {
+            break;
+        }
+        op = funcs[f++];
+    }
+    *rowp = row;
+    *rv = resultval;
+    *dv = diaval;
+    *rap = ra;
+    return f;
+}
+
+void ohs_euler_solve(ohs *h)
+{
+    double *res = h->results;
+    double *vm = h->vm;
+    const int32_t *funcs = h->a.funcs;
+    const double *raval = h->ravals;
+    int n = h->a.ncompts;
+    if (n == 1) {
+        vm[0] = res[0] / res[1];
+        return;
+    }
+    size_t row, ra;
+    double resultval, diaval;
+    size_t f = fwd_sweep(h, &row, &resultval, &diaval, &ra);
+    size_t vi = (size_t)n - 1;
+    vm[vi] = res[row] = resultval / diaval;
+    row -= 2;
+    resultval = res[row];
+    for (;;) {
+        int op = funcs[f++];
+        if (op == BACKWARD_ELIM) {
+            resultval -= raval[ra++] * res[funcs[f++]];
+        } else if (op == CALC_RESULTS) {
+            vi--;
+            vm[vi] = res[row] = resultval / res[row + 1];
+            row -= 2;
+            resultval = res[row];
+        } else if (op == SIBARRAY_ELIM) {
+            resultval -= raval[funcs[f]] * res[funcs[f + 1]];
+            f += 2;
+        } else {
+            break;
+        }
+    }
+}
+
+void ohs_crank_solve(ohs *h)
+{
+    double *res = h->results;
+    double *vm = h->vm;
+    const int32_t *funcs = h->a.funcs;
+    const double *raval = h->ravals;
+    int n = h->a.ncompts;
+    if (n == 1) {
+        double x = res[0] / res[1];
+        vm[0] = x + x - vm[0];
+        return;
+    }
+    size_t row, ra;
+    double resultval, diaval;
+    size_t f = fwd_sweep(h, &row, &resultval, &diaval, &ra);
+    size_t vi = (size_t)n - 1;
+    resultval = resultval / diaval;
+    vm[vi] = resultval + resultval - vm[vi];
+    res[row] = resultval;
+    row -= 2;
+    resultval = res[row];
+    for (;;) {
+        int op = funcs[f++];
+        if (op == BACKWARD_ELIM) {
+            resultval -= raval[ra++] * res[funcs[f++]];
+        } else if (op == CALC_RESULTS) {
+            resultval = resultval / res[row + 1];
+            vi--;
+            vm[vi] = resultval + resultval - vm[vi];
+            res[row] = resultval;
+            row -= 2;
+            resultval = res[row];
+        } else if (op == SIBARRAY_ELIM) {
+            resultval -= raval[funcs[f]] * res[funcs[f + 1]];
+            f += 2;
+        } else {
+            break;
+        }
+    }
+}
+
+int32_t ohs_step(ohs *h, int32_t nsteps)
+{
+    for (int32_t s = 0; s < nsteps; s++) {
+        if (ohs_chip_update(h)) return -1;
+        if (h->a.method == CRANK_INT) ohs_crank_solve(h);
+        else ohs_euler_solve(h);
+        h->step++;
+    }
+    return 0;
+}
+
+int32_t ohs_stepno(ohs *h) { return h->step; }
+
+void ohs_get(ohs *h, double *vm, double *chip, double *results, int32_t *ops, double *givals)
+{
+    if (vm) memcpy(vm, h->vm, sizeof(double) * h->a.ncompts);
+    if (chip) memcpy(chip, h->chip, sizeof(double) * h->a.nchips);
+    if (results) memcpy(results, h->results, sizeof(double) * 2 * h->a.ncompts);
+    if (ops) memcpy(ops, h->ops, sizeof(int32_t) * h->a.nops);
+    if (givals && h->givals) memcpy(givals, h->givals, sizeof(double) * h->a.ngivals);
+}
+
+void ohs_put(ohs *h, const double *vm, const double *chip, const int32_t *ops)
+{
+    if (vm) memcpy(h->vm, vm, sizeof(double) * h->a.ncompts);
+    if (chip) memcpy(h->chip, chip, sizeof(double) * h->a.nchips);
+    if (ops) memcpy(h->ops, ops, sizeof(int32_t) * h->a.nops);
+}
+
+int32_t ohs_nspikes(ohs *h) { return h->nspk; }
+void ohs_spikes(ohs *h, int32_t *opidx, int32_t *step)
+{
+    memcpy(opidx, h->spike_op, sizeof(int32_t) * h->nspk);
+    memcpy(step, h->spike_step, sizeof(int32_t) * h->nspk);
+}
+void ohs_clear_spikes(ohs *h) { h->nspk = 0; }
+
+typedef struct {
+    ohs **hs;
+    int32_t nh, nsteps, nthreads, tid, rc;
+} par_job;
+
+static void *par_run(void *arg)
+{
+    par_job *j = (par_job *)arg;
+    for (int32_t i = j->tid; i < j->nh; i += j->nthreads)
+        if (ohs_step(j->hs[i], j->nsteps)) j->rc = -1;
+    return NULL;
+}
+
+int32_t ohs_step_parallel(ohs **hs, int32_t nh, int32_t nsteps, int32_t nthreads)
+{
+    if (nthreads < 1) nthreads = 1;
+    pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * nthreads);
+    par_job *jobs = (par_job *)calloc(nthreads, sizeof(par_job));
+    for (int32_t t = 0; t < nthreads; t++) {
+        jobs[t].hs = hs; jobs[t].nh = nh; jobs[t].nsteps = nsteps;
+        jobs[t].nthreads = nthreads; jobs[t].tid = t;
+        pthread_create(&th[t], NULL, par_run, &jobs[t]);
+    }
+    int32_t rc = 0;
+    for (int32_t t = 0; t < nthreads; t++) {
+        pthread_join(th[t], NULL);
+        if (jobs[t].rc) rc = -1;
+    }
+    free(th);
+    free(jobs);
+    return rc;
+}

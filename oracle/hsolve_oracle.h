@@ -1,0 +1,92 @@
+/*
+ * hsolve_oracle.h -- CPU restatement of GENESIS's chanmode-4/5 hsolve step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity oracle: tests/, the smoke
+ * check and the CPU-baseline leg of bench.py call it to produce the reference
+ * answer.  No product code path links, loads or calls it.
+ *
+ * It walks the same Hsolve-format arrays as the reference CPU code, in the
+ * same array-of-structures order:
+ *   ohs_chip_update   genesis/src/hines/hines_chip.c:78-602 (CHANMODE4 with
+ *                     SOLVEINTERPOL, i.e. do_chip_hh4_update, calcmode 1)
+ *   ohs_euler_solve   genesis/src/hines/hines_solve.c:223-296
+ *   ohs_crank_solve   genesis/src/hines/hines_solve.c:140-218
+ *   ohs_step          phase order of genesis/src/hines/hines.c:222-270
+ * Compile with -O2 -ffp-contract=off and no fast-math.
+ */
+#ifndef HSOLVE_ORACLE_H
+#define HSOLVE_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ohs ohs;
+
+/* synchan event hook: called when a SYN2/SYN3 counter reaches 0
+ * (h_dosynchan, hines_conn.c:60-89).  It may add to chip[xslot] and must
+ * return the next event count (or -1 for none). */
+typedef int32_t (*ohs_syn_fn)(void *ctx, int32_t stabindex, int32_t child,
+                              int32_t xslot, int32_t step);
+/* uniform random number for SYN3_OP (urandom, hines_chip.c:516) */
+typedef double (*ohs_rand_fn)(void *ctx);
+
+typedef struct ohs_arrays {
+    int32_t ncompts, nchips, nops, nfuncs, nravals;
+    int32_t ncols, xdivs, ydivs, nconcs, nstab, ngivals, nelm_names;
+    int32_t method, chanmode, storemode, pad;
+    double  xmin, invdx, ymin, invdy, dt;
+    const int32_t *ops;
+    const double  *chip;
+    const double  *tablist;   /* (xdivs+2)*ncols, guard row included */
+    const double  *xvals;     /* xdivs+2 */
+    const double  *yvals;     /* ydivs+2 or NULL */
+    const double  *stablist;
+    const int32_t *funcs;
+    const double  *ravals;
+    const double  *vm;
+    const double  *givals;    /* may be NULL: then no givals are kept */
+    const double  *conc;      /* nconcs or NULL */
+} ohs_arrays;
+
+ohs    *ohs_create(const ohs_arrays *a);
+void    ohs_destroy(ohs *h);
+void    ohs_set_hooks(ohs *h, ohs_syn_fn syn, ohs_rand_fn rnd, void *ctx);
+
+/* One interpreter pass (do_chip_hh4_update).  Returns 0 or -1 on an
+ * undefined opcode. */
+int32_t ohs_chip_update(ohs *h);
+void    ohs_euler_solve(ohs *h);
+void    ohs_crank_solve(ohs *h);
+/* nsteps full steps: chip update then the elimination selected by method. */
+int32_t ohs_step(ohs *h, int32_t nsteps);
+
+/* state access (copies) */
+void    ohs_get(ohs *h, double *vm, double *chip, double *results, int32_t *ops,
+                double *givals);
+void    ohs_put(ohs *h, const double *vm, const double *chip, const int32_t *ops);
+
+/* spike events raised by SPIKE_OP: ops index of the SPIKE_OP and step. */
+int32_t ohs_nspikes(ohs *h);
+void    ohs_spikes(ohs *h, int32_t *opidx, int32_t *step);
+void    ohs_clear_spikes(ohs *h);
+int32_t ohs_stepno(ohs *h);
+
+/* Per-compartment operation trace of the last ohs_chip_update: for each
+ * compartment, a 64-bit FNV-1a hash over the opcodes and operands it
+ * executed in order.  Used to check that an ops walk visits the same ops. */
+void    ohs_trace_enable(ohs *h, int32_t on);
+void    ohs_trace(ohs *h, uint64_t *per_comp);
+
+/* Step nh independent solvers (disjoint neuron blocks, each its own hsolve)
+ * nsteps times on nthreads POSIX threads; solver i runs on thread i % nthreads.
+ * Used for the multi-core CPU baseline. */
+int32_t ohs_step_parallel(ohs **hs, int32_t nh, int32_t nsteps, int32_t nthreads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

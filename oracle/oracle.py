@@ -1,0 +1,200 @@
+"""ctypes binding of libhsolve_oracle.so -- TEST INFRASTRUCTURE (see __init__)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class Arrays(C.Structure):
+    _fields_ = [
+        ("ncompts", C.c_int32), ("nchips", C.c_int32), ("nops", C.c_int32),
+        ("nfuncs", C.c_int32), ("nravals", C.c_int32),
+        ("ncols", C.c_int32), ("xdivs", C.c_int32), ("ydivs", C.c_int32),
+        ("nconcs", C.c_int32), ("nstab", C.c_int32), ("ngivals", C.c_int32),
+        ("nelm_names", C.c_int32),
+        ("method", C.c_int32), ("chanmode", C.c_int32), ("storemode", C.c_int32),
+        ("pad", C.c_int32),
+        ("xmin", C.c_double), ("invdx", C.c_double), ("ymin", C.c_double),
+        ("invdy", C.c_double), ("dt", C.c_double),
+        ("ops", _i32p), ("chip", _f64p), ("tablist", _f64p), ("xvals", _f64p),
+        ("yvals", _f64p), ("stablist", _f64p), ("funcs", _i32p), ("ravals", _f64p),
+        ("vm", _f64p), ("givals", _f64p), ("conc", _f64p),
+    ]
+
+
+SYNFN = C.CFUNCTYPE(C.c_int32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32)
+RANDFN = C.CFUNCTYPE(C.c_double, C.c_void_p)
+
+_lib = None
+
+
+def oracle_lib():
+    global _lib
+    if _lib is None:
+        p = os.path.join(_HERE, "libhsolve_oracle.so")
+        if not os.path.exists(p):
+            raise OSError(f"{p} missing: build with `make -C oracle`")
+        L = C.CDLL(p)
+        L.ohs_create.argtypes = [C.POINTER(Arrays)]
+        L.ohs_create.restype = C.c_void_p
+        L.ohs_destroy.argtypes = [C.c_void_p]
+        L.ohs_set_hooks.argtypes = [C.c_void_p, SYNFN, RANDFN, C.c_void_p]
+        L.ohs_chip_update.argtypes = [C.c_void_p]
+        L.ohs_chip_update.restype = C.c_int32
+        L.ohs_euler_solve.argtypes = [C.c_void_p]
+        L.ohs_crank_solve.argtypes = [C.c_void_p]
+        L.ohs_step.argtypes = [C.c_void_p, C.c_int32]
+        L.ohs_step.restype = C.c_int32
+        L.ohs_get.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _i32p, _f64p]
+        L.ohs_put.argtypes = [C.c_void_p, _f64p, _f64p, _i32p]
+        L.ohs_nspikes.argtypes = [C.c_void_p]
+        L.ohs_nspikes.restype = C.c_int32
+        L.ohs_spikes.argtypes = [C.c_void_p, _i32p, _i32p]
+        L.ohs_clear_spikes.argtypes = [C.c_void_p]
+        L.ohs_stepno.argtypes = [C.c_void_p]
+        L.ohs_stepno.restype = C.c_int32
+        L.ohs_trace_enable.argtypes = [C.c_void_p, C.c_int32]
+        L.ohs_trace.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.ohs_step_parallel.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32]
+        L.ohs_step_parallel.restype = C.c_int32
+        _lib = L
+    return _lib
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+class Oracle:
+    """CPU reference solver over one set of Hsolve-format arrays."""
+
+    def __init__(self, m, *, givals: bool = True, conc=None, yvals=None,
+                 ydivs: int = 0, ymin: float = 0.0, invdy: float = 0.0, storemode: int = 0,
+                 nelm_names: int = 0):
+        L = oracle_lib()
+        self.m = m
+        # keep every array alive for the lifetime of the handle
+        self._keep = dict(
+            ops=np.ascontiguousarray(m.ops, np.int32), chip=np.ascontiguousarray(m.chip, np.float64),
+            tablist=np.ascontiguousarray(m.tablist, np.float64),
+            xvals=np.ascontiguousarray(m.xvals, np.float64),
+            stablist=np.ascontiguousarray(m.stablist, np.float64),
+            funcs=np.ascontiguousarray(m.funcs, np.int32),
+            ravals=np.ascontiguousarray(m.ravals, np.float64),
+            vm=np.ascontiguousarray(m.vm, np.float64),
+            givals=np.ascontiguousarray(m.givals, np.float64),
+        )
+        k = self._keep
+        a = Arrays()
+        a.ncompts, a.nchips, a.nops = m.ncompts, k["chip"].size, k["ops"].size
+        a.nfuncs, a.nravals = k["funcs"].size, k["ravals"].size
+        a.ncols, a.xdivs, a.ydivs = m.ncols, m.xdivs, ydivs
+        a.nstab, a.ngivals, a.nelm_names = k["stablist"].size // 4, k["givals"].size, nelm_names
+        a.method, a.chanmode, a.storemode = m.method, m.chanmode, storemode
+        a.xmin, a.invdx, a.ymin, a.invdy, a.dt = m.xmin, m.invdx, ymin, invdy, m.dt
+        a.ops, a.chip = _p(k["ops"], C.c_int32), _p(k["chip"], C.c_double)
+        a.tablist = _p(k["tablist"], C.c_double) if k["tablist"].size else None
+        a.xvals = _p(k["xvals"], C.c_double) if k["xvals"].size else None
+        if yvals is not None:
+            k["yvals"] = np.ascontiguousarray(yvals, np.float64)
+            a.yvals = _p(k["yvals"], C.c_double)
+        a.stablist = _p(k["stablist"], C.c_double) if k["stablist"].size else None
+        a.funcs, a.ravals = _p(k["funcs"], C.c_int32), _p(k["ravals"], C.c_double)
+        a.vm = _p(k["vm"], C.c_double)
+        a.givals = _p(k["givals"], C.c_double) if givals else None
+        if conc is not None:
+            k["conc"] = np.ascontiguousarray(conc, np.float64)
+            a.nconcs = k["conc"].size
+            a.conc = _p(k["conc"], C.c_double)
+        self._a = a
+        self.h = L.ohs_create(C.byref(a))
+        self._hooks = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            oracle_lib().ohs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_hooks(self, syn=None, rnd=None):
+        f1 = SYNFN(syn) if syn else SYNFN(0)
+        f2 = RANDFN(rnd) if rnd else RANDFN(0)
+        self._hooks = (f1, f2)
+        oracle_lib().ohs_set_hooks(self.h, f1, f2, None)
+
+    def step(self, n: int = 1):
+        if oracle_lib().ohs_step(self.h, n) != 0:
+            raise RuntimeError("oracle: undefined chip operation")
+
+    def chip_update(self):
+        if oracle_lib().ohs_chip_update(self.h) != 0:
+            raise RuntimeError("oracle: undefined chip operation")
+
+    def euler_solve(self):
+        oracle_lib().ohs_euler_solve(self.h)
+
+    def crank_solve(self):
+        oracle_lib().ohs_crank_solve(self.h)
+
+    @property
+    def vm(self):
+        v = np.empty(self.m.ncompts)
+        oracle_lib().ohs_get(self.h, _p(v, C.c_double), None, None, None, None)
+        return v
+
+    def state(self):
+        m = self.m
+        vm = np.empty(m.ncompts)
+        chip = np.empty(self._keep["chip"].size)
+        res = np.empty(2 * m.ncompts)
+        ops = np.empty(self._keep["ops"].size, np.int32)
+        giv = np.empty(self._keep["givals"].size)
+        oracle_lib().ohs_get(self.h, _p(vm, C.c_double), _p(chip, C.c_double), _p(res, C.c_double),
+                             _p(ops, C.c_int32), _p(giv, C.c_double))
+        return dict(vm=vm, chip=chip, results=res, ops=ops, givals=giv)
+
+    def put(self, vm=None, chip=None, ops=None):
+        keep = []
+        def pp(a, dt, ct):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return _p(a, ct)
+        oracle_lib().ohs_put(self.h, pp(vm, np.float64, C.c_double), pp(chip, np.float64, C.c_double),
+                             pp(ops, np.int32, C.c_int32))
+
+    def spikes(self):
+        L = oracle_lib()
+        n = L.ohs_nspikes(self.h)
+        a = np.empty(n, np.int32)
+        b = np.empty(n, np.int32)
+        if n:
+            L.ohs_spikes(self.h, _p(a, C.c_int32), _p(b, C.c_int32))
+        return a, b
+
+    def trace(self):
+        t = np.empty(self.m.ncompts, np.uint64)
+        oracle_lib().ohs_trace(self.h, t.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return t
+
+    def trace_enable(self, on=True):
+        oracle_lib().ohs_trace_enable(self.h, 1 if on else 0)
+
+
+def step_parallel(oracles, nsteps: int, nthreads: int):
+    L = oracle_lib()
+    arr = (C.c_void_p * len(oracles))(*[o.h for o in oracles])
+    if L.ohs_step_parallel(arr, len(oracles), nsteps, nthreads) != 0:
+        raise RuntimeError("oracle: undefined chip operation")
