@@ -1,0 +1,428 @@
+// hsh_ingest.cpp -- host-side analysis of Hsolve arrays into the device layout.
+//
+// Three analysers:
+//  * ops analyser: walks ops[] exactly as the reference interpreter does
+//    (genesis/src/hines/hines_chip.c:157-600), resolves the registers that
+//    the reference carries across compartment boundaries (the Ek register that
+//    CHAN_OP inherits, hines_chip_init.c:1029-1049; the table-row cache,
+//    hines_chip.c:178-190) and rewrites each compartment's program into device
+//    micro-ops.  Compartments whose programs are identical share a template.
+//    Anything the device interpreter cannot run is refused with a reason.
+//  * tree analyser: decodes funcs[]/ravals[] (hines_solve.c:223-296, emitted
+//    by hines_init.c:604-742, :1011-1172) into a parent-row array and the
+//    three coefficients of each non-root row, and checks that every tree is a
+//    contiguous row block whose kids are eliminated in ascending row order.
+//  * tiler: groups neurons with the same size and the same per-row templates
+//    into 64-lane tiles, so every wave executes a uniform program.
+#include <algorithm>
+#include <cstring>
+#include <map>
+
+#include "hsh_internal.h"
+
+namespace hsh {
+
+namespace {
+
+void refuse(Plan& p, int comp, int opcode, int opidx, int reason)
+{
+    p.refusals.push_back(hsh_refusal{comp, opcode, opidx, reason});
+}
+
+// Ek register state carried across compartments.
+enum EkKind { EK_NONE, EK_CONST, EK_DYN };
+
+}  // namespace
+
+bool analyse(const hsh_model& m, Plan& P, std::string& err)
+{
+    const int n = m.ncompts;
+    const int32_t* ops = m.ops;
+    const double* chip = m.chip;
+    if (n <= 0 || !ops || !chip || !m.vm || !m.funcs) { err = "empty or missing arrays"; return false; }
+    if (m.calcmode != 1) {
+        refuse(P, -1, 0, -1, HSH_R_CALCMODE);
+    }
+
+    // ------------------------------------------------------------------
+    // ops analyser
+    P.comp_tmpl.assign(n, -1);
+    P.comp_dsrc_off.assign(n + 1, 0);
+    P.comp_isrc_off.assign(n + 1, 0);
+    std::map<std::vector<int32_t>, int> tmpl_index;
+    std::vector<char> has_volt(n, 0);
+
+    int64_t p = 0, c = 0;
+    EkKind ek = EK_NONE;
+    int64_t eksrc = -1;       // chip index of the CHAN_EK_OP value in the register
+    int comp = -1;
+    auto bad = [&](const char* what) {
+        err = std::string("malformed ops[]: ") + what + " at ops index " + std::to_string(p);
+        return false;
+    };
+    for (;;) {
+        if (p >= m.nops) return bad("missing LCOMPT_OP");
+        int32_t head = ops[p++];
+        if (head == LCOMPT_OP) break;
+        if (head != COMPT_OP && head != FCOMPT_OP) return bad("expected COMPT_OP");
+        comp++;
+        if (comp >= n) return bad("more compartments than ncompts");
+        std::vector<int32_t> code;
+        std::vector<int64_t> dsrc;
+        std::vector<int64_t> isrc;
+        bool refused = false;
+        int ngates = 0;
+        bool ek_intra = false;    // Ek register set in this compartment
+        EkKind ek_in = ek;        // live-in Ek
+        (void)0;
+        if (c + 2 > m.nchips) return bad("chip[] too short");
+        dsrc.push_back(c); dsrc.push_back(c + 1);   // Em/Rm, inject
+        c += 2;
+        auto need_chip = [&](int k) { return c + k <= m.nchips; };
+        while (p < m.nops && ops[p] > LCOMPT_OP) {
+            int64_t opidx = p;
+            int32_t op = ops[p++];
+            bool to_cond = false;
+            switch (op) {
+            case NEWVOLT_OP:
+                code.push_back(U_NEWVOLT);
+                has_volt[comp] = 1;
+                break;
+            case SPIKE_OP:
+                if (!need_chip(1) || p + 2 > m.nops) return bad("SPIKE_OP operands");
+                code.push_back(U_SPIKE);
+                code.push_back((int32_t)isrc.size());
+                isrc.push_back(p);                   // counter (mutable)
+                isrc.push_back(p + 1);               // reload
+                isrc.push_back(-(1 + opidx));        // ops index, literal
+                dsrc.push_back(c);                   // threshold
+                c += 1;
+                p += 2;
+                break;
+            case CONC_VAL_OP: p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case DOMAIN_OP: c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case NEWCONC2_OP: p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case NEWCONC1_OP: refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case ELMMPUMP_OP: c += 3; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case FLUX_OP: refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case STORE_ALLI_OP: case STORE_ALLC_OP:
+                p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case SCALE_FLUX_OP: c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case GHK_OP:
+                c += 3; ek = EK_DYN; ek_intra = true; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true;
+                to_cond = true; break;
+            case SAME_GHK_OP:
+                c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; to_cond = true; break;
+            case DO_TABGHK_OP:
+                refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; to_cond = true; break;
+            case NERNST_OP:
+                c += 2; ek = EK_DYN; ek_intra = true; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true;
+                to_cond = true; break;
+            case CHAN_EK_OP:
+                if (!need_chip(2)) return bad("chip[] too short");
+                code.push_back(U_CHAN_EK);
+                dsrc.push_back(c); dsrc.push_back(c + 1);
+                ek = EK_CONST; eksrc = c; ek_intra = true;
+                c += 2;
+                to_cond = true;
+                break;
+            case CHAN_OP:
+                if (!need_chip(1)) return bad("chip[] too short");
+                if (ek_intra && ek == EK_DYN) {
+                    code.push_back(U_CHAN);
+                    dsrc.push_back(c);
+                } else if (!ek_intra && ek_in == EK_DYN) {
+                    refuse(P, comp, op, (int)opidx, HSH_R_EK_LIVEIN);
+                    refused = true;
+                } else {
+                    // The register holds a constant: the chip value of an
+                    // earlier CHAN_EK_OP (this compartment or a previous one),
+                    // or its initial 0.0.  Materialise it as an explicit slot
+                    // that reads the same chip value.
+                    code.push_back(U_CHAN_EK);
+                    if (eksrc >= 0) {
+                        dsrc.push_back(eksrc);
+                    } else {
+                        P.synth.push_back(0.0);
+                        dsrc.push_back(-(int64_t)P.synth.size());
+                    }
+                    dsrc.push_back(c);
+                    ek = EK_CONST; ek_intra = true;
+                }
+                c += 1;
+                to_cond = true;
+                break;
+            default:
+                refuse(P, comp, op, (int)opidx, HSH_R_OPCODE);
+                err = "undefined chip operation " + std::to_string(op) + " at ops index " +
+                      std::to_string(opidx);
+                return false;
+            }
+            if (!to_cond) continue;
+            // conductance ops until the current is added (hines_chip.c:404-590)
+            for (;;) {
+                if (p >= m.nops) return bad("unterminated channel");
+                int64_t kidx = p;
+                int32_t k = ops[p++];
+                if (k == IPOL1V_OP || k == IPOL1C_OP || k == IPOL2V_OP || k == IPOL2C_OP) {
+                    if (p + 2 > m.nops) return bad("IPOL operands");
+                    int32_t col = ops[p], pw = ops[p + 1];
+                    p += 2;
+                    if (pw != 0 && !need_chip(1)) return bad("chip[] too short");
+                    if (k != IPOL1V_OP) {
+                        refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
+                        if (pw != 0) c += 1;
+                        if (pw == 0) break;
+                        continue;
+                    }
+                    if (pw == 0) {   // tabcurrent / tab-GHK: Ek from a table
+                        refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
+                        ek = EK_DYN; ek_intra = true;
+                        break;
+                    }
+                    if (pw < -6 || pw > 6) return bad("gate power out of range");
+                    if (col < 0 || col + 1 >= m.ncols || !m.tablist)
+                        return bad("IPOL1V table column out of range");
+                    if (!has_volt[comp]) {
+                        refuse(P, comp, k, (int)kidx, HSH_R_TABLE_LIVEIN); refused = true;
+                    }
+                    code.push_back(U_IPOL1V);
+                    code.push_back(col);
+                    code.push_back(pw);
+                    dsrc.push_back(c);
+                    c += 1;
+                    ngates++;
+                    continue;
+                } else if (k == SYN2_OP || k == SYN3_OP) {
+                    if (k == SYN3_OP) c += 1;
+                    p += 3;
+                    c += 2;
+                    if (p < m.nops && ops[p] == MGBLOCK_OP) { p += 1; c += 3; }
+                    refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
+                    break;
+                } else if (k == ADD_CURR_OP) {
+                    code.push_back(U_ADDCURR);
+                    break;
+                } else {
+                    err = "undefined chip operation #2 " + std::to_string(k) + " at ops index " +
+                          std::to_string(kidx);
+                    return false;
+                }
+            }
+        }
+        if (c + 2 > m.nchips) return bad("chip[] too short at compartment end");
+        code.push_back(U_END);
+        dsrc.push_back(c); dsrc.push_back(c + 1);   // tbyc, diagterm
+        c += 2;
+        if (refused) continue;
+        Template t;
+        t.code = code;
+        t.nslots = (int)dsrc.size();
+        t.nislots = (int)isrc.size();
+        t.ngates = ngates;
+        std::vector<int32_t> key = code;
+        key.push_back(t.nslots);
+        key.push_back(t.nislots);
+        auto it = tmpl_index.find(key);
+        int ti;
+        if (it == tmpl_index.end()) {
+            ti = (int)P.templates.size();
+            tmpl_index[key] = ti;
+            P.templates.push_back(t);
+        } else {
+            ti = it->second;
+        }
+        P.comp_tmpl[comp] = ti;
+        P.comp_dsrc_off[comp] = (int64_t)P.dsrc.size();
+        P.comp_isrc_off[comp] = (int64_t)P.isrc.size();
+        P.dsrc.insert(P.dsrc.end(), dsrc.begin(), dsrc.end());
+        P.isrc.insert(P.isrc.end(), isrc.begin(), isrc.end());
+    }
+    if (comp != n - 1) { err = "ops[] holds " + std::to_string(comp + 1) + " compartments, ncompts is " + std::to_string(n); return false; }
+    if (c != m.nchips) { err = "chip[] length does not match the ops[] walk"; return false; }
+
+    // previous NEWVOLT compartment in op order (row-cache live-in)
+    P.pred.assign(n, -1);
+    {
+        int last = -1;
+        for (int i = 0; i < n; i++) {
+            if (has_volt[i]) { P.pred[i] = last; last = i; }
+        }
+    }
+
+    // ------------------------------------------------------------------
+    // tree analyser
+    P.parent.assign(n, -1);
+    P.ta.assign(n, 0.0); P.tcf.assign(n, 0.0); P.tcb.assign(n, 0.0);
+    std::vector<char> got_bwd(n, 0);
+    std::vector<int32_t> lastkid(n, -1);
+    {
+        const int32_t* f = m.funcs;
+        const double* ra = m.ravals;
+        int64_t fi = 0, ri = 0;
+        bool ok = true;
+        if (n > 1) {
+            int cur = 1;
+            for (;;) {
+                if (fi >= m.nfuncs) { err = "funcs[] missing forward FINISH"; return false; }
+                int32_t op = f[fi++];
+                if (op == F_ELIM) {
+                    if (fi >= m.nfuncs || ri + 2 > m.nravals) { err = "funcs[] truncated"; return false; }
+                    int32_t kr = f[fi++];
+                    if (kr < 0 || (kr & 1) || kr / 2 >= n || kr / 2 >= cur) {
+                        err = "FORWARD_ELIM row out of range"; return false;
+                    }
+                    int kid = kr / 2;
+                    if (P.parent[kid] != -1) { ok = false; refuse(P, kid, op, (int)(fi - 2), HSH_R_TREE); }
+                    if (kid <= lastkid[cur]) { ok = false; refuse(P, cur, op, (int)(fi - 2), HSH_R_TREE); }
+                    lastkid[cur] = kid;
+                    P.parent[kid] = cur;
+                    P.ta[kid] = ra[ri++];
+                    P.tcf[kid] = ra[ri++];
+                } else if (op == F_SET_DIAG) {
+                    cur += 1;
+                } else if (op == F_SKIP_DIAG) {
+                    cur += 2;
+                } else if (op == F_FINISH) {
+                    break;
+                } else {
+                    refuse(P, cur, op, (int)(fi - 1), HSH_R_FUNCS);
+                    ok = false;
+                    break;
+                }
+                if (cur >= n) { err = "funcs[] forward pass runs past the last row"; return false; }
+            }
+            if (ok) {
+                int row = n - 2;
+                for (;;) {
+                    if (fi >= m.nfuncs) { err = "funcs[] missing backward FINISH"; return false; }
+                    int32_t op = f[fi++];
+                    if (op == F_ELIM) {
+                        if (fi >= m.nfuncs || ri + 1 > m.nravals || row < 0) { err = "funcs[] truncated"; return false; }
+                        int32_t pr = f[fi++];
+                        if (pr != 2 * P.parent[row] || got_bwd[row]) {
+                            refuse(P, row, op, (int)(fi - 2), HSH_R_TREE);
+                            ok = false;
+                            break;
+                        }
+                        got_bwd[row] = 1;
+                        P.tcb[row] = ra[ri++];
+                    } else if (op == F_CALC) {
+                        row--;
+                    } else if (op == F_FINISH) {
+                        break;
+                    } else {
+                        refuse(P, row, op, (int)(fi - 1), HSH_R_FUNCS);
+                        ok = false;
+                        break;
+                    }
+                }
+                if (ok && row != -1) { err = "funcs[] backward pass does not cover every row"; return false; }
+                for (int r = 0; ok && r < n - 1; r++)
+                    if (P.parent[r] >= 0 && !got_bwd[r]) {
+                        refuse(P, r, 0, -1, HSH_R_TREE);
+                        ok = false;
+                    }
+            }
+        }
+        if (ok) {
+            // trees are contiguous row blocks ending at their root
+            std::vector<int32_t> root(n);
+            for (int r = n - 1; r >= 0; r--) root[r] = (P.parent[r] < 0) ? r : root[P.parent[r]];
+            int lo = 0;
+            for (int r = 0; r < n; r++) {
+                if (root[r] != r) {
+                    if (root[r] < r) { ok = false; refuse(P, r, 0, -1, HSH_R_TREE); break; }
+                    continue;
+                }
+                for (int q = lo; q <= r; q++)
+                    if (root[q] != r) { ok = false; refuse(P, q, 0, -1, HSH_R_TREE); break; }
+                if (!ok) break;
+                P.tree_lo.push_back(lo);
+                P.tree_hi.push_back(r);
+                lo = r + 1;
+            }
+        }
+    }
+    if (!P.refusals.empty()) return true;
+
+    // ------------------------------------------------------------------
+    // tiler
+    const int ntrees = (int)P.tree_lo.size();
+    std::map<std::vector<int32_t>, std::vector<int32_t>> groups;   // signature -> trees
+    for (int t = 0; t < ntrees; t++) {
+        std::vector<int32_t> sig;
+        sig.push_back(P.tree_hi[t] - P.tree_lo[t] + 1);
+        for (int r = P.tree_lo[t]; r <= P.tree_hi[t]; r++) sig.push_back(P.comp_tmpl[r]);
+        groups[sig].push_back(t);
+    }
+    // larger trees first: they run longest in the persistent kernel
+    std::vector<const std::pair<const std::vector<int32_t>, std::vector<int32_t>>*> order;
+    for (auto& g : groups) order.push_back(&g);
+    std::stable_sort(order.begin(), order.end(), [](auto* a, auto* b) { return a->first[0] > b->first[0]; });
+
+    // program array: templates concatenated
+    std::vector<int32_t> tprog(P.templates.size());
+    for (size_t t = 0; t < P.templates.size(); t++) {
+        tprog[t] = (int32_t)P.prog.size();
+        P.prog.insert(P.prog.end(), P.templates[t].code.begin(), P.templates[t].code.end());
+    }
+    P.comp_pos.assign(n, -1);
+    int64_t nS = 0, nI = 0;
+    for (auto* g : order) {
+        const std::vector<int32_t>& sig = g->first;
+        const std::vector<int32_t>& trees = g->second;
+        int R = sig[0];
+        P.max_rows = std::max(P.max_rows, R);
+        for (size_t t0 = 0; t0 < trees.size(); t0 += W) {
+            TileMeta tm;
+            tm.block0 = (int32_t)P.blocks.size();
+            tm.rows = R;
+            tm.lanes = (int32_t)std::min<size_t>(W, trees.size() - t0);
+            tm.pad = 0;
+            int tile = (int)P.tiles.size();
+            P.tiles.push_back(tm);
+            for (int r = 0; r < R; r++) {
+                int ti = sig[1 + r];
+                BlockMeta bm;
+                bm.prog = tprog[ti];
+                bm.dslot = (int32_t)nS;
+                bm.islot = (int32_t)nI;
+                bm.tile = tile;
+                int b = (int)P.blocks.size();
+                P.blocks.push_back(bm);
+                nS += (int64_t)P.templates[ti].nslots * W;
+                nI += (int64_t)P.templates[ti].nislots * W;
+                for (int l = 0; l < tm.lanes; l++) {
+                    int row = P.tree_lo[trees[t0 + l]] + r;
+                    P.comp_pos[row] = b * W + l;
+                }
+            }
+        }
+    }
+    if (nS > INT32_MAX || nI > INT32_MAX || (int64_t)P.blocks.size() * W > INT32_MAX) {
+        err = "model too large for 32-bit slot offsets";
+        return false;
+    }
+    P.nS = nS;
+    P.nI = nI;
+    P.pos_comp.assign(P.blocks.size() * W, -1);
+    for (int r = 0; r < n; r++) P.pos_comp[P.comp_pos[r]] = r;
+    // padding lanes duplicate lane 0 of their block
+    for (size_t b = 0; b < P.blocks.size(); b++)
+        for (int l = 1; l < W; l++)
+            if (P.pos_comp[b * W + l] < 0) P.pos_comp[b * W + l] = P.pos_comp[b * W];
+
+    // algorithmic bytes of one step (fp64, single pass): Vm r+w, gate states
+    // r+w, read-only slots, tree coefficients + parent for non-root rows.
+    double bytes = 0;
+    for (int r = 0; r < n; r++) {
+        const Template& t = P.templates[P.comp_tmpl[r]];
+        bytes += 16.0 + 16.0 * t.ngates + 8.0 * (t.nslots - t.ngates) + 8.0 * t.nislots;
+        if (P.parent[r] >= 0) bytes += 28.0;
+    }
+    P.bytes_per_step = bytes;
+    return true;
+}
+
+}  // namespace hsh

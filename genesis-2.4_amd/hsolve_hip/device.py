@@ -1,0 +1,260 @@
+"""ctypes binding of libhsolve_hip (include/hsolve_hip.h).
+
+``HipSolver`` mirrors the reference accelerator's per-hsolve life cycle
+(genesis/src/hines/cuda/cuda_hsolve.c:289-517): create from the hsolve's
+arrays (cuda_init / cuda_backend_init), step (cuda_chip_update with its
+"vm ready" return protocol), read state back (cuda_sync_chip), destroy
+(cuda_backend_cleanup).  There is no CPU fallback inside: if the library is
+missing or the device refuses the model, the caller sees an exception or the
+refusal list, exactly as GENESIS glue would see HSH_REFUSED and keep its own
+CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._paths import lib_path
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+HSH_OK, HSH_REFUSED = 0, 1
+PLAIN, BATCHED, CHANNELS = 0, 1, 2
+T_TOTAL, T_CHANNEL, T_ELIM = 0, 1, 2
+F_NO_AUTOTUNE = 1
+
+REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
+           5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode"}
+
+
+class HshModel(C.Structure):
+    _fields_ = [
+        ("ncompts", C.c_int32), ("nchips", C.c_int32), ("nops", C.c_int32),
+        ("nfuncs", C.c_int32), ("nravals", C.c_int32), ("ncols", C.c_int32),
+        ("xdivs", C.c_int32), ("nstab", C.c_int32), ("method", C.c_int32),
+        ("chanmode", C.c_int32), ("calcmode", C.c_int32), ("nconcs", C.c_int32),
+        ("dt", C.c_double), ("xmin", C.c_double), ("invdx", C.c_double),
+        ("ops", _i32p), ("chip", _f64p), ("tablist", _f64p), ("xvals", _f64p),
+        ("stablist", _f64p), ("funcs", _i32p), ("ravals", _f64p), ("vm", _f64p),
+    ]
+
+
+class HshOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("batch_k", C.c_int32), ("batch_waves", C.c_int32),
+                ("flags", C.c_int32)]
+
+
+class HshRefusal(C.Structure):
+    _fields_ = [("comp", C.c_int32), ("opcode", C.c_int32), ("op_index", C.c_int32),
+                ("reason", C.c_int32)]
+
+
+class HshStepOut(C.Structure):
+    _fields_ = [("vm_ready", C.c_int32), ("nspikes", C.c_int32), ("launches", C.c_int32),
+                ("pad", C.c_int32)]
+
+
+class HshInfo(C.Structure):
+    _fields_ = [("ntiles", C.c_int32), ("nblocks", C.c_int32), ("ntemplates", C.c_int32),
+                ("ntrees", C.c_int32), ("max_rows", C.c_int32), ("batch_k", C.c_int32),
+                ("batch_waves", C.c_int32), ("refused", C.c_int32),
+                ("bytes_per_step", C.c_double), ("lane_fill", C.c_double)]
+
+
+# every symbol include/hsolve_hip.h declares
+EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
+           "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get",
+           "hsh_last_time", "hsh_set_timing", "hsh_destroy", "hsh_last_error")
+
+_lib = None
+
+
+def lib():
+    """Load libhsolve_hip.so (raises OSError if it is not built)."""
+    global _lib
+    if _lib is None:
+        L = C.CDLL(lib_path("libhsolve_hip.so"))
+        L.hsh_create.argtypes = [C.POINTER(HshModel), C.POINTER(HshOptions), C.POINTER(C.c_void_p)]
+        L.hsh_create.restype = C.c_int32
+        L.hsh_refusals.argtypes = [C.c_void_p, C.POINTER(HshRefusal), C.c_int32, _i32p]
+        L.hsh_refusals.restype = C.c_int32
+        L.hsh_step.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(HshStepOut)]
+        L.hsh_step.restype = C.c_int32
+        L.hsh_sync.argtypes = [C.c_void_p]
+        L.hsh_sync.restype = C.c_int32
+        L.hsh_get_state.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _i32p]
+        L.hsh_get_state.restype = C.c_int32
+        L.hsh_put_state.argtypes = [C.c_void_p, _f64p, _f64p, _i32p]
+        L.hsh_put_state.restype = C.c_int32
+        L.hsh_set_chip.argtypes = [C.c_void_p, C.c_int32, _i32p, _f64p]
+        L.hsh_set_chip.restype = C.c_int32
+        L.hsh_drain_spikes.argtypes = [C.c_void_p, _i32p, _i32p, C.c_int32, _i32p]
+        L.hsh_drain_spikes.restype = C.c_int32
+        L.hsh_info_get.argtypes = [C.c_void_p, C.POINTER(HshInfo)]
+        L.hsh_info_get.restype = C.c_int32
+        L.hsh_last_time.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+        L.hsh_last_time.restype = C.c_int32
+        L.hsh_set_timing.argtypes = [C.c_void_p, C.c_int32]
+        L.hsh_set_timing.restype = C.c_int32
+        L.hsh_destroy.argtypes = [C.c_void_p]
+        L.hsh_destroy.restype = None
+        L.hsh_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+class HshError(RuntimeError):
+    pass
+
+
+class Refused(Exception):
+    """The device path refuses this model; ``refusals`` lists why."""
+
+    def __init__(self, msg, refusals):
+        super().__init__(msg)
+        self.refusals = refusals
+
+
+def _p(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise HshError(f"{what}: {lib().hsh_last_error().decode()} (status {rc})")
+    return rc
+
+
+class HipSolver:
+    """One hsolve on the GPU.  ``model`` is a :class:`hsolve_hip.model.Model`."""
+
+    def __init__(self, model, *, device: int = 0, batch_k: int = 0, batch_waves: int = 0,
+                 flags: int = 0):
+        L = lib()
+        self.m = model
+        k = dict(ops=np.ascontiguousarray(model.ops, np.int32),
+                 chip=np.ascontiguousarray(model.chip, np.float64),
+                 tablist=np.ascontiguousarray(model.tablist, np.float64),
+                 xvals=np.ascontiguousarray(model.xvals, np.float64),
+                 stablist=np.ascontiguousarray(model.stablist, np.float64),
+                 funcs=np.ascontiguousarray(model.funcs, np.int32),
+                 ravals=np.ascontiguousarray(model.ravals, np.float64),
+                 vm=np.ascontiguousarray(model.vm, np.float64))
+        mm = HshModel()
+        mm.ncompts, mm.nchips, mm.nops = model.ncompts, k["chip"].size, k["ops"].size
+        mm.nfuncs, mm.nravals, mm.ncols = k["funcs"].size, k["ravals"].size, model.ncols
+        mm.xdivs, mm.nstab = model.xdivs, k["stablist"].size // 4
+        mm.method, mm.chanmode, mm.calcmode, mm.nconcs = model.method, model.chanmode, 1, 0
+        mm.dt, mm.xmin, mm.invdx = model.dt, model.xmin, model.invdx
+        mm.ops, mm.chip = _p(k["ops"], C.c_int32), _p(k["chip"], C.c_double)
+        mm.tablist = _p(k["tablist"], C.c_double) if k["tablist"].size else None
+        mm.xvals = _p(k["xvals"], C.c_double) if k["xvals"].size else None
+        mm.stablist = _p(k["stablist"], C.c_double) if k["stablist"].size else None
+        mm.funcs, mm.ravals = _p(k["funcs"], C.c_int32), _p(k["ravals"], C.c_double)
+        mm.vm = _p(k["vm"], C.c_double)
+        opt = HshOptions(device, batch_k, batch_waves, flags)
+        h = C.c_void_p()
+        rc = L.hsh_create(C.byref(mm), C.byref(opt), C.byref(h))
+        self.h = h
+        if rc == HSH_REFUSED:
+            refs = self.refusals()
+            L.hsh_destroy(self.h)
+            self.h = None
+            raise Refused(L.hsh_last_error().decode(), refs)
+        _check(rc, "hsh_create")
+
+    # -- life cycle
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hsh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- queries
+    def refusals(self):
+        L = lib()
+        n = C.c_int32()
+        _check(L.hsh_refusals(self.h, None, 0, C.byref(n)), "hsh_refusals")
+        arr = (HshRefusal * max(1, n.value))()
+        _check(L.hsh_refusals(self.h, arr, n.value, C.byref(n)), "hsh_refusals")
+        return [(r.comp, r.opcode, r.op_index, REASONS.get(r.reason, str(r.reason)))
+                for r in arr[:n.value]]
+
+    def info(self) -> dict:
+        i = HshInfo()
+        _check(lib().hsh_info_get(self.h, C.byref(i)), "hsh_info_get")
+        return {f: getattr(i, f) for f, _ in HshInfo._fields_}
+
+    # -- stepping
+    def step(self, n: int = 1, mode: int = PLAIN) -> dict:
+        o = HshStepOut()
+        _check(lib().hsh_step(self.h, n, mode, C.byref(o)), "hsh_step")
+        return dict(vm_ready=o.vm_ready, launches=o.launches)
+
+    def sync(self):
+        _check(lib().hsh_sync(self.h), "hsh_sync")
+
+    def set_timing(self, on: bool = True):
+        _check(lib().hsh_set_timing(self.h, 1 if on else 0), "hsh_set_timing")
+
+    def last_time(self, which: int = T_TOTAL) -> float:
+        ms = C.c_double()
+        _check(lib().hsh_last_time(self.h, which, C.byref(ms)), "hsh_last_time")
+        return ms.value
+
+    # -- state
+    @property
+    def vm(self):
+        v = np.empty(self.m.ncompts)
+        _check(lib().hsh_get_state(self.h, _p(v, C.c_double), None, None, None), "hsh_get_state")
+        return v
+
+    def state(self) -> dict:
+        m = self.m
+        vm = np.empty(m.ncompts)
+        chip = np.empty(m.chip.size)
+        res = np.empty(2 * m.ncompts)
+        ops = np.empty(m.ops.size, np.int32)
+        _check(lib().hsh_get_state(self.h, _p(vm, C.c_double), _p(chip, C.c_double),
+                                   _p(res, C.c_double), _p(ops, C.c_int32)), "hsh_get_state")
+        return dict(vm=vm, chip=chip, results=res, ops=ops)
+
+    def put_state(self, vm=None, chip=None, ops=None):
+        keep = []
+
+        def pp(a, dt, ct):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dt)
+            keep.append(a)
+            return _p(a, ct)
+        _check(lib().hsh_put_state(self.h, pp(vm, np.float64, C.c_double),
+                                   pp(chip, np.float64, C.c_double), pp(ops, np.int32, C.c_int32)),
+               "hsh_put_state")
+
+    def set_chip(self, idx, val):
+        idx = np.ascontiguousarray(idx, np.int32)
+        val = np.ascontiguousarray(val, np.float64)
+        _check(lib().hsh_set_chip(self.h, idx.size, _p(idx, C.c_int32), _p(val, C.c_double)),
+               "hsh_set_chip")
+
+    def drain_spikes(self, cap: int = 1 << 20):
+        a = np.empty(cap, np.int32)
+        b = np.empty(cap, np.int32)
+        n = C.c_int32()
+        _check(lib().hsh_drain_spikes(self.h, _p(a, C.c_int32), _p(b, C.c_int32), cap, C.byref(n)),
+               "hsh_drain_spikes")
+        return a[:n.value].copy(), b[:n.value].copy()

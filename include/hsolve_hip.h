@@ -1,0 +1,173 @@
+/*
+ * hsolve_hip.h -- C ABI of libhsolve_hip, the MI355X (gfx950) backend for the
+ * chanmode-4/5 step of GENESIS's hsolve Hines solver.
+ *
+ * The library takes the arrays an hsolve holds after SETUP/RESET (the ops[]
+ * channel program, chip[] values, the composite gate table, the funcs[] /
+ * ravals[] elimination program and the Hines-ordered vm[]), copies them once
+ * into device-resident structure-of-arrays, and advances the solver on the
+ * GPU.  Plain pointers and sizes only; no GENESIS or torch types.
+ *
+ * Reference interfaces replaced (genesis/src/hines/cuda/):
+ *   hsh_create       cuda_backend_init (cuda_backend.cu:157-259) +
+ *                    cuda_backend_tree_init (cuda_backend.cu:396-444), and the
+ *                    refusal scan of cuda_init (cuda_hsolve.c:289-379)
+ *   hsh_refusals     the stderr opcode report of cuda_init (cuda_hsolve.c:321-341)
+ *   hsh_step         cuda_backend_perstep (cuda_backend.cu:263-316),
+ *                    cuda_backend_multiloop (:338-385) and
+ *                    cuda_backend_multiloop_tree (:461-527); return protocol of
+ *                    cuda_chip_update (cuda_hsolve.c:381-495, hines.c:225-252)
+ *   hsh_get_state    cuda_backend_sync_chip (cuda_backend.cu:529-538) /
+ *                    cuda_sync_chip (cuda_hsolve.h:24)
+ *   hsh_put_state    cuda_backend_upload_chip (cuda_backend.cu:318-336)
+ *   hsh_set_chip     per-step chip writes of h_in_msgs (hines_msgs.c:351-405)
+ *   hsh_destroy      cuda_backend_cleanup (cuda_backend.cu:540-568)
+ *
+ * Conventions: statuses are hsh_status codes (negative = error; HSH_REFUSED is
+ * not an error: the caller falls back to its CPU path for this solver).  No
+ * function calls exit().  One handle per hsolve; a handle is not thread-safe,
+ * distinct handles are.  Each handle owns one HIP stream and its graphs.
+ */
+#ifndef HSOLVE_HIP_H
+#define HSOLVE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hsh_status {
+    HSH_OK = 0,
+    HSH_REFUSED = 1,         /* model holds something the device path cannot run */
+    HSH_ERR_ARG = -1,        /* bad argument or malformed arrays */
+    HSH_ERR_DEVICE = -2,     /* HIP runtime error (message in hsh_last_error) */
+    HSH_ERR_NOMEM = -3,
+    HSH_ERR_STATE = -4,      /* call not valid in this state (e.g. refused handle) */
+    HSH_ERR_LIMIT = -5       /* a size exceeds a documented maximum */
+} hsh_status;
+
+/* integration methods (genesis/src/sim/sim_defs.h:174-175) */
+#define HSH_BEULER 10
+#define HSH_CRANK  11
+
+/* Hsolve-format input.  All arrays are host pointers, copied at hsh_create;
+ * the caller keeps ownership.  Sizes and layouts are those of struct
+ * hsolve_type (genesis/src/hines/hines_struct.h:126-417). */
+typedef struct hsh_model {
+    int32_t ncompts;         /* rows, Hines order */
+    int32_t nchips;
+    int32_t nops;            /* ops[] length including the LCOMPT_OP terminator */
+    int32_t nfuncs;
+    int32_t nravals;
+    int32_t ncols;           /* composite gate-table columns */
+    int32_t xdivs;           /* table rows are xdivs+1 (plus one guard row) */
+    int32_t nstab;           /* synapse types; stablist holds 4*nstab values */
+    int32_t method;          /* HSH_BEULER or HSH_CRANK */
+    int32_t chanmode;        /* 4 or 5 */
+    int32_t calcmode;        /* 1 = interpolated tables (the only supported mode) */
+    int32_t nconcs;          /* concentration pools (conc ops are refused) */
+    double  dt;
+    double  xmin;
+    double  invdx;
+    const int32_t *ops;      /* [nops] */
+    const double  *chip;     /* [nchips] */
+    const double  *tablist;  /* [(xdivs+2)*ncols]: row xdivs+1 is a guard row that
+                                repeats row xdivs (the reference reads that row
+                                when Vm >= xmax, with interpolation weight 0) */
+    const double  *xvals;    /* [xdivs+2] */
+    const double  *stablist; /* [4*nstab] or NULL */
+    const int32_t *funcs;    /* [nfuncs] */
+    const double  *ravals;   /* [nravals] */
+    const double  *vm;       /* [ncompts] */
+} hsh_model;
+
+typedef struct hsh_options {
+    int32_t device;          /* HIP device ordinal */
+    int32_t batch_k;         /* steps per batched launch; 0 = autotuned */
+    int32_t batch_waves;     /* waves per batched workgroup; 0 = autotuned */
+    int32_t flags;           /* HSH_F_* */
+} hsh_options;
+
+#define HSH_F_NO_AUTOTUNE 1  /* use batch_k / batch_waves (or defaults) as given */
+
+/* refusal reasons */
+#define HSH_R_OPCODE         1   /* opcode the device interpreter does not run */
+#define HSH_R_EK_LIVEIN      2   /* channel inherits a computed Ek from an earlier compartment */
+#define HSH_R_TABLE_LIVEIN   3   /* gate lookup without NEWVOLT in its compartment */
+#define HSH_R_REGISTER_LIVEIN 4  /* other register carried in from an earlier compartment */
+#define HSH_R_FUNCS          5   /* elimination program the device solver does not run
+                                    (sibling ops of symcompartments) */
+#define HSH_R_TREE           6   /* tree rows are not contiguous / kid order not ascending */
+#define HSH_R_CALCMODE       7   /* calcmode 0 (no interpolation) */
+
+typedef struct hsh_refusal {
+    int32_t comp;            /* row (Hines number), -1 for whole-solver reasons */
+    int32_t opcode;          /* offending ops[]/funcs[] code, 0 if none */
+    int32_t op_index;        /* index into ops[] or funcs[], -1 if none */
+    int32_t reason;          /* HSH_R_* */
+} hsh_refusal;
+
+/* step modes */
+#define HSH_PLAIN    0       /* per-step drop-in: channel kernel then elimination
+                                kernel, one step per pair of launches */
+#define HSH_BATCHED  1       /* persistent fused kernel, K steps per launch, replayed
+                                from a captured hipGraph */
+#define HSH_CHANNELS 2       /* channel interpreter only: results[] on the device,
+                                vm untouched (reference return-0 protocol) */
+
+typedef struct hsh_step_out {
+    int32_t vm_ready;        /* 1: vm[] is final for these steps (skip the CPU
+                                elimination); 0: only results[] were produced */
+    int32_t nspikes;         /* spike events raised during these steps */
+    int32_t launches;        /* kernel launches (or graph replays) issued */
+    int32_t pad;
+} hsh_step_out;
+
+typedef struct hsh_info {
+    int32_t ntiles;          /* 64-lane tiles of neurons */
+    int32_t nblocks;         /* (tile,row) blocks */
+    int32_t ntemplates;      /* distinct channel programs */
+    int32_t ntrees;          /* neurons */
+    int32_t max_rows;        /* largest neuron */
+    int32_t batch_k;         /* chosen K */
+    int32_t batch_waves;     /* chosen waves per workgroup */
+    int32_t refused;         /* 1 if the handle is refused */
+    double  bytes_per_step;  /* algorithmic HBM bytes of one plain step */
+    double  lane_fill;       /* real compartments / (blocks*64) */
+} hsh_info;
+
+hsh_status hsh_create(const hsh_model *m, const hsh_options *opt, struct hsh_handle **out);
+hsh_status hsh_refusals(struct hsh_handle *h, hsh_refusal *out, int32_t cap, int32_t *n);
+hsh_status hsh_step(struct hsh_handle *h, int32_t nsteps, int32_t mode, hsh_step_out *out);
+hsh_status hsh_sync(struct hsh_handle *h);
+/* Any pointer may be NULL.  Arrays are in the reference layout:
+ * vm[ncompts], chip[nchips], results[2*ncompts], ops[nops]. */
+hsh_status hsh_get_state(struct hsh_handle *h, double *vm, double *chip, double *results,
+                         int32_t *ops);
+hsh_status hsh_put_state(struct hsh_handle *h, const double *vm, const double *chip,
+                         const int32_t *ops);
+/* Overwrite n chip[] values (reference indices), e.g. INJECT messages. */
+hsh_status hsh_set_chip(struct hsh_handle *h, int32_t n, const int32_t *idx, const double *val);
+/* Spike events raised by SPIKE_OP since the last drain: ops[] index of the
+ * SPIKE_OP and the step number. */
+hsh_status hsh_drain_spikes(struct hsh_handle *h, int32_t *opidx, int32_t *step, int32_t cap,
+                            int32_t *n);
+hsh_status hsh_info_get(struct hsh_handle *h, hsh_info *info);
+/* Device time of the last hsh_step call (HIP events on the handle's stream),
+ * in ms, after hsh_sync.  kernel selects HSH_T_*. */
+#define HSH_T_TOTAL    0
+#define HSH_T_CHANNEL  1     /* plain mode: channel kernel only */
+#define HSH_T_ELIM     2     /* plain mode: elimination kernel only */
+hsh_status hsh_last_time(struct hsh_handle *h, int32_t kernel, double *ms);
+/* Enable per-kernel event timing for following hsh_step calls (adds events
+ * around every launch). */
+hsh_status hsh_set_timing(struct hsh_handle *h, int32_t on);
+void       hsh_destroy(struct hsh_handle *h);
+const char *hsh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

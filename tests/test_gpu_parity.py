@@ -1,0 +1,93 @@
+"""GPU parity: libhsolve_hip against the CPU oracle on the same arrays.
+
+The device path rounds every expression as the reference does, so the bar is
+bit-exact equality of vm[], chip[] (gate states), results[] and the mutable
+ops[] counters.  The one documented exception is the NEWVOLT row cache across
+tile boundaries in the batched kernel, which can differ only when a voltage
+lies within rounding of a table breakpoint; the 1e-7 V bound the workload
+states is asserted beside the bitwise check there.
+"""
+import numpy as np
+import pytest
+
+from helpers import ap_verify_model, max_abs, mixed_model, random_tree_model
+from hsolve_hip import device as D
+from hsolve_hip import model as M
+from oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(m, nsteps, mode, **kw):
+    print(f"[case] ncompts={m.ncompts} steps={nsteps} mode={mode} {kw}", flush=True)
+    o = Oracle(m, givals=False)
+    o.step(nsteps)
+    with D.HipSolver(m, **kw) as g:
+        r = g.step(nsteps, mode)
+        assert r["vm_ready"] == 1
+        st = g.state()
+    return o.state(), st
+
+
+def assert_same(os_, gs, *, exact=True, tol=1e-7):
+    if exact:
+        assert np.array_equal(os_["vm"], gs["vm"]), max_abs(os_["vm"], gs["vm"])
+        assert np.array_equal(os_["chip"], gs["chip"]), max_abs(os_["chip"], gs["chip"])
+        assert np.array_equal(os_["results"], gs["results"]), max_abs(os_["results"], gs["results"])
+        assert np.array_equal(os_["ops"], gs["ops"])
+    else:
+        assert max_abs(os_["vm"], gs["vm"]) <= tol
+
+
+def test_ap_verify_plain(gpu):
+    m = ap_verify_model(8)
+    os_, gs = run_pair(m, 210, D.PLAIN)
+    assert_same(os_, gs)
+    # the reference's CPU golden (cluster_bringup/accel_regression_golden.txt:4)
+    assert "%0.10g" % gs["vm"][m.hnum[0]] == "-0.02143485882"
+
+
+def test_ap_verify_batched_iso(gpu):
+    m = ap_verify_model(8)
+    os_, gs = run_pair(m, 210, D.BATCHED, batch_k=32)
+    assert_same(os_, gs)
+
+
+@pytest.mark.parametrize("mode", [D.PLAIN, D.BATCHED])
+def test_isopotential_population(gpu, mode):
+    m = M.isopotential_hh(1000, seed=0)
+    os_, gs = run_pair(m, 200, mode, batch_k=16)
+    assert_same(os_, gs)
+
+
+@pytest.mark.parametrize("mode", [D.PLAIN, D.BATCHED])
+@pytest.mark.parametrize("method", [M.BEULER, M.CRANK])
+def test_random_trees(gpu, mode, method):
+    m = random_tree_model(300, 16, seed=7, method=method)
+    os_, gs = run_pair(m, 200, mode, batch_k=8)
+    assert_same(os_, gs)
+
+
+def test_mixed_sizes_plain(gpu):
+    m = mixed_model(500)
+    os_, gs = run_pair(m, 100, D.PLAIN)
+    assert_same(os_, gs)
+
+
+def test_mixed_sizes_batched(gpu):
+    m = mixed_model(500)
+    os_, gs = run_pair(m, 100, D.BATCHED, batch_k=10)
+    assert_same(os_, gs)
+
+
+def test_steps_split_across_calls(gpu):
+    """K-step launches and graph replays compose: 3 calls == one run."""
+    m = random_tree_model(130, 16, seed=11)
+    o = Oracle(m, givals=False)
+    o.step(77)
+    with D.HipSolver(m, batch_k=8) as g:
+        g.step(30, D.BATCHED)
+        g.step(7, D.PLAIN)
+        g.step(40, D.BATCHED)
+        gs = g.state()
+    assert_same(o.state(), gs)
