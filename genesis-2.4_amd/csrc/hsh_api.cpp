@@ -267,8 +267,10 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     }
     h->device = opt ? opt->device : 0;
     if (opt && opt->batch_k > 0) h->batch_k = opt->batch_k;
+    // 0 = four-wave tiles with register-held tree coefficients (trees of up
+    // to 16 rows); otherwise waves per tile of the LDS-coefficient kernel
     if (opt && opt->batch_waves > 0) h->batch_waves = opt->batch_waves;
-    else h->batch_waves = std::min(16, std::max(1, h->plan.max_rows >= 16 ? 8 : h->plan.max_rows));
+    else h->batch_waves = h->plan.max_rows <= 16 ? 0 : std::min(16, std::max(1, h->plan.max_rows / 4));
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         delete h;
@@ -308,7 +310,7 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     HIPCHK_C(dalloc(&h->d_pred, npos));
     HIPCHK_C(dalloc(&h->d_spk, 2 * (size_t)SPIKE_CAP));
     HIPCHK_C(dalloc(&h->d_nspk, 1));
-    HIPCHK_C(dalloc(&h->d_step, 1));
+    HIPCHK_C(dalloc(&h->d_step, P.tiles.size()));
     const size_t ntab = (m->ncols > 0) ? (size_t)(m->xdivs + 2) * m->ncols : 1;
     const size_t nx = (m->ncols > 0) ? (size_t)(m->xdivs + 2) : 1;
     HIPCHK_C(dalloc(&h->d_tab, ntab));
@@ -344,7 +346,7 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         HIPCHK_C(hipMemcpy(h->d_xvals, m->xvals, nx * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK_C(hipMemset(h->d_nspk, 0, sizeof(int32_t)));
-    HIPCHK_C(hipMemset(h->d_step, 0, sizeof(int32_t)));
+    HIPCHK_C(hipMemset(h->d_step, 0, P.tiles.size() * sizeof(int32_t)));
     HIPCHK_C(hipMemset(h->d_R, 0, 2 * npos * sizeof(double)));
     if (upload_state(h, m->vm) != HSH_OK) return cleanup_fail(HSH_ERR_DEVICE);
     // reverse map chip index -> S positions
@@ -438,9 +440,9 @@ hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* o
             HIPCHK(launch_channel(h->dm, h->stream));
             if (h->timing) HIPCHK(hipEventRecord(h->evs[h->nev_used + 1], h->stream));
             if (mode == HSH_CHANNELS) {
-                HIPCHK(launch_step_advance(h->d_step, 1, h->stream));
+                HIPCHK(launch_step_advance(h->dm, 1, h->stream));
             } else {
-                HIPCHK(launch_elim(h->dm, h->stream));
+                HIPCHK(launch_elim(h->dm, h->plan.max_rows, h->stream));
             }
             if (h->timing) {
                 HIPCHK(hipEventRecord(h->evs[h->nev_used + 2], h->stream));

@@ -78,7 +78,7 @@ struct DevModel {
     const double* xvals;  // xdivs+2
     int32_t* spk;         // spike ring: [cap][2] (ops index, step)
     int32_t* nspk;        // spike count
-    int32_t* step;        // device step counter
+    int32_t* step;        // per-tile step counters
     int32_t spk_cap;
     int32_t ncols;
     int32_t xdivs;
@@ -134,10 +134,11 @@ bool analyse(const hsh_model& m, Plan& plan, std::string& err);
 
 // kernel launchers (hsh_kernels.hip)
 hipError_t launch_channel(const DevModel& d, hipStream_t s);
-hipError_t launch_elim(const DevModel& d, hipStream_t s);
+hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s);
+// waves == 0 selects the four-wave register-coefficient kernel (rows <= 16)
 hipError_t launch_batched(const DevModel& d, int ksteps, int waves, int max_rows, hipStream_t s);
 hipError_t launch_iso(const DevModel& d, int ksteps, hipStream_t s);
-hipError_t launch_step_advance(int32_t* step, int32_t by, hipStream_t s);
+hipError_t launch_step_advance(const DevModel& d, int32_t by, hipStream_t s);
 int batched_lds_bytes(int max_rows);
 
 }  // namespace hsh

@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_channel(DevModel d)
     const double Vm = d.V[pos];
     const int pp = d.pred[pos];
     const double Vp = pp >= 0 ? d.V[pp] : 0.0;
-    const int step = d.step[0];
+    const int step = d.step[bm.tile];
     CompOut o = run_comp(d, d.prog + bm.prog, d.S + bm.dslot + lane, d.I + bm.islot + lane, Vm,
                          pp >= 0, Vp, lane < tm.lanes, step);
     d.R[(size_t)b * 2 * W + lane] = o.rhs;
@@ -175,9 +175,9 @@ __global__ __launch_bounds__(256) void k_elim(DevModel d)
 {
     int t = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     t = __builtin_amdgcn_readfirstlane(t);
-    if (blockIdx.x == 0 && threadIdx.x == 0) d.step[0] += 1;
     if (t >= d.ntiles) return;
     const int lane = threadIdx.x & (W - 1);
+    if (lane == 0) d.step[t] += 1;
     const TileMeta tm = d.tiles[t];
     const int R = tm.rows, b0 = tm.block0;
     double* Rr = d.R + (size_t)b0 * 2 * W + lane;     // row r: rhs Rr[r*2W], diag Rr[r*2W+W]
@@ -234,7 +234,7 @@ __global__ __launch_bounds__(1024) void k_batched(DevModel d, int K)
     double* LB = lds + R * W;
     double* LD = lds + 2 * R * W;
     for (int r = wave; r < R; r += nw) LV[r * W + lane] = d.V[(size_t)(b0 + r) * W + lane];
-    const int step0 = d.step[0];
+    const int step0 = d.step[t];
     __syncthreads();
     for (int k = 0; k < K; k++) {
         for (int r = wave; r < R; r += nw) {
@@ -294,6 +294,190 @@ __global__ __launch_bounds__(1024) void k_batched(DevModel d, int K)
         d.R[b * 2 * W + lane] = LB[r * W + lane];
         d.R[b * 2 * W + W + lane] = LD[r * W + lane];
     }
+    if (threadIdx.x == 0) d.step[t] = step0 + K;
+}
+
+// Batched persistent kernel for small trees (rows <= 4*RPW): four waves per
+// tile, wave w owns rows w, w+4, ...  Each owner keeps its rows' tree
+// coefficients in registers for the whole launch, and the elimination walks
+// the rows in order -- the owner of row r folds it into its parent, then all
+// four waves meet at a barrier -- so the tile needs only 24 bytes of LDS per
+// compartment and every tile of the workload is resident at once.
+template <int RPW>
+__global__ __launch_bounds__(256) void k_batched_r(DevModel d, int K)
+{
+    constexpr int NW = 4;
+    constexpr int RMAX = NW * RPW;
+    extern __shared__ double lds[];
+    const int t = blockIdx.x;
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows, b0 = tm.block0;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & (W - 1);
+    const bool active = lane < tm.lanes;
+    double* LV = lds;
+    double* LB = lds + R * W;
+    double* LD = lds + 2 * R * W;
+    int par[RPW], pidx[RPW];
+    double ta[RPW], tcf[RPW], tcb[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int r = wave + NW * i;
+        par[i] = -1; pidx[i] = -1;
+        ta[i] = tcf[i] = tcb[i] = 0.0;
+        if (r < R) {
+            const size_t q = (size_t)(b0 + r) * W + lane;
+            par[i] = d.par[q];
+            ta[i] = d.ta[q];
+            tcf[i] = d.tcf[q];
+            tcb[i] = d.tcb[q];
+            LV[r * W + lane] = d.V[q];
+            const int pp = d.pred[q];
+            if (pp >= 0 && (pp >> 6) >= b0 && (pp >> 6) < b0 + R)
+                pidx[i] = ((pp >> 6) - b0) * W + (pp & (W - 1));
+        }
+    }
+    const bool is_last = ((size_t)(b0 + R - 1) * W + lane) == (size_t)d.last_pos;
+    const int step0 = d.step[t];
+    __syncthreads();
+    for (int k = 0; k < K; k++) {
+#pragma unroll
+        for (int i = 0; i < RPW; i++) {
+            const int r = wave + NW * i;
+            if (r < R) {
+                const int b = b0 + r;
+                const BlockMeta bm = d.blocks[b];
+                const double Vm = LV[r * W + lane];
+                const bool pok = pidx[i] >= 0;
+                const double Vp = pok ? LV[pidx[i]] : 0.0;
+                CompOut o = run_comp(d, d.prog + bm.prog, d.S + bm.dslot + lane, d.I + bm.islot + lane,
+                                     Vm, pok, Vp, active, step0 + k);
+                LB[r * W + lane] = o.rhs;
+                LD[r * W + lane] = o.diag;
+            }
+        }
+        __syncthreads();
+        double droot0 = 0.0;
+        if (wave == ((R - 1) & (NW - 1))) droot0 = LD[(R - 1) * W + lane];
+#pragma unroll
+        for (int r = 0; r < RMAX - 1; r++) {
+            if (r < R - 1) {
+            if (wave == (r & (NW - 1))) {
+                const int i = r / NW;
+                const int p = par[i];
+                if (p >= 0) {
+                    const double tt = ta[i] / LD[r * W + lane];
+                    LD[p * W + lane] -= tcf[i] * tt;
+                    LB[p * W + lane] -= LB[r * W + lane] * tt;
+                }
+            }
+            __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; r--) {
+            if (r < R) {
+                if (wave == (r & (NW - 1))) {
+                    const int i = r / NW;
+                    const int p = par[i];
+                    double bv = LB[r * W + lane];
+                    if (p >= 0) bv -= tcb[i] * LB[p * W + lane];
+                    const double x = bv / LD[r * W + lane];
+                    LB[r * W + lane] = x;
+                    if (d.crank) {
+                        const double v = LV[r * W + lane];
+                        LV[r * W + lane] = x + x - v;
+                    } else {
+                        LV[r * W + lane] = x;
+                    }
+                    if (r == R - 1 && is_last) LD[r * W + lane] = droot0;
+                }
+                __syncthreads();
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int r = wave + NW * i;
+        if (r < R) {
+            const size_t b = (size_t)(b0 + r);
+            d.V[b * W + lane] = LV[r * W + lane];
+            d.R[b * 2 * W + lane] = LB[r * W + lane];
+            d.R[b * 2 * W + W + lane] = LD[r * W + lane];
+        }
+    }
+    if (threadIdx.x == 0) d.step[t] = step0 + K;
+}
+
+// Plain drop-in elimination for trees of up to RMAX rows: one wave per tile,
+// the tile's rhs/diag staged in LDS (each lane touches only its own column),
+// tree coefficients loaded into registers before the sweep starts.
+template <int RMAX>
+__global__ __launch_bounds__(64) void k_elim_r(DevModel d)
+{
+    __shared__ double sB[RMAX * W], sD[RMAX * W];
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows, b0 = tm.block0;
+    if (lane == 0) d.step[t] += 1;
+    double* Rr = d.R + (size_t)b0 * 2 * W + lane;
+    const size_t p0 = (size_t)b0 * W + lane;
+    int par[RMAX];
+    double ta[RMAX], tcf[RMAX], tcb[RMAX];
+#pragma unroll
+    for (int r = 0; r < RMAX; r++) {
+        if (r < R) {
+            const size_t q = p0 + (size_t)r * W;
+            par[r] = d.par[q];
+            ta[r] = d.ta[q];
+            tcf[r] = d.tcf[q];
+            tcb[r] = d.tcb[q];
+            sB[r * W + lane] = Rr[(size_t)r * 2 * W];
+            sD[r * W + lane] = Rr[(size_t)r * 2 * W + W];
+        } else {
+            par[r] = -1;
+            ta[r] = tcf[r] = tcb[r] = 0.0;
+        }
+    }
+    const bool is_last = (p0 + (size_t)(R - 1) * W) == (size_t)d.last_pos;
+    const double droot0 = sD[(R - 1) * W + lane];
+#pragma unroll
+    for (int r = 0; r < RMAX - 1; r++) {
+        if (r < R - 1) {
+            const int p = par[r];
+            if (p >= 0) {
+                const double tt = ta[r] / sD[r * W + lane];
+                sD[p * W + lane] -= tcf[r] * tt;
+                sB[p * W + lane] -= sB[r * W + lane] * tt;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = RMAX - 1; r >= 0; r--) {
+        if (r < R) {
+            const size_t q = p0 + (size_t)r * W;
+            double bv = sB[r * W + lane];
+            const int p = par[r];
+            if (p >= 0) bv -= tcb[r] * sB[p * W + lane];
+            const double x = bv / sD[r * W + lane];
+            sB[r * W + lane] = x;
+            if (d.crank) {
+                const double v = d.V[q];
+                d.V[q] = x + x - v;
+            } else {
+                d.V[q] = x;
+            }
+        }
+    }
+    if (is_last) sD[(R - 1) * W + lane] = droot0;
+#pragma unroll
+    for (int r = 0; r < RMAX; r++) {
+        if (r < R) {
+            Rr[(size_t)r * 2 * W] = sB[r * W + lane];
+            Rr[(size_t)r * 2 * W + W] = sD[r * W + lane];
+        }
+    }
 }
 
 // Isopotential cells (every tree one compartment): one lane per cell, the
@@ -314,7 +498,7 @@ __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
     // the previous cell in op order, if it sits in this wave
     const bool pok = pp >= 0 && (pp >> 6) == b;
     const int plane = pok ? (pp & (W - 1)) : lane;
-    const int step0 = d.step[0];
+    const int step0 = d.step[t];
     CompOut o{0.0, 1.0};
     double x = 0.0;
     for (int k = 0; k < K; k++) {
@@ -329,11 +513,13 @@ __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
     // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
     d.R[(size_t)b * 2 * W + lane] = d.single ? o.rhs : x;
     d.R[(size_t)b * 2 * W + W + lane] = o.diag;
+    if (lane == 0) d.step[t] = step0 + K;
 }
 
-__global__ void k_step_advance(int32_t* step, int32_t by)
+__global__ void k_step_advance(int32_t* step, int32_t n, int32_t by)
 {
-    if (threadIdx.x == 0) step[0] += by;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) step[t] += by;
 }
 
 }  // namespace
@@ -347,23 +533,34 @@ hipError_t launch_channel(const DevModel& d, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_elim(const DevModel& d, hipStream_t s)
+hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s)
 {
-    dim3 grid((d.ntiles + 3) / 4);
-    hipLaunchKernelGGL(k_elim, grid, dim3(256), 0, s, d);
+    if (max_rows <= 16) {
+        hipLaunchKernelGGL(k_elim_r<16>, dim3(d.ntiles), dim3(64), 0, s, d);
+    } else {
+        dim3 grid((d.ntiles + 3) / 4);
+        hipLaunchKernelGGL(k_elim, grid, dim3(256), 0, s, d);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_batched(const DevModel& d, int ksteps, int waves, int max_rows, hipStream_t s)
 {
     if (max_rows > BATCH_MAX_ROWS) return hipErrorInvalidValue;
+    if (waves == 0 && max_rows <= 16) {
+        const int lds = batched_lds_bytes(max_rows);
+        if (max_rows <= 4)
+            hipLaunchKernelGGL(k_batched_r<1>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+        else if (max_rows <= 8)
+            hipLaunchKernelGGL(k_batched_r<2>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+        else
+            hipLaunchKernelGGL(k_batched_r<4>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+        return hipGetLastError();
+    }
     if (waves < 1) waves = 1;
     if (waves > 16) waves = 16;
     hipLaunchKernelGGL(k_batched, dim3(d.ntiles), dim3(waves * W), batched_lds_bytes(max_rows), s, d,
                        ksteps);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, d.step, ksteps);
     return hipGetLastError();
 }
 
@@ -371,15 +568,12 @@ hipError_t launch_iso(const DevModel& d, int ksteps, hipStream_t s)
 {
     dim3 grid((d.ntiles + 3) / 4);
     hipLaunchKernelGGL(k_iso, grid, dim3(256), 0, s, d, ksteps);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, d.step, ksteps);
     return hipGetLastError();
 }
 
-hipError_t launch_step_advance(int32_t* step, int32_t by, hipStream_t s)
+hipError_t launch_step_advance(const DevModel& d, int32_t by, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_advance, dim3(1), dim3(64), 0, s, step, by);
+    hipLaunchKernelGGL(k_step_advance, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d.step, d.ntiles, by);
     return hipGetLastError();
 }
 

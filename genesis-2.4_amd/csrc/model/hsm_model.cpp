@@ -596,7 +596,11 @@ extern "C" int32_t hsm_build_population(const hsm_pop* p, hsm_out** out)
 {
     *out = nullptr;
     if (p->n_neurons <= 0 || p->nsizes <= 0) return fail("hsm_build_population: empty population");
-    SplitMix64 rng(p->seed);
+    // one generator per neuron, seeded from (seed, global neuron index), so
+    // any contiguous sub-range [first_neuron, first_neuron+n_neurons) of a
+    // population is built identically on its own
+    SplitMix64 seeder(p->seed);
+    const uint64_t salt = seeder.next();
     const double PI = 3.141592653589793;
     std::vector<double> cum(p->nsizes);
     double tot = 0;
@@ -607,6 +611,8 @@ extern "C" int32_t hsm_build_population(const hsm_pop* p, hsm_out** out)
     std::vector<double> chan_gbar;
     const int max_kids = p->max_kids > 0 ? p->max_kids : 3;
     for (int nn = 0; nn < p->n_neurons; nn++) {
+        SplitMix64 rng(salt ^ (0x9E3779B97F4A7C15ull * (uint64_t)(p->first_neuron + nn + 1)));
+        rng.next();
         int size = p->sizes[0];
         if (p->nsizes > 1) {
             double u = rng.uniform() * tot;

@@ -147,7 +147,7 @@ typedef struct hsm_pop {
     double inject_lo, inject_hi;
     double erest_act;        /* HH table rest potential */
     int32_t float_fields;    /* as hsm_spec.float_fields */
-    int32_t pad;
+    int32_t first_neuron;    /* global index of the first neuron built */
 } hsm_pop;
 
 int32_t hsm_build_population(const hsm_pop *pop, hsm_out **out);

@@ -77,7 +77,7 @@ class Pop(C.Structure):
         ("CM", C.c_double), ("RM", C.c_double), ("RA", C.c_double), ("Em", C.c_double),
         ("gNa", C.c_double), ("gK", C.c_double), ("gKdr", C.c_double),
         ("inject_lo", C.c_double), ("inject_hi", C.c_double), ("erest_act", C.c_double),
-        ("float_fields", C.c_int32), ("pad", C.c_int32),
+        ("float_fields", C.c_int32), ("first_neuron", C.c_int32),
     ]
 
 
@@ -267,7 +267,7 @@ def population(n_neurons: int, sizes=(16,), probs=(1.0,), *, seed: int = 0, dt: 
                method: int = BEULER, soma=(30e-6, 30e-6), dend_len=(10e-6, 100e-6),
                dend_dia=(0.5e-6, 4e-6), CM=0.01, RM=0.33, RA=1.0, Em=-0.065,
                gNa=1200.0, gK=360.0, gKdr=50.0, inject=(0.2e-9, 1.0e-9),
-               max_kids: int = 3, float_fields: bool = True) -> Model:
+               max_kids: int = 3, float_fields: bool = True, first_neuron: int = 0) -> Model:
     """Random Hines trees: soma (root, Na/K) plus dendrites (leak + Kdr).
 
     Compartment i>0 takes a parent uniform over compartments 0..i-1, resampled
@@ -288,6 +288,7 @@ def population(n_neurons: int, sizes=(16,), probs=(1.0,), *, seed: int = 0, dt: 
     p.inject_lo, p.inject_hi = inject
     p.erest_act = Em
     p.float_fields = 1 if float_fields else 0
+    p.first_neuron = first_neuron
     outp = C.POINTER(Out)()
     rc = lib().hsm_build_population(C.byref(p), C.byref(outp))
     if rc != 0:
