@@ -401,6 +401,8 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     d.ntiles = (int32_t)P.tiles.size();
     d.last_pos = P.comp_pos[m->ncompts - 1];
     d.single = (m->ncompts == 1);
+    d.max_slots = 0;
+    for (const Template& t : P.templates) d.max_slots = std::max(d.max_slots, t.nslots);
     HIPCHK_C(hipEventCreate(&h->ev_a));
     HIPCHK_C(hipEventCreate(&h->ev_b));
 #undef HIPCHK_C

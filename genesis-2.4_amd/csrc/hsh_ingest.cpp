@@ -364,7 +364,10 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     // program array: templates concatenated
     std::vector<int32_t> tprog(P.templates.size());
     for (size_t t = 0; t < P.templates.size(); t++) {
+        // program = [nslots, micro-ops...]; the slot count lets a wave fetch
+        // the compartment's whole slot vector before interpreting it
         tprog[t] = (int32_t)P.prog.size();
+        P.prog.push_back(P.templates[t].nslots);
         P.prog.insert(P.prog.end(), P.templates[t].code.begin(), P.templates[t].code.end());
     }
     P.comp_pos.assign(n, -1);

@@ -88,6 +88,7 @@ struct DevModel {
     int32_t nblocks;
     int32_t ntiles;
     int32_t last_pos;     // device position of the last row (ncompts-1)
+    int32_t max_slots;    // largest program slot count
     int32_t single;       // ncompts == 1
 };
 

@@ -50,21 +50,50 @@ __device__ __forceinline__ void newvolt(const DevModel& d, double Vm, bool pred_
     vipol = v - (double)f;
 }
 
+// Slot access for run_prog.  With REG the compartment's whole slot vector is
+// fetched with back-to-back coalesced loads before the program runs (one
+// memory latency per compartment instead of one per micro-op) and indexed with
+// the wave-uniform slot cursor, which gfx950 does in VGPRs (s_set_gpr_idx);
+// gate states are written through to memory.  Without REG each slot is read
+// when its micro-op runs (programs with more than MAXS slots).
+constexpr int MAXS = 12;
+
 // One compartment's channel program (device micro-ops, see hsh_internal.h).
-// S and I point at this lane's slot 0; slot k lives at S[k*W].
-__device__ __forceinline__ CompOut run_comp(const DevModel& d, const int32_t* __restrict__ code,
-                                            double* __restrict__ S, int32_t* __restrict__ I,
-                                            double Vm, bool pred_ok, double Vp, bool active,
-                                            int step)
+// S and I point at this lane's slot 0; slot k lives at S[k*W].  code[0] is the
+// program's slot count, the micro-ops follow.
+template <bool REG>
+__device__ __forceinline__ CompOut run_prog(const DevModel& d, const int32_t* __restrict__ code,
+                                            double* __restrict__ Sb, int32_t* __restrict__ Ib,
+                                            int lane, double Vm, bool pred_ok, double Vp,
+                                            bool active, int step)
 {
+    // Sb / Ib: wave-uniform base of the block's slot area; slot c of this lane
+    // is Sb[c*W + lane]
+    double s[REG ? MAXS : 1];
+    if (REG) {
+        // unconditional loads (indices past the program's last slot re-read
+        // that slot, which the cache serves), so they issue back to back
+        const int last = code[0] - 1;
+#pragma unroll
+        for (int i = 0; i < MAXS; i++) {
+            const int j = __builtin_amdgcn_readfirstlane(i < last ? i : last);
+            s[i] = Sb[j * W + lane];
+        }
+    }
+    int32_t* I = Ib + lane;
+#define SGET(c) (REG ? s[(c)] : Sb[(c) * W + lane])
+    // a slot is never read again after the program writes it (the cursor
+    // only moves forward), so writes go to memory only and the register copy
+    // stays read-only
+#define SPUT(c, v) (Sb[(c) * W + lane] = (v))
     double sumgchan = 0.0;
-    double ichan = S[0] + S[W];
+    double ichan = SGET(0) + SGET(1);
     double Gk = 0.0, Ek = 0.0;
     double vipol = 0.0;
     const double* vtab = d.tab;
     const int ncols = d.ncols;
     int c = 2;
-    int pc = 0;
+    int pc = 1;
     for (;;) {
         const int op = code[pc++];
         if (op == U_IPOL1V) {
@@ -76,14 +105,14 @@ __device__ __forceinline__ CompOut run_comp(const DevModel& d, const int32_t* __
             X = X + vipol * (tab[ncols] - X);
             double Y = tab[1];
             Y = Y + vipol * (tab[ncols + 1] - Y);
-            double s = S[c * W];
+            const double sv = SGET(c);
             if (k > 0) {
-                X = (s * (2.0 - X) + Y) / X;
+                X = (sv * (2.0 - X) + Y) / X;
             } else {
                 X = Y / X;
                 k = -k;
             }
-            S[c * W] = X;
+            SPUT(c, X);
             c++;
             if (k == 1) {
                 Gk *= X;
@@ -106,8 +135,8 @@ __device__ __forceinline__ CompOut run_comp(const DevModel& d, const int32_t* __
                 Gk *= X;
             }
         } else if (op == U_CHAN_EK) {
-            Ek = S[c * W];
-            Gk = S[(c + 1) * W];
+            Ek = SGET(c);
+            Gk = SGET(c + 1);
             c += 2;
         } else if (op == U_ADDCURR) {
             sumgchan += Gk;
@@ -117,12 +146,12 @@ __device__ __forceinline__ CompOut run_comp(const DevModel& d, const int32_t* __
             newvolt(d, Vm, pred_ok, Vp, filo, vipol);
             vtab = d.tab + (size_t)filo * ncols;
         } else if (op == U_CHAN) {
-            Gk = S[c * W];
+            Gk = SGET(c);
             c += 1;
         } else if (op == U_SPIKE) {
             const int is = code[pc++];
             int cnt = I[is * W] - 1;
-            if (Vm > S[c * W]) {
+            if (Vm > SGET(c)) {
                 if (cnt <= 0) {
                     if (active) {
                         int q = atomicAdd(d.nspk, 1);
@@ -137,17 +166,22 @@ __device__ __forceinline__ CompOut run_comp(const DevModel& d, const int32_t* __
             c += 1;
             I[is * W] = cnt;
         } else {  // U_END
-            const double tbyc = S[c * W];
+            const double tbyc = SGET(c);
             CompOut o;
             o.rhs = Vm + ichan * tbyc;
-            o.diag = sumgchan * tbyc + S[(c + 1) * W];
+            o.diag = sumgchan * tbyc + SGET(c + 1);
             return o;
         }
     }
+#undef SGET
+#undef SPUT
 }
+
+
 
 // ---------------------------------------------------------------------------
 // Plain drop-in, part (a): one wave per block, all blocks in one launch.
+template <bool REG>
 __global__ __launch_bounds__(256) void k_channel(DevModel d)
 {
     int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -161,7 +195,7 @@ __global__ __launch_bounds__(256) void k_channel(DevModel d)
     const int pp = d.pred[pos];
     const double Vp = pp >= 0 ? d.V[pp] : 0.0;
     const int step = d.step[bm.tile];
-    CompOut o = run_comp(d, d.prog + bm.prog, d.S + bm.dslot + lane, d.I + bm.islot + lane, Vm,
+    CompOut o = run_prog<REG>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane, Vm,
                          pp >= 0, Vp, lane < tm.lanes, step);
     d.R[(size_t)b * 2 * W + lane] = o.rhs;
     d.R[(size_t)b * 2 * W + W + lane] = o.diag;
@@ -221,6 +255,7 @@ __global__ __launch_bounds__(256) void k_elim(DevModel d)
 // 64 eliminations (one per lane) between two barriers.
 constexpr int BATCH_MAX_ROWS = 64;
 
+template <bool REG>
 __global__ __launch_bounds__(1024) void k_batched(DevModel d, int K)
 {
     extern __shared__ double lds[];
@@ -252,7 +287,7 @@ __global__ __launch_bounds__(1024) void k_batched(DevModel d, int K)
                     Vp = LV[(pb - b0) * W + (pp & (W - 1))];
                 }
             }
-            CompOut o = run_comp(d, d.prog + bm.prog, d.S + bm.dslot + lane, d.I + bm.islot + lane,
+            CompOut o = run_prog<REG>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane,
                                  Vm, pok, Vp, lane < tm.lanes, step0 + k);
             LB[r * W + lane] = o.rhs;
             LD[r * W + lane] = o.diag;
@@ -341,7 +376,7 @@ __global__ __launch_bounds__(256) void k_batched_r(DevModel d, int K)
     const int step0 = d.step[t];
     __syncthreads();
     for (int k = 0; k < K; k++) {
-#pragma unroll
+#pragma unroll 1
         for (int i = 0; i < RPW; i++) {
             const int r = wave + NW * i;
             if (r < R) {
@@ -350,7 +385,7 @@ __global__ __launch_bounds__(256) void k_batched_r(DevModel d, int K)
                 const double Vm = LV[r * W + lane];
                 const bool pok = pidx[i] >= 0;
                 const double Vp = pok ? LV[pidx[i]] : 0.0;
-                CompOut o = run_comp(d, d.prog + bm.prog, d.S + bm.dslot + lane, d.I + bm.islot + lane,
+                CompOut o = run_prog<true>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane,
                                      Vm, pok, Vp, active, step0 + k);
                 LB[r * W + lane] = o.rhs;
                 LD[r * W + lane] = o.diag;
@@ -483,6 +518,7 @@ __global__ __launch_bounds__(64) void k_elim_r(DevModel d)
 // Isopotential cells (every tree one compartment): one lane per cell, the
 // voltage stays in a register for K steps.  V = rhs/diag is the reference's
 // single-row solve (hines_solve.c:236-238; CN :158-162).
+template <bool REG>
 __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
 {
     int t = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
@@ -503,7 +539,7 @@ __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
     double x = 0.0;
     for (int k = 0; k < K; k++) {
         const double Vp = __shfl(V, plane, W);
-        o = run_comp(d, d.prog + bm.prog, d.S + bm.dslot + lane, d.I + bm.islot + lane, V, pok, Vp,
+        o = run_prog<REG>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane, V, pok, Vp,
                      lane < tm.lanes, step0 + k);
         x = o.rhs / o.diag;
         if (d.crank) V = x + x - V;
@@ -529,7 +565,10 @@ int batched_lds_bytes(int max_rows) { return 3 * max_rows * W * (int)sizeof(doub
 hipError_t launch_channel(const DevModel& d, hipStream_t s)
 {
     dim3 grid((d.nblocks + 3) / 4);
-    hipLaunchKernelGGL(k_channel, grid, dim3(256), 0, s, d);
+    if (d.max_slots <= MAXS)
+        hipLaunchKernelGGL(k_channel<true>, grid, dim3(256), 0, s, d);
+    else
+        hipLaunchKernelGGL(k_channel<false>, grid, dim3(256), 0, s, d);
     return hipGetLastError();
 }
 
@@ -547,7 +586,8 @@ hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s)
 hipError_t launch_batched(const DevModel& d, int ksteps, int waves, int max_rows, hipStream_t s)
 {
     if (max_rows > BATCH_MAX_ROWS) return hipErrorInvalidValue;
-    if (waves == 0 && max_rows <= 16) {
+    const bool reg = d.max_slots <= MAXS;
+    if (waves == 0 && max_rows <= 16 && reg) {
         const int lds = batched_lds_bytes(max_rows);
         if (max_rows <= 4)
             hipLaunchKernelGGL(k_batched_r<1>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
@@ -559,15 +599,22 @@ hipError_t launch_batched(const DevModel& d, int ksteps, int waves, int max_rows
     }
     if (waves < 1) waves = 1;
     if (waves > 16) waves = 16;
-    hipLaunchKernelGGL(k_batched, dim3(d.ntiles), dim3(waves * W), batched_lds_bytes(max_rows), s, d,
-                       ksteps);
+    if (reg)
+        hipLaunchKernelGGL(k_batched<true>, dim3(d.ntiles), dim3(waves * W), batched_lds_bytes(max_rows),
+                           s, d, ksteps);
+    else
+        hipLaunchKernelGGL(k_batched<false>, dim3(d.ntiles), dim3(waves * W), batched_lds_bytes(max_rows),
+                           s, d, ksteps);
     return hipGetLastError();
 }
 
 hipError_t launch_iso(const DevModel& d, int ksteps, hipStream_t s)
 {
     dim3 grid((d.ntiles + 3) / 4);
-    hipLaunchKernelGGL(k_iso, grid, dim3(256), 0, s, d, ksteps);
+    if (d.max_slots <= MAXS)
+        hipLaunchKernelGGL(k_iso<true>, grid, dim3(256), 0, s, d, ksteps);
+    else
+        hipLaunchKernelGGL(k_iso<false>, grid, dim3(256), 0, s, d, ksteps);
     return hipGetLastError();
 }
 
