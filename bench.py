@@ -49,6 +49,10 @@ def parse():
     p.add_argument("--cpu-threads", type=int, default=0, help="threads of the all-cores CPU leg (0 = nproc)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and the oracle parity")
     p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--flags", type=int, default=0, help="hsh_options.flags for the batched handle")
+    p.add_argument("--copy-calibrate", type=int, default=0, metavar="MB",
+                   help="only run a device-to-device copy of MB megabytes (counter calibration "
+                        "for FETCH_SIZE/WRITE_SIZE) and exit")
     return p.parse_args()
 
 
@@ -66,6 +70,15 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+    if a.copy_calibrate:
+        nel = a.copy_calibrate * (1 << 20) // 8
+        src = torch.ones(nel, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibrate_bytes": nel * 8, "launches": 4}))
+        return
 
     t_e2e = time.perf_counter()
     m = M.population(a.neurons, (a.ncomp,), (1.0,), seed=a.seed)
@@ -75,7 +88,7 @@ def main():
           f"{m.chip.size} chip values, {m.ops.size} ops (built in {t_build:.2f}s)", file=sys.stderr, flush=True)
 
     # ---- batched path (value)
-    gb = D.HipSolver(m, batch_k=a.batch_k)
+    gb = D.HipSolver(m, batch_k=a.batch_k, flags=a.flags)
     info = gb.info()
     K = info["batch_k"]
     t_ready = time.perf_counter() - t_e2e

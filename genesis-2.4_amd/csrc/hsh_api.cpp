@@ -43,6 +43,7 @@ struct hsh_handle {
     hsh_model m{};
     Plan plan;
     bool refused = false;
+    bool host_only = false;
     int device = 0;
     hipStream_t stream = nullptr;
     // host copies in the reference layout
@@ -265,6 +266,12 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         g_err = buf;
         return HSH_REFUSED;
     }
+    if (opt && (opt->flags & HSH_F_HOST_ONLY)) {
+        h->host_only = true;
+        if (opt->batch_k > 0) h->batch_k = opt->batch_k;
+        *out = h;
+        return HSH_OK;
+    }
     h->device = opt ? opt->device : 0;
     if (opt && opt->batch_k > 0) h->batch_k = opt->batch_k;
     // 0 = four-wave tiles with register-held tree coefficients (trees of up
@@ -423,6 +430,7 @@ hsh_status hsh_refusals(hsh_handle* h, hsh_refusal* out, int32_t cap, int32_t* n
 hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* out)
 {
     if (!h) return fail(HSH_ERR_ARG, "hsh_step: null handle");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_step: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_step: handle refused the model; use the CPU path");
     if (nsteps < 0) return fail(HSH_ERR_ARG, "hsh_step: negative step count");
     hsh_step_out o{};
@@ -470,7 +478,7 @@ hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* o
 hsh_status hsh_sync(hsh_handle* h)
 {
     if (!h) return fail(HSH_ERR_ARG, "hsh_sync: null handle");
-    if (h->refused) return HSH_OK;
+    if (h->refused || h->host_only) return HSH_OK;
     HIPCHK(hipStreamSynchronize(h->stream));
     return HSH_OK;
 }
@@ -478,6 +486,7 @@ hsh_status hsh_sync(hsh_handle* h)
 hsh_status hsh_get_state(hsh_handle* h, double* vm, double* chip, double* results, int32_t* ops)
 {
     if (!h) return fail(HSH_ERR_ARG, "hsh_get_state: null handle");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_get_state: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_get_state: handle refused the model");
     const Plan& P = h->plan;
     const size_t npos = P.blocks.size() * W;
@@ -508,6 +517,7 @@ hsh_status hsh_get_state(hsh_handle* h, double* vm, double* chip, double* result
 hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, const int32_t* ops)
 {
     if (!h) return fail(HSH_ERR_ARG, "hsh_put_state: null handle");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_put_state: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_put_state: handle refused the model");
     HIPCHK(hipStreamSynchronize(h->stream));
     if (chip || ops) {
@@ -533,6 +543,7 @@ hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, co
 hsh_status hsh_set_chip(hsh_handle* h, int32_t n, const int32_t* idx, const double* val)
 {
     if (!h || (n > 0 && (!idx || !val))) return fail(HSH_ERR_ARG, "hsh_set_chip: null argument");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_set_chip: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_chip: handle refused the model");
     std::vector<std::pair<int64_t, double>> w;
     for (int32_t i = 0; i < n; i++) {
@@ -550,6 +561,7 @@ hsh_status hsh_set_chip(hsh_handle* h, int32_t n, const int32_t* idx, const doub
 hsh_status hsh_drain_spikes(hsh_handle* h, int32_t* opidx, int32_t* step, int32_t cap, int32_t* n)
 {
     if (!h || !n) return fail(HSH_ERR_ARG, "hsh_drain_spikes: null argument");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_drain_spikes: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_drain_spikes: handle refused the model");
     HIPCHK(hipStreamSynchronize(h->stream));
     int32_t cnt = 0;
@@ -599,7 +611,8 @@ hsh_status hsh_set_timing(hsh_handle* h, int32_t on)
 hsh_status hsh_last_time(hsh_handle* h, int32_t kernel, double* ms)
 {
     if (!h || !ms) return fail(HSH_ERR_ARG, "hsh_last_time: null argument");
-    if (!h->total_valid) return fail(HSH_ERR_STATE, "hsh_last_time: timing was not enabled for the last hsh_step");
+    if (h->host_only || !h->total_valid)
+        return fail(HSH_ERR_STATE, "hsh_last_time: timing was not enabled for the last hsh_step");
     HIPCHK(hipStreamSynchronize(h->stream));
     float t = 0.f;
     if (kernel == HSH_T_TOTAL) {

@@ -24,6 +24,7 @@ HSH_OK, HSH_REFUSED = 0, 1
 PLAIN, BATCHED, CHANNELS = 0, 1, 2
 T_TOTAL, T_CHANNEL, T_ELIM = 0, 1, 2
 F_NO_AUTOTUNE = 1
+F_HOST_ONLY = 2
 
 REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
            5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode"}
@@ -131,7 +132,7 @@ class HipSolver:
     """One hsolve on the GPU.  ``model`` is a :class:`hsolve_hip.model.Model`."""
 
     def __init__(self, model, *, device: int = 0, batch_k: int = 0, batch_waves: int = 0,
-                 flags: int = 0):
+                 flags: int = 0, host_only: bool = False):
         L = lib()
         self.m = model
         k = dict(ops=np.ascontiguousarray(model.ops, np.int32),
@@ -154,6 +155,8 @@ class HipSolver:
         mm.stablist = _p(k["stablist"], C.c_double) if k["stablist"].size else None
         mm.funcs, mm.ravals = _p(k["funcs"], C.c_int32), _p(k["ravals"], C.c_double)
         mm.vm = _p(k["vm"], C.c_double)
+        if host_only:
+            flags |= F_HOST_ONLY
         opt = HshOptions(device, batch_k, batch_waves, flags)
         h = C.c_void_p()
         rc = L.hsh_create(C.byref(mm), C.byref(opt), C.byref(h))

@@ -90,6 +90,9 @@ typedef struct hsh_options {
 } hsh_options;
 
 #define HSH_F_NO_AUTOTUNE 1  /* use batch_k / batch_waves (or defaults) as given */
+#define HSH_F_HOST_ONLY   2  /* analyse and plan only, allocate nothing on a device:
+                                hsh_refusals and hsh_info_get work, stepping and
+                                state calls return HSH_ERR_STATE */
 
 /* refusal reasons */
 #define HSH_R_OPCODE         1   /* opcode the device interpreter does not run */

@@ -1,0 +1,103 @@
+"""C ABI of libhsolve_hip on the CPU: the library loads, exports every symbol
+include/hsolve_hip.h declares, and its host-side analysers (ops walk, tree
+decode, tiling, refusals) behave as specified.  No kernel is launched here
+(HSH_F_HOST_ONLY handles allocate nothing on a device)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import ap_verify_model, mixed_model, random_tree_model
+from hsolve_hip import device as D
+from hsolve_hip import model as M
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "hsolve_hip.h")).read()
+    return set(re.findall(r"^\s*(?:hsh_status|void|const char \*)\s*(hsh_\w+)\s*\(", txt, re.M))
+
+
+def test_exports_match_header():
+    L = D.lib()
+    names = header_functions()
+    assert names == set(D.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def plan(m, **kw):
+    with D.HipSolver(m, host_only=True, **kw) as h:
+        return h.info()
+
+
+def test_plan_random_trees():
+    m = random_tree_model(130, 16)
+    info = plan(m)
+    assert info["ntrees"] == 130
+    assert info["max_rows"] == 16
+    assert info["ntiles"] == 3                  # ceil(130/64)
+    assert info["nblocks"] == 3 * 16
+    # one dendrite program (Ek materialised) and one soma program
+    assert info["ntemplates"] == 2
+    assert abs(info["lane_fill"] - 130 / 192) < 1e-12
+
+
+def test_plan_bytes_formula():
+    """Algorithmic bytes per compartment-step: 16 (Vm r+w) + 16 per gate
+    (state r+w) + 8 per read-only slot + 28 per non-root row (three axial
+    coefficients + int32 parent)."""
+    m = random_tree_model(64, 16, seed=3)
+    info = plan(m)
+    # soma: 11 slots (incl. materialised K Ek), 3 gates -> 16 + 48 + 8*8 = 128
+    # dendrite: 7 slots, 1 gate -> 16 + 16 + 6*8 + 28 = 108
+    want = 64 * (128 + 15 * 108)
+    assert info["bytes_per_step"] == want
+
+
+def test_plan_mixed_sizes_groups_tiles():
+    m = mixed_model(500)
+    info = plan(m)
+    assert info["max_rows"] == 64
+    assert info["ntrees"] == 500
+
+
+def test_refuses_synchan_with_reason():
+    comps = dict(parent=[-1, -1], Cm=[1e-11] * 2, Rm=[1e8] * 2, Ra=[1e7] * 2, Em=[-0.065] * 2,
+                 initVm=[-0.065] * 2, inject=[0.0] * 2)
+    m = M.build(comps, [], [], dt=1e-4, syntypes=[dict(Ek=0.0, tau1=5e-3, tau2=1e-3)],
+                syns=[(1, 0, 1e-9)])
+    with pytest.raises(D.Refused) as e:
+        D.HipSolver(m, host_only=True)
+    refs = e.value.refusals
+    assert len(refs) == 1
+    comp, opcode, opidx, why = refs[0]
+    assert opcode == 4100 and why == "opcode"
+    assert m.ops[opidx] == 4100
+    assert comp == m.hnum[1]
+
+
+def test_rejects_malformed_ops():
+    m = random_tree_model(3, 4)
+    bad = np.array(m.ops)
+    bad[-1] = 100          # no LCOMPT terminator
+    m.ops = bad
+    with pytest.raises(D.HshError):
+        D.HipSolver(m, host_only=True)
+
+
+def test_rejects_truncated_chip():
+    m = random_tree_model(3, 4)
+    m.chip = m.chip[:-3]
+    with pytest.raises(D.HshError):
+        D.HipSolver(m, host_only=True)
+
+
+def test_host_only_handle_refuses_stepping():
+    m = ap_verify_model(2)
+    with D.HipSolver(m, host_only=True) as h:
+        with pytest.raises(D.HshError):
+            h.step(1)

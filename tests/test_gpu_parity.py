@@ -71,8 +71,8 @@ def test_random_trees(gpu, mode, method):
 @pytest.mark.parametrize("ncomp", [3, 8, 16])
 @pytest.mark.parametrize("waves", [0, 4])
 def test_batched_kernel_variants(gpu, ncomp, waves):
-    """waves=0: four-wave tiles with register-held coefficients (RPW 1/2/4);
-    waves=4: the LDS-coefficient kernel."""
+    """waves=0: four-wave tiles (RPW 1/2/4) with register-held coefficients
+    and the row-owner elimination; waves=4: the generic LDS kernel."""
     m = random_tree_model(200, ncomp, seed=ncomp)
     os_, gs = run_pair(m, 64, D.BATCHED, batch_k=16, batch_waves=waves)
     assert_same(os_, gs)
