@@ -1,0 +1,20 @@
+/*
+ * hip_hsolve.h -- GENESIS-side entry points of the MI355X hsolve backend.
+ *
+ * Same role and return protocol as the CUDA backend's header
+ * (genesis/src/hines/cuda/cuda_hsolve.h:22-25): hines.c selects it with
+ *     #define ACCEL_CHIP_UPDATE hip_chip_update
+ * under USE_HIP (genesis/src/hines/hines.c:93-98).
+ */
+#ifndef HIP_HSOLVE_H
+#define HIP_HSOLVE_H
+
+#ifdef USE_HIP
+typedef struct hsolve_type Hsolve;
+
+int  hip_chip_update(Hsolve *hsolve);   /* 1: vm[] final (skip the CPU solve), 0: CPU path ran */
+void hip_sync_chip(Hsolve *hsolve);     /* device state -> vm/chip/results/ops (HGET, SAVE2) */
+void hip_release(Hsolve *hsolve);       /* free this solver's device state (h_delete) */
+#endif
+
+#endif
