@@ -151,11 +151,13 @@ def main():
                                "(HH soma Na/K, dendrite leak+Kdr), fp64, dt 20 us, BEULER",
                    "neurons": a.neurons, "compartments_per_neuron": a.ncomp, "compartments": n,
                    "batch_k": K, "batch_waves": info["batch_waves"], "tiles": info["ntiles"],
-                   "path": "batched persistent kernel in a captured hipGraph"},
+                   "path": "batched persistent kernel in a captured hipGraph",
+                   "specialised_kernels": bool(info["jit"]), "jit_compile_s": info["jit_compile_s"]},
         "plain": {"value": plain_value, "ms_per_step": wall_p * 1e3 / a.steps,
                   "channel_kernel_us": ch_ms * 1e3, "elim_kernel_us": el_ms * 1e3,
                   "batched_over_plain": value / plain_value},
-        "roofline": {"bound": "hbm", "kernel": "k_batched",
+        "roofline": {"bound": "hbm",
+                     "kernel": ("hsh_jit_batched_r4" if info["jit"] else "k_batched_r<4>"),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_compartment_step": bytes_step / n,

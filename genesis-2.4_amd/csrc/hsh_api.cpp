@@ -1,6 +1,7 @@
 // hsh_api.cpp -- C ABI of libhsolve_hip (declared in include/hsolve_hip.h).
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -59,6 +60,7 @@ struct hsh_handle {
     int32_t *d_I = nullptr, *d_par = nullptr, *d_pred = nullptr, *d_spk = nullptr, *d_nspk = nullptr,
             *d_step = nullptr;
     DevModel dm{};
+    JitModule jit;
     int batch_k = 32, batch_waves = 4;
     // graphs keyed by (K, launches)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
@@ -169,6 +171,7 @@ void free_device(hsh_handle* h)
                     h->d_nspk, h->d_step};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    jit_free(h->jit);
     for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& g : h->graph_src) (void)hipGraphDestroy(g.second);
     h->graphs.clear();
@@ -185,9 +188,9 @@ hsh_status batched_launch(hsh_handle* h, int K, int count)
 {
     const int rows = h->plan.max_rows;
     if (rows == 1) {
-        HIPCHK(launch_iso(h->dm, K, h->stream));
+        HIPCHK(launch_iso(h->dm, &h->jit, K, h->stream));
     } else {
-        HIPCHK(launch_batched(h->dm, K, h->batch_waves, rows, h->stream));
+        HIPCHK(launch_batched(h->dm, &h->jit, K, h->batch_waves, rows, h->stream));
     }
     (void)count;
     return HSH_OK;
@@ -413,6 +416,17 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     HIPCHK_C(hipEventCreate(&h->ev_a));
     HIPCHK_C(hipEventCreate(&h->ev_b));
 #undef HIPCHK_C
+    // kernels specialised to this model's channel programs; on failure the
+    // built-in interpreter kernels run (message kept in hsh_last_error)
+    const char* nojit = std::getenv("HSH_NO_JIT");
+    if (!(opt && (opt->flags & HSH_F_NO_JIT)) && !(nojit && nojit[0] == '1')) {
+        std::string jerr;
+        if (!jit_build(P, m->ncols, h->jit, jerr)) {
+            std::fprintf(stderr, "libhsolve_hip: specialised kernels unavailable (%s); using the interpreter kernels\n",
+                         jerr.c_str());
+            g_err = jerr;
+        }
+    }
     *out = h;
     return HSH_OK;
 }
@@ -447,7 +461,7 @@ hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* o
                 }
                 HIPCHK(hipEventRecord(h->evs[h->nev_used], h->stream));
             }
-            HIPCHK(launch_channel(h->dm, h->stream));
+            HIPCHK(launch_channel(h->dm, &h->jit, h->stream));
             if (h->timing) HIPCHK(hipEventRecord(h->evs[h->nev_used + 1], h->stream));
             if (mode == HSH_CHANNELS) {
                 HIPCHK(launch_step_advance(h->dm, 1, h->stream));
@@ -597,6 +611,8 @@ hsh_status hsh_info_get(hsh_handle* h, hsh_info* info)
     o.refused = h->refused ? 1 : 0;
     o.bytes_per_step = P.bytes_per_step;
     o.lane_fill = P.blocks.empty() ? 0.0 : (double)h->m.ncompts / ((double)P.blocks.size() * W);
+    o.jit = h->jit.ok ? 1 : 0;
+    o.jit_compile_s = h->jit.compile_s;
     *info = o;
     return HSH_OK;
 }
@@ -627,6 +643,23 @@ hsh_status hsh_last_time(hsh_handle* h, int32_t kernel, double* ms)
         acc += t;
     }
     *ms = acc;
+    return HSH_OK;
+}
+
+hsh_status hsh_jit_source(const hsh_model* m, char* buf, int64_t cap, int64_t* len)
+{
+    if (!m || !len) return fail(HSH_ERR_ARG, "hsh_jit_source: null argument");
+    Plan P;
+    std::string err;
+    if (!analyse(*m, P, err)) return fail(HSH_ERR_ARG, "hsh_jit_source: " + err);
+    if (!P.refusals.empty()) return fail(HSH_REFUSED, "hsh_jit_source: model refused");
+    const std::string src = jit_source(P, m->ncols);
+    *len = (int64_t)src.size();
+    if (buf && cap > 0) {
+        const size_t n = std::min<size_t>((size_t)cap - 1, src.size());
+        std::memcpy(buf, src.data(), n);
+        buf[n] = '\0';
+    }
     return HSH_OK;
 }
 

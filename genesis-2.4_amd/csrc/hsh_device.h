@@ -1,0 +1,447 @@
+// hsh_device.h -- device code shared by the built-in kernels (hsh_kernels.hip)
+// and the kernels libhsolve_hip specialises and compiles at run time
+// (hsh_jit.cpp).  Include after hip_runtime.h and hsh_devtypes.h.
+//
+// Arithmetic follows the reference CPU code expression by expression
+// (genesis/src/hines/hines_chip.c:157-600, hines_solve.c:140-296); compiled
+// with -ffp-contract=off, every result rounds as the reference's does, and
+// fp64 division is IEEE correctly rounded on gfx950.
+#pragma once
+
+namespace hsh {
+
+struct CompOut {
+    double rhs, diag;
+};
+
+// Table row and fraction for voltage Vm (hines_chip.c:174-203).  The
+// reference keeps the row of the previous NEWVOLT compartment (xlo/xhi cache)
+// and reuses it when Vm falls inside that row's [xvals[f], xvals[f+1]); the
+// previous compartment's own row is recomputed here from its voltage.
+__device__ __forceinline__ void newvolt(const DevModel& d, double Vm, bool pred_ok, double Vp,
+                                        int& filo, double& vipol)
+{
+    double vraw = (Vm - d.xmin) * d.invdx;
+    if (pred_ok) {
+        double pr = (Vp - d.xmin) * d.invdx;
+        if (pr < 0.0) pr = 0.0;
+        else if (pr > d.xdivs) pr = d.xdivs;
+        int fp = (int)pr;
+        fp = fp < 0 ? 0 : (fp > d.xdivs ? d.xdivs : fp);
+        double xlo = d.xvals[fp], xhi = d.xvals[fp + 1];
+        if (!((xlo > Vm) || (Vm >= xhi))) {
+            filo = fp;
+            vipol = vraw - (double)fp;
+            return;
+        }
+    }
+    double v = vraw;
+    if (v < 0.0) v = 0.0;
+    else if (v > d.xdivs) v = d.xdivs;
+    int f = (int)v;
+    f = f < 0 ? 0 : (f > d.xdivs ? d.xdivs : f);
+    filo = f;
+    vipol = v - (double)f;
+}
+
+// Gate state update from the interpolated B'/A' columns (hines_chip.c:425-434,
+// :484-489): integrating gate X' = (X(2-B')+A')/B', instantaneous X = A'/B'.
+__device__ __forceinline__ double gate_update(const double* __restrict__ vtab, int col, int ncols,
+                                              double vipol, double s, bool instant)
+{
+    const double* tab = vtab + col;
+    double X = tab[0];
+    X = X + vipol * (tab[ncols] - X);
+    double Y = tab[1];
+    Y = Y + vipol * (tab[ncols + 1] - Y);
+    if (!instant) return (s * (2.0 - X) + Y) / X;
+    return Y / X;
+}
+
+// Gk *= X^k with the reference's multiplication sequences (hines_chip.c:490-509).
+__device__ __forceinline__ double apply_power(double Gk, double X, int k)
+{
+    if (k == 1) {
+        Gk *= X;
+    } else if (k == 2) {
+        Gk *= X * X;
+    } else if (k == 3) {
+        Gk *= X * X * X;
+    } else if (k == 4) {
+        X *= X;
+        Gk *= X * X;
+    } else if (k == 5) {
+        Gk *= X;
+        X *= X;
+        X *= X;
+        Gk *= X;
+    } else {
+        X *= X;
+        Gk *= X;
+        X *= X;
+        Gk *= X;
+    }
+    return Gk;
+}
+
+// SPIKE_OP (hines_chip.c:266-276): counter-1 every step; when Vm > thresh and
+// the counter is spent, record an event and reload the counter.
+__device__ __forceinline__ void spike_op(const DevModel& d, int* __restrict__ Ib, int is, int lane,
+                                         double Vm, double thresh, bool active, int step)
+{
+    int cnt = Ib[is * W + lane] - 1;
+    if (Vm > thresh) {
+        if (cnt <= 0) {
+            if (active) {
+                int q = atomicAdd(d.nspk, 1);
+                if (q < d.spk_cap) {
+                    d.spk[2 * q] = Ib[(is + 2) * W + lane];
+                    d.spk[2 * q + 1] = step;
+                }
+            }
+            cnt = Ib[(is + 1) * W + lane];
+        }
+    }
+    Ib[is * W + lane] = cnt;
+}
+
+// ---------------------------------------------------------------------------
+// Generic micro-op interpreter.  With REG the compartment's whole slot vector
+// is fetched with back-to-back coalesced loads before the program runs (one
+// memory latency per compartment instead of one per micro-op) and indexed
+// with the wave-uniform slot cursor, which gfx950 does in VGPRs
+// (s_set_gpr_idx).  A slot is never read after the program writes it (the
+// cursor only moves forward), so writes go to memory only and the register
+// copy stays read-only.  Without REG each slot is read when its micro-op runs.
+constexpr int MAXS = 12;
+
+template <bool REG>
+__device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __restrict__ code,
+                                            double* __restrict__ Sb, int* __restrict__ Ib, int lane,
+                                            double Vm, bool pred_ok, double Vp, bool active, int step)
+{
+    double s[REG ? MAXS : 1];
+    if (REG) {
+        // unconditional loads (indices past the program's last slot re-read
+        // that slot, which the cache serves), so they issue back to back
+        const int last = code[0] - 1;
+#pragma unroll
+        for (int i = 0; i < MAXS; i++) {
+            const int j = __builtin_amdgcn_readfirstlane(i < last ? i : last);
+            s[i] = Sb[j * W + lane];
+        }
+    }
+#define SGET(c) (REG ? s[(c)] : Sb[(c) * W + lane])
+#define SPUT(c, v) (Sb[(c) * W + lane] = (v))
+    double sumgchan = 0.0;
+    double ichan = SGET(0) + SGET(1);
+    double Gk = 0.0, Ek = 0.0;
+    double vipol = 0.0;
+    const double* vtab = d.tab;
+    const int ncols = d.ncols;
+    int c = 2;
+    int pc = PROG_HEADER;
+    for (;;) {
+        const int op = code[pc++];
+        if (op == U_IPOL1V) {
+            const int col = code[pc];
+            const int k = code[pc + 1];
+            pc += 2;
+            const double X = gate_update(vtab, col, ncols, vipol, SGET(c), k < 0);
+            SPUT(c, X);
+            c++;
+            Gk = apply_power(Gk, X, k < 0 ? -k : k);
+        } else if (op == U_CHAN_EK) {
+            Ek = SGET(c);
+            Gk = SGET(c + 1);
+            c += 2;
+        } else if (op == U_ADDCURR) {
+            sumgchan += Gk;
+            ichan += Ek * Gk;
+        } else if (op == U_NEWVOLT) {
+            int filo;
+            newvolt(d, Vm, pred_ok, Vp, filo, vipol);
+            vtab = d.tab + (long long)filo * ncols;
+        } else if (op == U_CHAN) {
+            Gk = SGET(c);
+            c += 1;
+        } else if (op == U_SPIKE) {
+            const int is = code[pc++];
+            spike_op(d, Ib, is, lane, Vm, SGET(c), active, step);
+            c += 1;
+        } else {  // U_END
+            const double tbyc = SGET(c);
+            CompOut o;
+            o.rhs = Vm + ichan * tbyc;
+            o.diag = sumgchan * tbyc + SGET(c + 1);
+            return o;
+        }
+    }
+#undef SGET
+#undef SPUT
+}
+
+// Runner policy: the interpreter.  (Generated modules supply their own runner
+// with one straight-line function per template.)
+template <bool REG>
+struct InterpRun {
+    static __device__ __forceinline__ CompOut run(const DevModel& d, const BlockMeta& bm, int lane,
+                                                  double Vm, bool pok, double Vp, bool active,
+                                                  int step)
+    {
+        return run_prog<REG>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane, Vm, pok, Vp,
+                             active, step);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Kernel bodies (the __global__ wrappers pick the runner).
+
+// Plain drop-in, part (a): one wave per block, four blocks per workgroup.
+template <class RUN>
+__device__ __forceinline__ void body_channel(const DevModel& d)
+{
+    int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    b = __builtin_amdgcn_readfirstlane(b);
+    if (b >= d.nblocks) return;
+    const int lane = threadIdx.x & (W - 1);
+    const BlockMeta bm = d.blocks[b];
+    const TileMeta tm = d.tiles[bm.tile];
+    const int pos = b * W + lane;
+    const double Vm = d.V[pos];
+    const int pp = d.pred[pos];
+    const double Vp = pp >= 0 ? d.V[pp] : 0.0;
+    const int step = d.step[bm.tile];
+    CompOut o = RUN::run(d, bm, lane, Vm, pp >= 0, Vp, lane < tm.lanes, step);
+    d.R[(long long)b * 2 * W + lane] = o.rhs;
+    d.R[(long long)b * 2 * W + W + lane] = o.diag;
+}
+
+// Batched persistent kernel for small trees (rows <= 4*RPW): four waves per
+// tile, wave w owns rows w, w+4, ...  Each owner keeps its rows' tree
+// coefficients in registers for the whole launch, and the elimination walks
+// the rows in order -- the owner of row r folds it into its parent, then all
+// four waves meet at a barrier -- so the tile needs only 24 bytes of LDS per
+// compartment and every tile of the workload is resident at once.
+template <int RPW, class RUN>
+__device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double* lds)
+{
+    constexpr int NW = 4;
+    constexpr int RMAX = NW * RPW;
+    const int t = blockIdx.x;
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows, b0 = tm.block0;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & (W - 1);
+    const bool active = lane < tm.lanes;
+    double* LV = lds;
+    double* LB = lds + R * W;
+    double* LD = lds + 2 * R * W;
+    int par[RPW], pidx[RPW];
+    double ta[RPW], tcf[RPW], tcb[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int r = wave + NW * i;
+        par[i] = -1;
+        pidx[i] = -1;
+        ta[i] = tcf[i] = tcb[i] = 0.0;
+        if (r < R) {
+            const long long q = (long long)(b0 + r) * W + lane;
+            par[i] = d.par[q];
+            ta[i] = d.ta[q];
+            tcf[i] = d.tcf[q];
+            tcb[i] = d.tcb[q];
+            LV[r * W + lane] = d.V[q];
+            const int pp = d.pred[q];
+            if (pp >= 0 && (pp >> 6) >= b0 && (pp >> 6) < b0 + R)
+                pidx[i] = ((pp >> 6) - b0) * W + (pp & (W - 1));
+        }
+    }
+    const bool is_last = ((long long)(b0 + R - 1) * W + lane) == (long long)d.last_pos;
+    const int step0 = d.step[t];
+    __syncthreads();
+    for (int k = 0; k < K; k++) {
+#pragma unroll 1
+        for (int i = 0; i < RPW; i++) {
+            const int r = wave + NW * i;
+            if (r < R) {
+                const int b = b0 + r;
+                const BlockMeta bm = d.blocks[b];
+                const double Vm = LV[r * W + lane];
+                const bool pok = pidx[i] >= 0;
+                const double Vp = pok ? LV[pidx[i]] : 0.0;
+                CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, active, step0 + k);
+                LB[r * W + lane] = o.rhs;
+                LD[r * W + lane] = o.diag;
+            }
+        }
+        __syncthreads();
+        double droot0 = 0.0;
+        if (wave == ((R - 1) & (NW - 1))) droot0 = LD[(R - 1) * W + lane];
+#pragma unroll
+        for (int r = 0; r < RMAX - 1; r++) {
+            if (r < R - 1) {
+                if (wave == (r & (NW - 1))) {
+                    const int i = r / NW;
+                    const int p = par[i];
+                    if (p >= 0) {
+                        const double tt = ta[i] / LD[r * W + lane];
+                        LD[p * W + lane] -= tcf[i] * tt;
+                        LB[p * W + lane] -= LB[r * W + lane] * tt;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+#pragma unroll
+        for (int r = RMAX - 1; r >= 0; r--) {
+            if (r < R) {
+                if (wave == (r & (NW - 1))) {
+                    const int i = r / NW;
+                    const int p = par[i];
+                    double bv = LB[r * W + lane];
+                    if (p >= 0) bv -= tcb[i] * LB[p * W + lane];
+                    const double x = bv / LD[r * W + lane];
+                    LB[r * W + lane] = x;
+                    if (d.crank) {
+                        const double v = LV[r * W + lane];
+                        LV[r * W + lane] = x + x - v;
+                    } else {
+                        LV[r * W + lane] = x;
+                    }
+                    // the reference never stores its last row's eliminated
+                    // diagonal (hines_solve.c:275-276)
+                    if (r == R - 1 && is_last) LD[r * W + lane] = droot0;
+                }
+                __syncthreads();
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int r = wave + NW * i;
+        if (r < R) {
+            const long long b = b0 + r;
+            d.V[b * W + lane] = LV[r * W + lane];
+            d.R[b * 2 * W + lane] = LB[r * W + lane];
+            d.R[b * 2 * W + W + lane] = LD[r * W + lane];
+        }
+    }
+    if (threadIdx.x == 0) d.step[t] = step0 + K;
+}
+
+// Batched persistent kernel for trees of up to 64 rows: one workgroup per
+// tile, waves split the rows for the channel phase, wave 0 runs the 64
+// eliminations (one per lane) with coefficients read from L2.
+template <class RUN>
+__device__ __forceinline__ void body_batched(const DevModel& d, int K, double* lds)
+{
+    const int t = blockIdx.x;
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows, b0 = tm.block0;
+    const int nw = blockDim.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = threadIdx.x & (W - 1);
+    double* LV = lds;
+    double* LB = lds + R * W;
+    double* LD = lds + 2 * R * W;
+    for (int r = wave; r < R; r += nw) LV[r * W + lane] = d.V[(long long)(b0 + r) * W + lane];
+    const int step0 = d.step[t];
+    __syncthreads();
+    for (int k = 0; k < K; k++) {
+        for (int r = wave; r < R; r += nw) {
+            const int b = b0 + r;
+            const BlockMeta bm = d.blocks[b];
+            const int pos = b * W + lane;
+            const double Vm = LV[r * W + lane];
+            const int pp = d.pred[pos];
+            bool pok = false;
+            double Vp = 0.0;
+            if (pp >= 0) {
+                const int pb = pp >> 6;
+                if (pb >= b0 && pb < b0 + R) {
+                    pok = true;
+                    Vp = LV[(pb - b0) * W + (pp & (W - 1))];
+                }
+            }
+            CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, lane < tm.lanes, step0 + k);
+            LB[r * W + lane] = o.rhs;
+            LD[r * W + lane] = o.diag;
+        }
+        __syncthreads();
+        if (wave == 0) {
+            const long long p0 = (long long)b0 * W + lane;
+            const bool is_last = (p0 + (long long)(R - 1) * W) == (long long)d.last_pos;
+            const double droot0 = LD[(R - 1) * W + lane];
+            for (int r = 0; r < R - 1; r++) {
+                const int p = d.par[p0 + (long long)r * W];
+                if (p >= 0) {
+                    const double tt = d.ta[p0 + (long long)r * W] / LD[r * W + lane];
+                    LD[p * W + lane] -= d.tcf[p0 + (long long)r * W] * tt;
+                    LB[p * W + lane] -= LB[r * W + lane] * tt;
+                }
+            }
+            for (int r = R - 1; r >= 0; r--) {
+                const long long q = p0 + (long long)r * W;
+                double bv = LB[r * W + lane];
+                const int p = d.par[q];
+                if (p >= 0) bv -= d.tcb[q] * LB[p * W + lane];
+                const double x = bv / LD[r * W + lane];
+                LB[r * W + lane] = x;
+                if (d.crank) {
+                    const double v = LV[r * W + lane];
+                    LV[r * W + lane] = x + x - v;
+                } else {
+                    LV[r * W + lane] = x;
+                }
+            }
+            if (is_last) LD[(R - 1) * W + lane] = droot0;
+        }
+        __syncthreads();
+    }
+    for (int r = wave; r < R; r += nw) {
+        const long long b = b0 + r;
+        d.V[b * W + lane] = LV[r * W + lane];
+        d.R[b * 2 * W + lane] = LB[r * W + lane];
+        d.R[b * 2 * W + W + lane] = LD[r * W + lane];
+    }
+    if (threadIdx.x == 0) d.step[t] = step0 + K;
+}
+
+// Isopotential cells (every tree one compartment): one lane per cell, the
+// voltage stays in a register for K steps.  V = rhs/diag is the reference's
+// single-row solve (hines_solve.c:236-238; CN :158-162).
+template <class RUN>
+__device__ __forceinline__ void body_iso(const DevModel& d, int K)
+{
+    int t = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= d.ntiles) return;
+    const int lane = threadIdx.x & (W - 1);
+    const TileMeta tm = d.tiles[t];
+    const int b = tm.block0;
+    const BlockMeta bm = d.blocks[b];
+    const int pos = b * W + lane;
+    double V = d.V[pos];
+    const int pp = d.pred[pos];
+    // the previous cell in op order, if it sits in this wave
+    const bool pok = pp >= 0 && (pp >> 6) == b;
+    const int plane = pok ? (pp & (W - 1)) : lane;
+    const int step0 = d.step[t];
+    CompOut o{0.0, 1.0};
+    double x = 0.0;
+    for (int k = 0; k < K; k++) {
+        const double Vp = __shfl(V, plane, W);
+        o = RUN::run(d, bm, lane, V, pok, Vp, lane < tm.lanes, step0 + k);
+        x = o.rhs / o.diag;
+        if (d.crank) V = x + x - V;
+        else V = x;
+    }
+    d.V[pos] = V;
+    // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
+    d.R[(long long)b * 2 * W + lane] = d.single ? o.rhs : x;
+    d.R[(long long)b * 2 * W + W + lane] = o.diag;
+    if (lane == 0) d.step[t] = step0 + K;
+}
+
+}  // namespace hsh

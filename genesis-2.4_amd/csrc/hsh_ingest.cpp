@@ -364,10 +364,11 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     // program array: templates concatenated
     std::vector<int32_t> tprog(P.templates.size());
     for (size_t t = 0; t < P.templates.size(); t++) {
-        // program = [nslots, micro-ops...]; the slot count lets a wave fetch
-        // the compartment's whole slot vector before interpreting it
+        // program = [nslots, template id, micro-ops...]; the slot count lets a
+        // wave fetch the compartment's whole slot vector before interpreting it
         tprog[t] = (int32_t)P.prog.size();
         P.prog.push_back(P.templates[t].nslots);
+        P.prog.push_back((int32_t)t);
         P.prog.insert(P.prog.end(), P.templates[t].code.begin(), P.templates[t].code.end());
     }
     P.comp_pos.assign(n, -1);
@@ -387,11 +388,12 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
             P.tiles.push_back(tm);
             for (int r = 0; r < R; r++) {
                 int ti = sig[1 + r];
-                BlockMeta bm;
+                BlockMeta bm{};
                 bm.prog = tprog[ti];
                 bm.dslot = (int32_t)nS;
                 bm.islot = (int32_t)nI;
                 bm.tile = tile;
+                bm.tmpl = ti;
                 int b = (int)P.blocks.size();
                 P.blocks.push_back(bm);
                 nS += (int64_t)P.templates[ti].nslots * W;

@@ -8,11 +8,9 @@
 #include <vector>
 
 #include "hsolve_hip.h"
+#include "hsh_devtypes.h"
 
 namespace hsh {
-
-// Lanes per tile: one wave64 lane per neuron.
-constexpr int W = 64;
 
 // Reference opcodes (genesis/src/hines/hines_defs.h:178-222).
 enum RefOp : int32_t {
@@ -28,68 +26,6 @@ enum RefOp : int32_t {
 enum RefFunc : int32_t {
     F_ELIM = 0, F_SET_DIAG = 1, F_SKIP_DIAG = 2, F_FASTSIB = 3, F_COPY = 4, F_SIB = 5,
     F_CALC = 6, F_FINISH = 7,
-};
-
-// Device micro-ops.  A compartment's channel program is rewritten into a flat
-// list of these; the chip slots it reads are renumbered 0..nslots-1 in the
-// order the reference interpreter consumes them, so the device keeps the
-// reference's chip cursor discipline with a wave-uniform slot counter.
-enum UOp : int32_t {
-    U_END = 0,        // results: rhs = Vm + ichan*tbyc, diag = sumg*tbyc + diagterm
-    U_NEWVOLT = 1,    // table row + fraction for Vm, with the reference's row cache
-    U_CHAN_EK = 2,    // Ek = S[c++]; gmax = Gk = S[c++]
-    U_CHAN = 3,       // gmax = Gk = S[c++]  (Ek kept from this compartment)
-    U_IPOL1V = 4,     // operands: column, power (+/-1..6); gate state S[c++]
-    U_ADDCURR = 5,    // sumg += Gk; ichan += Ek*Gk; Ik = Gk*(Ek-Vm)
-    U_SPIKE = 6,      // operand: islot; threshold S[c++]; counter I[islot], reload
-                      // I[islot+1], ops index I[islot+2]
-};
-
-// Per-block metadata: a block is one row of one 64-neuron tile.
-struct BlockMeta {
-    int32_t prog;     // offset of the template's code in the program array
-    int32_t dslot;    // offset (doubles) of the block's slot area: nslots*64
-    int32_t islot;    // offset (ints) of the block's int slots: nislots*64
-    int32_t tile;
-};
-
-struct TileMeta {
-    int32_t block0;   // first block (row 0)
-    int32_t rows;     // rows of every neuron in the tile
-    int32_t lanes;    // active lanes (neurons)
-    int32_t pad;
-};
-
-// Device-side argument bundle.
-struct DevModel {
-    const int32_t* prog;
-    const BlockMeta* blocks;
-    const TileMeta* tiles;
-    double* S;            // chip slots, [block][slot][lane]
-    int32_t* I;           // int slots, [block][islot][lane]
-    double* V;            // vm, [block][lane]
-    double* R;            // results, [block][2][lane]
-    const int32_t* par;   // parent row within the tile, -1 = root; [block][lane]
-    const double* ta;     // A[parent][row] (forward numerator), [block][lane]
-    const double* tcf;    // A[row][parent] forward, [block][lane]
-    const double* tcb;    // A[row][parent] backward, [block][lane]
-    const int32_t* pred;  // position of the previous NEWVOLT compartment, -1 none
-    const double* tab;    // composite table (xdivs+2)*ncols
-    const double* xvals;  // xdivs+2
-    int32_t* spk;         // spike ring: [cap][2] (ops index, step)
-    int32_t* nspk;        // spike count
-    int32_t* step;        // per-tile step counters
-    int32_t spk_cap;
-    int32_t ncols;
-    int32_t xdivs;
-    int32_t crank;
-    double xmin;
-    double invdx;
-    int32_t nblocks;
-    int32_t ntiles;
-    int32_t last_pos;     // device position of the last row (ncompts-1)
-    int32_t max_slots;    // largest program slot count
-    int32_t single;       // ncompts == 1
 };
 
 struct Template {
@@ -133,12 +69,26 @@ struct Plan {
 // malformed model (message in err); refusals go into plan.refusals.
 bool analyse(const hsh_model& m, Plan& plan, std::string& err);
 
-// kernel launchers (hsh_kernels.hip)
-hipError_t launch_channel(const DevModel& d, hipStream_t s);
+// Kernels specialised to a model's channel programs and compiled at run time
+// (hsh_jit.cpp).  ok == false: the built-in interpreter kernels are used.
+struct JitModule {
+    hipModule_t mod = nullptr;
+    hipFunction_t channel = nullptr, batched_r[3] = {nullptr, nullptr, nullptr}, batched = nullptr,
+                  iso = nullptr;
+    bool ok = false;
+    double compile_s = 0.0;
+};
+std::string jit_source(const Plan& P, int ncols);
+bool jit_build(const Plan& P, int ncols, JitModule& jm, std::string& err);
+void jit_free(JitModule& jm);
+
+// kernel launchers (hsh_kernels.hip); jm may be null or not ok
+hipError_t launch_channel(const DevModel& d, const JitModule* jm, hipStream_t s);
 hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s);
 // waves == 0 selects the four-wave register-coefficient kernel (rows <= 16)
-hipError_t launch_batched(const DevModel& d, int ksteps, int waves, int max_rows, hipStream_t s);
-hipError_t launch_iso(const DevModel& d, int ksteps, hipStream_t s);
+hipError_t launch_batched(const DevModel& d, const JitModule* jm, int ksteps, int waves, int max_rows,
+                          hipStream_t s);
+hipError_t launch_iso(const DevModel& d, const JitModule* jm, int ksteps, hipStream_t s);
 hipError_t launch_step_advance(const DevModel& d, int32_t by, hipStream_t s);
 int batched_lds_bytes(int max_rows);
 

@@ -1,0 +1,210 @@
+// hsh_jit.cpp -- run-time specialisation of the channel programs.
+//
+// The interpreter (hsh_device.h run_prog) pays, per micro-op, a scalar load,
+// address arithmetic and a branch tree, and it cannot overlap the table
+// gathers of different gates.  A model has few distinct programs (the
+// 50,000 x 16 workload has two), so at hsh_create the library writes one
+// straight-line device function per program -- constant slot offsets, table
+// columns and powers, the same helper functions and expression order as the
+// interpreter -- compiles it with hipRTC for the device's architecture and
+// launches the same kernel bodies with it.  Compilation failures fall back to
+// the built-in interpreter kernels (still on the device).
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hsh_internal.h"
+
+namespace hsh {
+
+namespace {
+
+#include "hsh_jit_src.inc"   // kDevTypesSrc, kDeviceSrc: hsh_devtypes.h, hsh_device.h
+
+std::mutex g_jit_mu;
+std::map<std::string, std::vector<char>> g_code_cache;   // source -> code object
+
+std::string slot(int c) { return "s" + std::to_string(c); }
+
+std::string gen_template(const Template& t, int id, int ncols)
+{
+    std::string o;
+    o += "__device__ __forceinline__ CompOut tmpl_" + std::to_string(id) +
+         "(const DevModel& d, double* __restrict__ Sb, int* __restrict__ Ib, int lane, double Vm, "
+         "bool pred_ok, double Vp, bool active, int step)\n{\n";
+    for (int c = 0; c < t.nslots; c++)
+        o += "    const double " + slot(c) + " = Sb[" + std::to_string(c) + " * W + lane];\n";
+    o += "    double sumgchan = 0.0;\n    double ichan = s0 + s1;\n";
+    o += "    double Gk = 0.0, Ek = 0.0, vipol = 0.0;\n    const double* vtab = d.tab;\n";
+    o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)vipol; (void)vtab;\n";
+    o += "    (void)pred_ok; (void)Vp;\n";
+    const std::string NC = std::to_string(ncols);
+    int c = 2;
+    size_t pc = 0;
+    const std::vector<int32_t>& code = t.code;
+    while (pc < code.size()) {
+        const int op = code[pc++];
+        switch (op) {
+        case U_NEWVOLT:
+            o += "    { int filo; newvolt(d, Vm, pred_ok, Vp, filo, vipol); vtab = d.tab + (long long)filo * " +
+                 NC + "; }\n";
+            break;
+        case U_CHAN_EK:
+            o += "    Ek = " + slot(c) + "; Gk = " + slot(c + 1) + ";\n";
+            c += 2;
+            break;
+        case U_CHAN:
+            o += "    Gk = " + slot(c) + ";\n";
+            c += 1;
+            break;
+        case U_IPOL1V: {
+            const int col = code[pc], k = code[pc + 1];
+            pc += 2;
+            o += "    { const double X = gate_update(vtab, " + std::to_string(col) + ", " + NC +
+                 ", vipol, " + slot(c) + ", " + (k < 0 ? "true" : "false") + "); Sb[" +
+                 std::to_string(c) + " * W + lane] = X; Gk = apply_power(Gk, X, " +
+                 std::to_string(k < 0 ? -k : k) + "); }\n";
+            c += 1;
+            break;
+        }
+        case U_ADDCURR:
+            o += "    sumgchan += Gk; ichan += Ek * Gk;\n";
+            break;
+        case U_SPIKE: {
+            const int is = code[pc++];
+            o += "    spike_op(d, Ib, " + std::to_string(is) + ", lane, Vm, " + slot(c) +
+                 ", active, step);\n";
+            c += 1;
+            break;
+        }
+        default:   // U_END
+            o += "    { CompOut o; o.rhs = Vm + ichan * " + slot(c) + "; o.diag = sumgchan * " + slot(c) +
+                 " + " + slot(c + 1) + "; return o; }\n";
+            c += 2;
+            break;
+        }
+    }
+    o += "}\n";
+    return o;
+}
+
+std::string strip_pragma_once(const char* s)
+{
+    std::string r(s);
+    const std::string p = "#pragma once";
+    size_t i;
+    while ((i = r.find(p)) != std::string::npos) r.erase(i, p.size());
+    return r;
+}
+
+}  // namespace
+
+std::string jit_source(const Plan& P, int ncols)
+{
+    std::string src;
+    src += strip_pragma_once(kDevTypesSrc);
+    src += strip_pragma_once(kDeviceSrc);
+    src += "namespace hsh {\n";
+    for (size_t t = 0; t < P.templates.size(); t++) src += gen_template(P.templates[t], (int)t, ncols);
+    src += "struct JitRun {\n"
+           "    static __device__ __forceinline__ CompOut run(const DevModel& d, const BlockMeta& bm, "
+           "int lane, double Vm, bool pok, double Vp, bool active, int step)\n    {\n"
+           "        double* Sb = d.S + bm.dslot;\n        int* Ib = d.I + bm.islot;\n"
+           "        switch (bm.tmpl) {\n";
+    for (size_t t = 1; t < P.templates.size(); t++)
+        src += "        case " + std::to_string(t) + ": return tmpl_" + std::to_string(t) +
+               "(d, Sb, Ib, lane, Vm, pok, Vp, active, step);\n";
+    src += "        default: return tmpl_0(d, Sb, Ib, lane, Vm, pok, Vp, active, step);\n"
+           "        }\n    }\n};\n}  // namespace hsh\n";
+    src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_channel(hsh::DevModel d)\n"
+           "{ hsh::body_channel<hsh::JitRun>(d); }\n";
+    for (int r : {1, 2, 4})
+        src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_batched_r" + std::to_string(r) +
+               "(hsh::DevModel d, int K)\n{ extern __shared__ double lds[]; hsh::body_batched_r<" +
+               std::to_string(r) + ", hsh::JitRun>(d, K, lds); }\n";
+    src += "extern \"C\" __global__ __launch_bounds__(1024) void hsh_jit_batched(hsh::DevModel d, int K)\n"
+           "{ extern __shared__ double lds[]; hsh::body_batched<hsh::JitRun>(d, K, lds); }\n";
+    src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_iso(hsh::DevModel d, int K)\n"
+           "{ hsh::body_iso<hsh::JitRun>(d, K); }\n";
+    return src;
+}
+
+bool jit_build(const Plan& P, int ncols, JitModule& jm, std::string& err)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    const std::string src = jit_source(P, ncols);
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        err = "no device properties";
+        return false;
+    }
+    std::string arch = prop.gcnArchName;
+    const std::string key = arch + "\n" + src;
+    std::vector<char> code;
+    {
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        auto it = g_code_cache.find(key);
+        if (it != g_code_cache.end()) code = it->second;
+    }
+    if (code.empty()) {
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "hsh_jit.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+            err = "hiprtcCreateProgram failed";
+            return false;
+        }
+        const std::string archopt = "--offload-arch=" + arch.substr(0, arch.find(':'));
+        const char* opts[] = {archopt.c_str(), "-O3",      "-ffp-contract=off", "-fno-fast-math",
+                              "-std=c++17",    "-Xclang", "-target-feature",   "-Xclang",
+                              "-packed-fp32-ops"};
+        hiprtcResult rc = hiprtcCompileProgram(prog, (int)(sizeof opts / sizeof opts[0]), opts);
+        if (rc != HIPRTC_SUCCESS) {
+            size_t n = 0;
+            hiprtcGetProgramLogSize(prog, &n);
+            std::string log(n, '\0');
+            if (n) hiprtcGetProgramLog(prog, &log[0]);
+            err = std::string("hipRTC compile failed: ") + hiprtcGetErrorString(rc) + "\n" + log.substr(0, 2000);
+            hiprtcDestroyProgram(&prog);
+            return false;
+        }
+        size_t sz = 0;
+        hiprtcGetCodeSize(prog, &sz);
+        code.resize(sz);
+        hiprtcGetCode(prog, code.data());
+        hiprtcDestroyProgram(&prog);
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        g_code_cache[key] = code;
+    }
+    if (hipModuleLoadData(&jm.mod, code.data()) != hipSuccess) {
+        err = "hipModuleLoadData failed";
+        return false;
+    }
+    const char* names[] = {"hsh_jit_channel", "hsh_jit_batched_r1", "hsh_jit_batched_r2",
+                           "hsh_jit_batched_r4", "hsh_jit_batched", "hsh_jit_iso"};
+    hipFunction_t* slots[] = {&jm.channel, &jm.batched_r[0], &jm.batched_r[1], &jm.batched_r[2],
+                              &jm.batched, &jm.iso};
+    for (int i = 0; i < 6; i++) {
+        if (hipModuleGetFunction(slots[i], jm.mod, names[i]) != hipSuccess) {
+            err = std::string("hipModuleGetFunction failed: ") + names[i];
+            jit_free(jm);
+            return false;
+        }
+    }
+    jm.ok = true;
+    jm.compile_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+void jit_free(JitModule& jm)
+{
+    if (jm.mod) (void)hipModuleUnload(jm.mod);
+    jm = JitModule();
+}
+
+}  // namespace hsh

@@ -25,6 +25,7 @@ PLAIN, BATCHED, CHANNELS = 0, 1, 2
 T_TOTAL, T_CHANNEL, T_ELIM = 0, 1, 2
 F_NO_AUTOTUNE = 1
 F_HOST_ONLY = 2
+F_NO_JIT = 4
 
 REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
            5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode"}
@@ -61,13 +62,14 @@ class HshInfo(C.Structure):
     _fields_ = [("ntiles", C.c_int32), ("nblocks", C.c_int32), ("ntemplates", C.c_int32),
                 ("ntrees", C.c_int32), ("max_rows", C.c_int32), ("batch_k", C.c_int32),
                 ("batch_waves", C.c_int32), ("refused", C.c_int32),
-                ("bytes_per_step", C.c_double), ("lane_fill", C.c_double)]
+                ("bytes_per_step", C.c_double), ("lane_fill", C.c_double),
+                ("jit", C.c_int32), ("pad", C.c_int32), ("jit_compile_s", C.c_double)]
 
 
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
            "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get",
-           "hsh_last_time", "hsh_set_timing", "hsh_destroy", "hsh_last_error")
+           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_destroy", "hsh_last_error")
 
 _lib = None
 
@@ -99,6 +101,8 @@ def lib():
         L.hsh_last_time.restype = C.c_int32
         L.hsh_set_timing.argtypes = [C.c_void_p, C.c_int32]
         L.hsh_set_timing.restype = C.c_int32
+        L.hsh_jit_source.argtypes = [C.POINTER(HshModel), C.c_char_p, C.c_int64, C.POINTER(C.c_int64)]
+        L.hsh_jit_source.restype = C.c_int32
         L.hsh_destroy.argtypes = [C.c_void_p]
         L.hsh_destroy.restype = None
         L.hsh_last_error.restype = C.c_char_p
@@ -128,6 +132,41 @@ def _check(rc, what):
     return rc
 
 
+def _hsh_model(model):
+    """Build the hsh_model struct (and the arrays it points to) for a Model."""
+    k = dict(ops=np.ascontiguousarray(model.ops, np.int32),
+             chip=np.ascontiguousarray(model.chip, np.float64),
+             tablist=np.ascontiguousarray(model.tablist, np.float64),
+             xvals=np.ascontiguousarray(model.xvals, np.float64),
+             stablist=np.ascontiguousarray(model.stablist, np.float64),
+             funcs=np.ascontiguousarray(model.funcs, np.int32),
+             ravals=np.ascontiguousarray(model.ravals, np.float64),
+             vm=np.ascontiguousarray(model.vm, np.float64))
+    mm = HshModel()
+    mm.ncompts, mm.nchips, mm.nops = model.ncompts, k["chip"].size, k["ops"].size
+    mm.nfuncs, mm.nravals, mm.ncols = k["funcs"].size, k["ravals"].size, model.ncols
+    mm.xdivs, mm.nstab = model.xdivs, k["stablist"].size // 4
+    mm.method, mm.chanmode, mm.calcmode, mm.nconcs = model.method, model.chanmode, 1, 0
+    mm.dt, mm.xmin, mm.invdx = model.dt, model.xmin, model.invdx
+    mm.ops, mm.chip = _p(k["ops"], C.c_int32), _p(k["chip"], C.c_double)
+    mm.tablist = _p(k["tablist"], C.c_double) if k["tablist"].size else None
+    mm.xvals = _p(k["xvals"], C.c_double) if k["xvals"].size else None
+    mm.stablist = _p(k["stablist"], C.c_double) if k["stablist"].size else None
+    mm.funcs, mm.ravals = _p(k["funcs"], C.c_int32), _p(k["ravals"], C.c_double)
+    mm.vm = _p(k["vm"], C.c_double)
+    return mm, k
+
+
+def jit_source(model) -> str:
+    """Device source generated for this model's channel programs."""
+    mm, keep = _hsh_model(model)
+    n = C.c_int64()
+    _check(lib().hsh_jit_source(C.byref(mm), None, 0, C.byref(n)), "hsh_jit_source")
+    buf = C.create_string_buffer(n.value + 1)
+    _check(lib().hsh_jit_source(C.byref(mm), buf, n.value + 1, C.byref(n)), "hsh_jit_source")
+    return buf.value.decode()
+
+
 class HipSolver:
     """One hsolve on the GPU.  ``model`` is a :class:`hsolve_hip.model.Model`."""
 
@@ -135,26 +174,7 @@ class HipSolver:
                  flags: int = 0, host_only: bool = False):
         L = lib()
         self.m = model
-        k = dict(ops=np.ascontiguousarray(model.ops, np.int32),
-                 chip=np.ascontiguousarray(model.chip, np.float64),
-                 tablist=np.ascontiguousarray(model.tablist, np.float64),
-                 xvals=np.ascontiguousarray(model.xvals, np.float64),
-                 stablist=np.ascontiguousarray(model.stablist, np.float64),
-                 funcs=np.ascontiguousarray(model.funcs, np.int32),
-                 ravals=np.ascontiguousarray(model.ravals, np.float64),
-                 vm=np.ascontiguousarray(model.vm, np.float64))
-        mm = HshModel()
-        mm.ncompts, mm.nchips, mm.nops = model.ncompts, k["chip"].size, k["ops"].size
-        mm.nfuncs, mm.nravals, mm.ncols = k["funcs"].size, k["ravals"].size, model.ncols
-        mm.xdivs, mm.nstab = model.xdivs, k["stablist"].size // 4
-        mm.method, mm.chanmode, mm.calcmode, mm.nconcs = model.method, model.chanmode, 1, 0
-        mm.dt, mm.xmin, mm.invdx = model.dt, model.xmin, model.invdx
-        mm.ops, mm.chip = _p(k["ops"], C.c_int32), _p(k["chip"], C.c_double)
-        mm.tablist = _p(k["tablist"], C.c_double) if k["tablist"].size else None
-        mm.xvals = _p(k["xvals"], C.c_double) if k["xvals"].size else None
-        mm.stablist = _p(k["stablist"], C.c_double) if k["stablist"].size else None
-        mm.funcs, mm.ravals = _p(k["funcs"], C.c_int32), _p(k["ravals"], C.c_double)
-        mm.vm = _p(k["vm"], C.c_double)
+        mm, self._keep = _hsh_model(model)
         if host_only:
             flags |= F_HOST_ONLY
         opt = HshOptions(device, batch_k, batch_waves, flags)

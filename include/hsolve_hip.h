@@ -93,6 +93,9 @@ typedef struct hsh_options {
 #define HSH_F_HOST_ONLY   2  /* analyse and plan only, allocate nothing on a device:
                                 hsh_refusals and hsh_info_get work, stepping and
                                 state calls return HSH_ERR_STATE */
+#define HSH_F_NO_JIT      4  /* use the built-in micro-op interpreter kernels instead
+                                of kernels specialised to the model's channel programs
+                                and compiled at hsh_create (hipRTC) */
 
 /* refusal reasons */
 #define HSH_R_OPCODE         1   /* opcode the device interpreter does not run */
@@ -138,6 +141,9 @@ typedef struct hsh_info {
     int32_t refused;         /* 1 if the handle is refused */
     double  bytes_per_step;  /* algorithmic HBM bytes of one plain step */
     double  lane_fill;       /* real compartments / (blocks*64) */
+    int32_t jit;             /* 1: specialised kernels in use */
+    int32_t pad;
+    double  jit_compile_s;   /* time spent generating + compiling them */
 } hsh_info;
 
 hsh_status hsh_create(const hsh_model *m, const hsh_options *opt, struct hsh_handle **out);
@@ -166,6 +172,10 @@ hsh_status hsh_last_time(struct hsh_handle *h, int32_t kernel, double *ms);
 /* Enable per-kernel event timing for following hsh_step calls (adds events
  * around every launch). */
 hsh_status hsh_set_timing(struct hsh_handle *h, int32_t on);
+/* Diagnostic: the device source libhsolve_hip generates for this model's
+ * channel programs (what hsh_create compiles with hipRTC).  Writes up to cap
+ * bytes (NUL-terminated) and the full length to *len.  Needs no device. */
+hsh_status hsh_jit_source(const hsh_model *m, char *buf, int64_t cap, int64_t *len);
 void       hsh_destroy(struct hsh_handle *h);
 const char *hsh_last_error(void);
 

@@ -101,3 +101,27 @@ def test_host_only_handle_refuses_stepping():
     with D.HipSolver(m, host_only=True) as h:
         with pytest.raises(D.HshError):
             h.step(1)
+
+
+def test_generated_source_compiles_for_gfx950():
+    """The device source hsh_create generates for a model's channel programs
+    compiles with hipRTC for gfx950 (no device needed to compile)."""
+    L = C.CDLL("/opt/rocm/lib/libhiprtc.so")
+    m = random_tree_model(70, 16, seed=5)
+    src = D.jit_source(m)
+    assert "tmpl_0" in src and "tmpl_1" in src and "hsh_jit_batched_r4" in src
+    prog = C.c_void_p()
+    assert L.hiprtcCreateProgram(C.byref(prog), src.encode(), b"hsh_jit.hip", 0, None, None) == 0
+    opts = [b"--offload-arch=gfx950", b"-O3", b"-ffp-contract=off", b"-fno-fast-math", b"-std=c++17",
+            b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
+    rc = L.hiprtcCompileProgram(prog, len(opts), (C.c_char_p * len(opts))(*opts))
+    if rc != 0:
+        n = C.c_size_t()
+        L.hiprtcGetProgramLogSize(prog, C.byref(n))
+        log = C.create_string_buffer(n.value + 1)
+        L.hiprtcGetProgramLog(prog, log)
+        pytest.fail(log.value.decode()[:2000])
+    n = C.c_size_t()
+    L.hiprtcGetCodeSize(prog, C.byref(n))
+    assert n.value > 0
+    L.hiprtcDestroyProgram(C.byref(prog))

@@ -7,6 +7,8 @@ tile boundaries in the batched kernel, which can differ only when a voltage
 lies within rounding of a table breakpoint; the 1e-7 V bound the workload
 states is asserted beside the bitwise check there.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -18,11 +20,24 @@ from oracle import Oracle
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(params=["jit", "interp"], autouse=True)
+def kernels(request, monkeypatch):
+    """Every GPU test runs with the specialised kernels and with the built-in
+    interpreter kernels."""
+    if request.param == "interp":
+        monkeypatch.setenv("HSH_NO_JIT", "1")
+    else:
+        monkeypatch.delenv("HSH_NO_JIT", raising=False)
+    return request.param
+
+
 def run_pair(m, nsteps, mode, **kw):
     print(f"[case] ncompts={m.ncompts} steps={nsteps} mode={mode} {kw}", flush=True)
     o = Oracle(m, givals=False)
     o.step(nsteps)
     with D.HipSolver(m, **kw) as g:
+        want_jit = 0 if os.environ.get("HSH_NO_JIT") == "1" else 1
+        assert g.info()["jit"] == want_jit
         r = g.step(nsteps, mode)
         assert r["vm_ready"] == 1
         st = g.state()
