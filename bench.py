@@ -23,7 +23,11 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import shutil
+import sqlite3
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -34,6 +38,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 NEURONS, NCOMP = 50000, 16
+CAL_MB = 256                   # device copy used to scale FETCH_SIZE / WRITE_SIZE
 
 
 def parse():
@@ -53,14 +58,91 @@ def parse():
     p.add_argument("--copy-calibrate", type=int, default=0, metavar="MB",
                    help="only run a device-to-device copy of MB megabytes (counter calibration "
                         "for FETCH_SIZE/WRITE_SIZE) and exit")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per launch")
+    p.add_argument("--traffic-dir", default="",
+                   help="keep the rocprofv3 databases of the traffic passes under this directory")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="(child of the traffic measurement) copy calibration, then a few batched "
+                        "launches, and exit")
     return p.parse_args()
+
+
+def copy_calibration(torch, mb):
+    """Four device-to-device copies of ``mb`` MiB: the counter reading of a
+    copy of known size gives the scale of FETCH_SIZE and WRITE_SIZE."""
+    nel = mb * (1 << 20) // 8
+    src = torch.ones(nel, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(4):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return nel * 8
+
+
+def read_counter(db, counter):
+    """Mean per-dispatch value of ``counter`` for the batched kernel and for
+    the calibration copy, from a rocprofv3 --pmc database."""
+    con = sqlite3.connect(db)
+    kern, copy = [], []
+    for name, cn, val in con.execute("select kernel_name, counter_name, value from counters_collection"):
+        if cn != counter:
+            continue
+        if "batched" in name:
+            kern.append(val)
+        elif "copyBuffer" in name:
+            copy.append(val)
+    con.close()
+    if not kern or not copy:
+        raise RuntimeError(f"{counter}: no dispatches of the batched kernel or the copy in {db}")
+    return sum(kern) / len(kern), sum(copy) / len(copy)
+
+
+def measure_traffic(a, K):
+    """HBM bytes per batched launch: two child runs of this script under
+    ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE).  Each counter is
+    scaled by what it reads for a copy of CAL_MB MiB in the same run
+    (true bytes / counter), so the unit of the counter does not matter."""
+    exe = shutil.which("rocprofv3")
+    if exe is None:
+        return None, "rocprofv3 not found"
+    res = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    keep = os.path.abspath(a.traffic_dir) if a.traffic_dir else None
+    with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            d = os.path.join(keep or tmp, "pmc_" + counter)
+            cmd = [exe, "--pmc", counter, "-d", d, "-o", "p", "--", sys.executable, os.path.abspath(__file__),
+                   "--traffic-probe", "--neurons", str(a.neurons), "--ncomp", str(a.ncomp),
+                   "--seed", str(a.seed), "--batch-k", str(K), "--flags", str(a.flags)]
+            print(f"[bench] traffic pass {counter}: {' '.join(cmd)}", file=sys.stderr, flush=True)
+            try:
+                p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                   timeout=180)
+            except subprocess.TimeoutExpired:
+                return None, f"{counter} pass timed out"
+            if p.returncode != 0:
+                return None, f"{counter} pass exited {p.returncode}: {p.stdout.decode(errors='replace')[-300:]}"
+            dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
+            if not dbs:
+                return None, f"{counter}: no results database"
+            res[counter] = read_counter(dbs[0], counter)
+    cal_kib = CAL_MB * 1024.0
+    f_fetch = cal_kib / res["FETCH_SIZE"][1]
+    f_write = cal_kib / res["WRITE_SIZE"][1]
+    t = {"bytes_per_launch": (res["FETCH_SIZE"][0] * f_fetch + res["WRITE_SIZE"][0] * f_write) * 1024.0,
+         "fetch_kib_per_launch": res["FETCH_SIZE"][0], "write_kib_per_launch": res["WRITE_SIZE"][0],
+         "fetch_scale": f_fetch, "write_scale": f_write,
+         "calibration": f"{CAL_MB} MiB device copy read as {res['FETCH_SIZE'][1]:.0f} KiB FETCH_SIZE, "
+                        f"{res['WRITE_SIZE'][1]:.0f} KiB WRITE_SIZE"}
+    return t, None
 
 
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
-                                   "(neurons shard with no per-step exchange; see DESIGN.md)"}))
+                                   "(neurons shard across GPUs with no per-step exchange, one handle per GPU)"}))
         sys.exit(2)
 
     import torch
@@ -71,13 +153,16 @@ def main():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
     if a.copy_calibrate:
-        nel = a.copy_calibrate * (1 << 20) // 8
-        src = torch.ones(nel, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(4):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_calibrate_bytes": nel * 8, "launches": 4}))
+        nbytes = copy_calibration(torch, a.copy_calibrate)
+        print(json.dumps({"copy_calibrate_bytes": nbytes, "launches": 4}))
+        return
+    if a.traffic_probe:
+        copy_calibration(torch, CAL_MB)
+        m = M.population(a.neurons, (a.ncomp,), (1.0,), seed=a.seed)
+        with D.HipSolver(m, batch_k=a.batch_k, flags=a.flags) as g:
+            g.step(4 * g.info()["batch_k"], D.BATCHED)
+            g.sync()
+        print(json.dumps({"traffic_probe": "done"}))
         return
 
     t_e2e = time.perf_counter()
@@ -164,6 +249,19 @@ def main():
                      "launch_us": launch_s * 1e6, "steps_per_launch": K},
         "end_to_end_s": {"build_create_upload": t_ready, "total_batched_incl_warmup": t_e2e_total},
     }
+
+    # ---- HBM traffic of the batched kernel (two rocprofv3 --pmc child runs)
+    gb.close()
+    gp.close()
+    if not a.no_traffic:
+        t, why = measure_traffic(a, K)
+        if t is None:
+            print(f"[bench] traffic not measured: {why}", file=sys.stderr, flush=True)
+            out["roofline"]["traffic_note"] = why
+        else:
+            out["roofline"]["traffic"] = t["bytes_per_launch"]
+            out["roofline"]["traffic_detail"] = t
+            out["roofline"]["traffic_over_algorithmic"] = t["bytes_per_launch"] / (bytes_step * K)
 
     # ---- CPU baseline + parity (oracle)
     if not a.no_cpu:

@@ -1,5 +1,6 @@
 // hsh_api.cpp -- C ABI of libhsolve_hip (declared in include/hsolve_hip.h).
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -30,18 +31,42 @@ hsh_status fail(hsh_status st, const std::string& msg)
                                             hipGetErrorString(e_));                   \
     } while (0)
 
+#define STCHK(call)                         \
+    do {                                    \
+        hsh_status s_ = (call);             \
+        if (s_ != HSH_OK) return s_;        \
+    } while (0)
+
 template <class T> hipError_t dalloc(T** p, size_t n)
 {
     return hipMalloc((void**)p, std::max<size_t>(1, n) * sizeof(T));
 }
 
+template <class T> hipError_t dput(T* dst, const std::vector<T>& src)
+{
+    if (src.empty()) return hipSuccess;
+    return hipMemcpy(dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+}
+
 constexpr int SPIKE_CAP = 1 << 20;
 constexpr int GRAPH_MAX_LAUNCHES = 64;
+constexpr int MAX_LDS = 160 * 1024;     // gfx950 LDS per workgroup
+
+// Batched kernel classes: contiguous tile ranges (tiles are sorted by rows,
+// largest first) that one kernel advances, plus the packs.
+enum ClassKind { C_ISO, C_R1, C_R2, C_R4, C_WIDE, C_PLAINLOOP, C_PACKS };
+
+struct KClass {
+    int kind;
+    int t0, t1;      // tile range (C_PACKS: pack range)
+    int rows;        // largest rows in the range
+};
 
 }  // namespace
 
 struct hsh_handle {
     hsh_model m{};
+    hsh_options opt{};
     Plan plan;
     bool refused = false;
     bool host_only = false;
@@ -50,18 +75,28 @@ struct hsh_handle {
     // host copies in the reference layout
     std::vector<int32_t> ops;
     std::vector<double> chip;
-    std::vector<double> tablist, xvals;
-    // device buffers
+    // model-constant device buffers
     int32_t* d_prog = nullptr;
+    double *d_tab = nullptr, *d_xvals = nullptr;
+    int32_t *d_spk = nullptr, *d_nspk = nullptr;
+    // layout-dependent device buffers
     BlockMeta* d_blocks = nullptr;
     TileMeta* d_tiles = nullptr;
+    PackMeta* d_packs = nullptr;
     double *d_S = nullptr, *d_V = nullptr, *d_R = nullptr, *d_ta = nullptr, *d_tcf = nullptr,
-           *d_tcb = nullptr, *d_tab = nullptr, *d_xvals = nullptr;
-    int32_t *d_I = nullptr, *d_par = nullptr, *d_pred = nullptr, *d_spk = nullptr, *d_nspk = nullptr,
-            *d_step = nullptr;
+           *d_tcb = nullptr, *d_kta = nullptr, *d_ktcf = nullptr, *d_btcb = nullptr;
+    int32_t *d_I = nullptr, *d_par = nullptr, *d_pred = nullptr, *d_step = nullptr, *d_lvl = nullptr,
+            *d_fent = nullptr, *d_bkp = nullptr;
+    uint16_t* d_kidx = nullptr;
     DevModel dm{};
     JitModule jit;
+    std::vector<KClass> classes;
+    bool plain_loop = false;           // some tile too large for the batched kernels
     int batch_k = 32, batch_waves = 4;
+    int pack_waves = 4;
+    bool pack_stage = false;
+    bool tuned = false;
+    double autotune_s = 0.0;
     // graphs keyed by (K, launches)
     std::map<std::pair<int, int>, hipGraphExec_t> graphs;
     std::map<std::pair<int, int>, hipGraph_t> graph_src;
@@ -164,35 +199,295 @@ hsh_status download_chip_ops(hsh_handle* h)
     return HSH_OK;
 }
 
-void free_device(hsh_handle* h)
+// vm[] and results[] in the reference layout.
+hsh_status download_vm_results(hsh_handle* h, double* vm, double* results)
 {
-    void* ptrs[] = {h->d_prog, h->d_blocks, h->d_tiles, h->d_S, h->d_V, h->d_R, h->d_ta, h->d_tcf,
-                    h->d_tcb, h->d_tab, h->d_xvals, h->d_I, h->d_par, h->d_pred, h->d_spk,
-                    h->d_nspk, h->d_step};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    jit_free(h->jit);
+    const Plan& P = h->plan;
+    const size_t npos = P.blocks.size() * W;
+    std::vector<double> V(npos), R(2 * npos);
+    HIPCHK(hipMemcpy(V.data(), h->d_V, npos * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(R.data(), h->d_R, 2 * npos * sizeof(double), hipMemcpyDeviceToHost));
+    for (int g = 0; g < h->m.ncompts; g++) {
+        const int pos = P.comp_pos[g];
+        const int b = pos / W, l = pos % W;
+        if (vm) vm[g] = V[pos];
+        if (results) {
+            results[2 * g] = R[(size_t)b * 2 * W + l];
+            results[2 * g + 1] = R[(size_t)b * 2 * W + W + l];
+        }
+    }
+    return HSH_OK;
+}
+
+void free_graphs(hsh_handle* h)
+{
     for (auto& g : h->graphs) (void)hipGraphExecDestroy(g.second);
     for (auto& g : h->graph_src) (void)hipGraphDestroy(g.second);
     h->graphs.clear();
     h->graph_src.clear();
+}
+
+void free_layout(hsh_handle* h)
+{
+    free_graphs(h);
+    void* ptrs[] = {h->d_blocks, h->d_tiles, h->d_packs, h->d_S, h->d_V, h->d_R, h->d_ta, h->d_tcf,
+                    h->d_tcb, h->d_kta, h->d_ktcf, h->d_btcb, h->d_I, h->d_par, h->d_pred, h->d_step,
+                    h->d_lvl, h->d_fent, h->d_bkp, h->d_kidx};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_blocks = nullptr;
+    h->d_tiles = nullptr;
+    h->d_packs = nullptr;
+    h->d_S = h->d_V = h->d_R = h->d_ta = h->d_tcf = h->d_tcb = h->d_kta = h->d_ktcf = h->d_btcb = nullptr;
+    h->d_I = h->d_par = h->d_pred = h->d_step = h->d_lvl = h->d_fent = h->d_bkp = nullptr;
+    h->d_kidx = nullptr;
+}
+
+void free_device(hsh_handle* h)
+{
+    free_layout(h);
+    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_spk, h->d_nspk};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_prog = nullptr;
+    h->d_tab = h->d_xvals = nullptr;
+    h->d_spk = h->d_nspk = nullptr;
+    jit_free(h->jit);
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     h->evs.clear();
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
+    h->ev_a = h->ev_b = nullptr;
     if (h->stream) (void)hipStreamDestroy(h->stream);
     h->stream = nullptr;
 }
 
-hsh_status batched_launch(hsh_handle* h, int K, int count)
+// Tile classes for the batched step (tiles are sorted by rows, largest first).
+void compute_classes(hsh_handle* h)
 {
-    const int rows = h->plan.max_rows;
-    if (rows == 1) {
-        HIPCHK(launch_iso(h->dm, &h->jit, K, h->stream));
-    } else {
-        HIPCHK(launch_batched(h->dm, &h->jit, K, h->batch_waves, rows, h->stream));
+    const Plan& P = h->plan;
+    h->classes.clear();
+    h->plain_loop = false;
+    auto kind_of = [](int rows) {
+        if (rows == 1) return (int)C_ISO;
+        if (rows <= 4) return (int)C_R1;
+        if (rows <= 8) return (int)C_R2;
+        if (rows <= 16) return (int)C_R4;
+        if (rows <= 64) return (int)C_WIDE;
+        return (int)C_PLAINLOOP;
+    };
+    for (int t = 0; t < (int)P.tiles.size(); t++) {
+        const int k = kind_of(P.tiles[t].rows);
+        if (k == C_PLAINLOOP) h->plain_loop = true;
+        if (!h->classes.empty() && h->classes.back().kind == k) {
+            h->classes.back().t1 = t + 1;
+            h->classes.back().rows = std::max(h->classes.back().rows, P.tiles[t].rows);
+        } else {
+            h->classes.push_back(KClass{k, t, t + 1, P.tiles[t].rows});
+        }
     }
-    (void)count;
+    if (!P.packs.empty()) h->classes.push_back(KClass{C_PACKS, 0, (int)P.packs.size(), 0});
+}
+
+void set_devmodel(hsh_handle* h)
+{
+    const Plan& P = h->plan;
+    DevModel& d = h->dm;
+    d = DevModel{};
+    d.prog = h->d_prog;
+    d.blocks = h->d_blocks;
+    d.tiles = h->d_tiles;
+    d.S = h->d_S;
+    d.I = h->d_I;
+    d.V = h->d_V;
+    d.R = h->d_R;
+    d.par = h->d_par;
+    d.ta = h->d_ta;
+    d.tcf = h->d_tcf;
+    d.tcb = h->d_tcb;
+    d.pred = h->d_pred;
+    d.tab = h->d_tab;
+    d.xvals = h->d_xvals;
+    d.spk = h->d_spk;
+    d.nspk = h->d_nspk;
+    d.step = h->d_step;
+    d.packs = h->d_packs;
+    d.pstep = h->d_step + P.tiles.size();
+    d.lvl = h->d_lvl;
+    d.fent = h->d_fent;
+    d.kidx = h->d_kidx;
+    d.kta = h->d_kta;
+    d.ktcf = h->d_ktcf;
+    d.bkp = h->d_bkp;
+    d.btcb = h->d_btcb;
+    d.npacks = (int32_t)P.packs.size();
+    d.spk_cap = SPIKE_CAP;
+    d.ncols = h->m.ncols;
+    d.xdivs = h->m.xdivs;
+    d.crank = (h->m.method == HSH_CRANK);
+    d.xmin = h->m.xmin;
+    d.invdx = h->m.invdx;
+    d.nblocks = (int32_t)P.blocks.size();
+    d.ntiles = (int32_t)P.tiles.size();
+    d.last_pos = P.comp_pos[h->m.ncompts - 1];
+    d.single = (h->m.ncompts == 1);
+    d.max_slots = 0;
+    for (const Template& t : P.templates) d.max_slots = std::max(d.max_slots, t.nslots);
+}
+
+// Allocate and fill the layout-dependent buffers from the plan and the given
+// state (vm, results in the reference layout; chip/ops from the host copies).
+hsh_status alloc_layout(hsh_handle* h, const double* vm, const double* results)
+{
+    const Plan& P = h->plan;
+    const size_t npos = P.blocks.size() * W;
+    const size_t ncnt = P.tiles.size() + P.packs.size();
+    HIPCHK(dalloc(&h->d_blocks, P.blocks.size()));
+    HIPCHK(dalloc(&h->d_tiles, P.tiles.size()));
+    HIPCHK(dalloc(&h->d_packs, P.packs.size()));
+    HIPCHK(dalloc(&h->d_S, (size_t)P.nS));
+    HIPCHK(dalloc(&h->d_I, (size_t)std::max<int64_t>(1, P.nI)));
+    HIPCHK(dalloc(&h->d_V, npos));
+    HIPCHK(dalloc(&h->d_R, 2 * npos));
+    HIPCHK(dalloc(&h->d_par, npos));
+    HIPCHK(dalloc(&h->d_ta, npos));
+    HIPCHK(dalloc(&h->d_tcf, npos));
+    HIPCHK(dalloc(&h->d_tcb, npos));
+    HIPCHK(dalloc(&h->d_pred, npos));
+    HIPCHK(dalloc(&h->d_step, ncnt));
+    HIPCHK(dalloc(&h->d_lvl, P.lvl.size()));
+    HIPCHK(dalloc(&h->d_fent, P.fent.size()));
+    HIPCHK(dalloc(&h->d_kidx, P.kidx.size()));
+    HIPCHK(dalloc(&h->d_kta, P.kta.size()));
+    HIPCHK(dalloc(&h->d_ktcf, P.ktcf.size()));
+    HIPCHK(dalloc(&h->d_bkp, P.bkp.size()));
+    HIPCHK(dalloc(&h->d_btcb, P.btcb.size()));
+    // tree and predecessor arrays of the tile layout (row within the tile)
+    std::vector<int32_t> par(npos), pred(npos);
+    std::vector<double> ta(npos), tcf(npos), tcb(npos);
+    {
+        std::vector<int32_t> lo_of(h->m.ncompts);
+        for (size_t t = 0; t < P.tree_lo.size(); t++)
+            for (int r = P.tree_lo[t]; r <= P.tree_hi[t]; r++) lo_of[r] = P.tree_lo[t];
+        for (size_t q = 0; q < npos; q++) {
+            int g = P.pos_comp[q];
+            int pg = P.parent[g];
+            par[q] = pg >= 0 ? pg - lo_of[g] : -1;
+            ta[q] = P.ta[g];
+            tcf[q] = P.tcf[g];
+            tcb[q] = P.tcb[g];
+            const bool real = (P.comp_pos[g] == (int)q);
+            pred[q] = (real && P.pred[g] >= 0) ? P.comp_pos[P.pred[g]] : -1;
+        }
+    }
+    HIPCHK(dput(h->d_blocks, P.blocks));
+    HIPCHK(dput(h->d_tiles, P.tiles));
+    HIPCHK(dput(h->d_packs, P.packs));
+    HIPCHK(dput(h->d_par, par));
+    HIPCHK(dput(h->d_pred, pred));
+    HIPCHK(dput(h->d_ta, ta));
+    HIPCHK(dput(h->d_tcf, tcf));
+    HIPCHK(dput(h->d_tcb, tcb));
+    HIPCHK(dput(h->d_lvl, P.lvl));
+    HIPCHK(dput(h->d_fent, P.fent));
+    HIPCHK(dput(h->d_kidx, P.kidx));
+    HIPCHK(dput(h->d_kta, P.kta));
+    HIPCHK(dput(h->d_ktcf, P.ktcf));
+    HIPCHK(dput(h->d_bkp, P.bkp));
+    HIPCHK(dput(h->d_btcb, P.btcb));
+    std::vector<int32_t> steps(ncnt, h->host_step);
+    HIPCHK(dput(h->d_step, steps));
+    std::vector<double> R(2 * npos, 0.0);
+    if (results)
+        for (size_t q = 0; q < npos; q++) {
+            const int g = P.pos_comp[q];
+            const size_t b = q / W, l = q % W;
+            R[b * 2 * W + l] = results[2 * g];
+            R[b * 2 * W + W + l] = results[2 * g + 1];
+        }
+    HIPCHK(dput(h->d_R, R));
+    STCHK(upload_state(h, vm));
+    // reverse map chip index -> S positions
+    {
+        const int nchips = h->m.nchips;
+        std::vector<int64_t> cnt(nchips + 1, 0);
+        for (int g = 0; g < h->m.ncompts; g++) {
+            const Template& t = P.templates[P.comp_tmpl[g]];
+            for (int k = 0; k < t.nslots; k++) {
+                int64_t src = P.dsrc[P.comp_dsrc_off[g] + k];
+                if (src >= 0) cnt[src + 1]++;
+            }
+        }
+        for (int i = 0; i < nchips; i++) cnt[i + 1] += cnt[i];
+        h->c2s_off = cnt;
+        h->c2s.assign(cnt[nchips], 0);
+        std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+        for (int g = 0; g < h->m.ncompts; g++) {
+            const Template& t = P.templates[P.comp_tmpl[g]];
+            const int pos = P.comp_pos[g];
+            const BlockMeta& bm = P.blocks[pos / W];
+            for (int k = 0; k < t.nslots; k++) {
+                int64_t src = P.dsrc[P.comp_dsrc_off[g] + k];
+                if (src >= 0) h->c2s[cur[src]++] = (int64_t)bm.dslot + (int64_t)k * W + pos % W;
+            }
+        }
+    }
+    set_devmodel(h);
+    compute_classes(h);
+    return HSH_OK;
+}
+
+// Move the device state to a new layout.
+hsh_status relayout(hsh_handle* h, const LayoutOpts& o)
+{
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int n = h->m.ncompts;
+    std::vector<double> vm(n), res(2 * (size_t)n);
+    STCHK(download_vm_results(h, vm.data(), res.data()));
+    STCHK(download_chip_ops(h));
+    std::string err;
+    if (!build_layout(h->plan, o, err)) return fail(HSH_ERR_ARG, "layout: " + err);
+    free_layout(h);
+    return alloc_layout(h, vm.data(), res.data());
+}
+
+int pack_lds_bytes(const hsh_handle* h, bool stage)
+{
+    return (int)(stage ? h->plan.pack_lds_stage : h->plan.pack_lds);
+}
+
+bool stage_fits(const hsh_handle* h) { return h->plan.pack_lds_stage <= MAX_LDS; }
+
+DevModel tile_view(const DevModel& d, const KClass& c)
+{
+    DevModel v = d;
+    v.tiles = d.tiles + c.t0;
+    v.step = d.step + c.t0;
+    v.ntiles = c.t1 - c.t0;
+    return v;
+}
+
+hsh_status launch_class(hsh_handle* h, const KClass& c, int K, hipStream_t s)
+{
+    if (c.kind == C_PACKS) {
+        HIPCHK(launch_forest(h->dm, &h->jit, K, h->pack_waves, h->pack_stage, pack_lds_bytes(h, h->pack_stage), s));
+        return HSH_OK;
+    }
+    const DevModel v = tile_view(h->dm, c);
+    if (c.kind == C_ISO) {
+        HIPCHK(launch_iso(v, &h->jit, K, s));
+    } else {
+        const int waves = (c.kind == C_WIDE) ? std::max(1, h->batch_waves) : 0;
+        HIPCHK(launch_batched(v, &h->jit, K, waves, c.rows, s));
+    }
+    return HSH_OK;
+}
+
+// One batched launch of K steps: the kernel of every class, in turn, on the
+// handle's stream.
+hsh_status batched_launch(hsh_handle* h, int K)
+{
+    for (const KClass& c : h->classes) STCHK(launch_class(h, c, K, h->stream));
     return HSH_OK;
 }
 
@@ -209,7 +504,7 @@ hsh_status run_batched(hsh_handle* h, int nsteps, int* launches)
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
             hsh_status st = HSH_OK;
-            for (int i = 0; i < M && st == HSH_OK; i++) st = batched_launch(h, K, 1);
+            for (int i = 0; i < M && st == HSH_OK; i++) st = batched_launch(h, K);
             hipError_t ce = hipStreamEndCapture(h->stream, &g);
             if (st != HSH_OK) return st;
             if (ce != hipSuccess) return fail(HSH_ERR_DEVICE, std::string("HIP: graph capture failed: ") + hipGetErrorString(ce));
@@ -223,10 +518,156 @@ hsh_status run_batched(hsh_handle* h, int nsteps, int* launches)
         full -= M;
     }
     if (rem > 0) {
-        hsh_status st = batched_launch(h, rem, 1);
-        if (st != HSH_OK) return st;
+        STCHK(batched_launch(h, rem));
         (*launches)++;
     }
+    return HSH_OK;
+}
+
+// One plain step: channel programs of every block, then the eliminations.
+hsh_status plain_step(hsh_handle* h, bool channels_only)
+{
+    if (h->timing) {
+        while ((int)h->evs.size() < h->nev_used + 3) {
+            hipEvent_t e;
+            HIPCHK(hipEventCreate(&e));
+            h->evs.push_back(e);
+        }
+        HIPCHK(hipEventRecord(h->evs[h->nev_used], h->stream));
+    }
+    HIPCHK(launch_channel(h->dm, &h->jit, h->stream));
+    if (h->timing) HIPCHK(hipEventRecord(h->evs[h->nev_used + 1], h->stream));
+    if (channels_only) {
+        HIPCHK(launch_step_advance(h->dm, 1, h->stream));
+    } else {
+        if (h->dm.ntiles > 0) HIPCHK(launch_elim(h->dm, h->plan.max_rows, h->stream));
+        if (h->dm.npacks > 0) HIPCHK(launch_forest_elim(h->dm, pack_lds_bytes(h, false), h->stream));
+    }
+    if (h->timing) {
+        HIPCHK(hipEventRecord(h->evs[h->nev_used + 2], h->stream));
+        h->nev_used += 3;
+    }
+    return HSH_OK;
+}
+
+// Device time per step of `steps` batched steps (graph replays included).
+hsh_status time_batched(hsh_handle* h, int steps, double* ms_per_step)
+{
+    int launches = 0;
+    STCHK(run_batched(h, steps, &launches));        // warm (and capture)
+    HIPCHK(hipEventRecord(h->ev_a, h->stream));
+    STCHK(run_batched(h, steps, &launches));
+    HIPCHK(hipEventRecord(h->ev_b, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev_b));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, h->ev_a, h->ev_b));
+    *ms_per_step = (double)t / steps;
+    return HSH_OK;
+}
+
+// The autotuner: layout (tile_max_rows, pack_cap), pack workgroup shape
+// (waves, schedule in LDS) and K, each swept on this model with the others
+// held, fastest kept.  Options the caller fixed are not swept.  Only layouts
+// that one kernel runs are candidates (all trees in tiles of one class, or
+// all in packs); a layout that mixes classes runs only as configured.  The
+// state is saved first and restored after (spikes raised while tuning are
+// dropped).
+hsh_status autotune(hsh_handle* h)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int n = h->m.ncompts;
+    std::vector<double> vm0(n), res0(2 * (size_t)n);
+    STCHK(download_vm_results(h, vm0.data(), res0.data()));
+    STCHK(download_chip_ops(h));
+    const std::vector<double> chip0 = h->chip;
+    const std::vector<int32_t> ops0 = h->ops;
+    int32_t nspk0 = 0;
+    HIPCHK(hipMemcpy(&nspk0, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
+    const hsh_options& o = h->opt;
+    const Plan& P = h->plan;
+    const int K0 = o.batch_k > 0 ? o.batch_k : 32;
+    h->batch_k = K0;
+    // steps per trial: about 2e6 compartment-steps, at least 2 launches
+    const int trial = std::max(2 * K0, std::min(4096, (int)(2e6 / std::max(1, n)) / K0 * K0));
+
+    std::vector<int> tmr_c, cap_c, waves_c, stage_c;
+    if (o.tile_max_rows != 0) tmr_c = {o.tile_max_rows < 0 ? 0 : o.tile_max_rows};
+    else {
+        tmr_c = {16};
+        if (P.max_tree_rows > 16) tmr_c.push_back(64);
+        tmr_c.push_back(0);
+    }
+    if (o.pack_cap > 0) cap_c = {o.pack_cap};
+    else cap_c = {512, 1024, 2048};
+    waves_c = o.pack_waves > 0 ? std::vector<int>{o.pack_waves} : std::vector<int>{4, 8};
+    stage_c = o.pack_stage > 0 ? std::vector<int>{o.pack_stage == 2} : std::vector<int>{0, 1};
+
+    struct Best {
+        double ms = 1e300;
+        LayoutOpts lo;
+        int waves = 4;
+        bool stage = false;
+    } best;
+    for (int tmr : tmr_c) {
+        for (size_t ci = 0; ci < cap_c.size(); ci++) {
+            LayoutOpts lo;
+            lo.tile_max_rows = tmr;
+            lo.pack_cap = cap_c[ci];
+            STCHK(relayout(h, lo));
+            // pack size matters only when some tree went to a pack
+            const bool any_pack = !h->plan.packs.empty();
+            if (h->plain_loop) break;        // layouts with too-large tiles are not batched
+            if (h->classes.size() != 1) break;
+            for (int wv : (any_pack ? waves_c : std::vector<int>{waves_c[0]})) {
+                for (int st : (any_pack ? stage_c : std::vector<int>{0})) {
+                    if (st && !stage_fits(h)) continue;
+                    if (h->plan.pack_lds > MAX_LDS) continue;
+                    h->pack_waves = wv;
+                    h->pack_stage = st != 0;
+                    free_graphs(h);
+                    double ms = 0;
+                    STCHK(time_batched(h, trial, &ms));
+                    if (ms < best.ms) {
+                        best.ms = ms;
+                        best.lo = lo;
+                        best.waves = wv;
+                        best.stage = st != 0;
+                    }
+                }
+            }
+            if (!any_pack) break;
+        }
+    }
+    if (best.ms < 1e300) {
+        STCHK(relayout(h, best.lo));
+        h->pack_waves = best.waves;
+        h->pack_stage = best.stage;
+        if (o.batch_k <= 0) {
+            double bk = 1e300;
+            int kbest = K0;
+            for (int K : {1, 8, 32, 128}) {
+                h->batch_k = K;
+                free_graphs(h);
+                double ms = 0;
+                STCHK(time_batched(h, std::max(2 * K, trial / K * K), &ms));
+                if (ms < bk) {
+                    bk = ms;
+                    kbest = K;
+                }
+            }
+            h->batch_k = kbest;
+            free_graphs(h);
+        }
+    }
+    // restore the state
+    h->chip = chip0;
+    h->ops = ops0;
+    free_layout(h);
+    STCHK(alloc_layout(h, vm0.data(), res0.data()));
+    HIPCHK(hipMemcpy(h->d_nspk, &nspk0, sizeof(int32_t), hipMemcpyHostToDevice));
+    h->tuned = true;
+    h->autotune_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return HSH_OK;
 }
 
@@ -249,8 +690,12 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         return fail(HSH_ERR_ARG, "hsh_create: chanmode must be 4 or 5");
     if (m->method != HSH_BEULER && m->method != HSH_CRANK)
         return fail(HSH_ERR_ARG, "hsh_create: method must be HSH_BEULER or HSH_CRANK");
+    if (opt && (opt->pack_waves < 0 || opt->pack_waves > 8 || opt->pack_stage < 0 || opt->pack_stage > 2 ||
+                opt->pack_cap < 0 || opt->batch_k < 0 || opt->batch_waves < 0))
+        return fail(HSH_ERR_ARG, "hsh_create: option out of range");
     hsh_handle* h = new hsh_handle();
     h->m = *m;
+    if (opt) h->opt = *opt;
     h->ops.assign(m->ops, m->ops + m->nops);
     h->chip.assign(m->chip, m->chip + m->nchips);
     std::string err;
@@ -269,18 +714,33 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         g_err = buf;
         return HSH_REFUSED;
     }
-    if (opt && (opt->flags & HSH_F_HOST_ONLY)) {
+    // initial layout from the options (defaults where not given)
+    const hsh_options& o = h->opt;
+    LayoutOpts lo;
+    if (o.tile_max_rows != 0) lo.tile_max_rows = o.tile_max_rows < 0 ? 0 : o.tile_max_rows;
+    if (o.pack_cap > 0) lo.pack_cap = o.pack_cap;
+    if (!build_layout(h->plan, lo, err)) {
+        delete h;
+        return fail(HSH_ERR_ARG, "hsh_create: " + err);
+    }
+    if (o.batch_k > 0) h->batch_k = o.batch_k;
+    if (o.pack_waves > 0) h->pack_waves = o.pack_waves;
+    if (o.pack_stage > 0) h->pack_stage = (o.pack_stage == 2);
+    if (h->plan.pack_lds_stage > MAX_LDS) h->pack_stage = false;
+    if (h->plan.pack_lds > MAX_LDS) {
+        delete h;
+        return fail(HSH_ERR_ARG, "hsh_create: a pack needs more LDS than a workgroup has");
+    }
+    // waves per tile of the LDS kernel for trees of 17..64 rows
+    if (o.batch_waves > 0) h->batch_waves = o.batch_waves;
+    else h->batch_waves = std::min(16, std::max(1, h->plan.max_rows / 4));
+    if (o.flags & HSH_F_HOST_ONLY) {
         h->host_only = true;
-        if (opt->batch_k > 0) h->batch_k = opt->batch_k;
+        compute_classes(h);
         *out = h;
         return HSH_OK;
     }
-    h->device = opt ? opt->device : 0;
-    if (opt && opt->batch_k > 0) h->batch_k = opt->batch_k;
-    // 0 = four-wave tiles with register-held tree coefficients (trees of up
-    // to 16 rows); otherwise waves per tile of the LDS-coefficient kernel
-    if (opt && opt->batch_waves > 0) h->batch_waves = opt->batch_waves;
-    else h->batch_waves = h->plan.max_rows <= 16 ? 0 : std::min(16, std::max(1, h->plan.max_rows / 4));
+    h->device = o.device;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         delete h;
@@ -304,122 +764,31 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     } while (0)
     HIPCHK_C(hipSetDevice(h->device));
     HIPCHK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    HIPCHK_C(hipEventCreate(&h->ev_a));
+    HIPCHK_C(hipEventCreate(&h->ev_b));
     const Plan& P = h->plan;
-    const size_t npos = P.blocks.size() * W;
     HIPCHK_C(dalloc(&h->d_prog, P.prog.size()));
-    HIPCHK_C(dalloc(&h->d_blocks, P.blocks.size()));
-    HIPCHK_C(dalloc(&h->d_tiles, P.tiles.size()));
-    HIPCHK_C(dalloc(&h->d_S, (size_t)P.nS));
-    HIPCHK_C(dalloc(&h->d_I, (size_t)std::max<int64_t>(1, P.nI)));
-    HIPCHK_C(dalloc(&h->d_V, npos));
-    HIPCHK_C(dalloc(&h->d_R, 2 * npos));
-    HIPCHK_C(dalloc(&h->d_par, npos));
-    HIPCHK_C(dalloc(&h->d_ta, npos));
-    HIPCHK_C(dalloc(&h->d_tcf, npos));
-    HIPCHK_C(dalloc(&h->d_tcb, npos));
-    HIPCHK_C(dalloc(&h->d_pred, npos));
     HIPCHK_C(dalloc(&h->d_spk, 2 * (size_t)SPIKE_CAP));
     HIPCHK_C(dalloc(&h->d_nspk, 1));
-    HIPCHK_C(dalloc(&h->d_step, P.tiles.size()));
     const size_t ntab = (m->ncols > 0) ? (size_t)(m->xdivs + 2) * m->ncols : 1;
     const size_t nx = (m->ncols > 0) ? (size_t)(m->xdivs + 2) : 1;
     HIPCHK_C(dalloc(&h->d_tab, ntab));
     HIPCHK_C(dalloc(&h->d_xvals, nx));
-    // pack tree and predecessor arrays
-    std::vector<int32_t> par(npos), pred(npos);
-    std::vector<double> ta(npos), tcf(npos), tcb(npos);
-    {
-        std::vector<int32_t> lo_of(m->ncompts);
-        for (size_t t = 0; t < P.tree_lo.size(); t++)
-            for (int r = P.tree_lo[t]; r <= P.tree_hi[t]; r++) lo_of[r] = P.tree_lo[t];
-        for (size_t q = 0; q < npos; q++) {
-            int g = P.pos_comp[q];
-            int pg = P.parent[g];
-            par[q] = pg >= 0 ? pg - lo_of[g] : -1;
-            ta[q] = P.ta[g];
-            tcf[q] = P.tcf[g];
-            tcb[q] = P.tcb[g];
-            const bool real = (P.comp_pos[g] == (int)q);
-            pred[q] = (real && P.pred[g] >= 0) ? P.comp_pos[P.pred[g]] : -1;
-        }
-    }
-    HIPCHK_C(hipMemcpy(h->d_prog, P.prog.data(), P.prog.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_blocks, P.blocks.data(), P.blocks.size() * sizeof(BlockMeta), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_tiles, P.tiles.data(), P.tiles.size() * sizeof(TileMeta), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_par, par.data(), npos * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_pred, pred.data(), npos * sizeof(int32_t), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_ta, ta.data(), npos * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_tcf, tcf.data(), npos * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK_C(hipMemcpy(h->d_tcb, tcb.data(), npos * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK_C(dput(h->d_prog, P.prog));
     if (m->ncols > 0) {
         HIPCHK_C(hipMemcpy(h->d_tab, m->tablist, ntab * sizeof(double), hipMemcpyHostToDevice));
         HIPCHK_C(hipMemcpy(h->d_xvals, m->xvals, nx * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK_C(hipMemset(h->d_nspk, 0, sizeof(int32_t)));
-    HIPCHK_C(hipMemset(h->d_step, 0, P.tiles.size() * sizeof(int32_t)));
-    HIPCHK_C(hipMemset(h->d_R, 0, 2 * npos * sizeof(double)));
-    if (upload_state(h, m->vm) != HSH_OK) return cleanup_fail(HSH_ERR_DEVICE);
-    // reverse map chip index -> S positions
-    {
-        std::vector<int64_t> cnt(m->nchips + 1, 0);
-        for (int g = 0; g < m->ncompts; g++) {
-            const Template& t = P.templates[P.comp_tmpl[g]];
-            for (int k = 0; k < t.nslots; k++) {
-                int64_t src = P.dsrc[P.comp_dsrc_off[g] + k];
-                if (src >= 0) cnt[src + 1]++;
-            }
-        }
-        for (int i = 0; i < m->nchips; i++) cnt[i + 1] += cnt[i];
-        h->c2s_off = cnt;
-        h->c2s.assign(cnt[m->nchips], 0);
-        std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
-        for (int g = 0; g < m->ncompts; g++) {
-            const Template& t = P.templates[P.comp_tmpl[g]];
-            const int pos = P.comp_pos[g];
-            const BlockMeta& bm = P.blocks[pos / W];
-            for (int k = 0; k < t.nslots; k++) {
-                int64_t src = P.dsrc[P.comp_dsrc_off[g] + k];
-                if (src >= 0) h->c2s[cur[src]++] = (int64_t)bm.dslot + (int64_t)k * W + pos % W;
-            }
-        }
-    }
-    DevModel& d = h->dm;
-    d.prog = h->d_prog;
-    d.blocks = h->d_blocks;
-    d.tiles = h->d_tiles;
-    d.S = h->d_S;
-    d.I = h->d_I;
-    d.V = h->d_V;
-    d.R = h->d_R;
-    d.par = h->d_par;
-    d.ta = h->d_ta;
-    d.tcf = h->d_tcf;
-    d.tcb = h->d_tcb;
-    d.pred = h->d_pred;
-    d.tab = h->d_tab;
-    d.xvals = h->d_xvals;
-    d.spk = h->d_spk;
-    d.nspk = h->d_nspk;
-    d.step = h->d_step;
-    d.spk_cap = SPIKE_CAP;
-    d.ncols = m->ncols;
-    d.xdivs = m->xdivs;
-    d.crank = (m->method == HSH_CRANK);
-    d.xmin = m->xmin;
-    d.invdx = m->invdx;
-    d.nblocks = (int32_t)P.blocks.size();
-    d.ntiles = (int32_t)P.tiles.size();
-    d.last_pos = P.comp_pos[m->ncompts - 1];
-    d.single = (m->ncompts == 1);
-    d.max_slots = 0;
-    for (const Template& t : P.templates) d.max_slots = std::max(d.max_slots, t.nslots);
-    HIPCHK_C(hipEventCreate(&h->ev_a));
-    HIPCHK_C(hipEventCreate(&h->ev_b));
 #undef HIPCHK_C
+    {
+        hsh_status st = alloc_layout(h, m->vm, nullptr);
+        if (st != HSH_OK) return cleanup_fail(st);
+    }
     // kernels specialised to this model's channel programs; on failure the
     // built-in interpreter kernels run (message kept in hsh_last_error)
     const char* nojit = std::getenv("HSH_NO_JIT");
-    if (!(opt && (opt->flags & HSH_F_NO_JIT)) && !(nojit && nojit[0] == '1')) {
+    if (!(o.flags & HSH_F_NO_JIT) && !(nojit && nojit[0] == '1')) {
         std::string jerr;
         if (!jit_build(P, m->ncols, h->jit, jerr)) {
             std::fprintf(stderr, "libhsolve_hip: specialised kernels unavailable (%s); using the interpreter kernels\n",
@@ -441,46 +810,36 @@ hsh_status hsh_refusals(hsh_handle* h, hsh_refusal* out, int32_t cap, int32_t* n
     return HSH_OK;
 }
 
+hsh_status hsh_autotune(hsh_handle* h)
+{
+    if (!h) return fail(HSH_ERR_ARG, "hsh_autotune: null handle");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_autotune: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_autotune: handle refused the model");
+    return autotune(h);
+}
+
 hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* out)
 {
     if (!h) return fail(HSH_ERR_ARG, "hsh_step: null handle");
     if (h->host_only) return fail(HSH_ERR_STATE, "hsh_step: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_step: handle refused the model; use the CPU path");
     if (nsteps < 0) return fail(HSH_ERR_ARG, "hsh_step: negative step count");
+    if (mode != HSH_PLAIN && mode != HSH_CHANNELS && mode != HSH_BATCHED)
+        return fail(HSH_ERR_ARG, "hsh_step: unknown mode");
+    if (mode == HSH_BATCHED && !h->tuned && !(h->opt.flags & HSH_F_NO_AUTOTUNE) && nsteps > 0)
+        STCHK(autotune(h));
     hsh_step_out o{};
     if (h->timing) HIPCHK(hipEventRecord(h->ev_a, h->stream));
     h->nev_used = 0;
-    if (mode == HSH_PLAIN || mode == HSH_CHANNELS ||
-        (mode == HSH_BATCHED && h->plan.max_rows > 64)) {
+    if (mode == HSH_PLAIN || mode == HSH_CHANNELS || h->plain_loop) {
         for (int s = 0; s < nsteps; s++) {
-            if (h->timing) {
-                while ((int)h->evs.size() < h->nev_used + 3) {
-                    hipEvent_t e;
-                    HIPCHK(hipEventCreate(&e));
-                    h->evs.push_back(e);
-                }
-                HIPCHK(hipEventRecord(h->evs[h->nev_used], h->stream));
-            }
-            HIPCHK(launch_channel(h->dm, &h->jit, h->stream));
-            if (h->timing) HIPCHK(hipEventRecord(h->evs[h->nev_used + 1], h->stream));
-            if (mode == HSH_CHANNELS) {
-                HIPCHK(launch_step_advance(h->dm, 1, h->stream));
-            } else {
-                HIPCHK(launch_elim(h->dm, h->plan.max_rows, h->stream));
-            }
-            if (h->timing) {
-                HIPCHK(hipEventRecord(h->evs[h->nev_used + 2], h->stream));
-                h->nev_used += 3;
-            }
+            STCHK(plain_step(h, mode == HSH_CHANNELS));
             o.launches += 2;
         }
         o.vm_ready = (mode != HSH_CHANNELS);
-    } else if (mode == HSH_BATCHED) {
-        hsh_status st = run_batched(h, nsteps, &o.launches);
-        if (st != HSH_OK) return st;
-        o.vm_ready = 1;
     } else {
-        return fail(HSH_ERR_ARG, "hsh_step: unknown mode");
+        STCHK(run_batched(h, nsteps, &o.launches));
+        o.vm_ready = 1;
     }
     if (h->timing) HIPCHK(hipEventRecord(h->ev_b, h->stream));
     h->total_valid = h->timing;
@@ -502,26 +861,10 @@ hsh_status hsh_get_state(hsh_handle* h, double* vm, double* chip, double* result
     if (!h) return fail(HSH_ERR_ARG, "hsh_get_state: null handle");
     if (h->host_only) return fail(HSH_ERR_STATE, "hsh_get_state: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_get_state: handle refused the model");
-    const Plan& P = h->plan;
-    const size_t npos = P.blocks.size() * W;
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (vm || results) {
-        std::vector<double> V(npos), R(2 * npos);
-        HIPCHK(hipMemcpy(V.data(), h->d_V, npos * sizeof(double), hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(R.data(), h->d_R, 2 * npos * sizeof(double), hipMemcpyDeviceToHost));
-        for (int g = 0; g < h->m.ncompts; g++) {
-            const int pos = P.comp_pos[g];
-            const int b = pos / W, l = pos % W;
-            if (vm) vm[g] = V[pos];
-            if (results) {
-                results[2 * g] = R[(size_t)b * 2 * W + l];
-                results[2 * g + 1] = R[(size_t)b * 2 * W + W + l];
-            }
-        }
-    }
+    if (vm || results) STCHK(download_vm_results(h, vm, results));
     if (chip || ops) {
-        hsh_status st = download_chip_ops(h);
-        if (st != HSH_OK) return st;
+        STCHK(download_chip_ops(h));
         if (chip) std::memcpy(chip, h->chip.data(), h->chip.size() * sizeof(double));
         if (ops) std::memcpy(ops, h->ops.data(), h->ops.size() * sizeof(int32_t));
     }
@@ -537,15 +880,11 @@ hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, co
     if (chip || ops) {
         // refresh the host copies with the device's mutable state first, so
         // passing only one of chip/ops keeps the other current
-        hsh_status st = download_chip_ops(h);
-        if (st != HSH_OK) return st;
+        STCHK(download_chip_ops(h));
     }
     if (chip) std::memcpy(h->chip.data(), chip, h->chip.size() * sizeof(double));
     if (ops) std::memcpy(h->ops.data(), ops, h->ops.size() * sizeof(int32_t));
-    if (chip || ops) {
-        hsh_status st = upload_state(h, vm);
-        return st;
-    }
+    if (chip || ops) return upload_state(h, vm);
     if (vm) {
         std::vector<double> V;
         pack_V(h, vm, V);
@@ -612,7 +951,15 @@ hsh_status hsh_info_get(hsh_handle* h, hsh_info* info)
     o.bytes_per_step = P.bytes_per_step;
     o.lane_fill = P.blocks.empty() ? 0.0 : (double)h->m.ncompts / ((double)P.blocks.size() * W);
     o.jit = h->jit.ok ? 1 : 0;
+    o.npacks = (int32_t)P.packs.size();
     o.jit_compile_s = h->jit.compile_s;
+    o.tile_max_rows = P.layout.tile_max_rows;
+    o.pack_cap = P.layout.pack_cap;
+    o.pack_waves = h->pack_waves;
+    o.pack_stage = h->pack_stage ? 2 : 1;
+    o.max_tree_rows = P.max_tree_rows;
+    o.autotuned = h->tuned ? 1 : 0;
+    o.autotune_s = h->autotune_s;
     *info = o;
     return HSH_OK;
 }

@@ -206,13 +206,12 @@ __device__ __forceinline__ void body_channel(const DevModel& d)
     if (b >= d.nblocks) return;
     const int lane = threadIdx.x & (W - 1);
     const BlockMeta bm = d.blocks[b];
-    const TileMeta tm = d.tiles[bm.tile];
     const int pos = b * W + lane;
     const double Vm = d.V[pos];
     const int pp = d.pred[pos];
     const double Vp = pp >= 0 ? d.V[pp] : 0.0;
     const int step = d.step[bm.tile];
-    CompOut o = RUN::run(d, bm, lane, Vm, pp >= 0, Vp, lane < tm.lanes, step);
+    CompOut o = RUN::run(d, bm, lane, Vm, pp >= 0, Vp, lane < bm.lanes, step);
     d.R[(long long)b * 2 * W + lane] = o.rhs;
     d.R[(long long)b * 2 * W + W + lane] = o.diag;
 }
@@ -406,6 +405,162 @@ __device__ __forceinline__ void body_batched(const DevModel& d, int K, double* l
         d.R[b * 2 * W + W + lane] = LD[r * W + lane];
     }
     if (threadIdx.x == 0) d.step[t] = step0 + K;
+}
+
+// Packs (lane = compartment): whole neurons of any size, one workgroup per
+// pack.  V, rhs and diag of every pack-local position live in LDS for the
+// whole launch.  Per step: the waves run the channel programs of the pack's
+// blocks, then the Hines elimination runs level by level -- forward, each
+// parent pulls its children in ascending row order (the order in which the
+// reference's funcs[] program folds them in, so every parent sees the same
+// sequence of roundings); backward, roots solve and children follow their
+// parents -- with a barrier between levels.  CHAN = false is the plain
+// drop-in's elimination (rhs/diag read from results, one step).  STAGE copies
+// the pack's level schedule into LDS once per launch.
+template <class RUN, bool CHAN, bool STAGE>
+__device__ __forceinline__ void body_forest(const DevModel& d, int K, double* lds)
+{
+    const int pk = blockIdx.x;
+    const PackMeta pm = d.packs[pk];
+    const int NP = pm.nblk * W;
+    const int T = blockDim.x;
+    const int tid = threadIdx.x;
+    const int nw = T >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lane = tid & (W - 1);
+    double* LV = lds;
+    double* LB = lds + NP;
+    double* LD = lds + 2 * NP;
+    char* tail = (char*)(lds + 3 * NP);
+    int* LP = (int*)tail;                      // predecessor lp (-1: outside the pack)
+    if (CHAN) tail += (size_t)NP * sizeof(int);
+    const long long base = (long long)pm.block0 * W;
+    const int* lvl = d.lvl + pm.lv0;
+    const int* FE = d.fent + 2 * (long long)pm.fe0;
+    const unsigned short* KI = d.kidx + pm.ke0;
+    const double* KTA = d.kta + pm.ke0;
+    const double* KTF = d.ktcf + pm.ke0;
+    const int* BKP = d.bkp + pm.be0;
+    const double* BTC = d.btcb + pm.be0;
+    if (STAGE) {
+        // doubles first (8-byte aligned), then ints, then shorts
+        double* sKTA = (double*)tail;
+        double* sKTF = sKTA + pm.nke;
+        double* sBTC = sKTF + pm.nke;
+        int* sLVL = (int*)(sBTC + pm.nbe);
+        int* sFE = sLVL + (pm.nfl + pm.nbl + 2);
+        int* sBKP = sFE + 2 * pm.nfe;
+        unsigned short* sKI = (unsigned short*)(sBKP + pm.nbe);
+        for (int i = tid; i < pm.nke; i += T) {
+            sKTA[i] = KTA[i];
+            sKTF[i] = KTF[i];
+            sKI[i] = KI[i];
+        }
+        for (int i = tid; i < pm.nbe; i += T) {
+            sBTC[i] = BTC[i];
+            sBKP[i] = BKP[i];
+        }
+        for (int i = tid; i < 2 * pm.nfe; i += T) sFE[i] = FE[i];
+        for (int i = tid; i < pm.nfl + pm.nbl + 2; i += T) sLVL[i] = lvl[i];
+        KTA = sKTA;
+        KTF = sKTF;
+        BTC = sBTC;
+        lvl = sLVL;
+        FE = sFE;
+        BKP = sBKP;
+        KI = sKI;
+    }
+    for (int i = tid; i < NP; i += T) {
+        LV[i] = d.V[base + i];
+        if (CHAN) {
+            const int pp = d.pred[base + i];
+            LP[i] = (pp >= base && pp < base + NP) ? (int)(pp - base) : -1;
+        } else {
+            const long long r = (long long)(pm.block0 + (i >> 6)) * 2 * W + (i & (W - 1));
+            LB[i] = d.R[r];
+            LD[i] = d.R[r + W];
+        }
+    }
+    const int step0 = d.pstep[pk];
+    // The reference never stores its last row's eliminated diagonal
+    // (hines_solve.c:275-276): the thread that produces that row's diag keeps
+    // a copy and puts it back after the last step.
+    double droot0 = 0.0;
+    bool own_last = false;
+    if (!CHAN && pm.last_lp >= 0 && pm.last_lp % T == tid) {
+        own_last = true;
+        droot0 = LD[pm.last_lp];
+    }
+    __syncthreads();
+    for (int k = 0; k < K; k++) {
+        if (CHAN) {
+            for (int blk = wave; blk < pm.nblk; blk += nw) {
+                const BlockMeta bm = d.blocks[pm.block0 + blk];
+                const int i = blk * W + lane;
+                const double Vm = LV[i];
+                const int pi = LP[i];
+                const bool pok = pi >= 0;
+                const double Vp = pok ? LV[pi] : 0.0;
+                CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, lane < bm.lanes, step0 + k);
+                LB[i] = o.rhs;
+                LD[i] = o.diag;
+                if (i == pm.last_lp) {
+                    own_last = true;
+                    droot0 = o.diag;
+                }
+            }
+            __syncthreads();
+        }
+        for (int l = 0; l < pm.nfl; l++) {
+            const int hi = lvl[l + 1];
+            for (int e = lvl[l] + tid; e < hi; e += T) {
+                const int f0 = FE[2 * e];
+                const int kb = FE[2 * e + 1];
+                const int p = f0 & 0xffff;
+                const int ke = kb + (f0 >> 16);
+                double dp = LD[p], bp = LB[p];
+                for (int j = kb; j < ke; j++) {
+                    const int c = KI[j];
+                    const double tt = KTA[j] / LD[c];
+                    dp -= KTF[j] * tt;
+                    bp -= LB[c] * tt;
+                }
+                LD[p] = dp;
+                LB[p] = bp;
+            }
+            __syncthreads();
+        }
+        const int* blv = lvl + pm.nfl + 1;
+        for (int l = 0; l < pm.nbl; l++) {
+            const int hi = blv[l + 1];
+            for (int e = blv[l] + tid; e < hi; e += T) {
+                const int kp = BKP[e];
+                const int c = kp & 0xffff;
+                const int p = (kp >> 16) - 1;
+                double bv = LB[c];
+                if (p >= 0) bv -= BTC[e] * LB[p];
+                const double x = bv / LD[c];
+                // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
+                if (!d.single) LB[c] = x;
+                if (d.crank) {
+                    const double v = LV[c];
+                    LV[c] = x + x - v;
+                } else {
+                    LV[c] = x;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (own_last) LD[pm.last_lp] = droot0;
+    __syncthreads();
+    for (int i = tid; i < NP; i += T) {
+        d.V[base + i] = LV[i];
+        const long long r = (long long)(pm.block0 + (i >> 6)) * 2 * W + (i & (W - 1));
+        d.R[r] = LB[i];
+        d.R[r + W] = LD[i];
+    }
+    if (tid == 0) d.pstep[pk] = step0 + K;
 }
 
 // Isopotential cells (every tree one compartment): one lane per cell, the

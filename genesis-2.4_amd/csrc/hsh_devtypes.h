@@ -24,14 +24,17 @@ enum UOp : int {
 };
 constexpr int PROG_HEADER = 2;
 
-// Per-block metadata: a block is one row of one 64-neuron tile.
+// Per-block metadata.  A block is 64 compartments that run the same channel
+// program: one row of a 64-neuron tile (lane = neuron), or a run of
+// same-program compartments of a pack (lane = compartment).
 struct BlockMeta {
     int prog;     // offset of the block's program in the program array
     int dslot;    // offset (doubles) of the block's slot area: nslots*64
     int islot;    // offset (ints) of the block's int slots: nislots*64
-    int tile;
+    int tile;     // step-counter index: tile t, or ntiles + pack p
     int tmpl;     // template (program) id
-    int pad0, pad1, pad2;
+    int lanes;    // lanes holding a compartment (the rest pad)
+    int pad1, pad2;
 };
 
 struct TileMeta {
@@ -39,6 +42,24 @@ struct TileMeta {
     int rows;     // rows of every neuron in the tile
     int lanes;    // active lanes (neurons)
     int pad;
+};
+
+// A pack: whole neurons of any sizes whose compartments fill blocks
+// block0..block0+nblk-1, grouped by channel program.  Its Hines elimination
+// runs level by level over pack-local positions lp = (block-block0)*64+lane:
+//   forward  -- levels of height 1..H: each entry is a parent that pulls its
+//               children's contributions in ascending row order;
+//   backward -- levels of depth 0..D: roots solve, then children from their
+//               parents.
+// lvl[lv0 .. lv0+nfl] are the forward level bounds (entry indices relative to
+// fe0), lvl[lv0+nfl+1 .. lv0+nfl+1+nbl] the backward bounds (relative to be0).
+struct PackMeta {
+    int block0, nblk;
+    int lv0, nfl, nbl;
+    int fe0, nfe;     // forward entries: fent[2e] = parent lp | nkids << 16, fent[2e+1] = first kid
+    int ke0, nke;     // kid entries: kidx (lp), kta, ktcf (kid index relative to ke0)
+    int be0, nbe;     // backward entries: bkp = lp | (parent lp + 1) << 16, btcb
+    int last_lp;      // lp of the reference's last row (ncompts-1), -1 if elsewhere
 };
 
 // Device-side argument bundle (passed by value to every kernel).
@@ -59,7 +80,18 @@ struct DevModel {
     const double* xvals;  // xdivs+2
     int* spk;             // spike ring: [cap][2] (ops index, step)
     int* nspk;            // spike count
-    int* step;            // per-tile step counters
+    int* step;            // step counters: tiles, then packs
+    // packs (lane = compartment layout)
+    const PackMeta* packs;
+    int* pstep;           // step counters of the packs (step + ntiles)
+    const int* lvl;
+    const int* fent;
+    const unsigned short* kidx;
+    const double* kta;
+    const double* ktcf;
+    const int* bkp;
+    const double* btcb;
+    int npacks;
     int spk_cap;
     int ncols;
     int xdivs;

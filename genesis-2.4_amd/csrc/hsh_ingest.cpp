@@ -12,8 +12,10 @@
 //    by hines_init.c:604-742, :1011-1172) into a parent-row array and the
 //    three coefficients of each non-root row, and checks that every tree is a
 //    contiguous row block whose kids are eliminated in ascending row order.
-//  * tiler: groups neurons with the same size and the same per-row templates
-//    into 64-lane tiles, so every wave executes a uniform program.
+//  * layout (build_layout): groups neurons with the same size and the same
+//    per-row templates into 64-lane tiles (lane = neuron), and packs larger
+//    trees into workgroup-sized packs (lane = compartment) with a level
+//    schedule for their elimination; every wave executes a uniform program.
 #include <algorithm>
 #include <cstring>
 #include <map>
@@ -346,13 +348,80 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     }
     if (!P.refusals.empty()) return true;
 
-    // ------------------------------------------------------------------
-    // tiler
+    // program array: templates concatenated; program = [nslots, template id,
+    // micro-ops...] (the slot count lets a wave fetch the compartment's whole
+    // slot vector before interpreting it)
+    P.tprog.assign(P.templates.size(), 0);
+    for (size_t t = 0; t < P.templates.size(); t++) {
+        P.tprog[t] = (int32_t)P.prog.size();
+        P.prog.push_back(P.templates[t].nslots);
+        P.prog.push_back((int32_t)t);
+        P.prog.insert(P.prog.end(), P.templates[t].code.begin(), P.templates[t].code.end());
+    }
+    P.max_tree_rows = 0;
+    for (size_t t = 0; t < P.tree_lo.size(); t++)
+        P.max_tree_rows = std::max(P.max_tree_rows, P.tree_hi[t] - P.tree_lo[t] + 1);
+
+    // algorithmic bytes of one step (fp64, single pass): Vm r+w, gate states
+    // r+w, read-only slots, tree coefficients + parent for non-root rows.
+    double bytes = 0;
+    for (int r = 0; r < n; r++) {
+        const Template& t = P.templates[P.comp_tmpl[r]];
+        bytes += 16.0 + 16.0 * t.ngates + 8.0 * (t.nslots - t.ngates) + 8.0 * t.nislots;
+        if (P.parent[r] >= 0) bytes += 28.0;
+    }
+    P.bytes_per_step = bytes;
+    return build_layout(P, LayoutOpts{}, err);
+}
+
+// ----------------------------------------------------------------------
+// Layout.  Trees of up to o.tile_max_rows rows (and trees too large for a
+// pack) go to 64-neuron tiles, lane = neuron: neurons with the same size and
+// the same per-row programs share a tile, so block r of a tile is row r of 64
+// neurons.  Larger trees go to packs of about o.pack_cap compartments, lane =
+// compartment: the pack's compartments are grouped by program into blocks,
+// and the pack's elimination runs from a level schedule.
+bool build_layout(Plan& P, const LayoutOpts& o, std::string& err)
+{
+    const int n = (int)P.comp_tmpl.size();
+    P.layout = o;
+    P.blocks.clear();
+    P.tiles.clear();
+    P.packs.clear();
+    P.lvl.clear();
+    P.fent.clear();
+    P.kidx.clear();
+    P.kta.clear();
+    P.ktcf.clear();
+    P.bkp.clear();
+    P.btcb.clear();
+    P.max_rows = 0;
+    P.max_pack_pos = 0;
+    P.pack_lds = 0;
+    P.pack_lds_stage = 0;
+    P.comp_pos.assign(n, -1);
     const int ntrees = (int)P.tree_lo.size();
+    auto rows_of = [&](int t) { return P.tree_hi[t] - P.tree_lo[t] + 1; };
+    // a pack's level schedule keeps positions in 15 bits, and its V/rhs/diag
+    // and predecessor index (28 bytes a position) must fit the LDS
+    constexpr int PACK_MAX_POS = 4096;
+    auto packable = [&](int t) {
+        const int R = rows_of(t);
+        if (R <= o.tile_max_rows) return false;
+        // worst case: every row of the tree alone in its program's block
+        const int ntmpl = (int)P.templates.size();
+        return (int64_t)R + (int64_t)ntmpl * (W - 1) <= PACK_MAX_POS;
+    };
+
     std::map<std::vector<int32_t>, std::vector<int32_t>> groups;   // signature -> trees
+    std::vector<int32_t> big;
     for (int t = 0; t < ntrees; t++) {
+        if (packable(t)) {
+            big.push_back(t);
+            continue;
+        }
         std::vector<int32_t> sig;
-        sig.push_back(P.tree_hi[t] - P.tree_lo[t] + 1);
+        sig.push_back(rows_of(t));
         for (int r = P.tree_lo[t]; r <= P.tree_hi[t]; r++) sig.push_back(P.comp_tmpl[r]);
         groups[sig].push_back(t);
     }
@@ -361,18 +430,20 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     for (auto& g : groups) order.push_back(&g);
     std::stable_sort(order.begin(), order.end(), [](auto* a, auto* b) { return a->first[0] > b->first[0]; });
 
-    // program array: templates concatenated
-    std::vector<int32_t> tprog(P.templates.size());
-    for (size_t t = 0; t < P.templates.size(); t++) {
-        // program = [nslots, template id, micro-ops...]; the slot count lets a
-        // wave fetch the compartment's whole slot vector before interpreting it
-        tprog[t] = (int32_t)P.prog.size();
-        P.prog.push_back(P.templates[t].nslots);
-        P.prog.push_back((int32_t)t);
-        P.prog.insert(P.prog.end(), P.templates[t].code.begin(), P.templates[t].code.end());
-    }
-    P.comp_pos.assign(n, -1);
     int64_t nS = 0, nI = 0;
+    auto add_block = [&](int ti, int tile, int lanes) {
+        BlockMeta bm{};
+        bm.prog = P.tprog[ti];
+        bm.dslot = (int32_t)nS;
+        bm.islot = (int32_t)nI;
+        bm.tile = tile;
+        bm.tmpl = ti;
+        bm.lanes = lanes;
+        P.blocks.push_back(bm);
+        nS += (int64_t)P.templates[ti].nslots * W;
+        nI += (int64_t)P.templates[ti].nislots * W;
+        return (int)P.blocks.size() - 1;
+    };
     for (auto* g : order) {
         const std::vector<int32_t>& sig = g->first;
         const std::vector<int32_t>& trees = g->second;
@@ -387,23 +458,134 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
             int tile = (int)P.tiles.size();
             P.tiles.push_back(tm);
             for (int r = 0; r < R; r++) {
-                int ti = sig[1 + r];
-                BlockMeta bm{};
-                bm.prog = tprog[ti];
-                bm.dslot = (int32_t)nS;
-                bm.islot = (int32_t)nI;
-                bm.tile = tile;
-                bm.tmpl = ti;
-                int b = (int)P.blocks.size();
-                P.blocks.push_back(bm);
-                nS += (int64_t)P.templates[ti].nslots * W;
-                nI += (int64_t)P.templates[ti].nislots * W;
+                const int b = add_block(sig[1 + r], tile, tm.lanes);
                 for (int l = 0; l < tm.lanes; l++) {
                     int row = P.tree_lo[trees[t0 + l]] + r;
                     P.comp_pos[row] = b * W + l;
                 }
             }
         }
+    }
+
+    // packs: largest trees first, consecutive trees of one size together
+    std::stable_sort(big.begin(), big.end(), [&](int a, int b) { return rows_of(a) > rows_of(b); });
+    // height (leaves 0), depth (roots 0) and children (ascending row order:
+    // the reference eliminates kids in that order) of every row
+    std::vector<int32_t> height(n, 0), depth(n, 0), kid_off(n + 1, 0), kid_list;
+    for (int r = 0; r < n; r++)
+        if (P.parent[r] >= 0) {
+            height[P.parent[r]] = std::max(height[P.parent[r]], height[r] + 1);   // kids precede parents
+            kid_off[P.parent[r] + 1]++;
+        }
+    for (int r = n - 1; r >= 0; r--) depth[r] = P.parent[r] >= 0 ? depth[P.parent[r]] + 1 : 0;
+    for (int r = 0; r < n; r++) kid_off[r + 1] += kid_off[r];
+    kid_list.assign(kid_off[n], 0);
+    {
+        std::vector<int32_t> cur(kid_off.begin(), kid_off.end() - 1);
+        for (int r = 0; r < n; r++)
+            if (P.parent[r] >= 0) kid_list[cur[P.parent[r]]++] = r;
+    }
+    const int ntiles = (int)P.tiles.size();
+    const int cap = std::max(1, o.pack_cap);
+    for (size_t i0 = 0; i0 < big.size();) {
+        // trees of this pack
+        size_t i1 = i0;
+        int tot = 0;
+        const int ntmpl = (int)P.templates.size();
+        while (i1 < big.size()) {
+            const int R = rows_of(big[i1]);
+            if (i1 > i0 && (tot + R > cap || (int64_t)tot + R + (int64_t)ntmpl * (W - 1) > PACK_MAX_POS)) break;
+            tot += R;
+            i1++;
+        }
+        PackMeta pm{};
+        pm.block0 = (int32_t)P.blocks.size();
+        const int pk = (int)P.packs.size();
+        // blocks by program: compartments in tree order, rows ascending
+        std::vector<std::vector<int32_t>> by_t(ntmpl);
+        for (size_t i = i0; i < i1; i++)
+            for (int r = P.tree_lo[big[i]]; r <= P.tree_hi[big[i]]; r++) by_t[P.comp_tmpl[r]].push_back(r);
+        for (int ti = 0; ti < ntmpl; ti++) {
+            const auto& rows = by_t[ti];
+            for (size_t c0 = 0; c0 < rows.size(); c0 += W) {
+                const int lanes = (int)std::min<size_t>(W, rows.size() - c0);
+                const int b = add_block(ti, ntiles + pk, lanes);
+                for (int l = 0; l < lanes; l++) P.comp_pos[rows[c0 + l]] = b * W + l;
+            }
+        }
+        pm.nblk = (int32_t)P.blocks.size() - pm.block0;
+        const int base = pm.block0 * W;
+        const int NP = pm.nblk * W;
+        if (NP > PACK_MAX_POS) { err = "internal: pack exceeds its position limit"; return false; }
+        P.max_pack_pos = std::max(P.max_pack_pos, NP);
+        // level schedule
+        int H = 0, D = 0;
+        for (size_t i = i0; i < i1; i++)
+            for (int r = P.tree_lo[big[i]]; r <= P.tree_hi[big[i]]; r++) {
+                H = std::max(H, height[r]);
+                D = std::max(D, depth[r]);
+            }
+        auto lp = [&](int32_t r) { return P.comp_pos[r] - base; };
+        std::vector<std::vector<int32_t>> flev(H + 1), blev(D + 1);
+        for (size_t i = i0; i < i1; i++)
+            for (int r = P.tree_lo[big[i]]; r <= P.tree_hi[big[i]]; r++) {
+                if (height[r] > 0) flev[height[r]].push_back(r);
+                blev[depth[r]].push_back(r);
+            }
+        auto by_lp = [&](int32_t a, int32_t b) { return lp(a) < lp(b); };
+        pm.lv0 = (int32_t)P.lvl.size();
+        pm.nfl = H;
+        pm.nbl = D + 1;
+        pm.fe0 = (int32_t)(P.fent.size() / 2);
+        pm.ke0 = (int32_t)P.kidx.size();
+        pm.be0 = (int32_t)P.bkp.size();
+        int fe = 0, ke = 0, be = 0;
+        for (int h = 1; h <= H; h++) {
+            P.lvl.push_back(fe);
+            std::sort(flev[h].begin(), flev[h].end(), by_lp);
+            for (int32_t pr : flev[h]) {
+                const int32_t nk = kid_off[pr + 1] - kid_off[pr];
+                P.fent.push_back(lp(pr) | (nk << 16));
+                P.fent.push_back(ke);
+                for (int32_t q = kid_off[pr]; q < kid_off[pr + 1]; q++) {
+                    const int32_t c = kid_list[q];
+                    P.kidx.push_back((uint16_t)lp(c));
+                    P.kta.push_back(P.ta[c]);
+                    P.ktcf.push_back(P.tcf[c]);
+                    ke++;
+                }
+                fe++;
+            }
+        }
+        P.lvl.push_back(fe);
+        for (int dd = 0; dd <= D; dd++) {
+            P.lvl.push_back(be);
+            std::sort(blev[dd].begin(), blev[dd].end(), by_lp);
+            for (int32_t r : blev[dd]) {
+                const int32_t pr = P.parent[r];
+                P.bkp.push_back(lp(r) | ((pr >= 0 ? lp(pr) + 1 : 0) << 16));
+                P.btcb.push_back(pr >= 0 ? P.tcb[r] : 0.0);
+                be++;
+            }
+        }
+        P.lvl.push_back(be);
+        pm.nfe = fe;
+        pm.nke = ke;
+        pm.nbe = be;
+        pm.last_lp = -1;
+        P.packs.push_back(pm);
+        // LDS of the pack kernel: V, rhs, diag (+ predecessor index), and the
+        // staged schedule
+        const int64_t lds = (int64_t)NP * (3 * 8 + 4);
+        const int64_t stage = (int64_t)ke * (8 + 8 + 2) + (int64_t)be * (8 + 4) + (int64_t)fe * 8 +
+                              (int64_t)(H + D + 3) * 4 + 16;
+        P.pack_lds = std::max<int64_t>(P.pack_lds, lds);
+        P.pack_lds_stage = std::max<int64_t>(P.pack_lds_stage, lds + stage);
+        i0 = i1;
+    }
+    for (auto& pm : P.packs) {
+        const int last = P.comp_pos[n - 1];
+        if (last >= pm.block0 * W && last < (pm.block0 + pm.nblk) * W) pm.last_lp = last - pm.block0 * W;
     }
     if (nS > INT32_MAX || nI > INT32_MAX || (int64_t)P.blocks.size() * W > INT32_MAX) {
         err = "model too large for 32-bit slot offsets";
@@ -417,16 +599,6 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     for (size_t b = 0; b < P.blocks.size(); b++)
         for (int l = 1; l < W; l++)
             if (P.pos_comp[b * W + l] < 0) P.pos_comp[b * W + l] = P.pos_comp[b * W];
-
-    // algorithmic bytes of one step (fp64, single pass): Vm r+w, gate states
-    // r+w, read-only slots, tree coefficients + parent for non-root rows.
-    double bytes = 0;
-    for (int r = 0; r < n; r++) {
-        const Template& t = P.templates[P.comp_tmpl[r]];
-        bytes += 16.0 + 16.0 * t.ngates + 8.0 * (t.nslots - t.ngates) + 8.0 * t.nislots;
-        if (P.parent[r] >= 0) bytes += 28.0;
-    }
-    P.bytes_per_step = bytes;
     return true;
 }
 

@@ -35,6 +35,12 @@ struct Template {
     int ngates = 0;
 };
 
+// Layout choices (the autotuner sweeps them).
+struct LayoutOpts {
+    int tile_max_rows = 16;   // larger trees go to packs (0: every tree)
+    int pack_cap = 1024;      // compartments per pack (a larger tree fills one alone)
+};
+
 // Host-side result of analysing a model.
 struct Plan {
     std::vector<hsh_refusal> refusals;
@@ -54,27 +60,38 @@ struct Plan {
     std::vector<int32_t> tree_lo;         // per tree (neuron): first row; root = hi
     std::vector<int32_t> tree_hi;
     std::vector<int32_t> pred;            // previous NEWVOLT compartment (global row), -1
-    // layout
+    std::vector<int32_t> prog;            // programs of all templates
+    std::vector<int32_t> tprog;           // template -> offset in prog
+    int max_tree_rows = 0;
+    double bytes_per_step = 0;
+    // layout (build_layout)
+    LayoutOpts layout;
     std::vector<BlockMeta> blocks;
     std::vector<TileMeta> tiles;
-    std::vector<int32_t> prog;
+    std::vector<PackMeta> packs;
+    std::vector<int32_t> lvl, fent, bkp;
+    std::vector<uint16_t> kidx;
+    std::vector<double> kta, ktcf, btcb;
     std::vector<int32_t> comp_pos;        // row -> block*64+lane
     std::vector<int32_t> pos_comp;        // block*64+lane -> row (padding lanes: lane-0 row)
     int64_t nS = 0, nI = 0;
-    int max_rows = 0;
-    double bytes_per_step = 0;
+    int max_rows = 0;                     // rows of the largest tiled tree
+    int max_pack_pos = 0;                 // positions of the largest pack
+    int64_t pack_lds = 0, pack_lds_stage = 0;   // LDS bytes of the pack kernels
 };
 
 // Analyse ops[] / funcs[] and build the device layout.  Returns false on a
 // malformed model (message in err); refusals go into plan.refusals.
 bool analyse(const hsh_model& m, Plan& plan, std::string& err);
+// (Re)build the device layout of an analysed model.
+bool build_layout(Plan& plan, const LayoutOpts& o, std::string& err);
 
 // Kernels specialised to a model's channel programs and compiled at run time
 // (hsh_jit.cpp).  ok == false: the built-in interpreter kernels are used.
 struct JitModule {
     hipModule_t mod = nullptr;
     hipFunction_t channel = nullptr, batched_r[3] = {nullptr, nullptr, nullptr}, batched = nullptr,
-                  iso = nullptr;
+                  iso = nullptr, forest[2] = {nullptr, nullptr};
     bool ok = false;
     double compile_s = 0.0;
 };
@@ -89,6 +106,12 @@ hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s);
 hipError_t launch_batched(const DevModel& d, const JitModule* jm, int ksteps, int waves, int max_rows,
                           hipStream_t s);
 hipError_t launch_iso(const DevModel& d, const JitModule* jm, int ksteps, hipStream_t s);
+// packs: batched (K steps, channel programs + elimination), waves per pack,
+// stage = schedule copied to LDS; lds = bytes of dynamic LDS
+hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, bool stage, int lds,
+                         hipStream_t s);
+// packs, plain drop-in: elimination only (rhs/diag from results)
+hipError_t launch_forest_elim(const DevModel& d, int lds, hipStream_t s);
 hipError_t launch_step_advance(const DevModel& d, int32_t by, hipStream_t s);
 int batched_lds_bytes(int max_rows);
 

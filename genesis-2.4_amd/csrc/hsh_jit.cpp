@@ -132,6 +132,10 @@ std::string jit_source(const Plan& P, int ncols)
            "{ extern __shared__ double lds[]; hsh::body_batched<hsh::JitRun>(d, K, lds); }\n";
     src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_iso(hsh::DevModel d, int K)\n"
            "{ hsh::body_iso<hsh::JitRun>(d, K); }\n";
+    for (int st : {0, 1})
+        src += "extern \"C\" __global__ __launch_bounds__(512) void hsh_jit_forest" + std::to_string(st) +
+               "(hsh::DevModel d, int K)\n{ extern __shared__ double lds[]; hsh::body_forest<hsh::JitRun, true, " +
+               (st ? "true" : "false") + ">(d, K, lds); }\n";
     return src;
 }
 
@@ -186,10 +190,11 @@ bool jit_build(const Plan& P, int ncols, JitModule& jm, std::string& err)
         return false;
     }
     const char* names[] = {"hsh_jit_channel", "hsh_jit_batched_r1", "hsh_jit_batched_r2",
-                           "hsh_jit_batched_r4", "hsh_jit_batched", "hsh_jit_iso"};
+                           "hsh_jit_batched_r4", "hsh_jit_batched", "hsh_jit_iso",
+                           "hsh_jit_forest0", "hsh_jit_forest1"};
     hipFunction_t* slots[] = {&jm.channel, &jm.batched_r[0], &jm.batched_r[1], &jm.batched_r[2],
-                              &jm.batched, &jm.iso};
-    for (int i = 0; i < 6; i++) {
+                              &jm.batched, &jm.iso, &jm.forest[0], &jm.forest[1]};
+    for (int i = 0; i < 8; i++) {
         if (hipModuleGetFunction(slots[i], jm.mod, names[i]) != hipSuccess) {
             err = std::string("hipModuleGetFunction failed: ") + names[i];
             jit_free(jm);

@@ -2,9 +2,11 @@
 //
 // Data layout (HBM): every array is [block][slot][lane] with 64 lanes per
 // block, a block being one row of a tile of 64 neurons that share size and
-// per-row channel programs.  A wave owns a block (channel kernels) or a tile
-// (elimination), so all loads of one slot by one wave are one contiguous
-// 512-byte line and the program a wave interprets is wave-uniform.
+// per-row channel programs (lane = neuron), or 64 same-program compartments
+// of a pack of larger trees (lane = compartment).  A wave owns a block
+// (channel programs) or a tile (elimination), so all loads of one slot by one
+// wave are one contiguous 512-byte line and the program a wave interprets is
+// wave-uniform.
 //
 // The channel kernels here run the generic micro-op interpreter; when the
 // run-time specialised module of hsh_jit.cpp is available the launchers use
@@ -40,6 +42,19 @@ template <bool REG>
 __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
 {
     body_iso<InterpRun<REG>>(d, K);
+}
+
+template <bool REG, bool STAGE>
+__global__ __launch_bounds__(512) void k_forest(DevModel d, int K)
+{
+    extern __shared__ double lds[];
+    body_forest<InterpRun<REG>, true, STAGE>(d, K, lds);
+}
+
+__global__ __launch_bounds__(256) void k_forest_elim(DevModel d)
+{
+    extern __shared__ double lds[];
+    body_forest<InterpRun<false>, false, false>(d, 1, lds);
 }
 
 // Plain drop-in, part (b): one lane per neuron, one wave per tile.  Forward
@@ -78,7 +93,8 @@ __global__ __launch_bounds__(256) void k_elim(DevModel d)
         const int p = d.par[q];
         if (p >= 0) bv -= d.tcb[q] * Rr[(size_t)p * 2 * W];
         const double x = bv / Rr[(size_t)r * 2 * W + W];
-        Rr[(size_t)r * 2 * W] = x;
+        // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
+        if (!d.single) Rr[(size_t)r * 2 * W] = x;
         if (d.crank) {
             const double v = d.V[q];
             d.V[q] = x + x - v;
@@ -141,7 +157,7 @@ __global__ __launch_bounds__(64) void k_elim_r(DevModel d)
             const int p = par[r];
             if (p >= 0) bv -= tcb[r] * sB[p * W + lane];
             const double x = bv / sD[r * W + lane];
-            sB[r * W + lane] = x;
+            if (!d.single) sB[r * W + lane] = x;
             if (d.crank) {
                 const double v = d.V[q];
                 d.V[q] = x + x - v;
@@ -243,9 +259,35 @@ hipError_t launch_iso(const DevModel& d, const JitModule* jm, int ksteps, hipStr
     return hipGetLastError();
 }
 
+hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, bool stage, int lds,
+                         hipStream_t s)
+{
+    if (d.npacks <= 0) return hipSuccess;
+    waves = waves < 1 ? 1 : (waves > 8 ? 8 : waves);
+    const unsigned bx = (unsigned)(waves * W);
+    if (jm && jm->ok) return module_launch(jm->forest[stage ? 1 : 0], (unsigned)d.npacks, bx, (unsigned)lds, s, d, &ksteps);
+    const bool reg = d.max_slots <= MAXS;
+    if (stage) {
+        if (reg) hipLaunchKernelGGL((k_forest<true, true>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
+        else hipLaunchKernelGGL((k_forest<false, true>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
+    } else {
+        if (reg) hipLaunchKernelGGL((k_forest<true, false>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
+        else hipLaunchKernelGGL((k_forest<false, false>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_forest_elim(const DevModel& d, int lds, hipStream_t s)
+{
+    if (d.npacks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_forest_elim, dim3(d.npacks), dim3(256), lds, s, d);
+    return hipGetLastError();
+}
+
 hipError_t launch_step_advance(const DevModel& d, int32_t by, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_advance, dim3((d.ntiles + 255) / 256), dim3(256), 0, s, d.step, d.ntiles, by);
+    const int n = d.ntiles + d.npacks;
+    hipLaunchKernelGGL(k_step_advance, dim3((n + 255) / 256), dim3(256), 0, s, d.step, n, by);
     return hipGetLastError();
 }
 

@@ -45,7 +45,8 @@ class HshModel(C.Structure):
 
 class HshOptions(C.Structure):
     _fields_ = [("device", C.c_int32), ("batch_k", C.c_int32), ("batch_waves", C.c_int32),
-                ("flags", C.c_int32)]
+                ("flags", C.c_int32), ("tile_max_rows", C.c_int32), ("pack_cap", C.c_int32),
+                ("pack_waves", C.c_int32), ("pack_stage", C.c_int32)]
 
 
 class HshRefusal(C.Structure):
@@ -63,12 +64,15 @@ class HshInfo(C.Structure):
                 ("ntrees", C.c_int32), ("max_rows", C.c_int32), ("batch_k", C.c_int32),
                 ("batch_waves", C.c_int32), ("refused", C.c_int32),
                 ("bytes_per_step", C.c_double), ("lane_fill", C.c_double),
-                ("jit", C.c_int32), ("pad", C.c_int32), ("jit_compile_s", C.c_double)]
+                ("jit", C.c_int32), ("npacks", C.c_int32), ("jit_compile_s", C.c_double),
+                ("tile_max_rows", C.c_int32), ("pack_cap", C.c_int32), ("pack_waves", C.c_int32),
+                ("pack_stage", C.c_int32), ("max_tree_rows", C.c_int32), ("autotuned", C.c_int32),
+                ("autotune_s", C.c_double)]
 
 
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
-           "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get",
+           "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
            "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_destroy", "hsh_last_error")
 
 _lib = None
@@ -97,6 +101,8 @@ def lib():
         L.hsh_drain_spikes.restype = C.c_int32
         L.hsh_info_get.argtypes = [C.c_void_p, C.POINTER(HshInfo)]
         L.hsh_info_get.restype = C.c_int32
+        L.hsh_autotune.argtypes = [C.c_void_p]
+        L.hsh_autotune.restype = C.c_int32
         L.hsh_last_time.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         L.hsh_last_time.restype = C.c_int32
         L.hsh_set_timing.argtypes = [C.c_void_p, C.c_int32]
@@ -171,13 +177,18 @@ class HipSolver:
     """One hsolve on the GPU.  ``model`` is a :class:`hsolve_hip.model.Model`."""
 
     def __init__(self, model, *, device: int = 0, batch_k: int = 0, batch_waves: int = 0,
-                 flags: int = 0, host_only: bool = False):
+                 flags: int = 0, host_only: bool = False, tile_max_rows: int = 0, pack_cap: int = 0,
+                 pack_waves: int = 0, pack_stage: int = 0):
+        """Options as in hsh_options: 0 = autotuned (or the default under
+        F_NO_AUTOTUNE); ``tile_max_rows < 0`` puts every tree in packs;
+        ``pack_stage`` 1 = schedule read from memory, 2 = staged in LDS."""
         L = lib()
         self.m = model
         mm, self._keep = _hsh_model(model)
         if host_only:
             flags |= F_HOST_ONLY
-        opt = HshOptions(device, batch_k, batch_waves, flags)
+        opt = HshOptions(device, batch_k, batch_waves, flags, tile_max_rows, pack_cap, pack_waves,
+                         pack_stage)
         h = C.c_void_p()
         rc = L.hsh_create(C.byref(mm), C.byref(opt), C.byref(h))
         self.h = h
@@ -220,6 +231,10 @@ class HipSolver:
         i = HshInfo()
         _check(lib().hsh_info_get(self.h, C.byref(i)), "hsh_info_get")
         return {f: getattr(i, f) for f, _ in HshInfo._fields_}
+
+    def autotune(self):
+        _check(lib().hsh_autotune(self.h), "hsh_autotune")
+        return self.info()
 
     # -- stepping
     def step(self, n: int = 1, mode: int = PLAIN) -> dict:

@@ -22,6 +22,8 @@
  *   hsh_put_state    cuda_backend_upload_chip (cuda_backend.cu:318-336)
  *   hsh_set_chip     per-step chip writes of h_in_msgs (hines_msgs.c:351-405)
  *   hsh_destroy      cuda_backend_cleanup (cuda_backend.cu:540-568)
+ *   hsh_autotune     no counterpart: the reference multiloop launches one fixed
+ *                    shape (ceil(n_trees/64) x 64, cuda_backend.cu:461-527)
  *
  * Conventions: statuses are hsh_status codes (negative = error; HSH_REFUSED is
  * not an error: the caller falls back to its CPU path for this solver).  No
@@ -85,11 +87,21 @@ typedef struct hsh_model {
 typedef struct hsh_options {
     int32_t device;          /* HIP device ordinal */
     int32_t batch_k;         /* steps per batched launch; 0 = autotuned */
-    int32_t batch_waves;     /* waves per batched workgroup; 0 = autotuned */
+    int32_t batch_waves;     /* waves per workgroup of the tile kernel for trees of
+                                17..64 rows; 0 = rows/4 */
     int32_t flags;           /* HSH_F_* */
+    /* Layout.  Trees of up to tile_max_rows rows run in 64-neuron tiles (one
+     * lane per neuron); larger trees are packed, whole, into workgroup-sized
+     * packs (one lane per compartment, level-scheduled elimination). */
+    int32_t tile_max_rows;   /* 0 = autotuned; < 0 = every tree in packs */
+    int32_t pack_cap;        /* compartments per pack; 0 = autotuned */
+    int32_t pack_waves;      /* waves per pack workgroup (1..8); 0 = autotuned */
+    int32_t pack_stage;      /* pack schedule in LDS: 0 = autotuned, 1 = no, 2 = yes */
 } hsh_options;
 
-#define HSH_F_NO_AUTOTUNE 1  /* use batch_k / batch_waves (or defaults) as given */
+#define HSH_F_NO_AUTOTUNE 1  /* no autotuning: options as given, defaults for the rest
+                                (K 32, tile_max_rows 16, pack_cap 1024, 4 pack waves,
+                                schedule not staged) */
 #define HSH_F_HOST_ONLY   2  /* analyse and plan only, allocate nothing on a device:
                                 hsh_refusals and hsh_info_get work, stepping and
                                 state calls return HSH_ERR_STATE */
@@ -135,15 +147,22 @@ typedef struct hsh_info {
     int32_t nblocks;         /* (tile,row) blocks */
     int32_t ntemplates;      /* distinct channel programs */
     int32_t ntrees;          /* neurons */
-    int32_t max_rows;        /* largest neuron */
+    int32_t max_rows;        /* largest neuron in a tile */
     int32_t batch_k;         /* chosen K */
     int32_t batch_waves;     /* chosen waves per workgroup */
     int32_t refused;         /* 1 if the handle is refused */
     double  bytes_per_step;  /* algorithmic HBM bytes of one plain step */
     double  lane_fill;       /* real compartments / (blocks*64) */
     int32_t jit;             /* 1: specialised kernels in use */
-    int32_t pad;
+    int32_t npacks;          /* packs of larger trees */
     double  jit_compile_s;   /* time spent generating + compiling them */
+    int32_t tile_max_rows;   /* layout in use (see hsh_options) */
+    int32_t pack_cap;
+    int32_t pack_waves;
+    int32_t pack_stage;      /* 1 = no, 2 = yes */
+    int32_t max_tree_rows;   /* largest neuron */
+    int32_t autotuned;       /* 1 once the autotuner has run */
+    double  autotune_s;      /* time it took */
 } hsh_info;
 
 hsh_status hsh_create(const hsh_model *m, const hsh_options *opt, struct hsh_handle **out);
@@ -163,6 +182,11 @@ hsh_status hsh_set_chip(struct hsh_handle *h, int32_t n, const int32_t *idx, con
 hsh_status hsh_drain_spikes(struct hsh_handle *h, int32_t *opidx, int32_t *step, int32_t cap,
                             int32_t *n);
 hsh_status hsh_info_get(struct hsh_handle *h, hsh_info *info);
+/* Run the autotuner now (otherwise it runs before the first HSH_BATCHED step
+ * unless HSH_F_NO_AUTOTUNE): times candidate layouts, pack kernel shapes and
+ * K on the handle's own model and keeps the fastest.  The simulation state
+ * (vm, chip, ops, results, step count, pending spikes) is unchanged. */
+hsh_status hsh_autotune(struct hsh_handle *h);
 /* Device time of the last hsh_step call (HIP events on the handle's stream),
  * in ms, after hsh_sync.  kernel selects HSH_T_*. */
 #define HSH_T_TOTAL    0

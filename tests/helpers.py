@@ -50,5 +50,14 @@ def mixed_model(n_neurons: int, *, seed: int = 3) -> M.Model:
     return M.population(n_neurons, (1, 8, 16, 64), (0.4, 0.3, 0.2, 0.1), seed=seed)
 
 
+def c3_model(n_neurons: int, *, seed: int = 4, sizes=(1, 8, 16, 64, 256, 1024), method: int = M.BEULER,
+             dt: float = 2e-5) -> M.Model:
+    """Tree sizes from {1, 8, 16, 64, 256, 1024} with P proportional to
+    size^-0.67 (mean about 17 compartments per neuron)."""
+    p = np.array(sizes, float) ** -0.67
+    p /= p.sum()
+    return M.population(n_neurons, tuple(sizes), tuple(p), seed=seed, method=method, dt=dt)
+
+
 def max_abs(a, b) -> float:
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) if np.size(a) else 0.0

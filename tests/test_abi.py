@@ -9,7 +9,7 @@ import re
 import numpy as np
 import pytest
 
-from helpers import ap_verify_model, mixed_model, random_tree_model
+from helpers import ap_verify_model, c3_model, mixed_model, random_tree_model
 from hsolve_hip import device as D
 from hsolve_hip import model as M
 
@@ -61,8 +61,34 @@ def test_plan_bytes_formula():
 def test_plan_mixed_sizes_groups_tiles():
     m = mixed_model(500)
     info = plan(m)
-    assert info["max_rows"] == 64
+    assert info["max_tree_rows"] == 64
     assert info["ntrees"] == 500
+    # trees above 16 rows go to packs by default
+    assert info["max_rows"] == 16 and info["npacks"] > 0
+    wide = plan(m, tile_max_rows=64)
+    assert wide["max_rows"] == 64 and wide["npacks"] == 0
+    allp = plan(m, tile_max_rows=-1)
+    assert allp["ntiles"] == 0 and allp["npacks"] > 0
+    assert allp["tile_max_rows"] == 0
+
+
+def test_plan_packs_fill_blocks():
+    """Packs group compartments by program: with two programs (soma,
+    dendrite) a pack of 1024-row trees wastes at most two partial blocks."""
+    m = c3_model(400)
+    for cap in (512, 1024, 2048):
+        info = plan(m, pack_cap=cap, tile_max_rows=16)
+        assert info["pack_cap"] == cap
+        assert info["lane_fill"] > 0.8
+        assert info["npacks"] >= 1
+    assert plan(m)["max_tree_rows"] == 1024
+
+
+def test_rejects_bad_options():
+    m = random_tree_model(3, 4)
+    for kw in (dict(pack_waves=9), dict(pack_stage=3), dict(pack_cap=-1)):
+        with pytest.raises(D.HshError):
+            D.HipSolver(m, host_only=True, **kw)
 
 
 def test_refuses_synchan_with_reason():
