@@ -1,0 +1,353 @@
+// hsh_adjoint.hip -- reverse-mode (adjoint) kernels of the hsolve step.
+//
+// One reverse step t (state V^t, slots S^t -> V^{t+1}) runs three kernels on
+// the tile layout (block = one row of 64 neurons, lane = neuron):
+//   k_adj_rhs    recomputes rhs/diag of step t from (V^t, S^t), as the
+//                channel programs do (hines_chip.c:157-600), without writing
+//                the slots;
+//   k_adj_elim   the reverse of the Hines elimination (hines_solve.c:
+//                246-295): it recomputes the eliminated diagonals/rhs and the
+//                solution, then runs the back-substitution in reverse (rows
+//                ascending, leaf to root) and the forward elimination in
+//                reverse (rows descending) -- the transposed Hines sweep --
+//                giving the adjoints of rhs, diag and of the three
+//                coefficients of every row;
+//   k_adj_local  the reverse of each compartment's channel program: from the
+//                adjoints of rhs and diag, the adjoint of V^t, of every gate
+//                state at t, and gradients of the program's constants (gmax,
+//                Ek, Em/Rm, inject, tbyc, diagterm), accumulated over steps.
+// Gate tables are piecewise linear in Vm: d(table)/dVm = (next row - row) *
+// invdx inside the table, 0 where Vm is clamped.
+#include "hsh_internal.h"
+#include "hsh_device.h"
+
+namespace hsh {
+
+namespace {
+
+constexpr int MAXG = 3;     // gates per channel the reverse program keeps
+
+struct Interp {
+    int filo;
+    double vipol;
+    bool inside;
+};
+
+__device__ __forceinline__ Interp table_row(const DevModel& d, double Vm)
+{
+    Interp r;
+    const double raw = (Vm - d.xmin) * d.invdx;
+    double v = raw;
+    if (v < 0.0) v = 0.0;
+    else if (v > d.xdivs) v = d.xdivs;
+    int f = (int)v;
+    f = f < 0 ? 0 : (f > d.xdivs ? d.xdivs : f);
+    r.filo = f;
+    r.vipol = v - (double)f;
+    r.inside = raw > 0.0 && raw < (double)d.xdivs;
+    return r;
+}
+
+// rhs and diag of one compartment (the forward program, no slot writes).
+__device__ CompOut adj_forward_local(const DevModel& d, const int* __restrict__ code, const double* __restrict__ Sb,
+                                     int lane, double Vm)
+{
+    double sumg = 0.0;
+    double ichan = Sb[0 * W + lane] + Sb[1 * W + lane];
+    double Gk = 0.0, Ek = 0.0;
+    Interp ip{0, 0.0, false};
+    const double* vtab = d.tab;
+    const int ncols = d.ncols;
+    int c = 2, pc = PROG_HEADER;
+    for (;;) {
+        const int op = code[pc++];
+        if (op == U_IPOL1V) {
+            const int col = code[pc], k = code[pc + 1];
+            pc += 2;
+            const double X = gate_update(vtab, col, ncols, ip.vipol, Sb[c * W + lane], k < 0);
+            c++;
+            Gk = apply_power(Gk, X, k < 0 ? -k : k);
+        } else if (op == U_CHAN_EK) {
+            Ek = Sb[c * W + lane];
+            Gk = Sb[(c + 1) * W + lane];
+            c += 2;
+        } else if (op == U_ADDCURR) {
+            sumg += Gk;
+            ichan += Ek * Gk;
+        } else if (op == U_NEWVOLT) {
+            ip = table_row(d, Vm);
+            vtab = d.tab + (long long)ip.filo * ncols;
+        } else if (op == U_CHAN) {
+            Gk = Sb[c * W + lane];
+            c += 1;
+        } else if (op == U_SPIKE) {
+            pc++;
+            c += 1;
+        } else {
+            const double tbyc = Sb[c * W + lane];
+            CompOut o;
+            o.rhs = Vm + ichan * tbyc;
+            o.diag = sumg * tbyc + Sb[(c + 1) * W + lane];
+            return o;
+        }
+    }
+}
+
+__device__ __forceinline__ double ipow(double x, int p)
+{
+    double r = 1.0;
+    for (int i = 0; i < p; i++) r *= x;
+    return r;
+}
+
+// Reverse of one compartment's program.  bb, db: adjoints of its rhs and
+// diag; LSb: the slot adjoints (gate slots hold the adjoint of the gate state
+// at t+1 on entry and at t on return; other slots accumulate gradients).
+// Returns the adjoint of Vm at t contributed through rhs and the gates.
+__device__ double adj_reverse_local(const DevModel& d, const int* __restrict__ code, const double* __restrict__ Sb,
+                                    double* __restrict__ LSb, int lane, double Vm, double bb, double db)
+{
+    const int ns = code[0];
+    const double tbyc = Sb[(ns - 2) * W + lane];
+    const double lam_i = bb * tbyc;      // adjoint of ichan
+    const double lam_g = db * tbyc;      // adjoint of sumgchan
+    double vbar = bb;                    // rhs = Vm + ...
+    double sumg = 0.0;
+    double ichan = Sb[0 * W + lane] + Sb[1 * W + lane];
+    LSb[0 * W + lane] += lam_i;          // Em/Rm
+    LSb[1 * W + lane] += lam_i;          // inject
+    double Gk = 0.0, Ek = 0.0, gmax = 0.0;
+    int ekslot = -1, gslot = -1, ng = 0;
+    int gs[MAXG], gp[MAXG];
+    double gx[MAXG], gdx[MAXG], gdv[MAXG];
+    Interp ip{0, 0.0, false};
+    const double* vtab = d.tab;
+    const int ncols = d.ncols;
+    int c = 2, pc = PROG_HEADER;
+    for (;;) {
+        const int op = code[pc++];
+        if (op == U_IPOL1V) {
+            const int col = code[pc], k = code[pc + 1];
+            pc += 2;
+            const double* tab = vtab + col;
+            const double B = tab[0] + ip.vipol * (tab[ncols] - tab[0]);
+            const double dB = ip.inside ? (tab[ncols] - tab[0]) * d.invdx : 0.0;
+            const double dA = ip.inside ? (tab[ncols + 1] - tab[1]) * d.invdx : 0.0;
+            const double X0 = Sb[c * W + lane];
+            double X, dXdX, dXdV;
+            if (k < 0) {
+                X = gate_update(vtab, col, ncols, ip.vipol, X0, true);
+                dXdX = 0.0;
+                dXdV = (dA - X * dB) / B;
+            } else {
+                X = gate_update(vtab, col, ncols, ip.vipol, X0, false);
+                dXdX = (2.0 - B) / B;
+                dXdV = (dA - X0 * dB - X * dB) / B;
+            }
+            const int p = k < 0 ? -k : k;
+            Gk = apply_power(Gk, X, p);
+            if (ng < MAXG) {
+                gs[ng] = c;
+                gp[ng] = p;
+                gx[ng] = X;
+                gdx[ng] = dXdX;
+                gdv[ng] = dXdV;
+            }
+            ng++;
+            c++;
+        } else if (op == U_CHAN_EK) {
+            ekslot = c;
+            Ek = Sb[c * W + lane];
+            gslot = c + 1;
+            gmax = Gk = Sb[(c + 1) * W + lane];
+            c += 2;
+            ng = 0;
+        } else if (op == U_CHAN) {
+            gslot = c;
+            gmax = Gk = Sb[c * W + lane];
+            c += 1;
+            ng = 0;
+        } else if (op == U_ADDCURR) {
+            sumg += Gk;
+            ichan += Ek * Gk;
+            const double lamG = lam_i * Ek + lam_g;
+            if (ekslot >= 0) LSb[ekslot * W + lane] += lam_i * Gk;
+            double prod = 1.0;
+            for (int g = 0; g < ng && g < MAXG; g++) prod *= ipow(gx[g], gp[g]);
+            LSb[gslot * W + lane] += lamG * prod;
+            for (int g = 0; g < ng && g < MAXG; g++) {
+                double dG = gmax * (double)gp[g] * ipow(gx[g], gp[g] - 1);
+                for (int h = 0; h < ng && h < MAXG; h++)
+                    if (h != g) dG *= ipow(gx[h], gp[h]);
+                const double lamX = LSb[gs[g] * W + lane] + lamG * dG;
+                LSb[gs[g] * W + lane] = lamX * gdx[g];
+                vbar += lamX * gdv[g];
+            }
+            ng = 0;
+        } else if (op == U_NEWVOLT) {
+            ip = table_row(d, Vm);
+            vtab = d.tab + (long long)ip.filo * ncols;
+        } else if (op == U_SPIKE) {
+            pc++;
+            c += 1;
+        } else {
+            LSb[c * W + lane] += bb * ichan + db * sumg;     // tbyc
+            LSb[(c + 1) * W + lane] += db;                   // diagterm
+            return vbar;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_adj_rhs(AdjModel a)
+{
+    const DevModel& d = a.d;
+    int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    b = __builtin_amdgcn_readfirstlane(b);
+    if (b >= d.nblocks) return;
+    const int lane = threadIdx.x & (W - 1);
+    const BlockMeta bm = d.blocks[b];
+    const long long pos = (long long)b * W + lane;
+    const CompOut o = adj_forward_local(d, d.prog + bm.prog, a.St + bm.dslot, lane, a.Vt[pos]);
+    a.RB[pos] = o.rhs;
+    a.RD[pos] = o.diag;
+}
+
+__global__ __launch_bounds__(256) void k_adj_local(AdjModel a)
+{
+    const DevModel& d = a.d;
+    int b = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    b = __builtin_amdgcn_readfirstlane(b);
+    if (b >= d.nblocks) return;
+    const int lane = threadIdx.x & (W - 1);
+    const BlockMeta bm = d.blocks[b];
+    const long long pos = (long long)b * W + lane;
+    double vbar = adj_reverse_local(d, d.prog + bm.prog, a.St + bm.dslot, a.LS + bm.dslot, lane, a.Vt[pos],
+                                    a.BB[pos], a.DB[pos]);
+    // Crank-Nicolson: V^{t+1} = 2x - V^t
+    if (d.crank) vbar -= a.LV[pos];
+    a.LV[pos] = vbar;
+}
+
+// Reverse of the elimination, one lane per neuron, rows in global memory.
+__global__ __launch_bounds__(256) void k_adj_elim(AdjModel a)
+{
+    const DevModel& d = a.d;
+    int t = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
+    t = __builtin_amdgcn_readfirstlane(t);
+    if (t >= d.ntiles) return;
+    const int lane = threadIdx.x & (W - 1);
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows;
+    const long long p0 = (long long)tm.block0 * W + lane;
+#define Q(r) (p0 + (long long)(r) * W)
+    double* Dp = a.Dp;
+    double* Bp = a.Bp;
+    double* X = a.X;
+    double* XB = a.XB;
+    double* BB = a.BB;
+    double* DB = a.DB;
+    double* Gta = a.GT;
+    double* Gtf = a.GT + a.npos;
+    double* Gtb = a.GT + 2 * a.npos;
+    for (int r = 0; r < R; r++) {
+        Dp[Q(r)] = a.RD[Q(r)];
+        Bp[Q(r)] = a.RB[Q(r)];
+    }
+    // forward elimination and back-substitution, as the step ran them
+    for (int r = 0; r < R - 1; r++) {
+        const int p = d.par[Q(r)];
+        if (p >= 0) {
+            const double tt = d.ta[Q(r)] / Dp[Q(r)];
+            Dp[Q(p)] -= d.tcf[Q(r)] * tt;
+            Bp[Q(p)] -= Bp[Q(r)] * tt;
+        }
+    }
+    for (int r = R - 1; r >= 0; r--) {
+        const int p = d.par[Q(r)];
+        double bv = Bp[Q(r)];
+        if (p >= 0) bv -= d.tcb[Q(r)] * X[Q(p)];
+        X[Q(r)] = bv / Dp[Q(r)];
+    }
+    const double xs = d.crank ? 2.0 : 1.0;
+    for (int r = 0; r < R; r++) {
+        XB[Q(r)] = xs * a.LV[Q(r)];
+        BB[Q(r)] = 0.0;
+        DB[Q(r)] = 0.0;
+    }
+    // back-substitution in reverse: rows ascending (children before parents)
+    for (int r = 0; r < R; r++) {
+        const int p = d.par[Q(r)];
+        const double xb = XB[Q(r)];
+        const double dk = Dp[Q(r)];
+        BB[Q(r)] += xb / dk;
+        DB[Q(r)] -= xb * X[Q(r)] / dk;
+        if (p >= 0) {
+            Gtb[Q(r)] -= xb * X[Q(p)] / dk;
+            XB[Q(p)] -= xb * d.tcb[Q(r)] / dk;
+        }
+    }
+    // forward elimination in reverse: rows descending
+    for (int r = R - 2; r >= 0; r--) {
+        const int p = d.par[Q(r)];
+        if (p < 0) continue;
+        const double dk = Dp[Q(r)];
+        const double tt = d.ta[Q(r)] / dk;
+        const double bbp = BB[Q(p)], dbp = DB[Q(p)];
+        BB[Q(r)] -= bbp * tt;
+        const double tbar = -bbp * Bp[Q(r)] - dbp * d.tcf[Q(r)];
+        Gtf[Q(r)] -= dbp * tt;
+        Gta[Q(r)] += tbar / dk;
+        DB[Q(r)] -= tbar * tt / dk;
+    }
+#undef Q
+}
+
+__global__ void k_gather(const double* __restrict__ V, const int* __restrict__ pos, int n, double* __restrict__ out)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = V[pos[j]];
+}
+
+__global__ void k_inject(double* __restrict__ LV, const int* __restrict__ pos, int n, const double* __restrict__ g)
+{
+    // one thread per observed row: rows are distinct, so no two threads
+    // touch one position
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) LV[pos[j]] += g[j];
+}
+
+}  // namespace
+
+hipError_t launch_adj_rhs(const AdjModel& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_rhs, dim3((a.d.nblocks + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_elim(const AdjModel& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_elim, dim3((a.d.ntiles + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_local(const AdjModel& a, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_local, dim3((a.d.nblocks + 3) / 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const double* V, const int* pos, int n, double* out, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, V, pos, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_inject(double* LV, const int* pos, int n, const double* g, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_inject, dim3((n + 255) / 256), dim3(256), 0, s, LV, pos, n, g);
+    return hipGetLastError();
+}
+
+}  // namespace hsh

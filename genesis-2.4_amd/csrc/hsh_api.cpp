@@ -109,6 +109,20 @@ struct hsh_handle {
     // chip index -> device S positions (for hsh_set_chip)
     std::vector<int64_t> c2s_off, c2s;
     int host_step = 0;
+    // adjoint: checkpoints of the last hsh_adj_forward run, and buffers
+    struct Adj {
+        int nsteps = 0, ckpt = 0, ncp = 0, nobs = 0;
+        bool valid = false;
+        size_t npos = 0, nS = 0, nI = 0, nstep = 0;   // sizes the buffers were made for
+        int32_t* obs_pos = nullptr;
+        double* trace = nullptr;       // [nsteps][nobs]
+        double* dtrace = nullptr;
+        double *ckV = nullptr, *ckS = nullptr;       // [ncp][...]
+        int32_t *ckI = nullptr, *ckStep = nullptr;
+        double *segV = nullptr, *segS = nullptr;     // [ckpt][...]
+        double *RB = nullptr, *RD = nullptr, *LV = nullptr, *LS = nullptr, *BB = nullptr, *DB = nullptr,
+               *GT = nullptr, *Dp = nullptr, *Bp = nullptr, *X = nullptr, *XB = nullptr;
+    } adj;
 };
 
 namespace {
@@ -227,8 +241,19 @@ void free_graphs(hsh_handle* h)
     h->graph_src.clear();
 }
 
+void free_adj(hsh_handle* h)
+{
+    auto& A = h->adj;
+    void* ptrs[] = {A.obs_pos, A.trace, A.dtrace, A.ckV, A.ckS, A.ckI, A.ckStep, A.segV, A.segS, A.RB,
+                    A.RD, A.LV, A.LS, A.BB, A.DB, A.GT, A.Dp, A.Bp, A.X, A.XB};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    A = hsh_handle::Adj();
+}
+
 void free_layout(hsh_handle* h)
 {
+    free_adj(h);
     free_graphs(h);
     void* ptrs[] = {h->d_blocks, h->d_tiles, h->d_packs, h->d_S, h->d_V, h->d_R, h->d_ta, h->d_tcf,
                     h->d_tcb, h->d_kta, h->d_ktcf, h->d_btcb, h->d_I, h->d_par, h->d_pred, h->d_step,
@@ -671,6 +696,104 @@ hsh_status autotune(hsh_handle* h)
     return HSH_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Adjoint support
+
+constexpr int ADJ_MAXG = 3;     // gates per channel of the reverse program
+
+bool adj_supported(const Plan& P, std::string& why)
+{
+    for (size_t t = 0; t < P.templates.size(); t++) {
+        const std::vector<int32_t>& code = P.templates[t].code;
+        int ng = 0;
+        for (size_t pc = 0; pc < code.size();) {
+            const int op = code[pc++];
+            if (op == U_IPOL1V) {
+                pc += 2;
+                if (++ng > ADJ_MAXG) {
+                    why = "a channel with more than 3 gates";
+                    return false;
+                }
+            } else if (op == U_CHAN || op == U_CHAN_EK) {
+                ng = 0;
+            } else if (op == U_SPIKE) {
+                pc += 1;
+            } else if (op == U_END) {
+                break;
+            }
+        }
+    }
+    return true;
+}
+
+// The adjoint kernels run on the tile layout (one lane per neuron).
+hsh_status ensure_tiles(hsh_handle* h)
+{
+    if (h->plan.packs.empty()) return HSH_OK;
+    LayoutOpts lo = h->plan.layout;
+    lo.tile_max_rows = std::max(1, h->plan.max_tree_rows);
+    return relayout(h, lo);
+}
+
+hsh_status adj_alloc(hsh_handle* h, int nsteps, int ckpt, int nobs)
+{
+    auto& A = h->adj;
+    const Plan& P = h->plan;
+    const size_t npos = P.blocks.size() * W, nS = (size_t)std::max<int64_t>(1, P.nS);
+    const size_t nI = (size_t)std::max<int64_t>(1, P.nI), nstep = P.tiles.size() + P.packs.size();
+    const int ncp = (nsteps + ckpt - 1) / ckpt + 1;
+    free_adj(h);
+    A.nsteps = nsteps;
+    A.ckpt = ckpt;
+    A.ncp = ncp;
+    A.nobs = nobs;
+    A.npos = npos;
+    A.nS = nS;
+    A.nI = nI;
+    A.nstep = std::max<size_t>(1, nstep);
+    HIPCHK(dalloc(&A.obs_pos, (size_t)nobs));
+    HIPCHK(dalloc(&A.trace, (size_t)nsteps * nobs));
+    HIPCHK(dalloc(&A.dtrace, (size_t)nsteps * nobs));
+    HIPCHK(dalloc(&A.ckV, (size_t)ncp * npos));
+    HIPCHK(dalloc(&A.ckS, (size_t)ncp * nS));
+    HIPCHK(dalloc(&A.ckI, (size_t)ncp * nI));
+    HIPCHK(dalloc(&A.ckStep, (size_t)ncp * A.nstep));
+    HIPCHK(dalloc(&A.segV, (size_t)ckpt * npos));
+    HIPCHK(dalloc(&A.segS, (size_t)ckpt * nS));
+    double** bufs[] = {&A.RB, &A.RD, &A.LV, &A.BB, &A.DB, &A.Dp, &A.Bp, &A.X, &A.XB};
+    for (double** b : bufs) HIPCHK(dalloc(b, npos));
+    HIPCHK(dalloc(&A.LS, nS));
+    HIPCHK(dalloc(&A.GT, 3 * npos));
+    return HSH_OK;
+}
+
+hsh_status adj_save(hsh_handle* h, int i)
+{
+    auto& A = h->adj;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(A.ckV + (size_t)i * A.npos, h->d_V, A.npos * sizeof(double), k, h->stream));
+    HIPCHK(hipMemcpyAsync(A.ckS + (size_t)i * A.nS, h->d_S, (size_t)h->plan.nS * sizeof(double), k, h->stream));
+    HIPCHK(hipMemcpyAsync(A.ckI + (size_t)i * A.nI, h->d_I, (size_t)std::max<int64_t>(1, h->plan.nI) * sizeof(int32_t),
+                          k, h->stream));
+    HIPCHK(hipMemcpyAsync(A.ckStep + (size_t)i * A.nstep, h->d_step,
+                          (h->plan.tiles.size() + h->plan.packs.size()) * sizeof(int32_t), k, h->stream));
+    return HSH_OK;
+}
+
+hsh_status adj_load(hsh_handle* h, int i)
+{
+    auto& A = h->adj;
+    const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+    HIPCHK(hipMemcpyAsync(h->d_V, A.ckV + (size_t)i * A.npos, A.npos * sizeof(double), k, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_S, A.ckS + (size_t)i * A.nS, (size_t)h->plan.nS * sizeof(double), k, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_I, A.ckI + (size_t)i * A.nI, (size_t)std::max<int64_t>(1, h->plan.nI) * sizeof(int32_t),
+                          k, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_step, A.ckStep + (size_t)i * A.nstep,
+                          (h->plan.tiles.size() + h->plan.packs.size()) * sizeof(int32_t), k, h->stream));
+    return HSH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1006,6 +1129,154 @@ hsh_status hsh_jit_source(const hsh_model* m, char* buf, int64_t cap, int64_t* l
         const size_t n = std::min<size_t>((size_t)cap - 1, src.size());
         std::memcpy(buf, src.data(), n);
         buf[n] = '\0';
+    }
+    return HSH_OK;
+}
+
+
+hsh_status hsh_set_ravals(hsh_handle* h, const double* ravals)
+{
+    if (!h || !ravals) return fail(HSH_ERR_ARG, "hsh_set_ravals: null argument");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_ravals: handle refused the model");
+    Plan& P = h->plan;
+    for (int r = 0; r < h->m.ncompts; r++) {
+        if (P.ra_fwd[r] >= 0) {
+            P.ta[r] = ravals[P.ra_fwd[r]];
+            P.tcf[r] = ravals[P.ra_fwd[r] + 1];
+        }
+        if (P.ra_bwd[r] >= 0) P.tcb[r] = ravals[P.ra_bwd[r]];
+    }
+    if (h->host_only) return HSH_OK;
+    return relayout(h, P.layout);
+}
+
+hsh_status hsh_adj_forward(hsh_handle* h, int32_t nsteps, int32_t ckpt, int32_t nobs, const int32_t* obs,
+                           double* trace)
+{
+    if (!h) return fail(HSH_ERR_ARG, "hsh_adj_forward: null handle");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_adj_forward: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_adj_forward: handle refused the model");
+    if (nsteps <= 0 || ckpt <= 0 || nobs < 0 || (nobs > 0 && !obs))
+        return fail(HSH_ERR_ARG, "hsh_adj_forward: bad step count, checkpoint interval or observed rows");
+    {
+        std::vector<char> seen(h->m.ncompts, 0);
+        for (int j = 0; j < nobs; j++) {
+            if (obs[j] < 0 || obs[j] >= h->m.ncompts) return fail(HSH_ERR_ARG, "hsh_adj_forward: observed row out of range");
+            if (seen[obs[j]]++) return fail(HSH_ERR_ARG, "hsh_adj_forward: observed rows must be distinct");
+        }
+    }
+    std::string why;
+    if (!adj_supported(h->plan, why)) return fail(HSH_REFUSED, "hsh_adj_forward: the reverse program does not run " + why);
+    STCHK(ensure_tiles(h));
+    if (ckpt > nsteps) ckpt = nsteps;
+    STCHK(adj_alloc(h, nsteps, ckpt, nobs));
+    auto& A = h->adj;
+    std::vector<int32_t> pos(std::max(1, nobs));
+    for (int j = 0; j < nobs; j++) pos[j] = h->plan.comp_pos[obs[j]];
+    if (nobs > 0) HIPCHK(hipMemcpy(A.obs_pos, pos.data(), nobs * sizeof(int32_t), hipMemcpyHostToDevice));
+    const bool timing = h->timing;
+    h->timing = false;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    STCHK(adj_save(h, 0));
+    for (int t = 0; t < nsteps; t++) {
+        STCHK(plain_step(h, false));
+        HIPCHK(launch_gather(h->d_V, A.obs_pos, nobs, A.trace + (size_t)t * nobs, h->stream));
+        if ((t + 1) % ckpt == 0 && t + 1 < nsteps) STCHK(adj_save(h, (t + 1) / ckpt));
+    }
+    STCHK(adj_save(h, A.ncp - 1));
+    h->timing = timing;
+    h->host_step += nsteps;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (trace && nobs > 0)
+        HIPCHK(hipMemcpy(trace, A.trace, (size_t)nsteps * nobs * sizeof(double), hipMemcpyDeviceToHost));
+    A.valid = true;
+    return HSH_OK;
+}
+
+hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip, double* g_ravals, double* g_vm0)
+{
+    if (!h) return fail(HSH_ERR_ARG, "hsh_adj_backward: null handle");
+    auto& A = h->adj;
+    if (h->host_only || h->refused || !A.valid)
+        return fail(HSH_ERR_STATE, "hsh_adj_backward: no hsh_adj_forward run to differentiate");
+    if (!dtrace && A.nobs > 0) return fail(HSH_ERR_ARG, "hsh_adj_backward: null dtrace");
+    const Plan& P = h->plan;
+    const size_t npos = A.npos;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (A.nobs > 0)
+        HIPCHK(hipMemcpy(A.dtrace, dtrace, (size_t)A.nsteps * A.nobs * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemsetAsync(A.LV, 0, npos * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(A.LS, 0, A.nS * sizeof(double), h->stream));
+    HIPCHK(hipMemsetAsync(A.GT, 0, 3 * npos * sizeof(double), h->stream));
+    int32_t nspk0 = 0;
+    HIPCHK(hipMemcpy(&nspk0, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
+    const bool timing = h->timing;
+    h->timing = false;
+    AdjModel am{};
+    am.d = h->dm;
+    am.RB = A.RB;
+    am.RD = A.RD;
+    am.LV = A.LV;
+    am.LS = A.LS;
+    am.BB = A.BB;
+    am.DB = A.DB;
+    am.GT = A.GT;
+    am.Dp = A.Dp;
+    am.Bp = A.Bp;
+    am.X = A.X;
+    am.XB = A.XB;
+    am.npos = (long long)npos;
+    const int nseg = (A.nsteps + A.ckpt - 1) / A.ckpt;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int t0 = sg * A.ckpt, t1 = std::min(A.nsteps, t0 + A.ckpt);
+        STCHK(adj_load(h, sg));
+        for (int t = t0; t < t1; t++) {
+            const size_t j = (size_t)(t - t0);
+            HIPCHK(hipMemcpyAsync(A.segV + j * npos, h->d_V, npos * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+            HIPCHK(hipMemcpyAsync(A.segS + j * A.nS, h->d_S, (size_t)P.nS * sizeof(double), hipMemcpyDeviceToDevice,
+                                  h->stream));
+            STCHK(plain_step(h, false));
+        }
+        for (int t = t1 - 1; t >= t0; t--) {
+            const size_t j = (size_t)(t - t0);
+            HIPCHK(launch_inject(A.LV, A.obs_pos, A.nobs, A.dtrace + (size_t)t * A.nobs, h->stream));
+            am.Vt = A.segV + j * npos;
+            am.St = A.segS + j * A.nS;
+            HIPCHK(launch_adj_rhs(am, h->stream));
+            HIPCHK(launch_adj_elim(am, h->stream));
+            HIPCHK(launch_adj_local(am, h->stream));
+        }
+    }
+    STCHK(adj_load(h, A.ncp - 1));
+    HIPCHK(hipMemcpyAsync(h->d_nspk, &nspk0, sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    h->timing = timing;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> LV(npos), LS(A.nS), GT(3 * npos);
+    HIPCHK(hipMemcpy(LV.data(), A.LV, npos * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(LS.data(), A.LS, A.nS * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(GT.data(), A.GT, 3 * npos * sizeof(double), hipMemcpyDeviceToHost));
+    const int n = h->m.ncompts;
+    if (g_chip) std::fill(g_chip, g_chip + h->m.nchips, 0.0);
+    if (g_ravals) std::fill(g_ravals, g_ravals + h->m.nravals, 0.0);
+    for (int g = 0; g < n; g++) {
+        const int pos = P.comp_pos[g];
+        const int b = pos / W, l = pos % W;
+        if (g_vm0) g_vm0[g] = LV[pos];
+        if (g_chip) {
+            const BlockMeta& bm = P.blocks[b];
+            const Template& t = P.templates[P.comp_tmpl[g]];
+            for (int k = 0; k < t.nslots; k++) {
+                const int64_t src = P.dsrc[P.comp_dsrc_off[g] + k];
+                if (src >= 0) g_chip[src] += LS[(size_t)bm.dslot + (size_t)k * W + l];
+            }
+        }
+        if (g_ravals) {
+            if (P.ra_fwd[g] >= 0) {
+                g_ravals[P.ra_fwd[g]] += GT[pos];
+                g_ravals[P.ra_fwd[g] + 1] += GT[npos + pos];
+            }
+            if (P.ra_bwd[g] >= 0) g_ravals[P.ra_bwd[g]] += GT[2 * npos + pos];
+        }
     }
     return HSH_OK;
 }

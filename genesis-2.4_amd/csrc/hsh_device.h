@@ -411,10 +411,11 @@ __device__ __forceinline__ void body_batched(const DevModel& d, int K, double* l
 // pack.  V, rhs and diag of every pack-local position live in LDS for the
 // whole launch.  Per step: the waves run the channel programs of the pack's
 // blocks, then the Hines elimination runs level by level -- forward, each
-// parent pulls its children in ascending row order (the order in which the
-// reference's funcs[] program folds them in, so every parent sees the same
-// sequence of roundings); backward, roots solve and children follow their
-// parents -- with a barrier between levels.  CHAN = false is the plain
+// parent pulls its children in ascending row order, which is how the
+// reference's funcs[] program builds a row before storing it
+// (hines_solve.c:246-273), so every parent sees the same sequence of
+// roundings; backward, roots solve and children follow their parents
+// (:275-295) -- with a barrier between levels.  CHAN = false is the plain
 // drop-in's elimination (rhs/diag read from results, one step).  STAGE copies
 // the pack's level schedule into LDS once per launch.
 template <class RUN, bool CHAN, bool STAGE>

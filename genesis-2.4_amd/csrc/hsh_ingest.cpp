@@ -256,6 +256,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     // tree analyser
     P.parent.assign(n, -1);
     P.ta.assign(n, 0.0); P.tcf.assign(n, 0.0); P.tcb.assign(n, 0.0);
+    P.ra_fwd.assign(n, -1); P.ra_bwd.assign(n, -1);
     std::vector<char> got_bwd(n, 0);
     std::vector<int32_t> lastkid(n, -1);
     {
@@ -279,6 +280,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     if (kid <= lastkid[cur]) { ok = false; refuse(P, cur, op, (int)(fi - 2), HSH_R_TREE); }
                     lastkid[cur] = kid;
                     P.parent[kid] = cur;
+                    P.ra_fwd[kid] = (int32_t)ri;
                     P.ta[kid] = ra[ri++];
                     P.tcf[kid] = ra[ri++];
                 } else if (op == F_SET_DIAG) {
@@ -308,6 +310,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                             break;
                         }
                         got_bwd[row] = 1;
+                        P.ra_bwd[row] = (int32_t)ri;
                         P.tcb[row] = ra[ri++];
                     } else if (op == F_CALC) {
                         row--;

@@ -57,6 +57,8 @@ struct Plan {
     // tree
     std::vector<int32_t> parent;          // parent row (global), -1 root
     std::vector<double> ta, tcf, tcb;
+    std::vector<int32_t> ra_fwd;          // ravals index of (ta, tcf) of a non-root row, -1
+    std::vector<int32_t> ra_bwd;          // ravals index of tcb of a non-root row, -1
     std::vector<int32_t> tree_lo;         // per tree (neuron): first row; root = hi
     std::vector<int32_t> tree_hi;
     std::vector<int32_t> pred;            // previous NEWVOLT compartment (global row), -1
@@ -113,6 +115,31 @@ hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int
 // packs, plain drop-in: elimination only (rhs/diag from results)
 hipError_t launch_forest_elim(const DevModel& d, int lds, hipStream_t s);
 hipError_t launch_step_advance(const DevModel& d, int32_t by, hipStream_t s);
+
+// Adjoint (hsh_adjoint.hip).  Tile layout only.  Arrays are [position]
+// (block*64+lane) unless noted.
+struct AdjModel {
+    DevModel d;           // the model; d.S / d.V are not used
+    const double* Vt;     // V at step t
+    const double* St;     // slots at step t [nS]
+    double* RB;           // rhs of step t
+    double* RD;           // diag of step t
+    double* LV;           // adjoint of V (in: t+1, out: t)
+    double* LS;           // slot adjoints [nS]: gate states (in: t+1, out: t), constants (accumulated)
+    double* BB;           // adjoint of rhs
+    double* DB;           // adjoint of diag
+    double* GT;           // accumulated gradients of ta, tcf, tcb [3][npos]
+    double* Dp;           // scratch: eliminated diag, eliminated rhs, solution, adjoint of solution
+    double* Bp;
+    double* X;
+    double* XB;
+    long long npos;
+};
+hipError_t launch_adj_rhs(const AdjModel& a, hipStream_t s);
+hipError_t launch_adj_elim(const AdjModel& a, hipStream_t s);
+hipError_t launch_adj_local(const AdjModel& a, hipStream_t s);
+hipError_t launch_gather(const double* V, const int* pos, int n, double* out, hipStream_t s);
+hipError_t launch_inject(double* LV, const int* pos, int n, const double* g, hipStream_t s);
 int batched_lds_bytes(int max_rows);
 
 }  // namespace hsh

@@ -73,7 +73,8 @@ class HshInfo(C.Structure):
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
            "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
-           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_destroy", "hsh_last_error")
+           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_ravals", "hsh_adj_forward",
+           "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
 
 _lib = None
 
@@ -103,6 +104,12 @@ def lib():
         L.hsh_info_get.restype = C.c_int32
         L.hsh_autotune.argtypes = [C.c_void_p]
         L.hsh_autotune.restype = C.c_int32
+        L.hsh_set_ravals.argtypes = [C.c_void_p, _f64p]
+        L.hsh_set_ravals.restype = C.c_int32
+        L.hsh_adj_forward.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _f64p]
+        L.hsh_adj_forward.restype = C.c_int32
+        L.hsh_adj_backward.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+        L.hsh_adj_backward.restype = C.c_int32
         L.hsh_last_time.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         L.hsh_last_time.restype = C.c_int32
         L.hsh_set_timing.argtypes = [C.c_void_p, C.c_int32]
@@ -231,6 +238,34 @@ class HipSolver:
         i = HshInfo()
         _check(lib().hsh_info_get(self.h, C.byref(i)), "hsh_info_get")
         return {f: getattr(i, f) for f, _ in HshInfo._fields_}
+
+    def set_ravals(self, ravals):
+        r = np.ascontiguousarray(ravals, np.float64)
+        assert r.size == self.m.ravals.size
+        _check(lib().hsh_set_ravals(self.h, _p(r, C.c_double)), "hsh_set_ravals")
+
+    def adj_forward(self, nsteps: int, ckpt: int, obs):
+        """Run nsteps plain steps keeping checkpoints; returns vm[obs] after
+        every step as an (nsteps, nobs) array."""
+        obs = np.ascontiguousarray(obs, np.int32)
+        tr = np.empty((nsteps, obs.size))
+        rc = lib().hsh_adj_forward(self.h, nsteps, ckpt, obs.size, _p(obs, C.c_int32), _p(tr, C.c_double))
+        if rc == HSH_REFUSED:
+            raise Refused(lib().hsh_last_error().decode(), [])
+        _check(rc, "hsh_adj_forward")
+        self._adj_shape = tr.shape
+        return tr
+
+    def adj_backward(self, dtrace):
+        """Gradients (dL/dchip, dL/dravals, dL/dvm0) of the last adj_forward
+        run for dL/dtrace."""
+        g = np.ascontiguousarray(dtrace, np.float64).reshape(self._adj_shape)
+        gc = np.empty(self.m.chip.size)
+        gr = np.empty(self.m.ravals.size)
+        gv = np.empty(self.m.ncompts)
+        _check(lib().hsh_adj_backward(self.h, _p(g, C.c_double), _p(gc, C.c_double), _p(gr, C.c_double),
+                                      _p(gv, C.c_double)), "hsh_adj_backward")
+        return gc, gr, gv
 
     def autotune(self):
         _check(lib().hsh_autotune(self.h), "hsh_autotune")

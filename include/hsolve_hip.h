@@ -187,6 +187,28 @@ hsh_status hsh_info_get(struct hsh_handle *h, hsh_info *info);
  * K on the handle's own model and keeps the fastest.  The simulation state
  * (vm, chip, ops, results, step count, pending spikes) is unchanged. */
 hsh_status hsh_autotune(struct hsh_handle *h);
+/* Replace the elimination coefficients with ravals[nravals] in the layout of
+ * the model's funcs[] program (the values h_funcs_init computes from Cm and
+ * Ra, genesis/src/hines/hines_init.c:604-742, :1011-1172); the state is kept. */
+hsh_status hsh_set_ravals(struct hsh_handle *h, const double *ravals);
+/* Adjoint (reverse-mode gradients) of a run, for parameter fitting.  No
+ * counterpart in the reference.
+ * hsh_adj_forward runs nsteps plain steps from the current state, keeps a
+ * checkpoint of the state every ckpt steps, and writes vm of the rows obs[]
+ * (distinct) after every step to trace[nsteps*nobs] (host, may be NULL).  The
+ * state advances as with hsh_step.  Tabchannel programs only (up to three
+ * gates per channel); HSH_REFUSED otherwise.
+ * hsh_adj_backward takes dL/dtrace[nsteps*nobs] for the last forward run and
+ * writes dL/dchip[nchips], dL/dravals[nravals] and dL/dvm0[ncompts] (any may
+ * be NULL): constants of the channel programs (gmax, Ek, Em/Rm, inject, tbyc,
+ * diagterm) and the coefficients summed over the run, gate states and vm at
+ * its start.  It recomputes each segment from its checkpoint (memory: one
+ * state per checkpoint plus ckpt states) and leaves the state as the forward
+ * run left it. */
+hsh_status hsh_adj_forward(struct hsh_handle *h, int32_t nsteps, int32_t ckpt, int32_t nobs,
+                           const int32_t *obs, double *trace);
+hsh_status hsh_adj_backward(struct hsh_handle *h, const double *dtrace, double *g_chip,
+                            double *g_ravals, double *g_vm0);
 /* Device time of the last hsh_step call (HIP events on the handle's stream),
  * in ms, after hsh_sync.  kernel selects HSH_T_*. */
 #define HSH_T_TOTAL    0

@@ -11,6 +11,10 @@ of the reference CPU code:
 * elimination  -- genesis/src/hines/hines_solve.c:140-296
 * phase order  -- genesis/src/hines/hines.c:222-270
 
+``TangentLinear`` wraps ``hsolve_tl.c``: the same step (tabchannel subset)
+carrying the derivative of every value along one direction of the initial
+vm[], chip[] and ravals[]; the CPU reference for adjoint gradients.
+
 Pinning (what checks each function; see tests/test_oracle_*.py):
 
 * interpreter + elimination: the reference's own CPU goldens for
@@ -21,8 +25,11 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   trees (numpy.linalg.solve), Crank-Nicolson identity V = 2x - V_old.
 * SPIKE_OP counting: hand-computed refractory sequences.
 * SYN2_OP decay/event arithmetic: closed-form impulse response.
+* tangent-linear model: central finite differences of the oracle itself
+  (tests/test_oracle_tl.py); the closed-form derivative of the passive
+  isopotential recursion.
 * GHK / NERNST / MGBLOCK / conc ops: parity unpinned against the reference
   (no reference test reaches them; libm exp/log); checked only against
   the formulas at single points.
 """
-from .oracle import Oracle, oracle_lib  # noqa: F401
+from .oracle import Oracle, TangentLinear, oracle_lib  # noqa: F401

@@ -85,6 +85,21 @@ void    ohs_trace(ohs *h, uint64_t *per_comp);
  * Used for the multi-core CPU baseline. */
 int32_t ohs_step_parallel(ohs **hs, int32_t nh, int32_t nsteps, int32_t nthreads);
 
+/* Tangent-linear model (hsolve_tl.c): the same step carrying the derivative
+ * of every value along one direction of (vm0, chip, ravals).  Tabchannel
+ * subset only; ohs_tl_create returns NULL for other ops. */
+typedef struct ohs_tl ohs_tl;
+ohs_tl *ohs_tl_create(const ohs_arrays *a);
+void    ohs_tl_destroy(ohs_tl *t);
+/* tangents of vm[ncompts], chip[nchips], ravals[nravals] (NULL = zero) */
+void    ohs_tl_set_tangent(ohs_tl *t, const double *dvm, const double *dchip,
+                           const double *dravals);
+/* nsteps steps; after step s, trace[s*nobs+j] = vm[obs[j]] and dtrace the
+ * same for its tangent (either may be NULL) */
+int32_t ohs_tl_step(ohs_tl *t, int32_t nsteps, const int32_t *obs, int32_t nobs,
+                    double *trace, double *dtrace);
+void    ohs_tl_get(ohs_tl *t, double *vm, double *dvm, double *chip, double *dchip);
+
 #ifdef __cplusplus
 }
 #endif

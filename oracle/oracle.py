@@ -63,6 +63,13 @@ def oracle_lib():
         L.ohs_trace.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.ohs_step_parallel.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32]
         L.ohs_step_parallel.restype = C.c_int32
+        L.ohs_tl_create.argtypes = [C.POINTER(Arrays)]
+        L.ohs_tl_create.restype = C.c_void_p
+        L.ohs_tl_destroy.argtypes = [C.c_void_p]
+        L.ohs_tl_set_tangent.argtypes = [C.c_void_p, _f64p, _f64p, _f64p]
+        L.ohs_tl_step.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int32, _f64p, _f64p]
+        L.ohs_tl_step.restype = C.c_int32
+        L.ohs_tl_get.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -191,6 +198,43 @@ class Oracle:
 
     def trace_enable(self, on=True):
         oracle_lib().ohs_trace_enable(self.h, 1 if on else 0)
+
+
+class TangentLinear:
+    """Tangent-linear model of the step (hsolve_tl.c): the oracle's step with
+    the derivative of every value along one direction of (vm0, chip, ravals)."""
+
+    def __init__(self, m, dvm=None, dchip=None, dravals=None):
+        o = Oracle(m, givals=False)          # reuse the array marshalling
+        self._o = o
+        self.m = m
+        self.h = oracle_lib().ohs_tl_create(C.byref(o._a))
+        if not self.h:
+            raise ValueError("tangent-linear model: ops outside the tabchannel subset")
+        keep = [np.ascontiguousarray(x, np.float64) if x is not None else None
+                for x in (dvm, dchip, dravals)]
+        self._t = keep
+        oracle_lib().ohs_tl_set_tangent(self.h, *[_p(x, C.c_double) for x in keep])
+
+    def run(self, nsteps: int, obs):
+        """(trace, dtrace): vm[obs] and its tangent after each of nsteps steps."""
+        obs = np.ascontiguousarray(obs, np.int32)
+        tr = np.empty((nsteps, obs.size))
+        dtr = np.empty((nsteps, obs.size))
+        oracle_lib().ohs_tl_step(self.h, nsteps, _p(obs, C.c_int32), obs.size, _p(tr, C.c_double),
+                                 _p(dtr, C.c_double))
+        return tr, dtr
+
+    def close(self):
+        if getattr(self, "h", None):
+            oracle_lib().ohs_tl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def step_parallel(oracles, nsteps: int, nthreads: int):
