@@ -62,9 +62,19 @@ def parse():
                    help="skip the two rocprofv3 --pmc child runs that measure HBM bytes per launch")
     p.add_argument("--traffic-dir", default="",
                    help="keep the rocprofv3 databases of the traffic passes under this directory")
+    p.add_argument("--no-hetero", action="store_true",
+                   help="skip the heterogeneous-population section (100k neurons, 1..1024 compartments)")
+    p.add_argument("--hetero-neurons", type=int, default=100000)
+    p.add_argument("--hetero-steps", type=int, default=5000)
+    p.add_argument("--no-adjoint", action="store_true",
+                   help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
+    p.add_argument("--adj-neurons", type=int, default=1024)
+    p.add_argument("--adj-steps", type=int, default=2000)
     p.add_argument("--traffic-probe", action="store_true",
                    help="(child of the traffic measurement) copy calibration, then a few batched "
                         "launches, and exit")
+    p.add_argument("--probe-layout", default="16,1024,4,1",
+                   help="(traffic probe) tile_max_rows,pack_cap,pack_waves,pack_stage of the measured run")
     return p.parse_args()
 
 
@@ -80,7 +90,7 @@ def copy_calibration(torch, mb):
     return nel * 8
 
 
-def read_counter(db, counter):
+def read_counter(db, counter, kernel):
     """Mean per-dispatch value of ``counter`` for the batched kernel and for
     the calibration copy, from a rocprofv3 --pmc database."""
     con = sqlite3.connect(db)
@@ -88,7 +98,7 @@ def read_counter(db, counter):
     for name, cn, val in con.execute("select kernel_name, counter_name, value from counters_collection"):
         if cn != counter:
             continue
-        if "batched" in name:
+        if kernel in name:
             kern.append(val)
         elif "copyBuffer" in name:
             copy.append(val)
@@ -98,7 +108,7 @@ def read_counter(db, counter):
     return sum(kern) / len(kern), sum(copy) / len(copy)
 
 
-def measure_traffic(a, K):
+def measure_traffic(a, K, info, kernel):
     """HBM bytes per batched launch: two child runs of this script under
     ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE).  Each counter is
     scaled by what it reads for a copy of CAL_MB MiB in the same run
@@ -112,9 +122,12 @@ def measure_traffic(a, K):
     with tempfile.TemporaryDirectory(dir="/tmp") as tmp:
         for counter in ("FETCH_SIZE", "WRITE_SIZE"):
             d = os.path.join(keep or tmp, "pmc_" + counter)
+            # the child runs the same layout, kernel shape and K without tuning
+            lay = f"{info['tile_max_rows'] or -1},{info['pack_cap']},{info['pack_waves']},{info['pack_stage']}"
             cmd = [exe, "--pmc", counter, "-d", d, "-o", "p", "--", sys.executable, os.path.abspath(__file__),
                    "--traffic-probe", "--neurons", str(a.neurons), "--ncomp", str(a.ncomp),
-                   "--seed", str(a.seed), "--batch-k", str(K), "--flags", str(a.flags)]
+                   "--seed", str(a.seed), "--batch-k", str(K), "--flags", str(a.flags | 1),
+                   "--probe-layout", lay]
             print(f"[bench] traffic pass {counter}: {' '.join(cmd)}", file=sys.stderr, flush=True)
             try:
                 p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
@@ -126,7 +139,7 @@ def measure_traffic(a, K):
             dbs = [os.path.join(r, f) for r, _, fs in os.walk(d) for f in fs if f.endswith("_results.db")]
             if not dbs:
                 return None, f"{counter}: no results database"
-            res[counter] = read_counter(dbs[0], counter)
+            res[counter] = read_counter(dbs[0], counter, kernel)
     cal_kib = CAL_MB * 1024.0
     f_fetch = cal_kib / res["FETCH_SIZE"][1]
     f_write = cal_kib / res["WRITE_SIZE"][1]
@@ -136,6 +149,149 @@ def measure_traffic(a, K):
          "calibration": f"{CAL_MB} MiB device copy read as {res['FETCH_SIZE'][1]:.0f} KiB FETCH_SIZE, "
                         f"{res['WRITE_SIZE'][1]:.0f} KiB WRITE_SIZE"}
     return t, None
+
+
+def batched_kernel_name(info):
+    """Name of the kernel a single-class batched launch runs."""
+    jit = bool(info["jit"])
+    if info["ntiles"] == 0:
+        st = 1 if info["pack_stage"] == 2 else 0
+        return f"hsh_jit_forest{st}" if jit else "k_forest"
+    r = info["max_rows"]
+    if r == 1:
+        return "hsh_jit_iso" if jit else "k_iso"
+    if r <= 16:
+        rpw = 1 if r <= 4 else (2 if r <= 8 else 4)
+        return f"hsh_jit_batched_r{rpw}" if jit else f"k_batched_r<{rpw}>"
+    return "hsh_jit_batched" if jit else "k_batched"
+
+
+def hetero_section(a, D, M, Oracle):
+    """Heterogeneous population: tree sizes {1, 8, 16, 64, 256, 1024} with
+    P proportional to size^-0.67 (mean ~17 compartments), every tree in
+    packs (one kernel), K steps per captured launch; parity against the
+    oracle on a bounded number of steps."""
+    sizes = (1, 8, 16, 64, 256, 1024)
+    p = np.array(sizes, float) ** -0.67
+    p /= p.sum()
+    t0 = time.perf_counter()
+    m = M.population(a.hetero_neurons, sizes, tuple(p), seed=a.seed)
+    t_build = time.perf_counter() - t0
+    n = m.ncompts
+    opts = dict(batch_k=32, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=1024, pack_waves=8, pack_stage=2)
+    print(f"[bench] hetero: {a.hetero_neurons} neurons, {n} compartments (built {t_build:.1f}s)",
+          file=sys.stderr, flush=True)
+    res = {"workload": f"{a.hetero_neurons} random Hines trees, sizes {list(sizes)} with P ~ size^-0.67",
+           "compartments": n, "steps": a.hetero_steps, "layout": "packs of ~1024 compartments, level-scheduled "
+           "elimination, 8 waves, schedule in LDS, K=32"}
+    with D.HipSolver(m, **opts) as g:
+        info = g.info()
+        g.step(64, D.BATCHED)
+        g.sync()
+        g.set_timing(True)
+        t0 = time.perf_counter()
+        g.step(a.hetero_steps, D.BATCHED)
+        g.sync()
+        wall = time.perf_counter() - t0
+        dev_ms = g.last_time(D.T_TOTAL)
+        g.set_timing(False)
+        res["batched"] = {"value": a.hetero_steps * n / wall, "unit": "compartment-steps/s",
+                          "ms_per_step": wall * 1e3 / a.hetero_steps,
+                          "hbm_frac_algorithmic": info["bytes_per_step"] * a.hetero_steps / (dev_ms * 1e-3) / 1e9
+                          / HBM_PEAK_GBS,
+                          "packs": info["npacks"], "lane_fill": info["lane_fill"]}
+    with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
+        g.step(16, D.PLAIN)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(200, D.PLAIN)
+        g.sync()
+        wall = time.perf_counter() - t0
+        res["plain"] = {"value": 200 * n / wall, "unit": "compartment-steps/s", "ms_per_step": wall * 1e3 / 200}
+    if not a.no_cpu:
+        S = 40
+        o = Oracle(m, givals=False)
+        t0 = time.perf_counter()
+        o.step(S)
+        cpu = time.perf_counter() - t0
+        ref = o.vm
+        with D.HipSolver(m, **opts) as g:
+            g.step(S, D.BATCHED)
+            v = g.vm
+        res["parity"] = {"steps": S, "max_abs_dv_V": float(np.max(np.abs(v - ref))),
+                         "bitexact": bool(np.array_equal(v, ref))}
+        res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{S} steps of the population ({cpu:.1f} s)"}
+    return res
+
+
+def adjoint_section(a, torch, Oracle, TangentLinear):
+    """Adjoint parameter fit: 64-compartment random trees, soma-Vm MSE over
+    the run against CPU-solver targets at hidden per-neuron gbar/Ra/Cm
+    (+-30%), checkpoint every 100 steps; gradient checked against the CPU
+    tangent-linear model and central differences."""
+    from hsolve_hip import fit as F
+    N, T = a.adj_neurons, a.adj_steps
+    t0 = time.perf_counter()
+    pm = F.ParamMap(N, (64,), (1.0,), seed=a.seed)
+    rng = np.random.default_rng(a.seed + 1)
+    th_star = 1 + 0.3 * rng.uniform(-1, 1, (N, 5))
+    tgt_np, _ = TangentLinear(pm.model_at(th_star)).run(T, pm.soma)
+    tgt = torch.from_numpy(tgt_np)
+    print(f"[bench] adjoint: {N} x 64 compartments, {T} steps (setup + targets {time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    run = F.Runner(pm, T, ckpt=100)
+    n = pm.base.ncompts
+    theta = torch.ones(N, 5, dtype=torch.float64, requires_grad=True)
+    run(theta.detach())                     # warm (JIT, layout)
+    t0 = time.perf_counter()
+    tr = run(theta)
+    t_fwd = time.perf_counter() - t0
+    loss = torch.mean((tr - tgt) ** 2)
+    t0 = time.perf_counter()
+    loss.backward()
+    t_bwd = time.perf_counter() - t0
+    g = theta.grad.numpy()
+    w = (2 * (tr.detach() - tgt) / tr.numel()).numpy()
+    checks = []
+    # per parameter type: the neuron with the largest gradient component (the
+    # most spike-sensitive, the hardest case for difference quotients) and one
+    # drawn at random
+    pick_rng = np.random.default_rng(a.seed + 2)
+    picks = [(int(np.argmax(np.abs(g[:, j]))), j) for j in range(5)]
+    picks += [(int(pick_rng.integers(N)), j) for j in range(5)]
+    for (nn, j) in picks:
+        e = torch.zeros(N, 5, dtype=torch.float64)
+        e[nn, j] = 1
+        _, (dc, dr) = torch.func.jvp(pm.arrays, (torch.ones(N, 5, dtype=torch.float64),), (e,))
+        _, dtr = TangentLinear(pm.base, dchip=dc.numpy(), dravals=dr.numpy()).run(T, pm.soma)
+        g_tl = float(np.sum(w * dtr))
+        fd = {}
+        for h in (1e-6, 1e-8):
+            with torch.no_grad():
+                tp = torch.ones(N, 5, dtype=torch.float64)
+                tp[nn, j] += h
+                tm = torch.ones(N, 5, dtype=torch.float64)
+                tm[nn, j] -= h
+                fd[h] = (torch.mean((run(tp) - tgt) ** 2) - torch.mean((run(tm) - tgt) ** 2)).item() / (2 * h)
+        L0 = loss.item()
+        floor = {h: 10 * np.finfo(float).eps * L0 / h for h in fd}
+        checks.append({"param": f"{F.PARAMS[j]}[{nn}]", "adjoint": float(g[nn, j]), "tangent_linear": g_tl,
+                       "rel_vs_tl": abs(g[nn, j] - g_tl) / abs(g_tl),
+                       "rel_vs_fd_h1e-6": abs(g[nn, j] - fd[1e-6]) / abs(g[nn, j]),
+                       "rel_vs_fd_h1e-8": abs(g[nn, j] - fd[1e-8]) / abs(g[nn, j]),
+                       "fd_rounding_floor_rel": {f"h{h:.0e}": floor[h] / abs(g[nn, j]) for h in fd}})
+    run.close()
+    return {"workload": f"{N} random Hines trees x 64 compartments, {T} steps, dt 20 us, fp64, checkpoint "
+                        f"every 100 steps; loss = soma-Vm MSE vs CPU targets at hidden +-30% gNa/gK/gKdr/Ra/Cm",
+            "parameters": N * 5, "loss": loss.item(),
+            "forward_s": t_fwd, "backward_s": t_bwd,
+            "forward_cs_per_s": T * n / t_fwd, "backward_cs_per_s": T * n / t_bwd,
+            "gradient_checks": checks,
+            "max_rel_vs_tl": max(c["rel_vs_tl"] for c in checks),
+            "note": "central differences at h=1e-6 move spike upstrokes of gNa/Cm across gate-table intervals "
+                    "(piecewise-linear rates); at h=1e-8 they measure the tabulated model the adjoint "
+                    "differentiates"}
 
 
 def main():
@@ -159,7 +315,9 @@ def main():
     if a.traffic_probe:
         copy_calibration(torch, CAL_MB)
         m = M.population(a.neurons, (a.ncomp,), (1.0,), seed=a.seed)
-        with D.HipSolver(m, batch_k=a.batch_k, flags=a.flags) as g:
+        tmr, cap, wv, st = (int(x) for x in a.probe_layout.split(","))
+        with D.HipSolver(m, batch_k=a.batch_k, flags=a.flags, tile_max_rows=tmr, pack_cap=cap, pack_waves=wv,
+                         pack_stage=st) as g:
             g.step(4 * g.info()["batch_k"], D.BATCHED)
             g.sync()
         print(json.dumps({"traffic_probe": "done"}))
@@ -174,12 +332,12 @@ def main():
 
     # ---- batched path (value)
     gb = D.HipSolver(m, batch_k=a.batch_k, flags=a.flags)
-    info = gb.info()
-    K = info["batch_k"]
     t_ready = time.perf_counter() - t_e2e
-    gb.step(a.warmup, D.BATCHED)
+    gb.step(a.warmup, D.BATCHED)          # the first batched step runs the autotuner
     gb.sync()
     torch.cuda.synchronize()
+    info = gb.info()
+    K = info["batch_k"]
     gb.set_timing(True)
     t0 = time.perf_counter()
     r = gb.step(a.steps, D.BATCHED)
@@ -236,13 +394,15 @@ def main():
                                "(HH soma Na/K, dendrite leak+Kdr), fp64, dt 20 us, BEULER",
                    "neurons": a.neurons, "compartments_per_neuron": a.ncomp, "compartments": n,
                    "batch_k": K, "batch_waves": info["batch_waves"], "tiles": info["ntiles"],
+                   "packs": info["npacks"], "autotuned": bool(info["autotuned"]),
+                   "autotune_s": info["autotune_s"],
                    "path": "batched persistent kernel in a captured hipGraph",
                    "specialised_kernels": bool(info["jit"]), "jit_compile_s": info["jit_compile_s"]},
         "plain": {"value": plain_value, "ms_per_step": wall_p * 1e3 / a.steps,
                   "channel_kernel_us": ch_ms * 1e3, "elim_kernel_us": el_ms * 1e3,
                   "batched_over_plain": value / plain_value},
         "roofline": {"bound": "hbm",
-                     "kernel": ("hsh_jit_batched_r4" if info["jit"] else "k_batched_r<4>"),
+                     "kernel": batched_kernel_name(info),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "algorithmic_bytes_per_compartment_step": bytes_step / n,
@@ -254,7 +414,7 @@ def main():
     gb.close()
     gp.close()
     if not a.no_traffic:
-        t, why = measure_traffic(a, K)
+        t, why = measure_traffic(a, K, info, out["roofline"]["kernel"])
         if t is None:
             print(f"[bench] traffic not measured: {why}", file=sys.stderr, flush=True)
             out["roofline"]["traffic_note"] = why
@@ -311,6 +471,12 @@ def main():
                          "bitexact_batched": par["batched_bitexact"],
                          "bitexact_plain": par["plain_bitexact"], "tolerance_V": 1e-7}
         out["speedup_vs_cpu_1core"] = value / cpu1_rate
+    if not a.no_hetero:
+        from oracle import Oracle as _O
+        out["heterogeneous"] = hetero_section(a, D, M, _O)
+    if not a.no_adjoint:
+        from oracle import Oracle as _O, TangentLinear as _TL
+        out["adjoint"] = adjoint_section(a, torch, _O, _TL)
     print(json.dumps(out))
 
 

@@ -23,13 +23,11 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   recursion, steady state at rest, and hand-computed single-op cases.
 * elimination: brute-force dense solves of the same matrix on random tiny
   trees (numpy.linalg.solve), Crank-Nicolson identity V = 2x - V_old.
-* SPIKE_OP counting: hand-computed refractory sequences.
-* SYN2_OP decay/event arithmetic: closed-form impulse response.
 * tangent-linear model: central finite differences of the oracle itself
   (tests/test_oracle_tl.py); the closed-form derivative of the passive
   isopotential recursion.
-* GHK / NERNST / MGBLOCK / conc ops: parity unpinned against the reference
-  (no reference test reaches them; libm exp/log); checked only against
-  the formulas at single points.
+* parity unpinned (no reference test reaches them): GHK / NERNST / MGBLOCK /
+  conc ops (libm exp/log), SYN3 (urandom), SPIKE_OP and SYN2 event
+  arithmetic.
 """
 from .oracle import Oracle, TangentLinear, oracle_lib  # noqa: F401
