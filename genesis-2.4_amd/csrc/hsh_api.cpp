@@ -77,7 +77,14 @@ struct hsh_handle {
     std::vector<double> chip;
     // model-constant device buffers
     int32_t* d_prog = nullptr;
-    double *d_tab = nullptr, *d_xvals = nullptr;
+    double *d_tab = nullptr, *d_xvals = nullptr, *d_stab = nullptr;
+    // synaptic connections (hsh_set_synapses); rings are indexed by synchan id
+    double* d_ring = nullptr;
+    int32_t *d_out_off = nullptr, *d_out_dst = nullptr, *d_out_dly = nullptr;
+    double* d_out_w = nullptr;
+    int ring_len = 0;                  // 0: no connections
+    int max_k = 1 << 30;               // batched steps per launch allowed (min delay)
+    int nconn = 0;
     int32_t *d_spk = nullptr, *d_nspk = nullptr;
     // layout-dependent device buffers
     BlockMeta* d_blocks = nullptr;
@@ -271,11 +278,14 @@ void free_layout(hsh_handle* h)
 void free_device(hsh_handle* h)
 {
     free_layout(h);
-    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_spk, h->d_nspk};
+    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_stab, h->d_spk, h->d_nspk, h->d_ring, h->d_out_off,
+                    h->d_out_dst, h->d_out_dly, h->d_out_w};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_prog = nullptr;
-    h->d_tab = h->d_xvals = nullptr;
+    h->d_tab = h->d_xvals = h->d_stab = nullptr;
+    h->d_ring = h->d_out_w = nullptr;
+    h->d_out_off = h->d_out_dst = h->d_out_dly = nullptr;
     h->d_spk = h->d_nspk = nullptr;
     jit_free(h->jit);
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
@@ -346,6 +356,15 @@ void set_devmodel(hsh_handle* h)
     d.bkp = h->d_bkp;
     d.btcb = h->d_btcb;
     d.npacks = (int32_t)P.packs.size();
+    d.stab = h->d_stab;
+    d.ring = h->d_ring;
+    d.out_off = h->d_out_off;
+    d.out_dst = h->d_out_dst;
+    d.out_dly = h->d_out_dly;
+    d.out_w = h->d_out_w;
+    d.ring_mask = h->ring_len > 0 ? h->ring_len - 1 : 0;
+    d.nsrc = h->ring_len > 0 ? (int32_t)P.spike_ops.size() : 0;
+    d.dt = h->m.dt;
     d.spk_cap = SPIKE_CAP;
     d.ncols = h->m.ncols;
     d.xdivs = h->m.xdivs;
@@ -518,7 +537,7 @@ hsh_status batched_launch(hsh_handle* h, int K)
 
 hsh_status run_batched(hsh_handle* h, int nsteps, int* launches)
 {
-    const int K = std::max(1, h->batch_k);
+    const int K = std::max(1, std::min(h->batch_k, h->max_k));
     int full = nsteps / K;
     const int rem = nsteps % K;
     while (full > 0) {
@@ -609,9 +628,13 @@ hsh_status autotune(hsh_handle* h)
     const std::vector<int32_t> ops0 = h->ops;
     int32_t nspk0 = 0;
     HIPCHK(hipMemcpy(&nspk0, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
+    // pending synaptic events: trials consume and add them
+    std::vector<double> ring0((size_t)h->ring_len * h->plan.syn_ops.size());
+    if (!ring0.empty())
+        HIPCHK(hipMemcpy(ring0.data(), h->d_ring, ring0.size() * sizeof(double), hipMemcpyDeviceToHost));
     const hsh_options& o = h->opt;
     const Plan& P = h->plan;
-    const int K0 = o.batch_k > 0 ? o.batch_k : 32;
+    const int K0 = std::min(o.batch_k > 0 ? o.batch_k : 32, h->max_k);
     h->batch_k = K0;
     // steps per trial: about 2e6 compartment-steps, at least 2 launches
     const int trial = std::max(2 * K0, std::min(4096, (int)(2e6 / std::max(1, n)) / K0 * K0));
@@ -672,6 +695,7 @@ hsh_status autotune(hsh_handle* h)
             double bk = 1e300;
             int kbest = K0;
             for (int K : {1, 8, 32, 128}) {
+                if (K > h->max_k) continue;
                 h->batch_k = K;
                 free_graphs(h);
                 double ms = 0;
@@ -691,6 +715,8 @@ hsh_status autotune(hsh_handle* h)
     free_layout(h);
     STCHK(alloc_layout(h, vm0.data(), res0.data()));
     HIPCHK(hipMemcpy(h->d_nspk, &nspk0, sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!ring0.empty())
+        HIPCHK(hipMemcpy(h->d_ring, ring0.data(), ring0.size() * sizeof(double), hipMemcpyHostToDevice));
     h->tuned = true;
     h->autotune_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return HSH_OK;
@@ -719,6 +745,9 @@ bool adj_supported(const Plan& P, std::string& why)
                 ng = 0;
             } else if (op == U_SPIKE) {
                 pc += 1;
+            } else if (op == U_SYN2) {
+                why = "synaptic channels";
+                return false;
             } else if (op == U_END) {
                 break;
             }
@@ -897,6 +926,9 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     const size_t nx = (m->ncols > 0) ? (size_t)(m->xdivs + 2) : 1;
     HIPCHK_C(dalloc(&h->d_tab, ntab));
     HIPCHK_C(dalloc(&h->d_xvals, nx));
+    HIPCHK_C(dalloc(&h->d_stab, (size_t)std::max(1, 4 * m->nstab)));
+    if (m->nstab > 0 && m->stablist)
+        HIPCHK_C(hipMemcpy(h->d_stab, m->stablist, 4 * (size_t)m->nstab * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK_C(dput(h->d_prog, P.prog));
     if (m->ncols > 0) {
         HIPCHK_C(hipMemcpy(h->d_tab, m->tablist, ntab * sizeof(double), hipMemcpyHostToDevice));
@@ -1068,7 +1100,7 @@ hsh_status hsh_info_get(hsh_handle* h, hsh_info* info)
     o.ntemplates = (int32_t)P.templates.size();
     o.ntrees = (int32_t)P.tree_lo.size();
     o.max_rows = P.max_rows;
-    o.batch_k = h->batch_k;
+    o.batch_k = std::min(h->batch_k, h->max_k);
     o.batch_waves = h->batch_waves;
     o.refused = h->refused ? 1 : 0;
     o.bytes_per_step = P.bytes_per_step;
@@ -1133,6 +1165,90 @@ hsh_status hsh_jit_source(const hsh_model* m, char* buf, int64_t cap, int64_t* l
     return HSH_OK;
 }
 
+
+hsh_status hsh_set_synapses(hsh_handle* h, int32_t n, const int32_t* src_op, const int32_t* dst_op,
+                            const double* weight, const double* delay)
+{
+    if (!h) return fail(HSH_ERR_ARG, "hsh_set_synapses: null handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_synapses: handle refused the model");
+    if (n < 0 || (n > 0 && (!src_op || !dst_op || !weight || !delay)))
+        return fail(HSH_ERR_ARG, "hsh_set_synapses: bad arguments");
+    const Plan& P = h->plan;
+    std::map<int32_t, int32_t> src_id, dst_id;
+    for (size_t i = 0; i < P.spike_ops.size(); i++) src_id[P.spike_ops[i]] = (int32_t)i;
+    for (size_t i = 0; i < P.syn_ops.size(); i++) dst_id[P.syn_ops[i]] = (int32_t)i;
+    const double dt = h->m.dt;
+    std::vector<std::vector<int32_t>> by_src(P.spike_ops.size());
+    std::vector<int32_t> cdst(n), cdly(n);
+    std::vector<double> cw(n);
+    int min_d = 1 << 30, max_d = 0;
+    for (int32_t i = 0; i < n; i++) {
+        auto s = src_id.find(src_op[i]);
+        auto t = dst_id.find(dst_op[i]);
+        if (s == src_id.end()) return fail(HSH_ERR_ARG, "hsh_set_synapses: src_op " + std::to_string(src_op[i]) + " is not a SPIKE_OP");
+        if (t == dst_id.end()) return fail(HSH_ERR_ARG, "hsh_set_synapses: dst_op " + std::to_string(dst_op[i]) + " is not a SYN2_OP");
+        // weight and delay are float fields in GENESIS
+        const double w = (double)(float)weight[i];
+        const double dl = (double)(float)delay[i];
+        if (!(dl >= 0.0) || dl / dt > 1e6) return fail(HSH_ERR_ARG, "hsh_set_synapses: delay out of range");
+        // newconn/synchan.c:193-194; an op earlier in ops[] than the spike
+        // has already run this step, so the event reaches it one step later
+        const int steps = (int)(dl / dt + 0.9) + (dst_op[i] < src_op[i] ? 1 : 0);
+        if (steps < 1) return fail(HSH_ERR_ARG, "hsh_set_synapses: a connection shorter than one step");
+        cdst[i] = t->second;
+        cdly[i] = steps;
+        cw[i] = w / dt;
+        min_d = std::min(min_d, steps);
+        max_d = std::max(max_d, steps);
+        by_src[s->second].push_back(i);
+    }
+    if (h->host_only) {
+        h->max_k = n > 0 ? min_d : (1 << 30);
+        h->nconn = n;
+        return HSH_OK;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    void* ptrs[] = {h->d_ring, h->d_out_off, h->d_out_dst, h->d_out_dly, h->d_out_w};
+    for (void* p : ptrs)
+        if (p) HIPCHK(hipFree(p));
+    h->d_ring = h->d_out_w = nullptr;
+    h->d_out_off = h->d_out_dst = h->d_out_dly = nullptr;
+    h->ring_len = 0;
+    h->max_k = 1 << 30;
+    h->nconn = n;
+    if (n > 0) {
+        // arrivals of one launch (at most min_d steps) stay ahead of the
+        // slots that launch reads
+        int R = 1;
+        while (R < max_d + min_d + 1) R <<= 1;
+        std::vector<int32_t> off(P.spike_ops.size() + 1, 0), odst, odly;
+        std::vector<double> ow;
+        for (size_t sid = 0; sid < by_src.size(); sid++) {
+            for (int32_t i : by_src[sid]) {
+                odst.push_back(cdst[i]);
+                odly.push_back(cdly[i]);
+                ow.push_back(cw[i]);
+            }
+            off[sid + 1] = (int32_t)odst.size();
+        }
+        const size_t nring = (size_t)R * std::max<size_t>(1, P.syn_ops.size());
+        HIPCHK(dalloc(&h->d_ring, nring));
+        HIPCHK(hipMemset(h->d_ring, 0, nring * sizeof(double)));
+        HIPCHK(dalloc(&h->d_out_off, off.size()));
+        HIPCHK(dalloc(&h->d_out_dst, odst.size()));
+        HIPCHK(dalloc(&h->d_out_dly, odly.size()));
+        HIPCHK(dalloc(&h->d_out_w, ow.size()));
+        HIPCHK(dput(h->d_out_off, off));
+        HIPCHK(dput(h->d_out_dst, odst));
+        HIPCHK(dput(h->d_out_dly, odly));
+        HIPCHK(dput(h->d_out_w, ow));
+        h->ring_len = R;
+        h->max_k = min_d;
+    }
+    free_graphs(h);
+    set_devmodel(h);
+    return HSH_OK;
+}
 
 hsh_status hsh_set_ravals(hsh_handle* h, const double* ravals)
 {

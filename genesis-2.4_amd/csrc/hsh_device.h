@@ -85,7 +85,11 @@ __device__ __forceinline__ double apply_power(double Gk, double X, int k)
 }
 
 // SPIKE_OP (hines_chip.c:266-276): counter-1 every step; when Vm > thresh and
-// the counter is spent, record an event and reload the counter.
+// the counter is spent, record an event, deliver it to the spike's synaptic
+// connections and reload the counter.  Delivery adds weight/dt to the target
+// synchan's activation of the arrival step (its ring slot); every connection
+// is at least one step long, so no synchan reads a slot another wave is
+// adding to in the same step.
 __device__ __forceinline__ void spike_op(const DevModel& d, int* __restrict__ Ib, int is, int lane,
                                          double Vm, double thresh, bool active, int step)
 {
@@ -98,11 +102,43 @@ __device__ __forceinline__ void spike_op(const DevModel& d, int* __restrict__ Ib
                     d.spk[2 * q] = Ib[(is + 2) * W + lane];
                     d.spk[2 * q + 1] = step;
                 }
+                const int src = Ib[(is + 3) * W + lane];
+                if (d.ring_mask != 0 && src >= 0 && src < d.nsrc) {
+                    const long long rl = (long long)d.ring_mask + 1;
+                    for (int j = d.out_off[src]; j < d.out_off[src + 1]; j++)
+                        atomicAdd(d.ring + (long long)d.out_dst[j] * rl + ((step + d.out_dly[j]) & d.ring_mask),
+                                  d.out_w[j]);
+                }
             }
             cnt = Ib[(is + 1) * W + lane];
         }
     }
     Ib[is * W + lane] = cnt;
+}
+
+// SYN2_OP (hines_chip.c:511-550): X *= xconst2; an event arriving this step
+// adds dt * activation * stablist[k+3] (h_dosynchan, hines_conn.c:60-89);
+// the op counter decrements; Y = X*yconst1 + Y*yconst2.  Returns Y (the
+// caller multiplies it into Gk and adds the current).  Padding lanes repeat
+// lane 0's synchan and must not take its events.
+__device__ __forceinline__ double syn2_op(const DevModel& d, double* __restrict__ Sb, int* __restrict__ Ib, int c,
+                                          int k, int is, int lane, double X0, double Y0, bool active, int step)
+{
+    double X = X0 * d.stab[k];
+    if (d.ring_mask != 0 && active) {
+        const int sid = Ib[(is + 1) * W + lane];
+        double* slot = d.ring + (long long)sid * ((long long)d.ring_mask + 1) + (step & d.ring_mask);
+        const double act = *slot;
+        if (act != 0.0) {
+            X += d.dt * act * d.stab[k + 3];
+            *slot = 0.0;
+        }
+    }
+    Ib[is * W + lane] -= 1;
+    Sb[c * W + lane] = X;
+    const double Y = X * d.stab[k + 1] + Y0 * d.stab[k + 2];
+    Sb[(c + 1) * W + lane] = Y;
+    return Y;
 }
 
 // ---------------------------------------------------------------------------
@@ -169,6 +205,13 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             const int is = code[pc++];
             spike_op(d, Ib, is, lane, Vm, SGET(c), active, step);
             c += 1;
+        } else if (op == U_SYN2) {
+            const int k = code[pc], is = code[pc + 1];
+            pc += 2;
+            Gk *= syn2_op(d, Sb, Ib, c, k, is, lane, SGET(c), SGET(c + 1), active, step);
+            c += 2;
+            sumgchan += Gk;
+            ichan += Ek * Gk;
         } else {  // U_END
             const double tbyc = SGET(c);
             CompOut o;

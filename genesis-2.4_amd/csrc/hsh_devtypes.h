@@ -20,7 +20,9 @@ enum UOp : int {
     U_IPOL1V = 4,     // operands: column, power (+/-1..6); gate state S[c++]
     U_ADDCURR = 5,    // sumg += Gk; ichan += Ek*Gk
     U_SPIKE = 6,      // operand: islot; threshold S[c++]; counter I[islot], reload
-                      // I[islot+1], ops index I[islot+2]
+                      // I[islot+1], ops index I[islot+2], spike source I[islot+3]
+    U_SYN2 = 7,       // operands: stab index k, islot; X = S[c], Y = S[c+1];
+                      // counter I[islot], synchan id I[islot+1]; adds the current
 };
 constexpr int PROG_HEADER = 2;
 
@@ -92,6 +94,16 @@ struct DevModel {
     const int* bkp;
     const double* btcb;
     int npacks;
+    // synaptic events
+    const double* stab;   // stablist: xconst2, yconst1, yconst2, xconst1(/dt) per type
+    double* ring;         // per synchan: activation of steps (step & ring_mask)
+    const int* out_off;   // per spike source: first outgoing connection
+    const int* out_dst;   // connection: target synchan id
+    const int* out_dly;   // connection: delay in steps
+    const double* out_w;  // connection: weight / dt
+    int ring_mask;        // ring length - 1 (0: no connections)
+    int nsrc;             // spike sources with a connection table
+    double dt;
     int spk_cap;
     int ncols;
     int xdivs;

@@ -97,6 +97,8 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 isrc.push_back(p);                   // counter (mutable)
                 isrc.push_back(p + 1);               // reload
                 isrc.push_back(-(1 + opidx));        // ops index, literal
+                isrc.push_back(-(1 + (int64_t)P.spike_ops.size()));   // spike source id, literal
+                P.spike_ops.push_back((int32_t)opidx);
                 dsrc.push_back(c);                   // threshold
                 c += 1;
                 p += 2;
@@ -196,11 +198,37 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     ngates++;
                     continue;
                 } else if (k == SYN2_OP || k == SYN3_OP) {
-                    if (k == SYN3_OP) c += 1;
-                    p += 3;
+                    // [SYN2_OP, stablist index, event counter, child] (hines_chip.c:511-550)
+                    if (p + 3 > m.nops) return bad("SYN2_OP operands");
+                    const int32_t st = ops[p];
+                    const int64_t cntidx = p + 1;
+                    const bool mg = (p + 3 < m.nops && ops[p + 3] == MGBLOCK_OP);
+                    if (k == SYN3_OP || mg) {
+                        // SYN3 draws urandom(); the Mg block needs exp() of Vm
+                        if (k == SYN3_OP) c += 1;
+                        p += 3;
+                        c += 2;
+                        if (mg) { p += 1; c += 3; }
+                        refuse(P, comp, mg ? MGBLOCK_OP : k, (int)kidx, HSH_R_OPCODE); refused = true;
+                        break;
+                    }
+                    if (!need_chip(2)) return bad("chip[] too short");
+                    if (st < 0 || (int64_t)st + 3 >= 4 * (int64_t)m.nstab || !m.stablist)
+                        return bad("SYN2_OP stablist index out of range");
+                    if (ops[cntidx] >= 0) {
+                        refuse(P, comp, k, (int)kidx, HSH_R_SYN_PENDING); refused = true;
+                    }
+                    code.push_back(U_SYN2);
+                    code.push_back(st);
+                    code.push_back((int32_t)isrc.size());
+                    isrc.push_back(cntidx);                              // counter (mutable)
+                    isrc.push_back(-(1 + (int64_t)P.syn_ops.size()));    // synchan id, literal
+                    P.syn_ops.push_back((int32_t)kidx);
+                    dsrc.push_back(c);          // X
+                    dsrc.push_back(c + 1);      // Y
                     c += 2;
-                    if (p < m.nops && ops[p] == MGBLOCK_OP) { p += 1; c += 3; }
-                    refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
+                    p += 3;
+                    ngates += 2;
                     break;
                 } else if (k == ADD_CURR_OP) {
                     code.push_back(U_ADDCURR);

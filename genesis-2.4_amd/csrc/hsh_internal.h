@@ -62,6 +62,8 @@ struct Plan {
     std::vector<int32_t> tree_lo;         // per tree (neuron): first row; root = hi
     std::vector<int32_t> tree_hi;
     std::vector<int32_t> pred;            // previous NEWVOLT compartment (global row), -1
+    std::vector<int32_t> spike_ops;       // spike source id -> ops index of its SPIKE_OP
+    std::vector<int32_t> syn_ops;         // synchan id -> ops index of its SYN2_OP
     std::vector<int32_t> prog;            // programs of all templates
     std::vector<int32_t> tprog;           // template -> offset in prog
     int max_tree_rows = 0;

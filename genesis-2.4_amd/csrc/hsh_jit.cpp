@@ -83,6 +83,15 @@ std::string gen_template(const Template& t, int id, int ncols)
             c += 1;
             break;
         }
+        case U_SYN2: {
+            const int k = code[pc], is = code[pc + 1];
+            pc += 2;
+            o += "    Gk *= syn2_op(d, Sb, Ib, " + std::to_string(c) + ", " + std::to_string(k) + ", " +
+                 std::to_string(is) + ", lane, " + slot(c) + ", " + slot(c + 1) +
+                 ", active, step); sumgchan += Gk; ichan += Ek * Gk;\n";
+            c += 2;
+            break;
+        }
         default:   // U_END
             o += "    { CompOut o; o.rhs = Vm + ichan * " + slot(c) + "; o.diag = sumgchan * " + slot(c) +
                  " + " + slot(c + 1) + "; return o; }\n";

@@ -28,7 +28,7 @@ F_HOST_ONLY = 2
 F_NO_JIT = 4
 
 REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
-           5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode"}
+           5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode", 8: "pending synaptic event"}
 
 
 class HshModel(C.Structure):
@@ -73,7 +73,8 @@ class HshInfo(C.Structure):
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
            "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
-           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_ravals", "hsh_adj_forward",
+           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_set_ravals",
+           "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
 
 _lib = None
@@ -104,6 +105,8 @@ def lib():
         L.hsh_info_get.restype = C.c_int32
         L.hsh_autotune.argtypes = [C.c_void_p]
         L.hsh_autotune.restype = C.c_int32
+        L.hsh_set_synapses.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, _f64p]
+        L.hsh_set_synapses.restype = C.c_int32
         L.hsh_set_ravals.argtypes = [C.c_void_p, _f64p]
         L.hsh_set_ravals.restype = C.c_int32
         L.hsh_adj_forward.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _f64p]
@@ -238,6 +241,14 @@ class HipSolver:
         i = HshInfo()
         _check(lib().hsh_info_get(self.h, C.byref(i)), "hsh_info_get")
         return {f: getattr(i, f) for f, _ in HshInfo._fields_}
+
+    def set_synapses(self, src_op, dst_op, weight, delay):
+        """Synaptic connections delivered on the device (hsh_set_synapses):
+        SPIKE_OP ops index -> SYN2_OP ops index, weight, delay (s)."""
+        a = [np.ascontiguousarray(x, t) for x, t in ((src_op, np.int32), (dst_op, np.int32),
+                                                       (weight, np.float64), (delay, np.float64))]
+        _check(lib().hsh_set_synapses(self.h, a[0].size, _p(a[0], C.c_int32), _p(a[1], C.c_int32),
+                                      _p(a[2], C.c_double), _p(a[3], C.c_double)), "hsh_set_synapses")
 
     def set_ravals(self, ravals):
         r = np.ascontiguousarray(ravals, np.float64)

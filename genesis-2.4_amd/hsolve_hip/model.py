@@ -302,6 +302,53 @@ def isopotential_hh(n: int, *, seed: int = 0, dt: float = 20e-6, inject=(0.1e-9,
     return population(n, (1,), (1.0,), seed=seed, dt=dt, inject=inject)
 
 
+def network(n_cells: int = 4000, *, seed: int = 0, dt: float = 1e-4, frac_exc: float = 0.8,
+            p_conn: float = 0.02, g_exc: float = 6e-9, g_inh: float = 67e-9, tau_exc=(5e-3, 1e-3),
+            tau_inh=(10e-3, 1e-3), E_exc: float = 0.0, E_inh: float = -0.08, delay: float = 1.5e-3,
+            vm0=(-0.070, -0.050), inject=(0.0, 0.0), thresh: float = 0.0, abs_refract: float = 5e-3):
+    """Event-driven network of isopotential HH cells (30x30 um soma, Na/K
+    1200/360 S/m^2, Em -65 mV), one spikegen and two synchans (excitatory,
+    inhibitory; dual exponential, as the reference's VAnet2 synchans) per cell.
+    The first frac_exc of the cells are excitatory; each ordered pair (i, j),
+    i != j, is connected with probability p_conn to j's synchan of i's kind,
+    weight 1 (gmax carries the strength), delay ``delay``.  Initial Vm uniform
+    in vm0, inject uniform in ``inject``, from ``seed``.
+
+    Returns (model, (src_op, dst_op, weight, delay)) -- the connection table in
+    the form hsh_set_synapses takes."""
+    rng = np.random.default_rng(seed)
+    L = D = 30e-6
+    area = np.pi * D * L
+    n = n_cells
+    comps = dict(parent=[-1] * n, Cm=[0.01 * area] * n, Rm=[0.33 / area] * n, Ra=[1e6] * n,
+                 Em=[-0.065] * n, initVm=list(rng.uniform(vm0[0], vm0[1], n)),
+                 inject=list(rng.uniform(inject[0], inject[1], n)))
+    chans = []
+    for i in range(n):
+        chans.append((i, 0, 1200.0 * area))
+        chans.append((i, 1, 360.0 * area))
+    spikes = [(i, thresh, abs_refract) for i in range(n)]
+    syntypes = [dict(Ek=E_exc, tau1=tau_exc[0], tau2=tau_exc[1]),
+                dict(Ek=E_inh, tau1=tau_inh[0], tau2=tau_inh[1])]
+    syns = []
+    for i in range(n):
+        syns.append((i, 0, g_exc))
+        syns.append((i, 1, g_inh))
+    m = build(comps, hh_chantypes(-0.065), chans, dt=dt, spikes=spikes, syntypes=syntypes, syns=syns)
+    n_exc = int(round(frac_exc * n))
+    src, dst = [], []
+    conn = rng.random((n, n)) < p_conn
+    np.fill_diagonal(conn, False)
+    for i in range(n):
+        js = np.nonzero(conn[i])[0]
+        kind = 0 if i < n_exc else 1
+        src.extend([int(m.spike_op[i])] * js.size)
+        dst.extend(int(m.syn_op[2 * j + kind]) for j in js)
+    k = len(src)
+    table = (np.array(src, np.int32), np.array(dst, np.int32), np.ones(k), np.full(k, delay))
+    return m, table
+
+
 def tabinterp(table: np.ndarray, xdivs: int, xmin: float, invdx: float, x: float) -> float:
     t = np.ascontiguousarray(table, dtype=np.float64)
     return lib().hsm_tabinterp(_p(t, C.c_double), xdivs, xmin, invdx, x)

@@ -118,6 +118,7 @@ typedef struct hsh_options {
                                     (sibling ops of symcompartments) */
 #define HSH_R_TREE           6   /* tree rows are not contiguous / kid order not ascending */
 #define HSH_R_CALCMODE       7   /* calcmode 0 (no interpolation) */
+#define HSH_R_SYN_PENDING    8   /* SYN2_OP counter holds a pending event (>= 0) */
 
 typedef struct hsh_refusal {
     int32_t comp;            /* row (Hines number), -1 for whole-solver reasons */
@@ -187,6 +188,21 @@ hsh_status hsh_info_get(struct hsh_handle *h, hsh_info *info);
  * K on the handle's own model and keeps the fastest.  The simulation state
  * (vm, chip, ops, results, step count, pending spikes) is unchanged. */
 hsh_status hsh_autotune(struct hsh_handle *h);
+/* Synaptic connections, delivered on the device.  Each connection takes a
+ * spike raised by the SPIKE_OP at ops index src_op[i] to the SYN2_OP synchan
+ * at ops index dst_op[i] with weight[i] and delay[i] (s); weight and delay are
+ * rounded to float, as GENESIS stores them (newconn_struct.h:158-159).  A
+ * spike at step s adds weight/dt to the synchan's activation of step
+ * s + (int)(delay/dt + 0.9), one step later when the synchan's op precedes the
+ * spike's op in ops[] -- the arrival the reference's EVENT / HSEVENT / SYN2_OP
+ * counter produce for an event reaching an empty pending list
+ * (newconn/synchan.c:161-275, hines.c:352-357, hines_chip.c:514-528); SYN2_OP
+ * then adds dt*activation*stablist[k+3] to X (hines_conn.c:60-89).  Replaces
+ * the reference's per-spike host messaging (h_dospike_event, h_dosynchan).
+ * Batched launches run at most min(delay steps) steps, so every event a
+ * launch raises lands after it.  n = 0 removes all connections. */
+hsh_status hsh_set_synapses(struct hsh_handle *h, int32_t n, const int32_t *src_op,
+                            const int32_t *dst_op, const double *weight, const double *delay);
 /* Replace the elimination coefficients with ravals[nravals] in the layout of
  * the model's funcs[] program (the values h_funcs_init computes from Cm and
  * Ra, genesis/src/hines/hines_init.c:604-742, :1011-1172); the state is kept. */

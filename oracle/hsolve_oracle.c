@@ -75,6 +75,13 @@ struct ohs {
     void *ctx;
     int32_t trace_on;
     uint64_t *trace;
+    /* synaptic connections (ohs_set_synapses) */
+    int32_t ring_len;          /* 0: none */
+    int32_t *syn_of_op;        /* ops index -> synchan id, -1 */
+    int32_t *conn_off;         /* per ops index of a SPIKE_OP: first connection (CSR over ops indices) */
+    int32_t *conn_dst, *conn_dly;
+    double *conn_w;
+    double *ring;              /* per synchan id: ring_len activations */
 };
 
 static void *dupmem(const void *p, size_t n)
@@ -116,6 +123,8 @@ void ohs_destroy(ohs *h)
     free(h->ops); free(h->chip); free(h->vm); free(h->results); free(h->ravals);
     free(h->givals); free(h->conc); free(h->flux); free(h->codiag); free(h->itotal);
     free(h->spike_op); free(h->spike_step); free(h->trace);
+    free(h->syn_of_op); free(h->conn_off); free(h->conn_dst); free(h->conn_dly); free(h->conn_w);
+    free(h->ring);
     free(h);
 }
 
@@ -147,6 +156,70 @@ static void record_spike(ohs *h, int32_t opidx)
     h->spike_op[h->nspk] = opidx;
     h->spike_step[h->nspk] = h->step;
     h->nspk++;
+}
+
+/* A spike of the SPIKE_OP at ops index op adds weight/dt to the activation
+ * of each connected synchan at its arrival step, in connection order
+ * (newconn/synchan.c:340-351 sums weight/dt over the events of a step). */
+static void deliver(ohs *h, int32_t op)
+{
+    const int32_t mask = h->ring_len - 1;
+    for (int32_t j = h->conn_off[op]; j < h->conn_off[op + 1]; j++) {
+        double *slot = h->ring + (size_t)h->conn_dst[j] * h->ring_len + ((h->step + h->conn_dly[j]) & mask);
+        *slot += h->conn_w[j];
+    }
+}
+
+int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t *dst_op,
+                         const double *weight, const double *delay)
+{
+    const int32_t nops = h->a.nops;
+    const double dt = h->a.dt;
+    free(h->syn_of_op); free(h->conn_off); free(h->conn_dst); free(h->conn_dly); free(h->conn_w);
+    free(h->ring);
+    h->syn_of_op = NULL; h->conn_off = NULL; h->conn_dst = h->conn_dly = NULL; h->conn_w = NULL;
+    h->ring = NULL;
+    h->ring_len = 0;
+    if (n <= 0) return 0;
+    h->syn_of_op = (int32_t *)malloc(sizeof(int32_t) * nops);
+    int32_t nsyn = 0;
+    for (int32_t i = 0; i < nops; i++) h->syn_of_op[i] = -1;
+    int32_t mind = 1 << 30, maxd = 0;
+    int32_t *dly = (int32_t *)malloc(sizeof(int32_t) * n);
+    for (int32_t i = 0; i < n; i++) {
+        if (src_op[i] < 0 || src_op[i] >= nops || h->ops[src_op[i]] != SPIKE_OP) { free(dly); return -1; }
+        if (dst_op[i] < 0 || dst_op[i] >= nops || h->ops[dst_op[i]] != SYN2_OP) { free(dly); return -1; }
+        if (h->syn_of_op[dst_op[i]] < 0) h->syn_of_op[dst_op[i]] = nsyn++;
+        /* weight and delay are float fields of the synapse; arrival
+         * (int)(delay/dt + 0.9) steps later (newconn/synchan.c:193-194), one
+         * more when the synchan's op comes before the spike's in ops[] */
+        const double dl = (double)(float)delay[i];
+        dly[i] = (int32_t)(dl / dt + 0.9) + (dst_op[i] < src_op[i] ? 1 : 0);
+        if (dly[i] < 1) { free(dly); return -1; }
+        if (dly[i] < mind) mind = dly[i];
+        if (dly[i] > maxd) maxd = dly[i];
+    }
+    int32_t R = 1;
+    while (R < maxd + mind + 1) R <<= 1;
+    h->conn_off = (int32_t *)calloc((size_t)nops + 1, sizeof(int32_t));
+    h->conn_dst = (int32_t *)malloc(sizeof(int32_t) * n);
+    h->conn_dly = (int32_t *)malloc(sizeof(int32_t) * n);
+    h->conn_w = (double *)malloc(sizeof(double) * n);
+    for (int32_t i = 0; i < n; i++) h->conn_off[src_op[i] + 1]++;
+    for (int32_t i = 0; i < nops; i++) h->conn_off[i + 1] += h->conn_off[i];
+    int32_t *cur = (int32_t *)malloc(sizeof(int32_t) * ((size_t)nops + 1));
+    memcpy(cur, h->conn_off, sizeof(int32_t) * ((size_t)nops + 1));
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t q = cur[src_op[i]]++;
+        h->conn_dst[q] = h->syn_of_op[dst_op[i]];
+        h->conn_dly[q] = dly[i];
+        h->conn_w[q] = (double)(float)weight[i] / dt;
+    }
+    free(cur);
+    free(dly);
+    h->ring = (double *)calloc((size_t)R * (nsyn > 0 ? nsyn : 1), sizeof(double));
+    h->ring_len = R;
+    return mind;
 }
 
 /* do_chip_hh4_update with SOLVEINTERPOL (hines_chip.c:78-602).  The op
@@ -259,6 +332,7 @@ int32_t ohs_chip_update(ohs *h)
                 if (Vm > chip[c++]) {
                     if (ops[p] <= 0) {
                         record_spike(h, (int32_t)p - 1);
+                        if (h->ring_len > 0) deliver(h, (int32_t)p - 1);
                         ops[p] = ops[p + 1];
                     }
                 }
@@ -401,6 +475,7 @@ int32_t ohs_chip_update(ohs *h)
                         Gk *= X;
                     }
                 } else if (k >= SYN2_OP && k <= SYN3_OP) {
+                    const size_t synop = p - 1;
                     if (k == SYN3_OP) {
                         double u = h->rnd ? h->rnd(h->ctx) : 1.0;
                         if (u <= chip[c++]) chip[c] += stablist[ops[p] + 3];
@@ -408,6 +483,17 @@ int32_t ohs_chip_update(ohs *h)
                     k = ops[p++];
                     if (h->trace_on) tr = fnv(tr, k);
                     chip[c] *= stablist[k];
+                    if (h->ring_len > 0) {
+                        /* events from the connection table (ohs_set_synapses) */
+                        const int32_t sid = h->syn_of_op[synop];
+                        if (sid >= 0) {
+                            double *slot = h->ring + (size_t)sid * h->ring_len + (h->step & (h->ring_len - 1));
+                            if (*slot != 0.0) {
+                                chip[c] += A->dt * *slot * stablist[k + 3];
+                                *slot = 0.0;
+                            }
+                        }
+                    }
                     if (ops[p] == 0) {
                         int32_t nxt = -1;
                         if (h->syn) nxt = h->syn(h->ctx, k, ops[p + 1], (int32_t)c, h->step);

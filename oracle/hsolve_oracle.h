@@ -85,6 +85,17 @@ void    ohs_trace(ohs *h, uint64_t *per_comp);
  * Used for the multi-core CPU baseline. */
 int32_t ohs_step_parallel(ohs **hs, int32_t nh, int32_t nsteps, int32_t nthreads);
 
+/* Synaptic connections: a spike of the SPIKE_OP at ops index src_op[i] adds
+ * (float)weight[i]/dt to the activation of the SYN2_OP synchan at dst_op[i]
+ * at step s + (int)((float)delay[i]/dt + 0.9) (+1 when dst_op < src_op), and
+ * SYN2_OP adds dt*activation*stablist[k+3] to X at that step -- the arrival
+ * of an event that reaches an empty pending list in the reference
+ * (newconn/synchan.c:161-275, hines.c:352-357, hines_chip.c:514-528,
+ * hines_conn.c:60-89).  Returns the shortest connection in steps, or -1 on
+ * bad arguments. */
+int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t *dst_op,
+                         const double *weight, const double *delay);
+
 /* Tangent-linear model (hsolve_tl.c): the same step carrying the derivative
  * of every value along one direction of (vm0, chip, ravals).  Tabchannel
  * subset only; ohs_tl_create returns NULL for other ops. */

@@ -63,6 +63,8 @@ def oracle_lib():
         L.ohs_trace.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
         L.ohs_step_parallel.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32]
         L.ohs_step_parallel.restype = C.c_int32
+        L.ohs_set_synapses.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, _f64p]
+        L.ohs_set_synapses.restype = C.c_int32
         L.ohs_tl_create.argtypes = [C.POINTER(Arrays)]
         L.ohs_tl_create.restype = C.c_void_p
         L.ohs_tl_destroy.argtypes = [C.c_void_p]
@@ -181,6 +183,18 @@ class Oracle:
             return _p(a, ct)
         oracle_lib().ohs_put(self.h, pp(vm, np.float64, C.c_double), pp(chip, np.float64, C.c_double),
                              pp(ops, np.int32, C.c_int32))
+
+    def set_synapses(self, src_op, dst_op, weight, delay):
+        """Connection table (see ohs_set_synapses); returns the shortest
+        connection in steps."""
+        a = [np.ascontiguousarray(x, t) for x, t in ((src_op, np.int32), (dst_op, np.int32),
+                                                       (weight, np.float64), (delay, np.float64))]
+        self._syn = a
+        r = oracle_lib().ohs_set_synapses(self.h, a[0].size, _p(a[0], C.c_int32), _p(a[1], C.c_int32),
+                                          _p(a[2], C.c_double), _p(a[3], C.c_double))
+        if r < 0:
+            raise ValueError("oracle: bad synaptic connection")
+        return r
 
     def spikes(self):
         L = oracle_lib()

@@ -91,18 +91,40 @@ def test_rejects_bad_options():
             D.HipSolver(m, host_only=True, **kw)
 
 
-def test_refuses_synchan_with_reason():
+def synchan_model():
     comps = dict(parent=[-1, -1], Cm=[1e-11] * 2, Rm=[1e8] * 2, Ra=[1e7] * 2, Em=[-0.065] * 2,
                  initVm=[-0.065] * 2, inject=[0.0] * 2)
-    m = M.build(comps, [], [], dt=1e-4, syntypes=[dict(Ek=0.0, tau1=5e-3, tau2=1e-3)],
-                syns=[(1, 0, 1e-9)])
+    return M.build(comps, [], [], dt=1e-4, spikes=[(0, 0.0, 5e-3)],
+                   syntypes=[dict(Ek=0.0, tau1=5e-3, tau2=1e-3)], syns=[(1, 0, 1e-9)])
+
+
+def test_accepts_synchan_and_checks_connections():
+    m = synchan_model()
+    with D.HipSolver(m, host_only=True) as h:
+        assert h.refusals() == []
+        src, dst = int(m.spike_op[0]), int(m.syn_op[0])
+        h.set_synapses([src], [dst], [1.0], [1.5e-3])
+        assert h.info()["batch_k"] == (16 if dst < src else 15)   # K <= shortest connection
+        with pytest.raises(D.HshError):          # not a SPIKE_OP
+            h.set_synapses([dst], [dst], [1.0], [1.5e-3])
+        with pytest.raises(D.HshError):          # not a SYN2_OP
+            h.set_synapses([src], [src], [1.0], [1.5e-3])
+        with pytest.raises(D.HshError):          # shorter than a step
+            h.set_synapses([src], [dst], [1.0], [0.0] if dst > src else [-1.0])
+
+
+def test_refuses_pending_synaptic_event_with_reason():
+    m = synchan_model()
+    ops = np.array(m.ops)
+    op = int(m.syn_op[0])
+    ops[op + 2] = 5            # the reference's HSEVENT counter of a pending event
+    m.ops = ops
     with pytest.raises(D.Refused) as e:
         D.HipSolver(m, host_only=True)
     refs = e.value.refusals
     assert len(refs) == 1
     comp, opcode, opidx, why = refs[0]
-    assert opcode == 4100 and why == "opcode"
-    assert m.ops[opidx] == 4100
+    assert opcode == 4100 and opidx == op and why == "pending synaptic event"
     assert comp == m.hnum[1]
 
 
