@@ -66,6 +66,10 @@ def parse():
                    help="skip the heterogeneous-population section (100k neurons, 1..1024 compartments)")
     p.add_argument("--hetero-neurons", type=int, default=100000)
     p.add_argument("--hetero-steps", type=int, default=5000)
+    p.add_argument("--no-network", action="store_true",
+                   help="skip the network section (4000 HH cells, synchans, device event delivery)")
+    p.add_argument("--net-cells", type=int, default=4000)
+    p.add_argument("--net-steps", type=int, default=50000)
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
     p.add_argument("--adj-neurons", type=int, default=1024)
@@ -222,6 +226,55 @@ def hetero_section(a, D, M, Oracle):
                          "bitexact": bool(np.array_equal(v, ref))}
         res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{S} steps of the population ({cpu:.1f} s)"}
+    return res
+
+
+def network_section(a, D, M, Oracle):
+    """Event-driven network: isopotential HH cells, 80% excitatory, 2%
+    random connectivity, dual-exponential synchans (exc tau 5 ms, Erev 0;
+    inh tau 10 ms, Erev -80 mV; gmax 6 / 67 nS), delay 1.5 ms, threshold 0 mV,
+    Vm0 uniform -70..-50 mV, constant inject uniform 0.3..0.8 nA, dt 0.1 ms.
+    Spikes are delivered on the device; the raster of the whole run is
+    compared step-exact with the oracle's event mode."""
+    t0 = time.perf_counter()
+    m, table = M.network(a.net_cells, seed=a.seed, inject=(0.3e-9, 0.8e-9))
+    print(f"[bench] network: {a.net_cells} cells, {table[0].size} connections, {a.net_steps} steps "
+          f"(built {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    res = {"workload": f"{a.net_cells} isopotential HH cells (80% exc), 2% connectivity, dual-exponential "
+                       "synchans (5/10 ms, 6/67 nS), delay 1.5 ms, dt 0.1 ms, constant inject U(0.3, 0.8) nA",
+           "connections": int(table[0].size), "steps": a.net_steps,
+           "simulated_s": a.net_steps * m.dt}
+    for name, mode, kw in (("batched", D.BATCHED, dict()), ("plain", D.PLAIN, dict(flags=D.F_NO_AUTOTUNE))):
+        with D.HipSolver(m, **kw) as g:
+            g.set_synapses(*table)
+            if mode == D.BATCHED:
+                g.autotune()         # keeps the state: the timed run starts at step 0
+            g.sync()
+            t1 = time.perf_counter()
+            g.step(a.net_steps, mode)
+            g.sync()
+            wall = time.perf_counter() - t1
+            ops_, steps_ = g.drain_spikes()
+            info = g.info()
+            res[name] = {"value": a.net_steps * m.ncompts / wall, "unit": "compartment-steps/s",
+                         "ms_per_step": wall * 1e3 / a.net_steps, "spikes": int(ops_.size),
+                         "rate_hz": ops_.size / a.net_cells / (a.net_steps * m.dt), "batch_k": info["batch_k"]}
+            if name == "batched":
+                raster = (ops_, steps_)
+                vm_dev = g.vm
+    if not a.no_cpu:
+        o = Oracle(m, givals=False)
+        o.set_synapses(*table)
+        t1 = time.perf_counter()
+        o.step(a.net_steps)
+        cpu = time.perf_counter() - t1
+        w_op, w_step = o.spikes()
+        res["parity"] = {"steps": a.net_steps, "raster_identical": bool(np.array_equal(w_op, raster[0]) and
+                                                                        np.array_equal(w_step, raster[1])),
+                         "spikes_oracle": int(w_op.size),
+                         "max_abs_dv_V": float(np.max(np.abs(o.vm - vm_dev)))}
+        res["cpu_baseline"] = {"value": a.net_steps * m.ncompts / cpu, "unit": "compartment-steps/s", "cores": 1,
+                               "kind": "port", "sample": f"the whole run ({cpu:.1f} s)"}
     return res
 
 
@@ -474,6 +527,9 @@ def main():
     if not a.no_hetero:
         from oracle import Oracle as _O
         out["heterogeneous"] = hetero_section(a, D, M, _O)
+    if not a.no_network:
+        from oracle import Oracle as _O
+        out["network"] = network_section(a, D, M, _O)
     if not a.no_adjoint:
         from oracle import Oracle as _O, TangentLinear as _TL
         out["adjoint"] = adjoint_section(a, torch, _O, _TL)

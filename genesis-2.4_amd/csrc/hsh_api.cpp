@@ -694,7 +694,9 @@ hsh_status autotune(hsh_handle* h)
         if (o.batch_k <= 0) {
             double bk = 1e300;
             int kbest = K0;
-            for (int K : {1, 8, 32, 128}) {
+            std::vector<int> ks = {1, 8, 32, 128};
+            if (h->max_k < 128) ks.push_back(h->max_k);     // the longest launch connections allow
+            for (int K : ks) {
                 if (K > h->max_k) continue;
                 h->batch_k = K;
                 free_graphs(h);
