@@ -70,6 +70,10 @@ def parse():
                    help="skip the network section (4000 HH cells, synchans, device event delivery)")
     p.add_argument("--net-cells", type=int, default=4000)
     p.add_argument("--net-steps", type=int, default=50000)
+    p.add_argument("--no-calcium", action="store_true",
+                   help="skip the calcium section (Ca pools, Ca-gated K, GHK, table currents)")
+    p.add_argument("--ca-neurons", type=int, default=20000)
+    p.add_argument("--ca-steps", type=int, default=2000)
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
     p.add_argument("--adj-neurons", type=int, default=1024)
@@ -226,6 +230,66 @@ def hetero_section(a, D, M, Oracle):
                          "bitexact": bool(np.array_equal(v, ref))}
         res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{S} steps of the population ({cpu:.1f} s)"}
+    return res
+
+
+def calcium_section(a, D, M, Oracle):
+    """The 16-compartment population with calcium in every compartment: a
+    GHK Ca channel feeding a Ca_concen pool, a Ca-gated K channel (NEWCONC1 +
+    IPOL1C), a DOMAIN-gated K channel, table currents on Vm and on the pool.
+    Batched (autotuned) and plain throughput; parity against the oracle on a
+    bounded number of steps (GHK calls the device exp, so the bar there is
+    the 1e-7 V tolerance)."""
+    t0 = time.perf_counter()
+    m = M.ca_population(a.ca_neurons, (16,), (1.0,), features=M.CA_FEATURES[:6] + ("ghk",), seed=a.seed)
+    t_build = time.perf_counter() - t0
+    n = m.ncompts
+    print(f"[bench] calcium: {a.ca_neurons} neurons, {n} compartments (built {t_build:.1f}s)",
+          file=sys.stderr, flush=True)
+    res = {"workload": f"{a.ca_neurons} random Hines trees x 16 compartments, HH soma, Kdr dendrites; per "
+                       "compartment a GHK Ca channel, one Ca_concen pool, Ca- and DOMAIN-gated K, table currents",
+           "compartments": n, "steps": a.ca_steps}
+    with D.HipSolver(m) as g:
+        g.step(64, D.BATCHED)          # the autotuner runs here
+        g.sync()
+        info = g.info()
+        g.set_timing(True)
+        t0 = time.perf_counter()
+        g.step(a.ca_steps, D.BATCHED)
+        g.sync()
+        wall = time.perf_counter() - t0
+        dev_ms = g.last_time(D.T_TOTAL)
+        g.set_timing(False)
+        res["batched"] = {"value": a.ca_steps * n / wall, "unit": "compartment-steps/s",
+                          "ms_per_step": wall * 1e3 / a.ca_steps, "batch_k": info["batch_k"],
+                          "tile_max_rows": info["tile_max_rows"], "bytes_per_compartment_step":
+                          info["bytes_per_step"] / n,
+                          "hbm_frac_algorithmic": info["bytes_per_step"] * a.ca_steps / (dev_ms * 1e-3) / 1e9
+                          / HBM_PEAK_GBS}
+    with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
+        g.step(16, D.PLAIN)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(200, D.PLAIN)
+        g.sync()
+        wall = time.perf_counter() - t0
+        res["plain"] = {"value": 200 * n / wall, "unit": "compartment-steps/s", "ms_per_step": wall * 1e3 / 200}
+    if not a.no_cpu:
+        S = 40
+        o = Oracle(m, givals=False)
+        t0 = time.perf_counter()
+        o.step(S)
+        cpu = time.perf_counter() - t0
+        with D.HipSolver(m, batch_k=8, flags=D.F_NO_AUTOTUNE) as g:
+            g.step(S, D.BATCHED)
+            v, c = g.vm, g.conc()
+        ref, cref = o.vm, o.conc()
+        res["parity"] = {"steps": S, "max_abs_dv_V": float(np.max(np.abs(v - ref))),
+                         "max_rel_dconc": float(np.max(np.abs(c - cref) / np.abs(cref))),
+                         "bitexact": bool(np.array_equal(v, ref) and np.array_equal(c, cref)),
+                         "tolerance_V": 1e-7}
+        res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{S} steps of the calcium population ({cpu:.1f} s)"}
     return res
 
 
@@ -527,6 +591,9 @@ def main():
     if not a.no_hetero:
         from oracle import Oracle as _O
         out["heterogeneous"] = hetero_section(a, D, M, _O)
+    if not a.no_calcium:
+        from oracle import Oracle as _O
+        out["calcium"] = calcium_section(a, D, M, _O)
     if not a.no_network:
         from oracle import Oracle as _O
         out["network"] = network_section(a, D, M, _O)

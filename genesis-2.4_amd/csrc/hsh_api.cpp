@@ -72,7 +72,8 @@ struct hsh_handle {
     bool host_only = false;
     int device = 0;
     hipStream_t stream = nullptr;
-    // host copies in the reference layout
+    // host copies in the reference layout; chip holds the values slots are
+    // loaded from: chip[nchips] | conc[nconcs] | concchip[nconcchip]
     std::vector<int32_t> ops;
     std::vector<double> chip;
     // model-constant device buffers
@@ -371,6 +372,9 @@ void set_devmodel(hsh_handle* h)
     d.crank = (h->m.method == HSH_CRANK);
     d.xmin = h->m.xmin;
     d.invdx = h->m.invdx;
+    d.ymin = h->m.ymin;
+    d.invdy = h->m.invdy;
+    d.ydivs = h->m.ydivs;
     d.nblocks = (int32_t)P.blocks.size();
     d.ntiles = (int32_t)P.tiles.size();
     d.last_pos = P.comp_pos[h->m.ncompts - 1];
@@ -453,7 +457,7 @@ hsh_status alloc_layout(hsh_handle* h, const double* vm, const double* results)
     STCHK(upload_state(h, vm));
     // reverse map chip index -> S positions
     {
-        const int nchips = h->m.nchips;
+        const int nchips = (int)h->chip.size();
         std::vector<int64_t> cnt(nchips + 1, 0);
         for (int g = 0; g < h->m.ncompts; g++) {
             const Template& t = P.templates[P.comp_tmpl[g]];
@@ -747,8 +751,11 @@ bool adj_supported(const Plan& P, std::string& why)
                 ng = 0;
             } else if (op == U_SPIKE) {
                 pc += 1;
-            } else if (op == U_SYN2) {
+            } else if (op == U_SYN2 || op == U_SYN2MG) {
                 why = "synaptic channels";
+                return false;
+            } else if (op >= U_CONCVAL) {
+                why = "concentration-dependent ops";
                 return false;
             } else if (op == U_END) {
                 break;
@@ -844,6 +851,10 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         return fail(HSH_ERR_ARG, "hsh_create: chanmode must be 4 or 5");
     if (m->method != HSH_BEULER && m->method != HSH_CRANK)
         return fail(HSH_ERR_ARG, "hsh_create: method must be HSH_BEULER or HSH_CRANK");
+    if (m->nconcs < 0 || (m->nconcs > 0 && !m->conc) || m->nconcops < 0 ||
+        (m->nconcops > 0 && !m->concops) || m->nconcchip < 0 || (m->nconcchip > 0 && !m->concchip) ||
+        m->ydivs < 0)
+        return fail(HSH_ERR_ARG, "hsh_create: inconsistent concentration arrays");
     if (opt && (opt->pack_waves < 0 || opt->pack_waves > 8 || opt->pack_stage < 0 || opt->pack_stage > 2 ||
                 opt->pack_cap < 0 || opt->batch_k < 0 || opt->batch_waves < 0))
         return fail(HSH_ERR_ARG, "hsh_create: option out of range");
@@ -852,6 +863,8 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     if (opt) h->opt = *opt;
     h->ops.assign(m->ops, m->ops + m->nops);
     h->chip.assign(m->chip, m->chip + m->nchips);
+    if (m->nconcs > 0) h->chip.insert(h->chip.end(), m->conc, m->conc + m->nconcs);
+    if (m->nconcchip > 0) h->chip.insert(h->chip.end(), m->concchip, m->concchip + m->nconcchip);
     std::string err;
     if (!analyse(*m, h->plan, err)) {
         delete h;
@@ -924,7 +937,9 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     HIPCHK_C(dalloc(&h->d_prog, P.prog.size()));
     HIPCHK_C(dalloc(&h->d_spk, 2 * (size_t)SPIKE_CAP));
     HIPCHK_C(dalloc(&h->d_nspk, 1));
-    const size_t ntab = (m->ncols > 0) ? (size_t)(m->xdivs + 2) * m->ncols : 1;
+    // voltage rows and concentration rows share the table
+    const size_t trows = (size_t)std::max(m->xdivs, m->ydivs) + 2;
+    const size_t ntab = (m->ncols > 0) ? trows * m->ncols : 1;
     const size_t nx = (m->ncols > 0) ? (size_t)(m->xdivs + 2) : 1;
     HIPCHK_C(dalloc(&h->d_tab, ntab));
     HIPCHK_C(dalloc(&h->d_xvals, nx));
@@ -1022,7 +1037,7 @@ hsh_status hsh_get_state(hsh_handle* h, double* vm, double* chip, double* result
     if (vm || results) STCHK(download_vm_results(h, vm, results));
     if (chip || ops) {
         STCHK(download_chip_ops(h));
-        if (chip) std::memcpy(chip, h->chip.data(), h->chip.size() * sizeof(double));
+        if (chip) std::memcpy(chip, h->chip.data(), (size_t)h->m.nchips * sizeof(double));
         if (ops) std::memcpy(ops, h->ops.data(), h->ops.size() * sizeof(int32_t));
     }
     return HSH_OK;
@@ -1039,7 +1054,7 @@ hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, co
         // passing only one of chip/ops keeps the other current
         STCHK(download_chip_ops(h));
     }
-    if (chip) std::memcpy(h->chip.data(), chip, h->chip.size() * sizeof(double));
+    if (chip) std::memcpy(h->chip.data(), chip, (size_t)h->m.nchips * sizeof(double));
     if (ops) std::memcpy(h->ops.data(), ops, h->ops.size() * sizeof(int32_t));
     if (chip || ops) return upload_state(h, vm);
     if (vm) {
@@ -1048,6 +1063,28 @@ hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, co
         HIPCHK(hipMemcpy(h->d_V, V.data(), V.size() * sizeof(double), hipMemcpyHostToDevice));
     }
     return HSH_OK;
+}
+
+hsh_status hsh_get_conc(hsh_handle* h, double* conc)
+{
+    if (!h || !conc) return fail(HSH_ERR_ARG, "hsh_get_conc: null argument");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_get_conc: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_get_conc: handle refused the model");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    STCHK(download_chip_ops(h));
+    std::memcpy(conc, h->chip.data() + h->m.nchips, (size_t)std::max(0, h->m.nconcs) * sizeof(double));
+    return HSH_OK;
+}
+
+hsh_status hsh_put_conc(hsh_handle* h, const double* conc)
+{
+    if (!h || !conc) return fail(HSH_ERR_ARG, "hsh_put_conc: null argument");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_put_conc: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_put_conc: handle refused the model");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    STCHK(download_chip_ops(h));
+    std::memcpy(h->chip.data() + h->m.nchips, conc, (size_t)std::max(0, h->m.nconcs) * sizeof(double));
+    return upload_state(h, nullptr);
 }
 
 hsh_status hsh_set_chip(hsh_handle* h, int32_t n, const int32_t* idx, const double* val)

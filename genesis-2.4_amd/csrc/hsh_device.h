@@ -141,6 +141,89 @@ __device__ __forceinline__ double syn2_op(const DevModel& d, double* __restrict_
     return Y;
 }
 
+// Concentration table row and fraction (hines_chip.c:236-252), looked up
+// afresh (the reference's ylo/yhi row cache gives the same row except within
+// rounding of a breakpoint).
+__device__ __forceinline__ void newconc(const DevModel& d, double concen, int& cfilo, double& cipol)
+{
+    double v = (concen - d.ymin) * d.invdy;
+    if (v < 0.0) v = 0.0;
+    else if (v > d.ydivs) v = d.ydivs;
+    int f = (int)v;
+    f = f < 0 ? 0 : (f > d.ydivs ? d.ydivs : f);
+    cfilo = f;
+    cipol = v - (double)f;
+}
+
+// Table current (hines_chip.c:472-481): Ek = A(x), Gk *= B(x).
+__device__ __forceinline__ void tabcurr(const double* __restrict__ tb, int ncols, double ipol, double& Ek,
+                                        double& Gk)
+{
+    double X = tb[0];
+    X = X + ipol * (tb[ncols] - X);
+    double Y = tb[1];
+    Y = Y + ipol * (tb[ncols + 1] - Y);
+    Ek = Y;
+    Gk *= X;
+}
+
+// GHK_OP (hines_chip.c:334-365): slope conductance and the Ek that gives the
+// GHK current at Vm.  exp() is the device library's.
+__device__ __noinline__ void ghk_op(double Vm, double Cin, double Cout, double zFRT, double zF, double& Ek,
+                                    double& slope)
+{
+    double GHK = zFRT;
+    const double X = Vm * GHK;
+    const double Y = exp(-X);
+    const double E = zF;
+    double Ik = E * (Cin - (Y * Cout));
+    double G;
+    if (fabs(X) < 1.0e-6) {
+        G = 1.0 - 0.5 * X;
+        GHK = 0.5 * GHK * E * (Cin + Cout);
+        Ik = -Ik / G;
+    } else {
+        G = 1.0 - Y;
+        GHK = GHK * (Ik / G - E * GHK * (Cin - Cout) * Vm * Y / (G * G));
+        Ik = -X * Ik / G;
+    }
+    Ek = (Ik + GHK * Vm) / GHK;
+    slope = GHK;
+}
+
+// Pool flux registers with a run-time pool number (interpreter): selects over
+// constant indices keep them in VGPRs, where an indexed array or a reference
+// into one would live in scratch memory.
+struct Flux {
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0, f3 = 0.0;
+    __device__ __forceinline__ double get(int p) const
+    {
+        return p == 0 ? f0 : (p == 1 ? f1 : (p == 2 ? f2 : f3));
+    }
+    __device__ __forceinline__ void set(int p, double v)
+    {
+        if (p == 0) f0 = v;
+        else if (p == 1) f1 = v;
+        else if (p == 2) f2 = v;
+        else f3 = v;
+    }
+};
+
+// Mg block of a synaptic conductance (hines_chip.c:531-547): X = KMg_A
+// exp(Vm / KMg_B), Gk *= X / (X + [Mg]).  exp() is the device library's.
+__device__ __forceinline__ double mgblock(double Gk, double Vm, double A, double invB, double Mg)
+{
+    const double X = A * exp(Vm * invB);
+    return Gk * (X / (X + Mg));
+}
+
+// Ca_concen pool (hines_conc_chip.c:196-207): C = base + ((C - base)(2 - i')
+// + influx * B') / i'.
+__device__ __forceinline__ double caconc(double C, double base, double itau, double influx, double Bp)
+{
+    return base + ((C - base) * (2.0 - itau) + (influx * Bp)) / itau;
+}
+
 // ---------------------------------------------------------------------------
 // Generic micro-op interpreter.  With REG the compartment's whole slot vector
 // is fetched with back-to-back coalesced loads before the program runs (one
@@ -171,9 +254,11 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
 #define SPUT(c, v) (Sb[(c) * W + lane] = (v))
     double sumgchan = 0.0;
     double ichan = SGET(0) + SGET(1);
-    double Gk = 0.0, Ek = 0.0;
-    double vipol = 0.0;
+    double Gk = 0.0, Ek = 0.0, Ik = 0.0;
+    double vipol = 0.0, cipol = 0.0;
+    Flux flux;
     const double* vtab = d.tab;
+    const double* ctab = d.tab;
     const int ncols = d.ncols;
     int c = 2;
     int pc = PROG_HEADER;
@@ -194,6 +279,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
         } else if (op == U_ADDCURR) {
             sumgchan += Gk;
             ichan += Ek * Gk;
+            Ik = Gk * (Ek - Vm);
         } else if (op == U_NEWVOLT) {
             int filo;
             newvolt(d, Vm, pred_ok, Vp, filo, vipol);
@@ -205,13 +291,71 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             const int is = code[pc++];
             spike_op(d, Ib, is, lane, Vm, SGET(c), active, step);
             c += 1;
-        } else if (op == U_SYN2) {
+        } else if (op == U_SYN2 || op == U_SYN2MG) {
             const int k = code[pc], is = code[pc + 1];
             pc += 2;
             Gk *= syn2_op(d, Sb, Ib, c, k, is, lane, SGET(c), SGET(c + 1), active, step);
             c += 2;
+            if (op == U_SYN2MG) {
+                Gk = mgblock(Gk, Vm, SGET(c), SGET(c + 1), SGET(c + 2));
+                c += 3;
+            }
             sumgchan += Gk;
             ichan += Ek * Gk;
+            Ik = Gk * (Ek - Vm);
+        } else if (op == U_CONCVAL) {
+            flux.set(code[pc++], 0.0);
+        } else if (op == U_NEWCONC1 || op == U_DOMAIN) {
+            double concen;
+            if (op == U_NEWCONC1) {
+                concen = SGET(code[pc++]);
+            } else {
+                concen = SGET(c) * Ik;
+                c++;
+            }
+            int cf;
+            newconc(d, concen, cf, cipol);
+            ctab = d.tab + (long long)cf * ncols;
+        } else if (op == U_IPOL1C) {
+            const int col = code[pc];
+            const int k = code[pc + 1];
+            pc += 2;
+            const double X = gate_update(ctab, col, ncols, cipol, SGET(c), k < 0);
+            SPUT(c, X);
+            c++;
+            Gk = apply_power(Gk, X, k < 0 ? -k : k);
+        } else if (op == U_TABCURR) {
+            const int col = code[pc];
+            const int which = code[pc + 1];
+            pc += 2;
+            tabcurr((which ? ctab : vtab) + col, ncols, which ? cipol : vipol, Ek, Gk);
+            sumgchan += Gk;
+            ichan += Ek * Gk;
+            Ik = Gk * (Ek - Vm);
+        } else if (op == U_FLUX) {
+            const int p = code[pc++];
+            flux.set(p, flux.get(p) + Ik);
+        } else if (op == U_SCALEFLUX) {
+            const int p = code[pc++];
+            flux.set(p, flux.get(p) + SGET(c) * Ik);
+            c++;
+        } else if (op == U_NERNST) {
+            const int ci = code[pc], co = code[pc + 1];
+            pc += 2;
+            Ek = log(SGET(co) / SGET(ci)) * SGET(c);
+            Gk = SGET(c + 1);
+            c += 2;
+        } else if (op == U_GHK) {
+            const int ci = code[pc], co = code[pc + 1];
+            pc += 2;
+            double slope;
+            ghk_op(Vm, SGET(ci), SGET(co), SGET(c), SGET(c + 1), Ek, slope);
+            Gk = SGET(c + 2) * slope;
+            c += 3;
+        } else if (op == U_POOLCA) {
+            const int p = code[pc], cs = code[pc + 1], ks = code[pc + 2];
+            pc += 3;
+            SPUT(cs, caconc(SGET(cs), SGET(ks), SGET(ks + 1), flux.get(p), SGET(ks + 2)));
         } else {  // U_END
             const double tbyc = SGET(c);
             CompOut o;

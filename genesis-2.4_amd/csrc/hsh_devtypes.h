@@ -23,7 +23,23 @@ enum UOp : int {
                       // I[islot+1], ops index I[islot+2], spike source I[islot+3]
     U_SYN2 = 7,       // operands: stab index k, islot; X = S[c], Y = S[c+1];
                       // counter I[islot], synchan id I[islot+1]; adds the current
+    // concentration ops: a pool the compartment owns is a state slot; its
+    // flux is a register of the program (pool number p < MAXPOOL)
+    U_CONCVAL = 8,    // operand: p; flux[p] = 0
+    U_NEWCONC1 = 9,   // operand: slot of the pool's concentration; conc table row
+    U_DOMAIN = 10,    // conc = S[c++] * Ik; conc table row
+    U_IPOL1C = 11,    // operands: column, power; gate on the conc table row
+    U_TABCURR = 12,   // operands: column, table (0 voltage, 1 conc): Ek = A; Gk *= B; adds the current
+    U_FLUX = 13,      // operand: p; flux[p] += Ik
+    U_SCALEFLUX = 14, // operand: p; flux[p] += S[c++] * Ik
+    U_NERNST = 15,    // operands: Cin slot, Cout slot; Ek = log(Cout/Cin) * S[c++]; Gk = S[c++]
+    U_GHK = 16,       // operands: Cin slot, Cout slot; GHK slope conductance, Ek; Gk = S[c+2]*slope; c += 3
+    U_POOLCA = 17,    // operands: p, conc slot, first of 3 constant slots (base, 1+dt/2tau, dt B):
+                      // Ca_concen update
+    U_SYN2MG = 18,    // U_SYN2, then the Mg block before the current is added:
+                      // X = S[c+2] exp(Vm S[c+3]); Gk *= X / (X + S[c+4])
 };
+constexpr int MAXPOOL = 4;
 constexpr int PROG_HEADER = 2;
 
 // Per-block metadata.  A block is 64 compartments that run the same channel
@@ -104,6 +120,10 @@ struct DevModel {
     int ring_mask;        // ring length - 1 (0: no connections)
     int nsrc;             // spike sources with a connection table
     double dt;
+    // concentration-indexed rows of tab
+    double ymin;
+    double invdy;
+    int ydivs;
     int spk_cap;
     int ncols;
     int xdivs;

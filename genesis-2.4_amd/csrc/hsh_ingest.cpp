@@ -62,6 +62,31 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         err = std::string("malformed ops[]: ") + what + " at ops index " + std::to_string(p);
         return false;
     };
+    // Concentrations.  Slot sources beyond chip[]: nchips + k is conc[k],
+    // nchips + nconcs + j is concchip[j].  Ca_concen pools (concops: CACONC_OP k
+    // ... LCACONC_OP, hines_conc_chip.c:196-211) each take 3 concchip values.
+    const int64_t nchips = m.nchips, nconcs = std::max(0, m.nconcs);
+    std::vector<int64_t> pool_chip(nconcs, -1);     // k -> first concchip value, -1: not a Ca_concen pool
+    std::vector<int> pool_owner(nconcs, -1);       // k -> compartment whose program uses it
+    {
+        int64_t q = 0, j = 0;
+        bool ok = true;
+        while (q < m.nconcops && ok) {
+            const int32_t co = m.concops[q++];
+            if (co == LCACONC_OP) break;
+            if (co == CACONC_OP && q < m.nconcops) {
+                const int32_t k = m.concops[q++];
+                if (k < 0 || k >= nconcs || j + 3 > m.nconcchip) { err = "malformed concops[]"; return false; }
+                pool_chip[k] = j;
+                j += 3;
+            } else {
+                // pools with shells, buffers or pumps need the conc solve
+                refuse(P, -1, co, (int)(q - 1), HSH_R_OPCODE);
+                ok = false;
+            }
+        }
+    }
+    int creg = -1;            // conc register (CONC_VAL_OP), carried across compartments
     for (;;) {
         if (p >= m.nops) return bad("missing LCOMPT_OP");
         int32_t head = ops[p++];
@@ -75,6 +100,40 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         bool refused = false;
         int ngates = 0;
         bool ek_intra = false;    // Ek register set in this compartment
+        bool creg_intra = false;  // conc register set in this compartment
+        bool crow_intra = false;  // conc table row set in this compartment
+        // slots after tbyc/diagterm: pool concentrations, outside concentrations,
+        // pool constants; code positions that hold such a slot index
+        std::vector<int64_t> extra;
+        std::vector<std::pair<size_t, int>> fix;
+        std::vector<int> pools;   // local pool number -> k
+        auto extra_slot = [&](int64_t src) {
+            for (size_t i = 0; i < extra.size(); i++)
+                if (extra[i] == src) return (int)i;
+            extra.push_back(src);
+            return (int)extra.size() - 1;
+        };
+        auto pool_of = [&](int k) {
+            for (size_t i = 0; i < pools.size(); i++)
+                if (pools[i] == k) return (int)i;
+            pools.push_back(k);
+            return (int)pools.size() - 1;
+        };
+        auto push_extra = [&](int64_t src) {
+            fix.emplace_back(code.size(), extra_slot(src));
+            code.push_back(-1);
+        };
+        // true when the conductance ops from ops index q set Ek from a table
+        // (power-0 IPOL) before anything reads it
+        auto ek_dead = [&](int64_t q) {
+            while (q + 2 < m.nops) {
+                const int32_t k = ops[q];
+                if (k != IPOL1V_OP && k != IPOL1C_OP && k != IPOL2V_OP && k != IPOL2C_OP) return false;
+                if (ops[q + 2] == 0) return true;
+                q += 3;
+            }
+            return false;
+        };
         EkKind ek_in = ek;        // live-in Ek
         (void)0;
         if (c + 2 > m.nchips) return bad("chip[] too short");
@@ -103,25 +162,82 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 c += 1;
                 p += 2;
                 break;
-            case CONC_VAL_OP: p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
-            case DOMAIN_OP: c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case CONC_VAL_OP: {
+                if (p >= m.nops) return bad("CONC_VAL_OP operand");
+                const int32_t k = ops[p++];
+                if (k < 0 || k >= nconcs) return bad("CONC_VAL_OP pool index out of range");
+                creg = k;
+                creg_intra = true;
+                if (pool_owner[k] < 0) pool_owner[k] = comp;
+                if (pool_owner[k] != comp && pool_chip[k] >= 0) {
+                    // a pool several compartments feed: the device keeps a
+                    // pool's flux in the program of the compartment that owns it
+                    refuse(P, comp, op, (int)opidx, HSH_R_POOL);
+                    refused = true;
+                    break;
+                }
+                const int pn = pool_of(k);
+                if (pn >= MAXPOOL) { refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true; break; }
+                code.push_back(U_CONCVAL);
+                code.push_back(pn);
+                break;
+            }
+            case NEWCONC1_OP:
+                if (!creg_intra) { refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true; break; }
+                if (m.ydivs <= 0) return bad("NEWCONC1_OP without a concentration table");
+                code.push_back(U_NEWCONC1);
+                push_extra(nchips + creg);
+                crow_intra = true;
+                break;
+            case DOMAIN_OP:
+                if (!need_chip(1)) return bad("chip[] too short");
+                if (m.ydivs <= 0) return bad("DOMAIN_OP without a concentration table");
+                code.push_back(U_DOMAIN);
+                dsrc.push_back(c);
+                c += 1;
+                crow_intra = true;
+                break;
             case NEWCONC2_OP: p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
-            case NEWCONC1_OP: refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
             case ELMMPUMP_OP: c += 3; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
-            case FLUX_OP: refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case FLUX_OP:
+            case SCALE_FLUX_OP:
+                if (!creg_intra) { refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true;
+                                   if (op == SCALE_FLUX_OP) c += 1; break; }
+                code.push_back(op == FLUX_OP ? U_FLUX : U_SCALEFLUX);
+                code.push_back(pool_of(creg));
+                if (op == SCALE_FLUX_OP) {
+                    if (!need_chip(1)) return bad("chip[] too short");
+                    dsrc.push_back(c);
+                    c += 1;
+                }
+                break;
             case STORE_ALLI_OP: case STORE_ALLC_OP:
                 p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
-            case SCALE_FLUX_OP: c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
             case GHK_OP:
-                c += 3; ek = EK_DYN; ek_intra = true; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true;
-                to_cond = true; break;
+            case NERNST_OP: {
+                const int nc = (op == GHK_OP) ? 3 : 2;
+                if (!need_chip(nc)) return bad("chip[] too short");
+                if (creg_intra && creg + 1 >= nconcs) return bad("no outside concentration after the pool");
+                if (!creg_intra) {
+                    refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true;
+                } else if (pool_chip[creg + 1] >= 0) {
+                    // the outside concentration is a pool another program updates
+                    refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true;
+                } else {
+                    code.push_back(op == GHK_OP ? U_GHK : U_NERNST);
+                    push_extra(nchips + creg);
+                    push_extra(nchips + creg + 1);
+                    for (int i = 0; i < nc; i++) dsrc.push_back(c + i);
+                }
+                c += nc;
+                ek = EK_DYN; ek_intra = true;
+                to_cond = true;
+                break;
+            }
             case SAME_GHK_OP:
                 c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; to_cond = true; break;
             case DO_TABGHK_OP:
                 refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; to_cond = true; break;
-            case NERNST_OP:
-                c += 2; ek = EK_DYN; ek_intra = true; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true;
-                to_cond = true; break;
             case CHAN_EK_OP:
                 if (!need_chip(2)) return bad("chip[] too short");
                 code.push_back(U_CHAN_EK);
@@ -132,7 +248,9 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 break;
             case CHAN_OP:
                 if (!need_chip(1)) return bad("chip[] too short");
-                if (ek_intra && ek == EK_DYN) {
+                if ((ek_intra && ek == EK_DYN) || ek_dead(p)) {
+                    // Ek computed in this compartment, or overwritten by a
+                    // table current before the current is added
                     code.push_back(U_CHAN);
                     dsrc.push_back(c);
                 } else if (!ek_intra && ek_in == EK_DYN) {
@@ -173,24 +291,29 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     int32_t col = ops[p], pw = ops[p + 1];
                     p += 2;
                     if (pw != 0 && !need_chip(1)) return bad("chip[] too short");
-                    if (k != IPOL1V_OP) {
+                    if (k != IPOL1V_OP && k != IPOL1C_OP) {
                         refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
                         if (pw != 0) c += 1;
                         if (pw == 0) break;
                         continue;
                     }
-                    if (pw == 0) {   // tabcurrent / tab-GHK: Ek from a table
-                        refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
+                    if (pw < -6 || pw > 6) return bad("gate power out of range");
+                    if (col < 0 || col + 1 >= m.ncols || !m.tablist)
+                        return bad("IPOL table column out of range");
+                    const bool onconc = (k == IPOL1C_OP);
+                    if (onconc ? !crow_intra : !has_volt[comp]) {
+                        refuse(P, comp, k, (int)kidx, HSH_R_TABLE_LIVEIN); refused = true;
+                    }
+                    if (pw == 0) {
+                        // table current (hines_chip.c:472-481): Ek from the
+                        // table, Gk scaled by it, the current added
+                        code.push_back(U_TABCURR);
+                        code.push_back(col);
+                        code.push_back(onconc ? 1 : 0);
                         ek = EK_DYN; ek_intra = true;
                         break;
                     }
-                    if (pw < -6 || pw > 6) return bad("gate power out of range");
-                    if (col < 0 || col + 1 >= m.ncols || !m.tablist)
-                        return bad("IPOL1V table column out of range");
-                    if (!has_volt[comp]) {
-                        refuse(P, comp, k, (int)kidx, HSH_R_TABLE_LIVEIN); refused = true;
-                    }
-                    code.push_back(U_IPOL1V);
+                    code.push_back(onconc ? U_IPOL1C : U_IPOL1V);
                     code.push_back(col);
                     code.push_back(pw);
                     dsrc.push_back(c);
@@ -203,22 +326,22 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     const int32_t st = ops[p];
                     const int64_t cntidx = p + 1;
                     const bool mg = (p + 3 < m.nops && ops[p + 3] == MGBLOCK_OP);
-                    if (k == SYN3_OP || mg) {
-                        // SYN3 draws urandom(); the Mg block needs exp() of Vm
-                        if (k == SYN3_OP) c += 1;
+                    if (k == SYN3_OP) {
+                        // SYN3 draws urandom() in the reference's serial op order
+                        c += 1;
                         p += 3;
                         c += 2;
                         if (mg) { p += 1; c += 3; }
-                        refuse(P, comp, mg ? MGBLOCK_OP : k, (int)kidx, HSH_R_OPCODE); refused = true;
+                        refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
                         break;
                     }
-                    if (!need_chip(2)) return bad("chip[] too short");
+                    if (!need_chip(mg ? 5 : 2)) return bad("chip[] too short");
                     if (st < 0 || (int64_t)st + 3 >= 4 * (int64_t)m.nstab || !m.stablist)
                         return bad("SYN2_OP stablist index out of range");
                     if (ops[cntidx] >= 0) {
                         refuse(P, comp, k, (int)kidx, HSH_R_SYN_PENDING); refused = true;
                     }
-                    code.push_back(U_SYN2);
+                    code.push_back(mg ? U_SYN2MG : U_SYN2);
                     code.push_back(st);
                     code.push_back((int32_t)isrc.size());
                     isrc.push_back(cntidx);                              // counter (mutable)
@@ -229,6 +352,11 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     c += 2;
                     p += 3;
                     ngates += 2;
+                    if (mg) {                   // KMg_A, 1/KMg_B, [Mg] (hines_chip_init.c:1180-1187)
+                        for (int i = 0; i < 3; i++) dsrc.push_back(c + i);
+                        c += 3;
+                        p += 1;
+                    }
                     break;
                 } else if (k == ADD_CURR_OP) {
                     code.push_back(U_ADDCURR);
@@ -241,10 +369,32 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
             }
         }
         if (c + 2 > m.nchips) return bad("chip[] too short at compartment end");
+        // the Ca_concen update of the pools this compartment owns, after all
+        // of its ops have read and fed them (the reference updates pools after
+        // the whole chip update, hines.c:222-270; a pool owned by one
+        // compartment sees no difference)
+        for (size_t pn = 0; pn < pools.size() && !refused; pn++) {
+            const int k = pools[pn];
+            if (pool_chip[k] < 0) continue;
+            code.push_back(U_POOLCA);
+            code.push_back((int32_t)pn);
+            push_extra(nchips + k);
+            push_extra(nchips + nconcs + pool_chip[k]);
+            extra_slot(nchips + nconcs + pool_chip[k] + 1);
+            extra_slot(nchips + nconcs + pool_chip[k] + 2);
+        }
         code.push_back(U_END);
         dsrc.push_back(c); dsrc.push_back(c + 1);   // tbyc, diagterm
         c += 2;
         if (refused) continue;
+        {
+            // the three pool constants must be consecutive slots
+            const int base = (int)dsrc.size();
+            for (auto& f : fix) code[f.first] = base + f.second;
+            for (int64_t src : extra) dsrc.push_back(src);
+            for (size_t pn = 0; pn < pools.size(); pn++)
+                if (pool_chip[pools[pn]] >= 0) ngates++;      // the pool concentration is state
+        }
         Template t;
         t.code = code;
         t.nslots = (int)dsrc.size();
@@ -270,6 +420,8 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     }
     if (comp != n - 1) { err = "ops[] holds " + std::to_string(comp + 1) + " compartments, ncompts is " + std::to_string(n); return false; }
     if (c != m.nchips) { err = "chip[] length does not match the ops[] walk"; return false; }
+    for (int64_t k = 0; k < nconcs; k++)
+        if (pool_chip[k] >= 0 && pool_owner[k] < 0) refuse(P, -1, CACONC_OP, -1, HSH_R_POOL);
 
     // previous NEWVOLT compartment in op order (row-cache live-in)
     P.pred.assign(n, -1);

@@ -20,7 +20,7 @@ enum RefOp : int32_t {
     SCALE_FLUX_OP = 3301, STORE_ALLI_OP = 3501, STORE_ALLC_OP = 3502, IPOL1V_OP = 4001,
     IPOL1C_OP = 4002, IPOL2V_OP = 4003, IPOL2C_OP = 4004, SYN2_OP = 4100, SYN3_OP = 4101,
     MGBLOCK_OP = 4200, NEWVOLT_OP = 5100, NEWCONC1_OP = 5110, NEWCONC2_OP = 5111,
-    DOMAIN_OP = 5112,
+    DOMAIN_OP = 5112, CACONC_OP = 10200, LCACONC_OP = 10202,
 };
 // Reference funcs codes (hines_defs.h:119-128).
 enum RefFunc : int32_t {

@@ -41,8 +41,11 @@ std::string gen_template(const Template& t, int id, int ncols)
     for (int c = 0; c < t.nslots; c++)
         o += "    const double " + slot(c) + " = Sb[" + std::to_string(c) + " * W + lane];\n";
     o += "    double sumgchan = 0.0;\n    double ichan = s0 + s1;\n";
-    o += "    double Gk = 0.0, Ek = 0.0, vipol = 0.0;\n    const double* vtab = d.tab;\n";
-    o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)vipol; (void)vtab;\n";
+    o += "    double Gk = 0.0, Ek = 0.0, Ik = 0.0, vipol = 0.0, cipol = 0.0;\n";
+    o += "    double flux[MAXPOOL] = {0.0, 0.0, 0.0, 0.0};\n";
+    o += "    const double* vtab = d.tab;\n    const double* ctab = d.tab;\n";
+    o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)Ik; (void)vipol; (void)cipol;\n";
+    o += "    (void)flux; (void)vtab; (void)ctab;\n";
     o += "    (void)pred_ok; (void)Vp;\n";
     const std::string NC = std::to_string(ncols);
     int c = 2;
@@ -74,7 +77,7 @@ std::string gen_template(const Template& t, int id, int ncols)
             break;
         }
         case U_ADDCURR:
-            o += "    sumgchan += Gk; ichan += Ek * Gk;\n";
+            o += "    sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
             break;
         case U_SPIKE: {
             const int is = code[pc++];
@@ -83,13 +86,80 @@ std::string gen_template(const Template& t, int id, int ncols)
             c += 1;
             break;
         }
-        case U_SYN2: {
+        case U_SYN2:
+        case U_SYN2MG: {
             const int k = code[pc], is = code[pc + 1];
             pc += 2;
             o += "    Gk *= syn2_op(d, Sb, Ib, " + std::to_string(c) + ", " + std::to_string(k) + ", " +
-                 std::to_string(is) + ", lane, " + slot(c) + ", " + slot(c + 1) +
-                 ", active, step); sumgchan += Gk; ichan += Ek * Gk;\n";
+                 std::to_string(is) + ", lane, " + slot(c) + ", " + slot(c + 1) + ", active, step);";
             c += 2;
+            if (op == U_SYN2MG) {
+                o += " Gk = mgblock(Gk, Vm, " + slot(c) + ", " + slot(c + 1) + ", " + slot(c + 2) + ");";
+                c += 3;
+            }
+            o += " sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
+            break;
+        }
+        case U_CONCVAL:
+            o += "    flux[" + std::to_string(code[pc++]) + "] = 0.0;\n";
+            break;
+        case U_NEWCONC1:
+        case U_DOMAIN: {
+            std::string conc;
+            if (op == U_NEWCONC1) {
+                conc = slot(code[pc++]);
+            } else {
+                conc = slot(c) + " * Ik";
+                c += 1;
+            }
+            o += "    { int cf; newconc(d, " + conc + ", cf, cipol); ctab = d.tab + (long long)cf * " + NC + "; }\n";
+            break;
+        }
+        case U_IPOL1C: {
+            const int col = code[pc], k = code[pc + 1];
+            pc += 2;
+            o += "    { const double X = gate_update(ctab, " + std::to_string(col) + ", " + NC +
+                 ", cipol, " + slot(c) + ", " + (k < 0 ? "true" : "false") + "); Sb[" +
+                 std::to_string(c) + " * W + lane] = X; Gk = apply_power(Gk, X, " +
+                 std::to_string(k < 0 ? -k : k) + "); }\n";
+            c += 1;
+            break;
+        }
+        case U_TABCURR: {
+            const int col = code[pc], which = code[pc + 1];
+            pc += 2;
+            o += std::string("    tabcurr(") + (which ? "ctab" : "vtab") + " + " + std::to_string(col) + ", " + NC +
+                 ", " + (which ? "cipol" : "vipol") +
+                 ", Ek, Gk); sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
+            break;
+        }
+        case U_FLUX:
+            o += "    flux[" + std::to_string(code[pc++]) + "] += Ik;\n";
+            break;
+        case U_SCALEFLUX:
+            o += "    flux[" + std::to_string(code[pc++]) + "] += " + slot(c) + " * Ik;\n";
+            c += 1;
+            break;
+        case U_NERNST: {
+            const int ci = code[pc], co = code[pc + 1];
+            pc += 2;
+            o += "    Ek = log(" + slot(co) + " / " + slot(ci) + ") * " + slot(c) + "; Gk = " + slot(c + 1) + ";\n";
+            c += 2;
+            break;
+        }
+        case U_GHK: {
+            const int ci = code[pc], co = code[pc + 1];
+            pc += 2;
+            o += "    { double sl; ghk_op(Vm, " + slot(ci) + ", " + slot(co) + ", " + slot(c) + ", " + slot(c + 1) +
+                 ", Ek, sl); Gk = " + slot(c + 2) + " * sl; }\n";
+            c += 3;
+            break;
+        }
+        case U_POOLCA: {
+            const int p = code[pc], cs = code[pc + 1], ks = code[pc + 2];
+            pc += 3;
+            o += "    Sb[" + std::to_string(cs) + " * W + lane] = caconc(" + slot(cs) + ", " + slot(ks) + ", " +
+                 slot(ks + 1) + ", flux[" + std::to_string(p) + "], " + slot(ks + 2) + ");\n";
             break;
         }
         default:   // U_END

@@ -70,6 +70,25 @@ static int hip_init(Hsolve *hsolve)
     m.funcs = hsolve->funcs;
     m.ravals = hsolve->ravals;
     m.vm = hsolve->vm;
+    /* concentrations: Ca_concen pools run on the device, in the program of
+     * the compartment that feeds them */
+    m.nconcs = hsolve->nconcs;
+    m.conc = hsolve->conc;
+    m.nconcops = hsolve->concops ? hsolve->nconcops : 0;
+    m.concops = hsolve->concops;
+    m.nconcchip = hsolve->nconcchips;
+    m.concchip = hsolve->concchip;
+    m.ydivs = hsolve->ydivs;
+    m.ymin = hsolve->ymin;
+    m.invdy = hsolve->invdy;
+    if (hsolve->ydivs > hsolve->xdivs || hsolve->ndiffs) {
+        /* concentration rows beyond the composite table, or a conc solve
+         * (difshells): CPU path */
+        fprintf(stderr, "HIP: %s: concentration tables or shells the device path does not run; "
+                "CPU path for this solver\n", Pathname(hsolve));
+        hsolve->accel_state = HIP_REFUSED;
+        return -1;
+    }
     m.stablist = hsolve->stablist;
     m.nstab = 0;
     for (i = 0; i + 1 < hsolve->nops; i++)      /* synapse types referenced by ops[] */
@@ -150,6 +169,13 @@ void hip_sync_chip(Hsolve *hsolve)
     HipState *st = (HipState *)hsolve->accel_state;
     if (!st || st == (HipState *)HIP_REFUSED) return;
     hsh_get_state(st->h, hsolve->vm, hsolve->chip, hsolve->results, hsolve->ops);
+    if (hsolve->nconcs > 0) hsh_get_conc(st->h, hsolve->conc);
+}
+
+int hip_owns_conc(Hsolve *hsolve)
+{
+    HipState *st = (HipState *)hsolve->accel_state;
+    return st != NULL && st != (HipState *)HIP_REFUSED;
 }
 
 void hip_release(Hsolve *hsolve)

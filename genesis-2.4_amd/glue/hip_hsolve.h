@@ -15,6 +15,7 @@ typedef struct hsolve_type Hsolve;
 int  hip_chip_update(Hsolve *hsolve);   /* 1: vm[] final (skip the CPU solve), 0: CPU path ran */
 void hip_sync_chip(Hsolve *hsolve);     /* device state -> vm/chip/results/ops (HGET, SAVE2) */
 void hip_release(Hsolve *hsolve);       /* free this solver's device state (h_delete) */
+int  hip_owns_conc(Hsolve *hsolve);     /* 1: the device updates conc[] (skip do_h4_conc_chip_update) */
 #endif
 
 #endif

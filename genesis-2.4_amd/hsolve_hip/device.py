@@ -28,7 +28,8 @@ F_HOST_ONLY = 2
 F_NO_JIT = 4
 
 REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
-           5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode", 8: "pending synaptic event"}
+           5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode", 8: "pending synaptic event",
+           9: "shared pool"}
 
 
 class HshModel(C.Structure):
@@ -40,6 +41,9 @@ class HshModel(C.Structure):
         ("dt", C.c_double), ("xmin", C.c_double), ("invdx", C.c_double),
         ("ops", _i32p), ("chip", _f64p), ("tablist", _f64p), ("xvals", _f64p),
         ("stablist", _f64p), ("funcs", _i32p), ("ravals", _f64p), ("vm", _f64p),
+        ("ydivs", C.c_int32), ("nconcops", C.c_int32), ("nconcchip", C.c_int32),
+        ("ymin", C.c_double), ("invdy", C.c_double),
+        ("conc", _f64p), ("concops", _i32p), ("concchip", _f64p),
     ]
 
 
@@ -72,7 +76,7 @@ class HshInfo(C.Structure):
 
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
-           "hsh_put_state", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
+           "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
            "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_set_ravals",
            "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
@@ -157,12 +161,15 @@ def _hsh_model(model):
              stablist=np.ascontiguousarray(model.stablist, np.float64),
              funcs=np.ascontiguousarray(model.funcs, np.int32),
              ravals=np.ascontiguousarray(model.ravals, np.float64),
-             vm=np.ascontiguousarray(model.vm, np.float64))
+             vm=np.ascontiguousarray(model.vm, np.float64),
+             conc=np.ascontiguousarray(model.conc, np.float64),
+             concops=np.ascontiguousarray(model.concops, np.int32),
+             concchip=np.ascontiguousarray(model.concchip, np.float64))
     mm = HshModel()
     mm.ncompts, mm.nchips, mm.nops = model.ncompts, k["chip"].size, k["ops"].size
     mm.nfuncs, mm.nravals, mm.ncols = k["funcs"].size, k["ravals"].size, model.ncols
     mm.xdivs, mm.nstab = model.xdivs, k["stablist"].size // 4
-    mm.method, mm.chanmode, mm.calcmode, mm.nconcs = model.method, model.chanmode, 1, 0
+    mm.method, mm.chanmode, mm.calcmode = model.method, model.chanmode, 1
     mm.dt, mm.xmin, mm.invdx = model.dt, model.xmin, model.invdx
     mm.ops, mm.chip = _p(k["ops"], C.c_int32), _p(k["chip"], C.c_double)
     mm.tablist = _p(k["tablist"], C.c_double) if k["tablist"].size else None
@@ -170,6 +177,11 @@ def _hsh_model(model):
     mm.stablist = _p(k["stablist"], C.c_double) if k["stablist"].size else None
     mm.funcs, mm.ravals = _p(k["funcs"], C.c_int32), _p(k["ravals"], C.c_double)
     mm.vm = _p(k["vm"], C.c_double)
+    mm.nconcs, mm.nconcops, mm.nconcchip = k["conc"].size, k["concops"].size, k["concchip"].size
+    mm.ydivs, mm.ymin, mm.invdy = model.ydivs, model.ymin, model.invdy
+    mm.conc = _p(k["conc"], C.c_double) if k["conc"].size else None
+    mm.concops = _p(k["concops"], C.c_int32) if k["concops"].size else None
+    mm.concchip = _p(k["concchip"], C.c_double) if k["concchip"].size else None
     return mm, k
 
 
@@ -315,6 +327,20 @@ class HipSolver:
         _check(lib().hsh_get_state(self.h, _p(vm, C.c_double), _p(chip, C.c_double),
                                    _p(res, C.c_double), _p(ops, C.c_int32)), "hsh_get_state")
         return dict(vm=vm, chip=chip, results=res, ops=ops)
+
+    def conc(self) -> np.ndarray:
+        """Pool concentrations conc[nconcs] (reference indices)."""
+        out = np.empty(self.m.conc.size)
+        if out.size:
+            _check(lib().hsh_get_conc(self.h, _p(out, C.c_double)), "hsh_get_conc")
+        return out
+
+    def put_conc(self, conc):
+        a = np.ascontiguousarray(conc, np.float64)
+        if a.size != self.m.conc.size:
+            raise ValueError(f"conc has {a.size} values, the model {self.m.conc.size}")
+        if a.size:
+            _check(lib().hsh_put_conc(self.h, _p(a, C.c_double)), "hsh_put_conc")
 
     def put_state(self, vm=None, chip=None, ops=None):
         keep = []

@@ -130,6 +130,14 @@ class Model:
     syn_op: np.ndarray
     syn_chip: np.ndarray
     extra: dict = field(default_factory=dict)
+    # concentrations (hines_struct.h:315-376): pool values, the Ca_concen
+    # program (CACONC_OP k ... LCACONC_OP) and its three values per pool
+    conc: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    concops: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    concchip: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    ydivs: int = 0
+    ymin: float = 0.0
+    invdy: float = 0.0
 
     @property
     def nchips(self):
@@ -352,3 +360,195 @@ def network(n_cells: int = 4000, *, seed: int = 0, dt: float = 1e-4, frac_exc: f
 def tabinterp(table: np.ndarray, xdivs: int, xmin: float, invdx: float, x: float) -> float:
     t = np.ascontiguousarray(table, dtype=np.float64)
     return lib().hsm_tabinterp(_p(t, C.c_double), xdivs, xmin, invdx, x)
+
+
+# ---------------------------------------------------------------------------
+# Calcium: pools, Ca-dependent channels, Nernst / GHK reversal potentials
+
+# opcodes (genesis/src/hines/hines_defs.h)
+_COMPT, _FCOMPT, _LCOMPT = 100, 101, 102
+_CONC_VAL, _NERNST, _GHK = 1000, 2000, 2010
+_CHAN, _CHAN_EK, _ADD_CURR, _FLUX, _SCALE_FLUX = 3000, 3001, 3200, 3300, 3301
+_IPOL1V, _IPOL1C, _NEWVOLT, _NEWCONC1, _DOMAIN = 4001, 4002, 5100, 5110, 5112
+_CACONC, _LCACONC = 10200, 10202
+
+CA_FEATURES = ("flux", "scale", "kc", "domain", "tabcurr", "ctabcurr", "nernst", "ghk")
+
+
+def _segments(m: Model):
+    """Per compartment (ops order): (head op, [(op, operands, chip values)],
+    Em/Rm and inject, tbyc and diagterm) of a model whose programs hold only
+    NEWVOLT / CHAN_EK / CHAN / IPOL1V / ADD_CURR ops."""
+    ops, chip = m.ops, m.chip
+    segs = []
+    p = c = 0
+    while True:
+        head = int(ops[p]); p += 1
+        if head == _LCOMPT:
+            break
+        pre = list(chip[c:c + 2]); c += 2
+        body = []
+        while int(ops[p]) not in (_COMPT, _FCOMPT, _LCOMPT):
+            o = int(ops[p]); p += 1
+            if o == _CHAN_EK:
+                body.append((o, [], list(chip[c:c + 2]))); c += 2
+            elif o == _CHAN:
+                body.append((o, [], list(chip[c:c + 1]))); c += 1
+            elif o == _IPOL1V:
+                nc = 1 if ops[p + 1] != 0 else 0
+                body.append((o, [int(ops[p]), int(ops[p + 1])], list(chip[c:c + nc])))
+                c += nc
+                p += 2
+            elif o in (_NEWVOLT, _ADD_CURR):
+                body.append((o, [], []))
+            else:
+                raise ValueError(f"_segments: op {o} not in the supported subset")
+        segs.append((head, body, pre, list(chip[c:c + 2])))
+        c += 2
+    return segs
+
+
+def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATURES, seed: int = 0,
+                  dt: float = 20e-6, method: int = BEULER, ydivs: int = 1000, ymax: float = 3e-3,
+                  ca_base: float = 5e-5, tau: float = 10e-3, cout: float = 2.0,
+                  inject=(0.2e-9, 1.0e-9), **kw) -> Model:
+    """The population workload (random Hines trees, HH soma, Kdr dendrites)
+    with calcium in every compartment, emitted in the order the reference's
+    SETUP writes it (hines_chip_init.c:806-1235, hines_conc_init.c:779-795):
+
+    * a Ca channel (HH m gate, power 2): Ek fixed ("flux"), from NERNST_OP
+      ("nernst") or from the GHK slope conductance ("ghk"), its current fed
+      into the compartment's Ca_concen pool with FLUX_OP (SCALE_FLUX_OP with
+      "scale" or "ghk");
+    * "kc": a Ca-dependent K channel, Z gate on the pool concentration
+      (NEWCONC1_OP, IPOL1C_OP);
+    * "domain": a second one whose Z gate reads the Ca current of the channel
+      before it (DOMAIN_OP);
+    * "tabcurr" / "ctabcurr": table currents on Vm / on the pool (power-0 IPOL);
+    * one Ca_concen pool per compartment (CACONC_OP: base, 1 + dt/(2 tau),
+      dt B) and a constant outside concentration after it in conc[].
+
+    Concentrations in mM (mol/m^3), the conc table on [0, ymax] with ydivs
+    intervals, sharing tablist with the voltage columns.
+    """
+    feats = set(features)
+    bad = feats - set(CA_FEATURES)
+    if bad:
+        raise ValueError(f"unknown features {sorted(bad)}")
+    if "nernst" in feats and "ghk" in feats:
+        raise ValueError("nernst and ghk both set the Ca channel's Ek; choose one")
+    base = population(n_neurons, sizes, probs, seed=seed, dt=dt, method=method, inject=inject, **kw)
+    rng = np.random.default_rng(seed + 7919)
+    xdivs, ncols0 = base.xdivs, base.ncols
+    rows = max(xdivs, ydivs) + 2
+    tab0 = base.tablist.reshape(xdivs + 2, ncols0)
+    tab = np.zeros((rows, ncols0 + 6))
+    tab[:xdivs + 2, :ncols0] = tab0
+    tab[xdivs + 2:, :ncols0] = tab0[xdivs + 1]
+    invdy = ydivs / ymax
+    C_r = np.arange(ydivs + 1) / invdy
+    V_r = base.xmin + np.arange(xdivs + 1) / base.invdx
+    col_z, col_ct, col_t = ncols0, ncols0 + 2, ncols0 + 4
+    # K_C Z gate: alpha = 500 C/(C + 1e-3), beta = 100 (1/s), as the
+    # non-instant columns 1 + dt/2 (alpha + beta), dt alpha
+    al = 500.0 * C_r / (C_r + 1e-3)
+    be = 100.0
+    zB, zA = 1.0 + dt / 2.0 * (al + be), dt * al
+    def put(col, x, n):
+        tab[:n + 1, col] = x
+        tab[n + 1:, col] = x[-1]
+    put(col_z, zB, ydivs)
+    put(col_z + 1, zA, ydivs)
+    # table currents: G_tab and E_tab
+    put(col_ct, C_r / (C_r + 5e-4), ydivs)
+    put(col_ct + 1, -0.08 + 10.0 * C_r, ydivs)
+    put(col_t, 1.0 / (1.0 + np.exp(-(V_r + 0.04) / 0.005)), xdivs)
+    put(col_t + 1, -0.03 + 0.1 * V_r, xdivs)
+
+    def ss(col, v):
+        """Steady state A/B of a non-instant column at row coordinate v."""
+        r = min(max(int(v), 0), tab.shape[0] - 2)
+        f = v - r
+        B = tab[r, col] + f * (tab[r + 1, col] - tab[r, col])
+        A = tab[r, col + 1] + f * (tab[r + 1, col + 1] - tab[r, col + 1])
+        return A / (2.0 * (B - 1.0))
+
+    RM, Em = kw.get("RM", 0.33), kw.get("Em", -0.065)
+    # the Na m gate column (the soma's power-3 gate) drives the Ca channel
+    col_m = next(int(base.ops[i + 1]) for i in range(base.ops.size - 2)
+                 if base.ops[i] == _IPOL1V and base.ops[i + 2] == 3)
+    segs = _segments(base)
+    ops, chip, conc, concops, concchip = [], [], [], [], []
+    compops, compchips = [], []
+    F, R, T = 96485.3329, 8.314462618, 308.15
+    zFRT = 2.0 * F / (R * T)
+    # The Ek register carries across compartments: CHAN_OP inherits the last
+    # Ek, and the channels added here change it, so a base CHAN_OP whose
+    # inherited value no longer holds becomes a CHAN_EK_OP, as SETUP writes it
+    # (hines_chip_init.c:1029-1049).
+    reg = base_reg = 0.0
+    for r, (head, body, pre, post) in enumerate(segs):
+        compops.append(len(ops)); compchips.append(len(chip))
+        ops.append(head); chip += pre
+        for (o, opnd, vals) in body:
+            if o == _CHAN_EK:
+                base_reg = reg = vals[0]
+            elif o == _CHAN and reg != base_reg:
+                o, vals = _CHAN_EK, [base_reg] + vals
+                reg = base_reg
+            ops += [o] + opnd
+            chip += vals
+        v0 = (base.vm[r] - base.xmin) * base.invdx
+        k = len(conc)
+        area = float(pre[0]) * RM / Em       # Em/Rm = Em area / RM
+        vol = area * 0.2e-6
+        B = 1.0 / (2.0 * F * vol)
+        conc += [ca_base, cout]
+        g_ca = 5.0 * area * rng.uniform(0.5, 1.5)
+        m0 = ss(col_m, v0)
+        if "nernst" in feats:
+            ops += [_CONC_VAL, k, _NERNST, _IPOL1V, col_m, 2, _ADD_CURR, _FLUX]
+            chip += [1.0 / zFRT, g_ca, m0]
+            reg = None
+        elif "ghk" in feats:
+            perm = g_ca / (2.0 * F * zFRT * cout)
+            ops += [_CONC_VAL, k, _GHK, _IPOL1V, col_m, 2, _ADD_CURR, _SCALE_FLUX]
+            chip += [zFRT, 2.0 * F, perm, m0, 1.0]
+            reg = None
+        else:
+            ops += [_CHAN_EK, _IPOL1V, col_m, 2, _ADD_CURR, _CONC_VAL, k]
+            chip += [0.12, g_ca, m0]
+            reg = 0.12
+            if "scale" in feats:
+                ops += [_SCALE_FLUX]
+                chip += [0.8]
+            else:
+                ops += [_FLUX]
+        if "domain" in feats:
+            ops += [_DOMAIN, _CHAN_EK, _IPOL1C, col_z, 1, _ADD_CURR]
+            chip += [2.0e5, -0.09, 5.0 * area, ss(col_z, 0.0)]
+            reg = -0.09
+        if "kc" in feats:
+            ops += [_NEWCONC1, _CHAN_EK, _IPOL1C, col_z, 1, _ADD_CURR]
+            chip += [-0.09, 10.0 * area, ss(col_z, ca_base * invdy)]
+            reg = -0.09
+        if "tabcurr" in feats:
+            ops += [_CHAN, _IPOL1V, col_t, 0]
+            chip += [2.0 * area]
+            reg = None
+        if "ctabcurr" in feats:
+            ops += [_NEWCONC1, _CHAN, _IPOL1C, col_ct, 0]
+            chip += [1.0 * area]
+            reg = None
+        chip += post
+        concops += [_CACONC, k]
+        concchip += [ca_base, 1.0 + dt / (2.0 * tau), dt * B]
+    ops.append(_LCOMPT)
+    concops.append(_LCACONC)
+    import dataclasses
+    return dataclasses.replace(
+        base, ops=np.array(ops, np.int32), chip=np.array(chip, np.float64), ncols=ncols0 + 6,
+        tablist=np.ascontiguousarray(tab.reshape(-1)), compops=np.array(compops, np.int32),
+        compchips=np.array(compchips, np.int32), conc=np.array(conc, np.float64),
+        concops=np.array(concops, np.int32), concchip=np.array(concchip, np.float64),
+        ydivs=ydivs, ymin=0.0, invdy=invdy, givals=np.zeros(0))

@@ -68,7 +68,7 @@ typedef struct hsh_model {
     int32_t method;          /* HSH_BEULER or HSH_CRANK */
     int32_t chanmode;        /* 4 or 5 */
     int32_t calcmode;        /* 1 = interpolated tables (the only supported mode) */
-    int32_t nconcs;          /* concentration pools (conc ops are refused) */
+    int32_t nconcs;          /* entries of conc[] (0: no concentrations) */
     double  dt;
     double  xmin;
     double  invdx;
@@ -82,6 +82,22 @@ typedef struct hsh_model {
     const int32_t *funcs;    /* [nfuncs] */
     const double  *ravals;   /* [nravals] */
     const double  *vm;       /* [ncompts] */
+    /* Concentrations (hines_struct.h:315-376); all zero / NULL without
+     * them.  The concentration-indexed rows of the gate table share tablist
+     * with the voltage rows: with ydivs > 0 tablist holds
+     * (max(xdivs, ydivs)+2)*ncols values, the last row a guard row that
+     * repeats row max(xdivs, ydivs). */
+    int32_t ydivs;           /* conc table rows are ydivs+1; a row is found from
+                                ymin/invdy (hines_chip.c:238-247), so yvals[] is
+                                not needed */
+    int32_t nconcops;        /* concops[] length including LCACONC_OP */
+    int32_t nconcchip;
+    double  ymin;
+    double  invdy;
+    const double  *conc;     /* [nconcs] pool concentrations (Cin, Cout pairs) */
+    const int32_t *concops;  /* [nconcops]: (CACONC_OP k)* LCACONC_OP; other pool
+                                ops (shells, buffers, pumps) are refused */
+    const double  *concchip; /* [nconcchip]: Ca_base, 1 + dt/(2 tau), dt B per pool */
 } hsh_model;
 
 typedef struct hsh_options {
@@ -119,6 +135,9 @@ typedef struct hsh_options {
 #define HSH_R_TREE           6   /* tree rows are not contiguous / kid order not ascending */
 #define HSH_R_CALCMODE       7   /* calcmode 0 (no interpolation) */
 #define HSH_R_SYN_PENDING    8   /* SYN2_OP counter holds a pending event (>= 0) */
+#define HSH_R_POOL           9   /* concentration pool fed or read by several compartments,
+                                    updated with no compartment feeding it, or more
+                                    than 4 pools in one compartment */
 
 typedef struct hsh_refusal {
     int32_t comp;            /* row (Hines number), -1 for whole-solver reasons */
@@ -166,6 +185,9 @@ typedef struct hsh_info {
     double  autotune_s;      /* time it took */
 } hsh_info;
 
+/* opaque solver handle */
+struct hsh_handle;
+
 hsh_status hsh_create(const hsh_model *m, const hsh_options *opt, struct hsh_handle **out);
 hsh_status hsh_refusals(struct hsh_handle *h, hsh_refusal *out, int32_t cap, int32_t *n);
 hsh_status hsh_step(struct hsh_handle *h, int32_t nsteps, int32_t mode, hsh_step_out *out);
@@ -176,6 +198,11 @@ hsh_status hsh_get_state(struct hsh_handle *h, double *vm, double *chip, double 
                          int32_t *ops);
 hsh_status hsh_put_state(struct hsh_handle *h, const double *vm, const double *chip,
                          const int32_t *ops);
+/* Pool concentrations conc[nconcs] (reference indices).  The Ca_concen
+ * update runs on the device in the program of the compartment that feeds the
+ * pool (do_h4_conc_chip_update, hines.c:239-240, hines_conc_chip.c:196-207). */
+hsh_status hsh_get_conc(struct hsh_handle *h, double *conc);
+hsh_status hsh_put_conc(struct hsh_handle *h, const double *conc);
 /* Overwrite n chip[] values (reference indices), e.g. INJECT messages. */
 hsh_status hsh_set_chip(struct hsh_handle *h, int32_t n, const int32_t *idx, const double *val);
 /* Spike events raised by SPIKE_OP since the last drain: ops[] index of the

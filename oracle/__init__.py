@@ -8,6 +8,8 @@ The product library (libhsolve_hip) never loads it.
 of the reference CPU code:
 
 * interpreter  -- genesis/src/hines/hines_chip.c:78-602 (do_chip_hh4_update)
+* Ca_concen    -- genesis/src/hines/hines_conc_chip.c:196-211
+                  (do_h4_conc_chip_update, CACONC_OP / LCACONC_OP only)
 * elimination  -- genesis/src/hines/hines_solve.c:140-296
 * phase order  -- genesis/src/hines/hines.c:222-270
 
@@ -26,8 +28,16 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
 * tangent-linear model: central finite differences of the oracle itself
   (tests/test_oracle_tl.py); the closed-form derivative of the passive
   isopotential recursion.
-* parity unpinned (no reference test reaches them): GHK / NERNST / MGBLOCK /
-  conc ops (libm exp/log), SYN3 (urandom), SPIKE_OP and SYN2 event
-  arithmetic.
+* SPIKE_OP and synaptic events: refractory spacing, arrival step and the
+  SYN2 impulse response in closed form, the Mg block against Jahr-Stevens
+  (tests/test_oracle_events.py).
+* concentrations: the Ca_concen recursion under constant influx in closed
+  form, the Nernst equilibrium, the GHK current and its slope against the
+  textbook GHK equation (both branches), concentration-gated channels
+  settling on the interpolated table (incl. the row cache and the clamps),
+  DOMAIN_OP and table currents (tests/test_oracle_conc.py).
+* parity unpinned (no reference test or closed form reaches them here):
+  SYN3 (urandom), 2-D tables (NEWCONC2, IPOL2V/IPOL2C), ELMMPUMP, TABGHK.
+  The device path refuses all of them.
 """
 from .oracle import Oracle, TangentLinear, oracle_lib  # noqa: F401

@@ -6,6 +6,8 @@
  * shapes, the reference CPU code:
  *   - channel-opcode interpreter: genesis/src/hines/hines_chip.c:78-602
  *     (built as do_chip_hh4_update via hines_4chip.c:19-24);
+ *   - Ca_concen pools: genesis/src/hines/hines_conc_chip.c:196-211
+ *     (do_h4_conc_chip_update; CACONC_OP and LCACONC_OP only);
  *   - Euler / Crank-Nicolson elimination: genesis/src/hines/hines_solve.c
  *     :223-296 and :140-218;
  *   - step phase order: genesis/src/hines/hines.c:222-270.
@@ -48,6 +50,8 @@
 #define NEWCONC1_OP 5110
 #define NEWCONC2_OP 5111
 #define DOMAIN_OP 5112
+#define CACONC_OP 10200   /* concops[] codes: hines_defs.h:214-215 */
+#define LCACONC_OP 10202
 /* funcs codes: hines_defs.h:119-128 */
 #define FORWARD_ELIM 0
 #define BACKWARD_ELIM 0
@@ -676,10 +680,42 @@ void ohs_crank_solve(ohs *h)
     }
 }
 
+/* Ca_concen pools (hines_conc_chip.c:196-207, CHANMODE4: the flux is not
+ * reset here; CONC_VAL_OP resets it in the chip pass).  The reference runs
+ * this pass after the whole chip update and before the solve
+ * (hines.c:238-240). */
+int32_t ohs_conc_chip_update(ohs *h)
+{
+    const int32_t *op = h->a.concops;
+    const double *cc = h->a.concchip;
+    if (!op || h->a.nconcops <= 0) return 0;
+    for (int32_t q = 0; q < h->a.nconcops;) {
+        const int32_t code = op[q++];
+        if (code == LCACONC_OP) return 0;
+        if (code != CACONC_OP) return -1;
+        const int32_t k = op[q++];
+        const double base = *cc++;
+        const double itau = *cc++;
+        h->conc[k] = base + ((h->conc[k] - base) * (2.0 - itau) + (h->flux[k] * *cc++)) / itau;
+    }
+    return 0;
+}
+
+void ohs_get_conc(ohs *h, double *conc)
+{
+    if (h->a.nconcs > 0) memcpy(conc, h->conc, sizeof(double) * h->a.nconcs);
+}
+
+void ohs_put_conc(ohs *h, const double *conc)
+{
+    if (h->a.nconcs > 0) memcpy(h->conc, conc, sizeof(double) * h->a.nconcs);
+}
+
 int32_t ohs_step(ohs *h, int32_t nsteps)
 {
     for (int32_t s = 0; s < nsteps; s++) {
         if (ohs_chip_update(h)) return -1;
+        if (ohs_conc_chip_update(h)) return -1;
         if (h->a.method == CRANK_INT) ohs_crank_solve(h);
         else ohs_euler_solve(h);
         h->step++;

@@ -49,6 +49,9 @@ typedef struct ohs_arrays {
     const double  *vm;
     const double  *givals;    /* may be NULL: then no givals are kept */
     const double  *conc;      /* nconcs or NULL */
+    const int32_t *concops;   /* nconcops or NULL: (CACONC_OP k)* LCACONC_OP */
+    const double  *concchip;  /* nconcchip or NULL: Ca_base, 1 + dt/(2 tau), dt B per pool */
+    int32_t nconcops, nconcchip;
 } ohs_arrays;
 
 ohs    *ohs_create(const ohs_arrays *a);
@@ -60,7 +63,13 @@ void    ohs_set_hooks(ohs *h, ohs_syn_fn syn, ohs_rand_fn rnd, void *ctx);
 int32_t ohs_chip_update(ohs *h);
 void    ohs_euler_solve(ohs *h);
 void    ohs_crank_solve(ohs *h);
-/* nsteps full steps: chip update then the elimination selected by method. */
+/* Ca_concen pass over concops[] (do_h4_conc_chip_update).  Returns 0 or -1
+ * on a pool opcode other than CACONC_OP / LCACONC_OP. */
+int32_t ohs_conc_chip_update(ohs *h);
+void    ohs_get_conc(ohs *h, double *conc);
+void    ohs_put_conc(ohs *h, const double *conc);
+/* nsteps full steps: chip update, the Ca_concen pass, then the elimination
+ * selected by method. */
 int32_t ohs_step(ohs *h, int32_t nsteps);
 
 /* state access (copies) */

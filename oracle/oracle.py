@@ -25,6 +25,7 @@ class Arrays(C.Structure):
         ("ops", _i32p), ("chip", _f64p), ("tablist", _f64p), ("xvals", _f64p),
         ("yvals", _f64p), ("stablist", _f64p), ("funcs", _i32p), ("ravals", _f64p),
         ("vm", _f64p), ("givals", _f64p), ("conc", _f64p),
+        ("concops", _i32p), ("concchip", _f64p), ("nconcops", C.c_int32), ("nconcchip", C.c_int32),
     ]
 
 
@@ -51,6 +52,10 @@ def oracle_lib():
         L.ohs_crank_solve.argtypes = [C.c_void_p]
         L.ohs_step.argtypes = [C.c_void_p, C.c_int32]
         L.ohs_step.restype = C.c_int32
+        L.ohs_conc_chip_update.argtypes = [C.c_void_p]
+        L.ohs_conc_chip_update.restype = C.c_int32
+        L.ohs_get_conc.argtypes = [C.c_void_p, _f64p]
+        L.ohs_put_conc.argtypes = [C.c_void_p, _f64p]
         L.ohs_get.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _i32p, _f64p]
         L.ohs_put.argtypes = [C.c_void_p, _f64p, _f64p, _i32p]
         L.ohs_nspikes.argtypes = [C.c_void_p]
@@ -103,6 +108,11 @@ class Oracle:
         a = Arrays()
         a.ncompts, a.nchips, a.nops = m.ncompts, k["chip"].size, k["ops"].size
         a.nfuncs, a.nravals = k["funcs"].size, k["ravals"].size
+        # concentration arrays: the model's unless given here
+        if conc is None and getattr(m, "conc", None) is not None and np.size(m.conc):
+            conc = m.conc
+        if not ydivs and getattr(m, "ydivs", 0):
+            ydivs, ymin, invdy = m.ydivs, m.ymin, m.invdy
         a.ncols, a.xdivs, a.ydivs = m.ncols, m.xdivs, ydivs
         a.nstab, a.ngivals, a.nelm_names = k["stablist"].size // 4, k["givals"].size, nelm_names
         a.method, a.chanmode, a.storemode = m.method, m.chanmode, storemode
@@ -110,6 +120,14 @@ class Oracle:
         a.ops, a.chip = _p(k["ops"], C.c_int32), _p(k["chip"], C.c_double)
         a.tablist = _p(k["tablist"], C.c_double) if k["tablist"].size else None
         a.xvals = _p(k["xvals"], C.c_double) if k["xvals"].size else None
+        if yvals is None and ydivs > 0:
+            # accumulated as SETUP fills them (hines_chip_init.c:1291-1297),
+            # plus one guard entry for the row-cache bound at the top row
+            yvals = np.empty(ydivs + 2)
+            x, dx = ymin, 1.0 / invdy
+            for j in range(ydivs + 2):
+                yvals[j] = x
+                x += dx
         if yvals is not None:
             k["yvals"] = np.ascontiguousarray(yvals, np.float64)
             a.yvals = _p(k["yvals"], C.c_double)
@@ -121,6 +139,11 @@ class Oracle:
             k["conc"] = np.ascontiguousarray(conc, np.float64)
             a.nconcs = k["conc"].size
             a.conc = _p(k["conc"], C.c_double)
+        if getattr(m, "concops", None) is not None and np.size(m.concops):
+            k["concops"] = np.ascontiguousarray(m.concops, np.int32)
+            k["concchip"] = np.ascontiguousarray(m.concchip, np.float64)
+            a.concops, a.nconcops = _p(k["concops"], C.c_int32), k["concops"].size
+            a.concchip, a.nconcchip = _p(k["concchip"], C.c_double), k["concchip"].size
         self._a = a
         self.h = L.ohs_create(C.byref(a))
         self._hooks = None
@@ -172,6 +195,18 @@ class Oracle:
         oracle_lib().ohs_get(self.h, _p(vm, C.c_double), _p(chip, C.c_double), _p(res, C.c_double),
                              _p(ops, C.c_int32), _p(giv, C.c_double))
         return dict(vm=vm, chip=chip, results=res, ops=ops, givals=giv)
+
+    def conc(self) -> np.ndarray:
+        out = np.empty(self._a.nconcs)
+        if out.size:
+            oracle_lib().ohs_get_conc(self.h, _p(out, C.c_double))
+        return out
+
+    def put_conc(self, conc):
+        a = np.ascontiguousarray(conc, np.float64)
+        assert a.size == self._a.nconcs
+        if a.size:
+            oracle_lib().ohs_put_conc(self.h, _p(a, C.c_double))
 
     def put(self, vm=None, chip=None, ops=None):
         keep = []

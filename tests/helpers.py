@@ -61,3 +61,80 @@ def c3_model(n_neurons: int, *, seed: int = 4, sizes=(1, 8, 16, 64, 256, 1024), 
 
 def max_abs(a, b) -> float:
     return float(np.max(np.abs(np.asarray(a) - np.asarray(b)))) if np.size(a) else 0.0
+
+
+def spliced(base: M.Model, add, *, conc=(), concops=(), concchip=(), extra_cols=None, ydivs=0,
+            ymin=0.0, invdy=0.0) -> M.Model:
+    """A copy of ``base`` (NEWVOLT / CHAN_EK / CHAN / IPOL1V / ADD_CURR
+    programs) with ``add(row) -> (ops, chips)`` appended to each compartment's
+    program, before its tbyc/diagterm; the concentration arrays as given and
+    ``extra_cols`` (rows x 2k) appended to the gate table."""
+    import dataclasses
+    segs = M._segments(base)
+    ops, chip, compops, compchips = [], [], [], []
+    for r, (head, body, pre, post) in enumerate(segs):
+        compops.append(len(ops)); compchips.append(len(chip))
+        ops.append(head); chip += pre
+        for (o, opnd, vals) in body:
+            ops += [o] + opnd
+            chip += vals
+        a_ops, a_chip = add(r)
+        ops += list(a_ops); chip += list(a_chip)
+        chip += post
+    ops.append(M._LCOMPT)
+    tab, ncols = base.tablist, base.ncols
+    if extra_cols is not None:
+        t0 = base.tablist.reshape(base.xdivs + 2, base.ncols) if base.ncols else np.zeros((0, 0))
+        rows = max(t0.shape[0], extra_cols.shape[0])
+        t = np.zeros((rows, base.ncols + extra_cols.shape[1]))
+        if base.ncols:
+            t[:t0.shape[0], :base.ncols] = t0
+            t[t0.shape[0]:, :base.ncols] = t0[-1]
+        t[:extra_cols.shape[0], base.ncols:] = extra_cols
+        t[extra_cols.shape[0]:, base.ncols:] = extra_cols[-1]
+        tab, ncols = np.ascontiguousarray(t.reshape(-1)), t.shape[1]
+    xvals, xdivs, xmin, invdx = base.xvals, base.xdivs, base.xmin, base.invdx
+    if not base.ncols and extra_cols is not None:
+        # a table with only concentration columns still needs a voltage axis
+        xdivs, xmin = extra_cols.shape[0] - 2, -0.1
+        invdx = xdivs / 0.15
+        xvals = xmin + np.arange(xdivs + 2) / invdx
+    return dataclasses.replace(
+        base, ops=np.array(ops, np.int32), chip=np.array(chip, np.float64), tablist=tab, ncols=ncols,
+        xdivs=xdivs, xvals=xvals, xmin=xmin, invdx=invdx, compops=np.array(compops, np.int32),
+        compchips=np.array(compchips, np.int32), conc=np.array(conc, np.float64),
+        concops=np.array(concops, np.int32), concchip=np.array(concchip, np.float64), ydivs=ydivs,
+        ymin=ymin, invdy=invdy, givals=np.zeros(0))
+
+
+MGBLOCK_OP = 4200
+
+
+def with_mgblock(m: M.Model, which, A: float = 3.57, invB: float = 62.0, Mg: float = 1.2):
+    """A copy of ``m`` whose synchans ``which`` (indices into m.syn_op) carry
+    a Mg block: MGBLOCK_OP after the four SYN2_OP words and the chip values
+    KMg_A, 1/KMg_B, [Mg] after X and Y, as SETUP writes them
+    (hines_chip_init.c:1180-1187).  Returns (model, remap) where remap maps
+    old ops indices to new ones (for connection tables)."""
+    import dataclasses
+    which = sorted(set(int(w) for w in which))
+    op_at = [int(m.syn_op[w]) + 4 for w in which]          # insertion points in ops[]
+    ch_at = [int(m.syn_chip[w]) + 2 for w in which]        # insertion points in chip[]
+    ops = np.insert(np.asarray(m.ops), op_at, MGBLOCK_OP).astype(np.int32)
+    chip = np.asarray(m.chip, np.float64)
+    chip = np.insert(chip, np.repeat(ch_at, 3), np.tile([A, invB, Mg], len(ch_at)))
+    op_at_a, ch_at_a = np.array(op_at), np.array(ch_at)
+
+    def remap(q):
+        q = np.asarray(q)
+        return q + np.searchsorted(op_at_a, q, side="right")
+
+    def remap_chip(q):
+        q = np.asarray(q)
+        return q + 3 * np.searchsorted(ch_at_a, q, side="right")
+    out = dataclasses.replace(
+        m, ops=ops, chip=chip, compops=remap(m.compops).astype(np.int32),
+        compchips=remap_chip(m.compchips).astype(np.int32),
+        spike_op=remap(m.spike_op).astype(np.int32), syn_op=remap(m.syn_op).astype(np.int32),
+        syn_chip=remap_chip(m.syn_chip).astype(np.int32))
+    return out, remap

@@ -113,6 +113,14 @@ def test_accepts_synchan_and_checks_connections():
             h.set_synapses([src], [dst], [1.0], [0.0] if dst > src else [-1.0])
 
 
+def test_accepts_mg_block():
+    from helpers import with_mgblock
+    m, remap = with_mgblock(synchan_model(), [0])
+    with D.HipSolver(m, host_only=True) as h:
+        assert h.refusals() == []
+        h.set_synapses([int(m.spike_op[0])], [int(m.syn_op[0])], [1.0], [1.5e-3])
+
+
 def test_refuses_pending_synaptic_event_with_reason():
     m = synchan_model()
     ops = np.array(m.ops)
@@ -173,3 +181,21 @@ def test_generated_source_compiles_for_gfx950():
     L.hiprtcGetCodeSize(prog, C.byref(n))
     assert n.value > 0
     L.hiprtcDestroyProgram(C.byref(prog))
+
+
+def test_header_is_clean_c(tmp_path):
+    """include/hsolve_hip.h compiles as C99 and C++ with warnings as errors,
+    and a C caller can pass the opaque handle between calls."""
+    import subprocess
+    src = tmp_path / "use.c"
+    src.write_text('#include "hsolve_hip.h"\n'
+                   'int f(const hsh_model *m) {\n'
+                   '    struct hsh_handle *h = 0; hsh_step_out o; double c[1];\n'
+                   '    if (hsh_create(m, 0, &h) != HSH_OK) return 1;\n'
+                   '    hsh_step(h, 1, HSH_PLAIN, &o); hsh_get_conc(h, c);\n'
+                   '    hsh_destroy(h); return 0;\n}\n')
+    inc = os.path.join(ROOT, "include")
+    for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
+        r = subprocess.run([cc, std, "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I", inc,
+                            "-x", "c" if cc == "gcc" else "c++", str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

@@ -82,3 +82,34 @@ def test_event_arrival_both_op_orders(gpu, mode):
             got = g.drain_spikes()
         want = o.spikes()
         assert np.array_equal(got[1], want[1]) and np.array_equal(got[0], want[0])
+
+
+@pytest.mark.parametrize("mode,kw", [
+    (D.PLAIN, dict(flags=D.F_NO_AUTOTUNE)),
+    (D.BATCHED, dict(batch_k=15, flags=D.F_NO_AUTOTUNE)),
+    (D.BATCHED, dict(batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=256)),
+], ids=["plain", "iso-K15", "packs-K8"])
+def test_network_with_mg_block(gpu, mode, kw):
+    """Excitatory synchans with a Mg block (NMDA-like): the block calls the
+    device's exp, so vm is held to 1e-7 V; the raster must still be the
+    oracle's."""
+    from helpers import with_mgblock
+    m0, table0 = M.network(400, seed=3, inject=(0.3e-9, 0.8e-9))
+    m, remap = with_mgblock(m0, range(0, m0.syn_op.size, 2))
+    table = (remap(table0[0]), remap(table0[1]), table0[2], table0[3])
+    T = 1500
+    print(f"[case] network + Mg block, mode {mode} {kw}", flush=True)
+    o = Oracle(m, givals=False)
+    o.set_synapses(*table)
+    o.step(T)
+    want_op, want_step = o.spikes()
+    assert want_op.size > 100
+    with D.HipSolver(m, **kw) as g:
+        g.set_synapses(*table)
+        g.step(T, mode)
+        got_op, got_step = g.drain_spikes()
+        gs = g.state()
+    os_ = o.state()
+    assert float(np.max(np.abs(os_["vm"] - gs["vm"]))) <= 1e-7
+    assert np.array_equal(os_["ops"], gs["ops"])
+    assert np.array_equal(got_step, want_step) and np.array_equal(got_op, want_op)

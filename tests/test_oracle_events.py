@@ -79,3 +79,30 @@ def test_event_arrival_step_and_impulse_response():
         assert np.allclose(X[t0:], wantX, rtol=1e-12, atol=0)
         assert np.allclose(Y[t0:], wantY, rtol=1e-11, atol=0)
     assert orders == {True, False}
+
+
+def test_mgblock_is_jahr_stevens():
+    """MGBLOCK_OP: g = g_syn / (1 + [Mg]/A exp(-V/B)) (Jahr & Stevens 1990),
+    with KMg_A = A, 1/KMg_B = 1/B in the chip values."""
+    from helpers import with_mgblock
+    A, invB, Mg = 3.57, 62.0, 1.2
+    for V in (-0.08, -0.03, 0.0, 0.02):
+        comps = dict(parent=[-1], Cm=[1e-11], Rm=[1e8], Ra=[1e6], Em=[-0.065], initVm=[V], inject=[0.0])
+        m0 = M.build(comps, [], [], dt=DT, syntypes=[dict(Ek=0.0, tau1=5e-3, tau2=1e-3)], syns=[(0, 0, 1e-9)],
+                     float_fields=False)
+        m, _ = with_mgblock(m0, [0], A, invB, Mg)
+        xs = int(m.syn_chip[0])
+        chip = m.chip.copy()
+        chip[xs], chip[xs + 1] = 0.7, 0.3                      # X, Y
+        m.chip = chip
+        o = Oracle(m, givals=False)
+        o.chip_update()
+        k = int(m.ops[m.syn_op[0] + 1])
+        x2, y1, y2, _ = m.stablist[k:k + 4]
+        Y1 = 0.7 * x2 * y1 + 0.3 * y2
+        g_syn = chip[xs - 1] * Y1
+        want = g_syn / (1 + Mg / A * np.exp(-V * invB))
+        st = o.state()
+        tbyc, dterm = m.chip[-2], m.chip[-1]
+        G = (st["results"][1] - dterm) / tbyc
+        assert abs(G - want) <= 1e-9 * want, V
