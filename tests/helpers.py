@@ -123,7 +123,7 @@ def with_mgblock(m: M.Model, which, A: float = 3.57, invB: float = 62.0, Mg: flo
     ops = np.insert(np.asarray(m.ops), op_at, MGBLOCK_OP).astype(np.int32)
     chip = np.asarray(m.chip, np.float64)
     chip = np.insert(chip, np.repeat(ch_at, 3), np.tile([A, invB, Mg], len(ch_at)))
-    op_at_a, ch_at_a = np.array(op_at), np.array(ch_at)
+    op_at_a, ch_at_a = np.sort(op_at), np.sort(ch_at)
 
     def remap(q):
         q = np.asarray(q)
