@@ -86,6 +86,7 @@ struct hsh_handle {
     int ring_len = 0;                  // 0: no connections
     int max_k = 1 << 30;               // batched steps per launch allowed (min delay)
     int nconn = 0;
+    long long tab_max = 0;             // largest start offset of a register-based 2-D lookup
     int32_t *d_spk = nullptr, *d_nspk = nullptr;
     // layout-dependent device buffers
     BlockMeta* d_blocks = nullptr;
@@ -372,6 +373,7 @@ void set_devmodel(hsh_handle* h)
     d.crank = (h->m.method == HSH_CRANK);
     d.xmin = h->m.xmin;
     d.invdx = h->m.invdx;
+    d.tab_max = h->tab_max;
     d.ymin = h->m.ymin;
     d.invdy = h->m.invdy;
     d.ydivs = h->m.ydivs;
@@ -755,7 +757,7 @@ bool adj_supported(const Plan& P, std::string& why)
                 why = "synaptic channels";
                 return false;
             } else if (op >= U_CONCVAL) {
-                why = "concentration-dependent ops";
+                why = "concentration-dependent ops, 2-D tables or GHK";
                 return false;
             } else if (op == U_END) {
                 break;
@@ -940,8 +942,13 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     // voltage rows and concentration rows share the table
     const size_t trows = (size_t)std::max(m->xdivs, m->ydivs) + 2;
     const size_t ntab = (m->ncols > 0) ? trows * m->ncols : 1;
+    // zero padding past the table: a 2-D lookup at the top concentration row
+    // reads up to 2*ydivs+3 values past its row (with weight 0)
+    const size_t tpad = 2 * (size_t)std::max(0, m->ydivs) + 4 + (size_t)std::max(0, m->ncols);
     const size_t nx = (m->ncols > 0) ? (size_t)(m->xdivs + 2) : 1;
-    HIPCHK_C(dalloc(&h->d_tab, ntab));
+    HIPCHK_C(dalloc(&h->d_tab, ntab + tpad));
+    HIPCHK_C(hipMemset(h->d_tab, 0, (ntab + tpad) * sizeof(double)));
+    h->tab_max = (long long)(ntab + tpad) - std::max(0, m->ncols) - 4;
     HIPCHK_C(dalloc(&h->d_xvals, nx));
     HIPCHK_C(dalloc(&h->d_stab, (size_t)std::max(1, 4 * m->nstab)));
     if (m->nstab > 0 && m->stablist)

@@ -155,16 +155,34 @@ __device__ __forceinline__ void newconc(const DevModel& d, double concen, int& c
     cipol = v - (double)f;
 }
 
-// Table current (hines_chip.c:472-481): Ek = A(x), Gk *= B(x).
-__device__ __forceinline__ void tabcurr(const double* __restrict__ tb, int ncols, double ipol, double& Ek,
-                                        double& Gk)
+// Table lookup of the IPOL ops (hines_chip.c:419-458, :567-584).  which:
+// 0 = 1-D on the voltage row, 1 = 1-D on the concentration row, 2 = 2-D over
+// voltage and concentration, 3 = 2-D over two concentrations from the table
+// register (the position the previous lookup left, hines_chip.c:569).  The
+// table register moves to one past the column read, as the reference's does.
+__device__ __forceinline__ void ipol_xy(const DevModel& d, int which, int col, const double* vtab,
+                                        const double* ctab, const double*& tabr, int cfilo, double vipol,
+                                        double cipol, double ipol, double& X, double& Y)
 {
-    double X = tb[0];
-    X = X + ipol * (tb[ncols] - X);
-    double Y = tb[1];
-    Y = Y + ipol * (tb[ncols + 1] - Y);
-    Ek = Y;
-    Gk *= X;
+    const int nc = d.ncols;
+    const double* tb;
+    if (which <= 1) {
+        tb = (which == 0 ? vtab : ctab) + col;
+        const double f = which == 0 ? vipol : cipol;
+        X = tb[0];
+        X = X + f * (tb[nc] - X);
+        Y = tb[1];
+        Y = Y + f * (tb[nc + 1] - Y);
+    } else {
+        long long off = (long long)((which == 2 ? vtab : tabr) - d.tab) + 2LL * cfilo + col;
+        if (which == 3) off = off < 0 ? 0 : (off > d.tab_max ? d.tab_max : off);
+        tb = d.tab + off;
+        const double f = which == 2 ? vipol : ipol;
+        const double* tn = tb + nc;
+        X = (1.0 - f) * ((1.0 - cipol) * tb[0] + cipol * tb[2]) + f * ((1.0 - cipol) * tn[0] + cipol * tn[2]);
+        Y = (1.0 - f) * ((1.0 - cipol) * tb[1] + cipol * tb[3]) + f * ((1.0 - cipol) * tn[1] + cipol * tn[3]);
+    }
+    tabr = tb + 1;
 }
 
 // GHK_OP (hines_chip.c:334-365): slope conductance and the Ek that gives the
@@ -257,6 +275,9 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
     double Gk = 0.0, Ek = 0.0, Ik = 0.0;
     double vipol = 0.0, cipol = 0.0;
     Flux flux;
+    int cfilo = 0;
+    double ipol = 0.0, ghk = 0.0;
+    const double* tabr = d.tab;
     const double* vtab = d.tab;
     const double* ctab = d.tab;
     const int ncols = d.ncols;
@@ -269,6 +290,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             const int k = code[pc + 1];
             pc += 2;
             const double X = gate_update(vtab, col, ncols, vipol, SGET(c), k < 0);
+            tabr = vtab + col + 1;
             SPUT(c, X);
             c++;
             Gk = apply_power(Gk, X, k < 0 ? -k : k);
@@ -313,25 +335,49 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
                 concen = SGET(c) * Ik;
                 c++;
             }
-            int cf;
-            newconc(d, concen, cf, cipol);
-            ctab = d.tab + (long long)cf * ncols;
+            newconc(d, concen, cfilo, cipol);
+            ctab = d.tab + (long long)cfilo * ncols;
+        } else if (op == U_NEWCONC2) {
+            tabr = ctab;
+            ipol = cipol;
+            newconc(d, SGET(code[pc++]), cfilo, cipol);
+            ctab = d.tab + (long long)cfilo * ncols;
         } else if (op == U_IPOL1C) {
             const int col = code[pc];
             const int k = code[pc + 1];
             pc += 2;
             const double X = gate_update(ctab, col, ncols, cipol, SGET(c), k < 0);
+            tabr = ctab + col + 1;
             SPUT(c, X);
             c++;
             Gk = apply_power(Gk, X, k < 0 ? -k : k);
-        } else if (op == U_TABCURR) {
+        } else if (op == U_IPOLG) {
+            const int which = code[pc], col = code[pc + 1], k = code[pc + 2];
+            pc += 3;
+            double X, Y;
+            ipol_xy(d, which, col, vtab, ctab, tabr, cfilo, vipol, cipol, ipol, X, Y);
+            X = (k > 0) ? (SGET(c) * (2.0 - X) + Y) / X : Y / X;
+            SPUT(c, X);
+            c++;
+            Gk = apply_power(Gk, X, k < 0 ? -k : k);
+        } else if (op == U_TABCURR || op == U_TABGHK) {
             const int col = code[pc];
             const int which = code[pc + 1];
             pc += 2;
-            tabcurr((which ? ctab : vtab) + col, ncols, which ? cipol : vipol, Ek, Gk);
-            sumgchan += Gk;
-            ichan += Ek * Gk;
-            Ik = Gk * (Ek - Vm);
+            double X, Y;
+            ipol_xy(d, which, col, vtab, ctab, tabr, cfilo, vipol, cipol, ipol, X, Y);
+            Ek = Y;
+            if (op == U_TABGHK) {
+                ghk = X;
+            } else {
+                Gk *= X;
+                sumgchan += Gk;
+                ichan += Ek * Gk;
+                Ik = Gk * (Ek - Vm);
+            }
+        } else if (op == U_SAMEGHK) {
+            Gk = SGET(c) * ghk;
+            c++;
         } else if (op == U_FLUX) {
             const int p = code[pc++];
             flux.set(p, flux.get(p) + Ik);
@@ -350,6 +396,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             pc += 2;
             double slope;
             ghk_op(Vm, SGET(ci), SGET(co), SGET(c), SGET(c + 1), Ek, slope);
+            ghk = slope;
             Gk = SGET(c + 2) * slope;
             c += 3;
         } else if (op == U_POOLCA) {

@@ -38,6 +38,12 @@ enum UOp : int {
                       // Ca_concen update
     U_SYN2MG = 18,    // U_SYN2, then the Mg block before the current is added:
                       // X = S[c+2] exp(Vm S[c+3]); Gk *= X / (X + S[c+4])
+    U_NEWCONC2 = 19,  // operand: conc slot; table register = conc row, ipol = cipol, then the
+                      // second concentration's row
+    U_IPOLG = 20,     // operands: which (2: Vm x conc, 3: conc x conc), column, power; 2-D gate
+    U_TABGHK = 21,    // operands: column, which (0..3); Ek = A, GHK slope = B (no current)
+    U_SAMEGHK = 22,   // Gk = S[c++] * GHK slope
+    // U_TABCURR's "which" takes the same four values as U_IPOLG / U_TABGHK
 };
 constexpr int MAXPOOL = 4;
 constexpr int PROG_HEADER = 2;
@@ -94,7 +100,8 @@ struct DevModel {
     const double* tcf;    // A[row][parent] forward, [block][lane]
     const double* tcb;    // A[row][parent] backward, [block][lane]
     const int* pred;      // position of the previous NEWVOLT compartment, -1 none
-    const double* tab;    // composite table (xdivs+2)*ncols
+    const double* tab;    // composite table (max(xdivs, ydivs)+2)*ncols, zero padded
+    long long tab_max;    // largest start offset a 2-D lookup through the table register may use
     const double* xvals;  // xdivs+2
     int* spk;             // spike ring: [cap][2] (ops index, step)
     int* nspk;            // spike count

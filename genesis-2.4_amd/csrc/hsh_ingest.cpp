@@ -102,6 +102,10 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         bool ek_intra = false;    // Ek register set in this compartment
         bool creg_intra = false;  // conc register set in this compartment
         bool crow_intra = false;  // conc table row set in this compartment
+        bool tabr_intra = false;  // table register (last lookup position) set in this compartment
+        bool ipol_intra = false;  // first-concentration fraction (NEWCONC2) set in this compartment
+        bool ghk_intra = false;   // GHK slope register set in this compartment
+        bool tabghk = false;      // DO_TABGHK_OP: the next table lookup sets Ek and the slope
         // slots after tbyc/diagterm: pool concentrations, outside concentrations,
         // pool constants; code positions that hold such a slot index
         std::vector<int64_t> extra;
@@ -197,7 +201,23 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 c += 1;
                 crow_intra = true;
                 break;
-            case NEWCONC2_OP: p += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case NEWCONC2_OP: {
+                // the C1 row becomes the table register, then the row of
+                // conc[k] (hines_chip.c:224-232)
+                if (p >= m.nops) return bad("NEWCONC2_OP operand");
+                const int32_t k = ops[p++];
+                if (k < 0 || k >= nconcs) return bad("NEWCONC2_OP pool index out of range");
+                if (m.ydivs <= 0) return bad("NEWCONC2_OP without a concentration table");
+                if (pool_chip[k] >= 0 && pool_owner[k] != comp) {
+                    refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true;
+                    break;
+                }
+                code.push_back(U_NEWCONC2);
+                push_extra(nchips + k);
+                tabr_intra = ipol_intra = crow_intra;
+                crow_intra = true;
+                break;
+            }
             case ELMMPUMP_OP: c += 3; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
             case FLUX_OP:
             case SCALE_FLUX_OP:
@@ -225,6 +245,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true;
                 } else {
                     code.push_back(op == GHK_OP ? U_GHK : U_NERNST);
+                    if (op == GHK_OP) ghk_intra = true;
                     push_extra(nchips + creg);
                     push_extra(nchips + creg + 1);
                     for (int i = 0; i < nc; i++) dsrc.push_back(c + i);
@@ -235,9 +256,21 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 break;
             }
             case SAME_GHK_OP:
-                c += 1; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; to_cond = true; break;
+                // gmax times the slope of the last GHK_OP / table GHK (hines_chip.c:358-365)
+                if (!need_chip(1)) return bad("chip[] too short");
+                if (!ghk_intra) {
+                    refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true;
+                } else {
+                    code.push_back(U_SAMEGHK);
+                    dsrc.push_back(c);
+                }
+                c += 1;
+                to_cond = true;
+                break;
             case DO_TABGHK_OP:
-                refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; to_cond = true; break;
+                tabghk = true;
+                to_cond = true;
+                break;
             case CHAN_EK_OP:
                 if (!need_chip(2)) return bad("chip[] too short");
                 code.push_back(U_CHAN_EK);
@@ -291,29 +324,51 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                     int32_t col = ops[p], pw = ops[p + 1];
                     p += 2;
                     if (pw != 0 && !need_chip(1)) return bad("chip[] too short");
-                    if (k != IPOL1V_OP && k != IPOL1C_OP) {
-                        refuse(P, comp, k, (int)kidx, HSH_R_OPCODE); refused = true;
-                        if (pw != 0) c += 1;
-                        if (pw == 0) break;
-                        continue;
-                    }
                     if (pw < -6 || pw > 6) return bad("gate power out of range");
-                    if (col < 0 || col + 1 >= m.ncols || !m.tablist)
+                    const bool twod = (k == IPOL2V_OP || k == IPOL2C_OP);
+                    if (col < 0 || col + (twod ? 3 : 1) >= m.ncols || !m.tablist)
                         return bad("IPOL table column out of range");
-                    const bool onconc = (k == IPOL1C_OP);
-                    if (onconc ? !crow_intra : !has_volt[comp]) {
+                    // which table: 0 voltage row, 1 concentration row, 2 voltage x
+                    // concentration, 3 two concentrations through the table register
+                    const int which = k == IPOL1V_OP ? 0 : k == IPOL1C_OP ? 1 : k == IPOL2V_OP ? 2 : 3;
+                    bool live = which == 0 ? has_volt[comp] : crow_intra;
+                    if (which == 2) live = live && has_volt[comp];
+                    if (which == 3) live = live && ipol_intra && tabr_intra;
+                    if (!live) {
                         refuse(P, comp, k, (int)kidx, HSH_R_TABLE_LIVEIN); refused = true;
+                    }
+                    tabr_intra = true;
+                    if (tabghk) {
+                        // table GHK (hines_chip.c:473-477): Ek and the slope
+                        // from the table, no current yet
+                        tabghk = false;
+                        if (pw != 0) {
+                            refuse(P, comp, DO_TABGHK_OP, (int)kidx, HSH_R_OPCODE); refused = true;
+                            c += 1;
+                            continue;
+                        }
+                        code.push_back(U_TABGHK);
+                        code.push_back(col);
+                        code.push_back(which);
+                        ghk_intra = true;
+                        ek = EK_DYN; ek_intra = true;
+                        break;
                     }
                     if (pw == 0) {
                         // table current (hines_chip.c:472-481): Ek from the
                         // table, Gk scaled by it, the current added
                         code.push_back(U_TABCURR);
                         code.push_back(col);
-                        code.push_back(onconc ? 1 : 0);
+                        code.push_back(which);
                         ek = EK_DYN; ek_intra = true;
                         break;
                     }
-                    code.push_back(onconc ? U_IPOL1C : U_IPOL1V);
+                    if (which <= 1) {
+                        code.push_back(which == 1 ? U_IPOL1C : U_IPOL1V);
+                    } else {
+                        code.push_back(U_IPOLG);
+                        code.push_back(which);
+                    }
                     code.push_back(col);
                     code.push_back(pw);
                     dsrc.push_back(c);
@@ -366,6 +421,11 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                           std::to_string(kidx);
                     return false;
                 }
+            }
+            if (tabghk) {
+                // DO_TABGHK_OP not followed by a power-0 table lookup
+                refuse(P, comp, DO_TABGHK_OP, -1, HSH_R_OPCODE); refused = true;
+                tabghk = false;
             }
         }
         if (c + 2 > m.nchips) return bad("chip[] too short at compartment end");

@@ -43,9 +43,10 @@ std::string gen_template(const Template& t, int id, int ncols)
     o += "    double sumgchan = 0.0;\n    double ichan = s0 + s1;\n";
     o += "    double Gk = 0.0, Ek = 0.0, Ik = 0.0, vipol = 0.0, cipol = 0.0;\n";
     o += "    double flux[MAXPOOL] = {0.0, 0.0, 0.0, 0.0};\n";
-    o += "    const double* vtab = d.tab;\n    const double* ctab = d.tab;\n";
+    o += "    const double* vtab = d.tab;\n    const double* ctab = d.tab;\n    const double* tabr = d.tab;\n";
+    o += "    int cfilo = 0;\n    double ipol = 0.0, ghk = 0.0;\n";
     o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)Ik; (void)vipol; (void)cipol;\n";
-    o += "    (void)flux; (void)vtab; (void)ctab;\n";
+    o += "    (void)flux; (void)vtab; (void)ctab; (void)tabr; (void)cfilo; (void)ipol; (void)ghk;\n";
     o += "    (void)pred_ok; (void)Vp;\n";
     const std::string NC = std::to_string(ncols);
     int c = 2;
@@ -70,9 +71,9 @@ std::string gen_template(const Template& t, int id, int ncols)
             const int col = code[pc], k = code[pc + 1];
             pc += 2;
             o += "    { const double X = gate_update(vtab, " + std::to_string(col) + ", " + NC +
-                 ", vipol, " + slot(c) + ", " + (k < 0 ? "true" : "false") + "); Sb[" +
-                 std::to_string(c) + " * W + lane] = X; Gk = apply_power(Gk, X, " +
-                 std::to_string(k < 0 ? -k : k) + "); }\n";
+                 ", vipol, " + slot(c) + ", " + (k < 0 ? "true" : "false") + "); tabr = vtab + " +
+                 std::to_string(col + 1) + "; Sb[" + std::to_string(c) +
+                 " * W + lane] = X; Gk = apply_power(Gk, X, " + std::to_string(k < 0 ? -k : k) + "); }\n";
             c += 1;
             break;
         }
@@ -112,25 +113,51 @@ std::string gen_template(const Template& t, int id, int ncols)
                 conc = slot(c) + " * Ik";
                 c += 1;
             }
-            o += "    { int cf; newconc(d, " + conc + ", cf, cipol); ctab = d.tab + (long long)cf * " + NC + "; }\n";
+            o += "    newconc(d, " + conc + ", cfilo, cipol); ctab = d.tab + (long long)cfilo * " + NC + ";\n";
             break;
         }
+        case U_NEWCONC2:
+            o += "    tabr = ctab; ipol = cipol; newconc(d, " + slot(code[pc++]) +
+                 ", cfilo, cipol); ctab = d.tab + (long long)cfilo * " + NC + ";\n";
+            break;
+        case U_IPOLG: {
+            const int which = code[pc], col = code[pc + 1], k = code[pc + 2];
+            pc += 3;
+            o += "    { double X, Y; ipol_xy(d, " + std::to_string(which) + ", " + std::to_string(col) +
+                 ", vtab, ctab, tabr, cfilo, vipol, cipol, ipol, X, Y); X = " +
+                 (k > 0 ? "(" + slot(c) + " * (2.0 - X) + Y) / X" : std::string("Y / X")) + "; Sb[" +
+                 std::to_string(c) + " * W + lane] = X; Gk = apply_power(Gk, X, " +
+                 std::to_string(k < 0 ? -k : k) + "); }\n";
+            c += 1;
+            break;
+        }
+        case U_TABGHK: {
+            const int col = code[pc], which = code[pc + 1];
+            pc += 2;
+            o += "    { double X, Y; ipol_xy(d, " + std::to_string(which) + ", " + std::to_string(col) +
+                 ", vtab, ctab, tabr, cfilo, vipol, cipol, ipol, X, Y); Ek = Y; ghk = X; }\n";
+            break;
+        }
+        case U_SAMEGHK:
+            o += "    Gk = " + slot(c) + " * ghk;\n";
+            c += 1;
+            break;
         case U_IPOL1C: {
             const int col = code[pc], k = code[pc + 1];
             pc += 2;
             o += "    { const double X = gate_update(ctab, " + std::to_string(col) + ", " + NC +
-                 ", cipol, " + slot(c) + ", " + (k < 0 ? "true" : "false") + "); Sb[" +
-                 std::to_string(c) + " * W + lane] = X; Gk = apply_power(Gk, X, " +
-                 std::to_string(k < 0 ? -k : k) + "); }\n";
+                 ", cipol, " + slot(c) + ", " + (k < 0 ? "true" : "false") + "); tabr = ctab + " +
+                 std::to_string(col + 1) + "; Sb[" + std::to_string(c) +
+                 " * W + lane] = X; Gk = apply_power(Gk, X, " + std::to_string(k < 0 ? -k : k) + "); }\n";
             c += 1;
             break;
         }
         case U_TABCURR: {
             const int col = code[pc], which = code[pc + 1];
             pc += 2;
-            o += std::string("    tabcurr(") + (which ? "ctab" : "vtab") + " + " + std::to_string(col) + ", " + NC +
-                 ", " + (which ? "cipol" : "vipol") +
-                 ", Ek, Gk); sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
+            o += "    { double X, Y; ipol_xy(d, " + std::to_string(which) + ", " + std::to_string(col) +
+                 ", vtab, ctab, tabr, cfilo, vipol, cipol, ipol, X, Y); Ek = Y; Gk *= X; }"
+                 " sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
             break;
         }
         case U_FLUX:
@@ -151,7 +178,7 @@ std::string gen_template(const Template& t, int id, int ncols)
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
             o += "    { double sl; ghk_op(Vm, " + slot(ci) + ", " + slot(co) + ", " + slot(c) + ", " + slot(c + 1) +
-                 ", Ek, sl); Gk = " + slot(c + 2) + " * sl; }\n";
+                 ", Ek, sl); ghk = sl; Gk = " + slot(c + 2) + " * sl; }\n";
             c += 3;
             break;
         }

@@ -372,7 +372,9 @@ _CHAN, _CHAN_EK, _ADD_CURR, _FLUX, _SCALE_FLUX = 3000, 3001, 3200, 3300, 3301
 _IPOL1V, _IPOL1C, _NEWVOLT, _NEWCONC1, _DOMAIN = 4001, 4002, 5100, 5110, 5112
 _CACONC, _LCACONC = 10200, 10202
 
-CA_FEATURES = ("flux", "scale", "kc", "domain", "tabcurr", "ctabcurr", "nernst", "ghk")
+CA_FEATURES = ("flux", "scale", "kc", "domain", "tabcurr", "ctabcurr", "nernst", "ghk", "kc2d", "c1c2",
+               "tabghk", "sameghk")
+_IPOL2V, _IPOL2C, _NEWCONC2, _SAME_GHK, _DO_TABGHK = 4003, 4004, 5111, 3002, 3003
 
 
 def _segments(m: Model):
@@ -409,7 +411,7 @@ def _segments(m: Model):
 
 
 def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATURES, seed: int = 0,
-                  dt: float = 20e-6, method: int = BEULER, ydivs: int = 1000, ymax: float = 3e-3,
+                  dt: float = 20e-6, method: int = BEULER, ydivs: int = 1000, ymax: float = 0.05,
                   ca_base: float = 5e-5, tau: float = 10e-3, cout: float = 2.0,
                   inject=(0.2e-9, 1.0e-9), **kw) -> Model:
     """The population workload (random Hines trees, HH soma, Kdr dendrites)
@@ -425,8 +427,15 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
     * "domain": a second one whose Z gate reads the Ca current of the channel
       before it (DOMAIN_OP);
     * "tabcurr" / "ctabcurr": table currents on Vm / on the pool (power-0 IPOL);
+    * "kc2d": a K channel whose gate is a 2-D table over Vm and the pool
+      (IPOL2V); "c1c2": one over the pool and a second, constant
+      concentration (NEWCONC2 + IPOL2C through the table register);
+    * "tabghk": a channel whose Ek and GHK slope come from a voltage table
+      (DO_TABGHK_OP, power-0 IPOL, SAME_GHK_OP); "sameghk" (with "ghk"): a
+      second channel on the Ca channel's GHK slope (SAME_GHK_OP);
     * one Ca_concen pool per compartment (CACONC_OP: base, 1 + dt/(2 tau),
-      dt B) and a constant outside concentration after it in conc[].
+      dt B), a constant outside concentration after it in conc[] and a third,
+      constant concentration (1 mM) for "c1c2".
 
     Concentrations in mM (mol/m^3), the conc table on [0, ymax] with ydivs
     intervals, sharing tablist with the voltage columns.
@@ -437,12 +446,16 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
         raise ValueError(f"unknown features {sorted(bad)}")
     if "nernst" in feats and "ghk" in feats:
         raise ValueError("nernst and ghk both set the Ca channel's Ek; choose one")
+    if "sameghk" in feats and "ghk" not in feats:
+        raise ValueError("sameghk reuses the Ca channel's GHK slope; add ghk")
     base = population(n_neurons, sizes, probs, seed=seed, dt=dt, method=method, inject=inject, **kw)
     rng = np.random.default_rng(seed + 7919)
     xdivs, ncols0 = base.xdivs, base.ncols
     rows = max(xdivs, ydivs) + 2
     tab0 = base.tablist.reshape(xdivs + 2, ncols0)
-    tab = np.zeros((rows, ncols0 + 6))
+    w2 = 2 * (ydivs + 1)                          # columns of one 2-D gate
+    n2d = ("kc2d" in feats) + ("c1c2" in feats)
+    tab = np.zeros((rows, ncols0 + 6 + n2d * w2))
     tab[:xdivs + 2, :ncols0] = tab0
     tab[xdivs + 2:, :ncols0] = tab0[xdivs + 1]
     invdy = ydivs / ymax
@@ -464,6 +477,29 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
     put(col_ct + 1, -0.08 + 10.0 * C_r, ydivs)
     put(col_t, 1.0 / (1.0 + np.exp(-(V_r + 0.04) / 0.005)), xdivs)
     put(col_t + 1, -0.03 + 0.1 * V_r, xdivs)
+    # 2-D gates: B' at col + 2j, A' at col + 2j + 1 for concentration row j
+    nxt = ncols0 + 6
+    col_2v = col_cc = -1
+    if "kc2d" in feats:
+        col_2v, nxt = nxt, nxt + w2
+        a2 = (300.0 * C_r / (C_r + 1e-3))[None, :] * np.exp(V_r / 0.03)[:, None]
+        tab[:xdivs + 1, col_2v:col_2v + w2:2] = 1.0 + dt / 2.0 * (a2 + 80.0)
+        tab[:xdivs + 1, col_2v + 1:col_2v + w2:2] = dt * a2
+        tab[xdivs + 1:, col_2v:col_2v + w2] = tab[xdivs, col_2v:col_2v + w2]
+    if "c1c2" in feats:
+        col_cc, nxt = nxt, nxt + w2
+        a2 = (400.0 * C_r / (C_r + 5e-4))[:, None] * (1.0 / (1.0 + C_r / 1e-3))[None, :]
+        tab[:ydivs + 1, col_cc:col_cc + w2:2] = 1.0 + dt / 2.0 * (a2 + 150.0)
+        tab[:ydivs + 1, col_cc + 1:col_cc + w2:2] = dt * a2
+        tab[ydivs + 1:, col_cc:col_cc + w2] = tab[ydivs, col_cc:col_cc + w2]
+
+    def ss2(col, r_a, f_a, r_b, f_b):
+        """Steady state of a non-instant 2-D gate (rows r_a, fraction f_a;
+        concentration row r_b, fraction f_b)."""
+        t0, t1 = tab[r_a, col + 2 * r_b:], tab[r_a + 1, col + 2 * r_b:]
+        B = (1 - f_a) * ((1 - f_b) * t0[0] + f_b * t0[2]) + f_a * ((1 - f_b) * t1[0] + f_b * t1[2])
+        A = (1 - f_a) * ((1 - f_b) * t0[1] + f_b * t0[3]) + f_a * ((1 - f_b) * t1[1] + f_b * t1[3])
+        return A / (2.0 * (B - 1.0))
 
     def ss(col, v):
         """Steady state A/B of a non-instant column at row coordinate v."""
@@ -503,7 +539,7 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
         area = float(pre[0]) * RM / Em       # Em/Rm = Em area / RM
         vol = area * 0.2e-6
         B = 1.0 / (2.0 * F * vol)
-        conc += [ca_base, cout]
+        conc += [ca_base, cout] + ([1e-3] if "c1c2" in feats else [])
         g_ca = 5.0 * area * rng.uniform(0.5, 1.5)
         m0 = ss(col_m, v0)
         if "nernst" in feats:
@@ -532,6 +568,24 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
             ops += [_NEWCONC1, _CHAN_EK, _IPOL1C, col_z, 1, _ADD_CURR]
             chip += [-0.09, 10.0 * area, ss(col_z, ca_base * invdy)]
             reg = -0.09
+        cb = ca_base * invdy
+        if "kc2d" in feats:
+            ops += [_NEWCONC1, _CHAN_EK, _IPOL2V, col_2v, 1, _ADD_CURR]
+            chip += [-0.09, 2.0 * area, ss2(col_2v, int(v0), v0 - int(v0), int(cb), cb - int(cb))]
+            reg = -0.09
+        if "c1c2" in feats:
+            c2 = 1e-3 * invdy
+            ops += [_NEWCONC1, _NEWCONC2, k + 2, _CHAN_EK, _IPOL2C, col_cc, 2, _ADD_CURR]
+            chip += [-0.085, 1.5 * area, ss2(col_cc, int(cb), cb - int(cb), int(c2), c2 - int(c2))]
+            reg = -0.085
+        if "sameghk" in feats:
+            ops += [_SAME_GHK, _IPOL1V, col_m, 1, _ADD_CURR]
+            chip += [0.3 * perm, m0]
+            reg = None
+        if "tabghk" in feats:
+            ops += [_DO_TABGHK, _IPOL1V, col_t, 0, _SAME_GHK, _ADD_CURR]
+            chip += [1.5 * area]
+            reg = None
         if "tabcurr" in feats:
             ops += [_CHAN, _IPOL1V, col_t, 0]
             chip += [2.0 * area]
@@ -547,7 +601,7 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
     concops.append(_LCACONC)
     import dataclasses
     return dataclasses.replace(
-        base, ops=np.array(ops, np.int32), chip=np.array(chip, np.float64), ncols=ncols0 + 6,
+        base, ops=np.array(ops, np.int32), chip=np.array(chip, np.float64), ncols=tab.shape[1],
         tablist=np.ascontiguousarray(tab.reshape(-1)), compops=np.array(compops, np.int32),
         compchips=np.array(compchips, np.int32), conc=np.array(conc, np.float64),
         concops=np.array(concops, np.int32), concchip=np.array(concchip, np.float64),

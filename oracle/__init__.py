@@ -36,8 +36,9 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   textbook GHK equation (both branches), concentration-gated channels
   settling on the interpolated table (incl. the row cache and the clamps),
   DOMAIN_OP and table currents (tests/test_oracle_conc.py).
+* 2-D tables (IPOL2V; NEWCONC2 + IPOL2C) settling on the bilinearly
+  interpolated table, table GHK and SAME_GHK (tests/test_oracle_conc.py).
 * parity unpinned (no reference test or closed form reaches them here):
-  SYN3 (urandom), 2-D tables (NEWCONC2, IPOL2V/IPOL2C), ELMMPUMP, TABGHK.
-  The device path refuses all of them.
+  SYN3 (urandom), ELMMPUMP.  The device path refuses both.
 """
 from .oracle import Oracle, TangentLinear, oracle_lib  # noqa: F401

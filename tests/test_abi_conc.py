@@ -47,7 +47,7 @@ def test_ca_generated_source_compiles_for_gfx950():
     L = C.CDLL("/opt/rocm/lib/libhiprtc.so")
     m = M.ca_population(20, (1, 8), (0.5, 0.5), features=M.CA_FEATURES[:6] + ("ghk",), seed=1)
     src = D.jit_source(m)
-    for frag in ("caconc(", "newconc(", "ghk_op(", "tabcurr(", "flux[0] += "):
+    for frag in ("caconc(", "newconc(", "ghk_op(", "ipol_xy(", "flux[0] += "):
         assert frag in src, frag
     prog = C.c_void_p()
     assert L.hiprtcCreateProgram(C.byref(prog), src.encode(), b"hsh_jit.hip", 0, None, None) == 0
@@ -121,3 +121,30 @@ def test_refuses_more_than_four_pools_in_a_compartment():
     m = spliced(passive(1), lambda r: (ops, chip), conc=[1e-4] * 5, concops=concops,
                 concchip=[5e-5, 1.01, 1e6] * 5)
     assert [(r[1], r[3]) for r in refusals(m)] == [(CONC_VAL, "shared pool")]
+
+
+IPOL1V, IPOL2C, NEWCONC2, SAME_GHK, DO_TABGHK, NEWVOLT = 4001, 4004, 5111, 3002, 3003, 5100
+
+
+def test_2d_and_table_ghk_accepted():
+    m = M.ca_population(10, (1, 8), (0.5, 0.5), features=("flux", "kc", "kc2d", "c1c2", "tabghk", "ghk",
+                                                             "sameghk"), seed=3, ydivs=20)
+    assert refusals(m) == []
+
+
+def test_refuses_2d_lookup_without_its_registers():
+    # IPOL2C needs NEWCONC2 (first-concentration fraction, table register) in its compartment
+    cols2 = np.tile([[1.01, 0.001]], (12, 12))
+    progs = {0: ([CONC_VAL, 0, NEWCONC1, NEWCONC2, 1], []),
+             1: ([CONC_VAL, 0, NEWCONC1, CHAN_EK, IPOL2C, 0, 1, ADD_CURR], [-0.09, 1e-10, 0.5])}
+    m = spliced(passive(2), lambda r: progs[r], conc=[1e-4, 2e-4], extra_cols=cols2, ydivs=10, invdy=1e4)
+    assert [(r[1], r[3]) for r in refusals(m)] == [(IPOL2C, "table live-in")]
+
+
+def test_refuses_ghk_slope_from_earlier_compartment_and_bad_table_ghk():
+    progs = {0: ([CONC_VAL, 0, GHK, ADD_CURR], [75.0, 1.9e5, 1e-15]), 1: ([SAME_GHK, ADD_CURR], [1e-15])}
+    m = spliced(passive(2), lambda r: progs[r], conc=[1e-4, 2.0])
+    assert [(r[1], r[3]) for r in refusals(m)] == [(SAME_GHK, "register live-in")]
+    # DO_TABGHK_OP must be followed by a power-0 lookup
+    m = spliced(passive(1), lambda r: ([NEWVOLT, DO_TABGHK, IPOL1V, 0, 1, ADD_CURR], [0.5]), extra_cols=cols())
+    assert [(r[1], r[3]) for r in refusals(m)] == [(DO_TABGHK, "opcode")]

@@ -112,3 +112,24 @@ def test_adjoint_refuses_concentrations(gpu):
     with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
         with pytest.raises(D.Refused, match="concentration"):
             g.adj_forward(10, 5, [0])
+
+
+EXACT2D = ("flux", "kc", "kc2d", "c1c2", "tabghk", "domain", "tabcurr", "ctabcurr")
+
+
+@pytest.mark.parametrize("mode,kw", LAYOUTS, ids=IDS)
+def test_calcium_2d_tables_exact(gpu, mode, kw):
+    """2-D gates over (Vm, Ca) and (Ca, second concentration through the
+    table register), table GHK: bit-exact."""
+    m = M.ca_population(150, (1, 8, 16, 40), (0.4, 0.3, 0.2, 0.1), features=EXACT2D, seed=6, ydivs=50)
+    w, g = run_pair(m, 400, mode, **kw)
+    assert_exact(w, g)
+
+
+@pytest.mark.parametrize("mode,kw", [LAYOUTS[0], LAYOUTS[3]], ids=[IDS[0], IDS[3]])
+def test_calcium_same_ghk(gpu, mode, kw):
+    m = M.ca_population(150, (1, 8, 16), (0.4, 0.3, 0.3), features=("flux", "kc", "ghk", "sameghk", "kc2d"),
+                        seed=7, ydivs=50)
+    w, g = run_pair(m, 1000, mode, **kw)
+    assert float(np.max(np.abs(w["vm"] - g["vm"]))) <= 1e-7
+    assert float(np.max(np.abs(w["conc"] - g["conc"]) / np.abs(w["conc"]))) <= 1e-6

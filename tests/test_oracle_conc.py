@@ -202,3 +202,96 @@ def test_table_currents():
 
 
 NEWVOLT_OP = 5100
+
+
+IPOL2V, IPOL2C, NEWCONC2, SAME_GHK, DO_TABGHK = 4003, 4004, 5111, 3002, 3003
+NY2 = 8                 # concentration rows of the 2-D test tables
+
+
+def table2d(rows, seed):
+    """A 2-D gate block: B' at 2j, A' at 2j + 1 for concentration row j, rows
+    along the first axis; values differ along both axes (a transposed read
+    gives another number)."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(50, 500, (rows, NY2 + 1)) + 30 * np.arange(rows)[:, None]
+    blk = np.empty((rows, 2 * (NY2 + 1) + 2))
+    blk[:, 0:2 * (NY2 + 1):2] = 1 + DT / 2 * (a + 4000.0)
+    blk[:, 1:2 * (NY2 + 1):2] = DT * a
+    blk[:, -2:] = blk[:, -4:-2]
+    return blk
+
+
+def bilinear(blk, r, f, j, g, off):
+    t0, t1 = blk[r], blk[r + 1]
+    B = (1 - f) * ((1 - g) * t0[2 * j] + g * t0[2 * j + 2]) + f * ((1 - g) * t1[2 * j] + g * t1[2 * j + 2])
+    A = (1 - f) * ((1 - g) * t0[2 * j + 1] + g * t0[2 * j + 3]) + f * ((1 - g) * t1[2 * j + 1] + g * t1[2 * j + 3])
+    return A / (2 * (B - 1))
+
+
+def test_2d_gate_on_voltage_and_concentration():
+    """IPOL2V: the gate settles on the table interpolated bilinearly at the
+    voltage row/fraction and the concentration row/fraction."""
+    inv_y = NY2 / 1e-3
+    V, C = -0.0533, 6.1e-4
+    rows = 152
+    base = passive(1, Em=V, V0=[V])
+    blk = table2d(rows, 1)
+    m = spliced(base, lambda r: ([NEWVOLT_OP, CONC_VAL, 0, NEWCONC1, CHAN_EK, IPOL2V, 0, 1, ADD_CURR],
+                                 [V, 1e-12, 0.5]),
+                conc=[C], extra_cols=blk, ydivs=NY2, ymin=0.0, invdy=inv_y)
+    o = Oracle(m, givals=False)
+    o.step(400)
+    v = (V - m.xmin) * m.invdx
+    c = C * inv_y
+    want = bilinear(blk, int(v), v - int(v), int(c), c - int(c), 0)
+    got = o.state()["chip"][seg(m, 0)[0] + 4]
+    assert abs(got - want) <= 1e-12 * want
+    assert abs(o.vm[0] - V) <= 1e-15
+
+
+def test_2d_gate_on_two_concentrations():
+    """NEWCONC2 + IPOL2C: rows from the first concentration (the table
+    register NEWCONC2 saves), columns from the second."""
+    inv_y = NY2 / 1e-3
+    C1, C2 = 3.3e-4, 8.6e-4
+    blk = table2d(NY2 + 2, 2)
+    base = passive(1)
+    m = spliced(base, lambda r: ([CONC_VAL, 0, NEWCONC1, NEWCONC2, 1, CHAN_EK, IPOL2C, 0, 1, ADD_CURR],
+                                 [-0.065, 1e-12, 0.5]),
+                conc=[C1, C2], extra_cols=blk, ydivs=NY2, ymin=0.0, invdy=inv_y)
+    o = Oracle(m, givals=False)
+    o.step(400)
+    a, b = C1 * inv_y, C2 * inv_y
+    want = bilinear(blk, int(a), a - int(a), int(b), b - int(b), 0)
+    got = o.state()["chip"][seg(m, 0)[0] + 4]
+    assert abs(got - want) <= 1e-12 * want
+    # swapping the two concentrations gives another value (the test sees the axes)
+    assert abs(bilinear(blk, int(b), b - int(b), int(a), a - int(a), 0) - want) > 1e-6 * want
+
+
+def test_table_ghk_and_same_ghk():
+    """DO_TABGHK_OP + power-0 IPOL + SAME_GHK_OP: Ek = E(V), Gk = gmax G(V);
+    SAME_GHK_OP after GHK_OP: a second gmax on the same slope."""
+    V, g = -0.047, 2e-9
+    xdivs, xmin, invdx = 100, -0.1, 100 / 0.15
+    Vr = xmin + np.arange(xdivs + 2) / invdx
+    cols = np.stack([1 / (1 + np.exp(-(Vr + 0.04) / 0.005)), -0.03 + 0.1 * Vr], axis=1)
+    base = passive(1, V0=[V])
+    m = spliced(base, lambda r: ([NEWVOLT_OP, DO_TABGHK, IPOL1V, 0, 0, SAME_GHK, ADD_CURR], [g]), extra_cols=cols)
+    o = Oracle(m, givals=False)
+    o.chip_update()
+    v = (V - xmin) * invdx
+    r, f = int(v), v - int(v)
+    Gt = cols[r, 0] + f * (cols[r + 1, 0] - cols[r, 0])
+    Et = cols[r, 1] + f * (cols[r + 1, 1] - cols[r, 1])
+    G, EG = gk_ek(m, o, 0, V)
+    assert abs(G - g * Gt) <= 1e-9 * G and abs(EG - g * Gt * Et) <= 1e-9 * abs(EG)
+    # GHK channel P1 then SAME_GHK with P2: the current is (P1 + P2) times the GHK current per P
+    Cin, Cout, P1, P2 = 1e-4, 2.0, 3e-15, 1e-15
+    m = spliced(passive(1, V0=[V]), lambda r: ([CONC_VAL, 0, GHK, ADD_CURR, SAME_GHK, ADD_CURR],
+                                               [ZFRT, 2 * F, P1, P2]), conc=[Cin, Cout])
+    o = Oracle(m, givals=False)
+    o.chip_update()
+    G, EG = gk_ek(m, o, 0, V)
+    want_I = ghk_current(V, P1 + P2, Cin, Cout)
+    assert abs((EG - G * V) - want_I) <= 1e-9 * abs(want_I)
