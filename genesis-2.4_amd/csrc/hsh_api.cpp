@@ -88,6 +88,7 @@ struct hsh_handle {
     int nconn = 0;
     long long tab_max = 0;             // largest start offset of a register-based 2-D lookup
     int32_t *d_spk = nullptr, *d_nspk = nullptr;
+    double* d_giv = nullptr;           // givals output (models created with givals)
     // layout-dependent device buffers
     BlockMeta* d_blocks = nullptr;
     TileMeta* d_tiles = nullptr;
@@ -177,6 +178,23 @@ void pack_V(hsh_handle* h, const double* vm, std::vector<double>& V)
     const Plan& P = h->plan;
     V.assign(P.blocks.size() * W, 0.0);
     for (size_t q = 0; q < V.size(); q++) V[q] = vm[P.pos_comp[q]];
+}
+
+// givals around runs that must not show (autotuning, adjoint recomputation)
+hsh_status save_givals(hsh_handle* h, std::vector<double>& g)
+{
+    g.clear();
+    if (!h->d_giv) return HSH_OK;
+    g.resize((size_t)h->m.ngivals);
+    HIPCHK(hipMemcpy(g.data(), h->d_giv, g.size() * sizeof(double), hipMemcpyDeviceToHost));
+    return HSH_OK;
+}
+
+hsh_status load_givals(hsh_handle* h, const std::vector<double>& g)
+{
+    if (!h->d_giv || g.empty()) return HSH_OK;
+    HIPCHK(hipMemcpy(h->d_giv, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice));
+    return HSH_OK;
 }
 
 hsh_status upload_state(hsh_handle* h, const double* vm)
@@ -281,7 +299,7 @@ void free_device(hsh_handle* h)
 {
     free_layout(h);
     void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_stab, h->d_spk, h->d_nspk, h->d_ring, h->d_out_off,
-                    h->d_out_dst, h->d_out_dly, h->d_out_w};
+                    h->d_out_dst, h->d_out_dly, h->d_out_w, h->d_giv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_prog = nullptr;
@@ -289,6 +307,7 @@ void free_device(hsh_handle* h)
     h->d_ring = h->d_out_w = nullptr;
     h->d_out_off = h->d_out_dst = h->d_out_dly = nullptr;
     h->d_spk = h->d_nspk = nullptr;
+    h->d_giv = nullptr;
     jit_free(h->jit);
     for (hipEvent_t e : h->evs) (void)hipEventDestroy(e);
     h->evs.clear();
@@ -368,6 +387,8 @@ void set_devmodel(hsh_handle* h)
     d.nsrc = h->ring_len > 0 ? (int32_t)P.spike_ops.size() : 0;
     d.dt = h->m.dt;
     d.spk_cap = SPIKE_CAP;
+    d.giv = P.giv ? h->d_giv : nullptr;
+    d.chanmode5 = P.chanmode5 ? 1 : 0;
     d.ncols = h->m.ncols;
     d.xdivs = h->m.xdivs;
     d.crank = (h->m.method == HSH_CRANK);
@@ -634,6 +655,8 @@ hsh_status autotune(hsh_handle* h)
     const std::vector<int32_t> ops0 = h->ops;
     int32_t nspk0 = 0;
     HIPCHK(hipMemcpy(&nspk0, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> giv0;
+    STCHK(save_givals(h, giv0));
     // pending synaptic events: trials consume and add them
     std::vector<double> ring0((size_t)h->ring_len * h->plan.syn_ops.size());
     if (!ring0.empty())
@@ -723,6 +746,7 @@ hsh_status autotune(hsh_handle* h)
     free_layout(h);
     STCHK(alloc_layout(h, vm0.data(), res0.data()));
     HIPCHK(hipMemcpy(h->d_nspk, &nspk0, sizeof(int32_t), hipMemcpyHostToDevice));
+    STCHK(load_givals(h, giv0));
     if (!ring0.empty())
         HIPCHK(hipMemcpy(h->d_ring, ring0.data(), ring0.size() * sizeof(double), hipMemcpyHostToDevice));
     h->tuned = true;
@@ -857,6 +881,8 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         (m->nconcops > 0 && !m->concops) || m->nconcchip < 0 || (m->nconcchip > 0 && !m->concchip) ||
         m->ydivs < 0)
         return fail(HSH_ERR_ARG, "hsh_create: inconsistent concentration arrays");
+    if (m->ngivals < 0 || (m->ngivals > 0 && !m->givals))
+        return fail(HSH_ERR_ARG, "hsh_create: ngivals > 0 but no givals");
     if (opt && (opt->pack_waves < 0 || opt->pack_waves > 8 || opt->pack_stage < 0 || opt->pack_stage > 2 ||
                 opt->pack_cap < 0 || opt->batch_k < 0 || opt->batch_waves < 0))
         return fail(HSH_ERR_ARG, "hsh_create: option out of range");
@@ -867,6 +893,7 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     h->chip.assign(m->chip, m->chip + m->nchips);
     if (m->nconcs > 0) h->chip.insert(h->chip.end(), m->conc, m->conc + m->nconcs);
     if (m->nconcchip > 0) h->chip.insert(h->chip.end(), m->concchip, m->concchip + m->nconcchip);
+    if (m->ngivals > 0 && m->givals) h->chip.insert(h->chip.end(), m->givals, m->givals + m->ngivals);
     std::string err;
     if (!analyse(*m, h->plan, err)) {
         delete h;
@@ -959,6 +986,10 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         HIPCHK_C(hipMemcpy(h->d_xvals, m->xvals, nx * sizeof(double), hipMemcpyHostToDevice));
     }
     HIPCHK_C(hipMemset(h->d_nspk, 0, sizeof(int32_t)));
+    if (P.giv) {
+        HIPCHK_C(dalloc(&h->d_giv, (size_t)m->ngivals));
+        HIPCHK_C(hipMemcpy(h->d_giv, m->givals, (size_t)m->ngivals * sizeof(double), hipMemcpyHostToDevice));
+    }
 #undef HIPCHK_C
     {
         hsh_status st = alloc_layout(h, m->vm, nullptr);
@@ -1069,6 +1100,17 @@ hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, co
         pack_V(h, vm, V);
         HIPCHK(hipMemcpy(h->d_V, V.data(), V.size() * sizeof(double), hipMemcpyHostToDevice));
     }
+    return HSH_OK;
+}
+
+hsh_status hsh_get_givals(hsh_handle* h, double* givals)
+{
+    if (!h || !givals) return fail(HSH_ERR_ARG, "hsh_get_givals: null argument");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_get_givals: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_get_givals: handle refused the model");
+    if (!h->d_giv) return fail(HSH_ERR_STATE, "hsh_get_givals: the model was created without givals");
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(hipMemcpy(givals, h->d_giv, (size_t)h->m.ngivals * sizeof(double), hipMemcpyDeviceToHost));
     return HSH_OK;
 }
 
@@ -1372,6 +1414,8 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     HIPCHK(hipMemsetAsync(A.GT, 0, 3 * npos * sizeof(double), h->stream));
     int32_t nspk0 = 0;
     HIPCHK(hipMemcpy(&nspk0, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<double> giv0;
+    STCHK(save_givals(h, giv0));
     const bool timing = h->timing;
     h->timing = false;
     AdjModel am{};
@@ -1411,6 +1455,8 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     }
     STCHK(adj_load(h, A.ncp - 1));
     HIPCHK(hipMemcpyAsync(h->d_nspk, &nspk0, sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    STCHK(load_givals(h, giv0));
     h->timing = timing;
     HIPCHK(hipStreamSynchronize(h->stream));
     std::vector<double> LV(npos), LS(A.nS), GT(3 * npos);

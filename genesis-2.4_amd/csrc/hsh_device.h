@@ -227,6 +227,21 @@ struct Flux {
     }
 };
 
+// givals of one current (hines_chip.c:536-564): Gk, Ik, Ek at gi, gi+1,
+// gi+2 (chanmode 5: Gk and Ik per gmax).
+__device__ __forceinline__ void giv_put3(const DevModel& d, int& gi, double Gk, double Ik, double Ek, double gmax)
+{
+    if (d.chanmode5) {
+        d.giv[gi] = Gk / gmax;
+        d.giv[gi + 1] = Ik / gmax;
+    } else {
+        d.giv[gi] = Gk;
+        d.giv[gi + 1] = Ik;
+    }
+    d.giv[gi + 2] = Ek;
+    gi += 3;
+}
+
 // Mg block of a synaptic conductance (hines_chip.c:531-547): X = KMg_A
 // exp(Vm / KMg_B), Gk *= X / (X + [Mg]).  exp() is the device library's.
 __device__ __forceinline__ double mgblock(double Gk, double Vm, double A, double invB, double Mg)
@@ -281,6 +296,10 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
     const double* vtab = d.tab;
     const double* ctab = d.tab;
     const int ncols = d.ncols;
+    // givals: cursor, membrane current, gmax register (chanmode 5)
+    const bool giv = d.giv != nullptr && active;
+    int gi = 0;
+    double Im = 0.0, gmax = 0.0;
     int c = 2;
     int pc = PROG_HEADER;
     for (;;) {
@@ -296,18 +315,22 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             Gk = apply_power(Gk, X, k < 0 ? -k : k);
         } else if (op == U_CHAN_EK) {
             Ek = SGET(c);
-            Gk = SGET(c + 1);
+            gmax = Gk = SGET(c + 1);
             c += 2;
         } else if (op == U_ADDCURR) {
             sumgchan += Gk;
             ichan += Ek * Gk;
             Ik = Gk * (Ek - Vm);
+            if (giv) {
+                giv_put3(d, gi, Gk, Ik, Ek, gmax);
+                Im += Ik;
+            }
         } else if (op == U_NEWVOLT) {
             int filo;
             newvolt(d, Vm, pred_ok, Vp, filo, vipol);
             vtab = d.tab + (long long)filo * ncols;
         } else if (op == U_CHAN) {
-            Gk = SGET(c);
+            gmax = Gk = SGET(c);
             c += 1;
         } else if (op == U_SPIKE) {
             const int is = code[pc++];
@@ -319,12 +342,17 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             Gk *= syn2_op(d, Sb, Ib, c, k, is, lane, SGET(c), SGET(c + 1), active, step);
             c += 2;
             if (op == U_SYN2MG) {
+                if (giv) giv_put3(d, gi, Gk, Gk * (Ek - Vm), Ek, gmax);   // unblocked
                 Gk = mgblock(Gk, Vm, SGET(c), SGET(c + 1), SGET(c + 2));
                 c += 3;
             }
             sumgchan += Gk;
             ichan += Ek * Gk;
             Ik = Gk * (Ek - Vm);
+            if (giv) {
+                giv_put3(d, gi, Gk, Ik, Ek, gmax);
+                Im += Ik;
+            }
         } else if (op == U_CONCVAL) {
             flux.set(code[pc++], 0.0);
         } else if (op == U_NEWCONC1 || op == U_DOMAIN) {
@@ -374,10 +402,24 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
                 sumgchan += Gk;
                 ichan += Ek * Gk;
                 Ik = Gk * (Ek - Vm);
+                if (giv) {
+                    giv_put3(d, gi, Gk, Ik, Ek, gmax);
+                    Im += Ik;
+                }
             }
         } else if (op == U_SAMEGHK) {
-            Gk = SGET(c) * ghk;
+            gmax = SGET(c);
+            Gk = gmax * ghk;
             c++;
+        } else if (op == U_GIV0) {
+            const int rs = code[pc], is = code[pc + 1];
+            pc += 2;
+            if (giv) {
+                gi = Ib[is * W + lane];
+                Im = SGET(0) - Vm / SGET(rs);
+                d.giv[gi + 1] = Im;
+                gi += 2;
+            }
         } else if (op == U_FLUX) {
             const int p = code[pc++];
             flux.set(p, flux.get(p) + Ik);
@@ -389,7 +431,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
             Ek = log(SGET(co) / SGET(ci)) * SGET(c);
-            Gk = SGET(c + 1);
+            gmax = Gk = SGET(c + 1);
             c += 2;
         } else if (op == U_GHK) {
             const int ci = code[pc], co = code[pc + 1];
@@ -397,13 +439,15 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             double slope;
             ghk_op(Vm, SGET(ci), SGET(co), SGET(c), SGET(c + 1), Ek, slope);
             ghk = slope;
-            Gk = SGET(c + 2) * slope;
+            gmax = SGET(c + 2);
+            Gk = gmax * slope;
             c += 3;
         } else if (op == U_POOLCA) {
             const int p = code[pc], cs = code[pc + 1], ks = code[pc + 2];
             pc += 3;
             SPUT(cs, caconc(SGET(cs), SGET(ks), SGET(ks + 1), flux.get(p), SGET(ks + 2)));
         } else {  // U_END
+            if (giv) d.giv[gi] = Im;
             const double tbyc = SGET(c);
             CompOut o;
             o.rhs = Vm + ichan * tbyc;

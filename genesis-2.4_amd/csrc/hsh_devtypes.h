@@ -43,6 +43,7 @@ enum UOp : int {
     U_IPOLG = 20,     // operands: which (2: Vm x conc, 3: conc x conc), column, power; 2-D gate
     U_TABGHK = 21,    // operands: column, which (0..3); Ek = A, GHK slope = B (no current)
     U_SAMEGHK = 22,   // Gk = S[c++] * GHK slope
+    U_GIV0 = 23,      // operands: Rm slot, islot of the compartment's givals base: leak current
     // U_TABCURR's "which" takes the same four values as U_IPOLG / U_TABGHK
 };
 constexpr int MAXPOOL = 4;
@@ -132,6 +133,8 @@ struct DevModel {
     double invdy;
     int ydivs;
     int spk_cap;
+    double* giv;          // givals output (nullptr: not produced)
+    int chanmode5;        // givals per gmax
     int ncols;
     int xdivs;
     int crank;

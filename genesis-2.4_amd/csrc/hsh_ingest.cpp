@@ -87,6 +87,12 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         }
     }
     int creg = -1;            // conc register (CONC_VAL_OP), carried across compartments
+    // givals: written in op order from givals[1] (hines_chip.c:114); value-space
+    // sources of its entries follow concchip[]
+    P.giv = (m.givals != nullptr && m.ngivals > 0);
+    P.chanmode5 = (m.chanmode == 5);
+    const int64_t gsrc0 = nchips + nconcs + std::max(0, m.nconcchip);
+    int64_t gcur = 1;
     for (;;) {
         if (p >= m.nops) return bad("missing LCOMPT_OP");
         int32_t head = ops[p++];
@@ -143,6 +149,15 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         if (c + 2 > m.nchips) return bad("chip[] too short");
         dsrc.push_back(c); dsrc.push_back(c + 1);   // Em/Rm, inject
         c += 2;
+        if (P.giv) {
+            // leak current from Rm (givals[g]) into givals[g+1] (hines_chip.c:164-165)
+            if (gcur + 2 > m.ngivals) return bad("givals[] too short");
+            code.push_back(U_GIV0);
+            push_extra(gsrc0 + gcur);
+            code.push_back((int32_t)isrc.size());
+            isrc.push_back(-(1 + gcur));
+            gcur += 2;
+        }
         auto need_chip = [&](int k) { return c + k <= m.nchips; };
         while (p < m.nops && ops[p] > LCOMPT_OP) {
             int64_t opidx = p;
@@ -361,6 +376,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                         code.push_back(col);
                         code.push_back(which);
                         ek = EK_DYN; ek_intra = true;
+                        if (P.giv) gcur += 3;
                         break;
                     }
                     if (which <= 1) {
@@ -412,9 +428,11 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                         c += 3;
                         p += 1;
                     }
+                    if (P.giv) gcur += mg ? 6 : 3;   // the unblocked values first (hines_chip.c:533-542)
                     break;
                 } else if (k == ADD_CURR_OP) {
                     code.push_back(U_ADDCURR);
+                    if (P.giv) gcur += 3;
                     break;
                 } else {
                     err = "undefined chip operation #2 " + std::to_string(k) + " at ops index " +
@@ -446,6 +464,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         code.push_back(U_END);
         dsrc.push_back(c); dsrc.push_back(c + 1);   // tbyc, diagterm
         c += 2;
+        if (P.giv) gcur += 1;                       // total membrane current
         if (refused) continue;
         {
             // the three pool constants must be consecutive slots
@@ -480,6 +499,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     }
     if (comp != n - 1) { err = "ops[] holds " + std::to_string(comp + 1) + " compartments, ncompts is " + std::to_string(n); return false; }
     if (c != m.nchips) { err = "chip[] length does not match the ops[] walk"; return false; }
+    if (P.giv && gcur > m.ngivals) { err = "givals[] shorter than the ops[] walk writes"; return false; }
     for (int64_t k = 0; k < nconcs; k++)
         if (pool_chip[k] >= 0 && pool_owner[k] < 0) refuse(P, -1, CACONC_OP, -1, HSH_R_POOL);
 

@@ -44,6 +44,8 @@ struct LayoutOpts {
 // Host-side result of analysing a model.
 struct Plan {
     std::vector<hsh_refusal> refusals;
+    bool giv = false;                     // produce givals (chanmode-4/5 output stream)
+    bool chanmode5 = false;
     std::vector<Template> templates;
     // per compartment (Hines row)
     std::vector<int32_t> comp_tmpl;

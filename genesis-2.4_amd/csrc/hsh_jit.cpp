@@ -32,7 +32,7 @@ std::map<std::string, std::vector<char>> g_code_cache;   // source -> code objec
 
 std::string slot(int c) { return "s" + std::to_string(c); }
 
-std::string gen_template(const Template& t, int id, int ncols)
+std::string gen_template(const Template& t, int id, int ncols, bool giv)
 {
     std::string o;
     o += "__device__ __forceinline__ CompOut tmpl_" + std::to_string(id) +
@@ -48,6 +48,11 @@ std::string gen_template(const Template& t, int id, int ncols)
     o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)Ik; (void)vipol; (void)cipol;\n";
     o += "    (void)flux; (void)vtab; (void)ctab; (void)tabr; (void)cfilo; (void)ipol; (void)ghk;\n";
     o += "    (void)pred_ok; (void)Vp;\n";
+    // givals (only in modules made for a model that asks for them)
+    const std::string GM = giv ? "gmax = " : "";
+    const std::string PUT3 = giv ? " if (giv) { giv_put3(d, gi, Gk, Ik, Ek, gmax); Im += Ik; }" : "";
+    if (giv) o += "    const bool giv = active;\n    int gi = 0;\n    double Im = 0.0, gmax = 0.0;\n"
+                  "    (void)gmax; (void)Im;\n";
     const std::string NC = std::to_string(ncols);
     int c = 2;
     size_t pc = 0;
@@ -60,13 +65,21 @@ std::string gen_template(const Template& t, int id, int ncols)
                  NC + "; }\n";
             break;
         case U_CHAN_EK:
-            o += "    Ek = " + slot(c) + "; Gk = " + slot(c + 1) + ";\n";
+            o += "    Ek = " + slot(c) + "; " + GM + "Gk = " + slot(c + 1) + ";\n";
             c += 2;
             break;
         case U_CHAN:
-            o += "    Gk = " + slot(c) + ";\n";
+            o += "    " + GM + "Gk = " + slot(c) + ";\n";
             c += 1;
             break;
+        case U_GIV0: {
+            const int rs = code[pc], is = code[pc + 1];
+            pc += 2;
+            if (giv)
+                o += "    if (giv) { gi = Ib[" + std::to_string(is) + " * W + lane]; Im = s0 - Vm / " + slot(rs) +
+                     "; d.giv[gi + 1] = Im; gi += 2; }\n";
+            break;
+        }
         case U_IPOL1V: {
             const int col = code[pc], k = code[pc + 1];
             pc += 2;
@@ -78,7 +91,7 @@ std::string gen_template(const Template& t, int id, int ncols)
             break;
         }
         case U_ADDCURR:
-            o += "    sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
+            o += "    sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);" + PUT3 + "\n";
             break;
         case U_SPIKE: {
             const int is = code[pc++];
@@ -95,10 +108,11 @@ std::string gen_template(const Template& t, int id, int ncols)
                  std::to_string(is) + ", lane, " + slot(c) + ", " + slot(c + 1) + ", active, step);";
             c += 2;
             if (op == U_SYN2MG) {
+                if (giv) o += " if (giv) giv_put3(d, gi, Gk, Gk * (Ek - Vm), Ek, gmax);";
                 o += " Gk = mgblock(Gk, Vm, " + slot(c) + ", " + slot(c + 1) + ", " + slot(c + 2) + ");";
                 c += 3;
             }
-            o += " sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
+            o += " sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);" + PUT3 + "\n";
             break;
         }
         case U_CONCVAL:
@@ -139,7 +153,7 @@ std::string gen_template(const Template& t, int id, int ncols)
             break;
         }
         case U_SAMEGHK:
-            o += "    Gk = " + slot(c) + " * ghk;\n";
+            o += "    " + (giv ? "gmax = " + slot(c) + "; " : std::string()) + "Gk = " + slot(c) + " * ghk;\n";
             c += 1;
             break;
         case U_IPOL1C: {
@@ -157,7 +171,7 @@ std::string gen_template(const Template& t, int id, int ncols)
             pc += 2;
             o += "    { double X, Y; ipol_xy(d, " + std::to_string(which) + ", " + std::to_string(col) +
                  ", vtab, ctab, tabr, cfilo, vipol, cipol, ipol, X, Y); Ek = Y; Gk *= X; }"
-                 " sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);\n";
+                 " sumgchan += Gk; ichan += Ek * Gk; Ik = Gk * (Ek - Vm);" + PUT3 + "\n";
             break;
         }
         case U_FLUX:
@@ -170,7 +184,8 @@ std::string gen_template(const Template& t, int id, int ncols)
         case U_NERNST: {
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
-            o += "    Ek = log(" + slot(co) + " / " + slot(ci) + ") * " + slot(c) + "; Gk = " + slot(c + 1) + ";\n";
+            o += "    Ek = log(" + slot(co) + " / " + slot(ci) + ") * " + slot(c) + "; " + GM + "Gk = " +
+                 slot(c + 1) + ";\n";
             c += 2;
             break;
         }
@@ -178,7 +193,8 @@ std::string gen_template(const Template& t, int id, int ncols)
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
             o += "    { double sl; ghk_op(Vm, " + slot(ci) + ", " + slot(co) + ", " + slot(c) + ", " + slot(c + 1) +
-                 ", Ek, sl); ghk = sl; Gk = " + slot(c + 2) + " * sl; }\n";
+                 ", Ek, sl); ghk = sl; " + (giv ? "gmax = " + slot(c + 2) + "; " : std::string()) + "Gk = " +
+                 slot(c + 2) + " * sl; }\n";
             c += 3;
             break;
         }
@@ -190,6 +206,7 @@ std::string gen_template(const Template& t, int id, int ncols)
             break;
         }
         default:   // U_END
+            if (giv) o += "    if (giv) d.giv[gi] = Im;\n";
             o += "    { CompOut o; o.rhs = Vm + ichan * " + slot(c) + "; o.diag = sumgchan * " + slot(c) +
                  " + " + slot(c + 1) + "; return o; }\n";
             c += 2;
@@ -217,7 +234,7 @@ std::string jit_source(const Plan& P, int ncols)
     src += strip_pragma_once(kDevTypesSrc);
     src += strip_pragma_once(kDeviceSrc);
     src += "namespace hsh {\n";
-    for (size_t t = 0; t < P.templates.size(); t++) src += gen_template(P.templates[t], (int)t, ncols);
+    for (size_t t = 0; t < P.templates.size(); t++) src += gen_template(P.templates[t], (int)t, ncols, P.giv);
     src += "struct JitRun {\n"
            "    static __device__ __forceinline__ CompOut run(const DevModel& d, const BlockMeta& bm, "
            "int lane, double Vm, bool pok, double Vp, bool active, int step)\n    {\n"

@@ -81,6 +81,9 @@ static int hip_init(Hsolve *hsolve)
     m.ydivs = hsolve->ydivs;
     m.ymin = hsolve->ymin;
     m.invdy = hsolve->invdy;
+    /* chanmode-4/5 output stream: written on the device, pulled on demand */
+    m.ngivals = hsolve->givals ? hsolve->ngivals : 0;
+    m.givals = hsolve->givals;
     if (hsolve->ydivs > hsolve->xdivs || hsolve->ndiffs) {
         /* concentration rows beyond the composite table, or a conc solve
          * (difshells): CPU path */
@@ -170,6 +173,7 @@ void hip_sync_chip(Hsolve *hsolve)
     if (!st || st == (HipState *)HIP_REFUSED) return;
     hsh_get_state(st->h, hsolve->vm, hsolve->chip, hsolve->results, hsolve->ops);
     if (hsolve->nconcs > 0) hsh_get_conc(st->h, hsolve->conc);
+    if (hsolve->givals && hsolve->ngivals > 0) hsh_get_givals(st->h, hsolve->givals);
 }
 
 int hip_owns_conc(Hsolve *hsolve)

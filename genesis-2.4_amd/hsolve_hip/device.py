@@ -44,6 +44,7 @@ class HshModel(C.Structure):
         ("ydivs", C.c_int32), ("nconcops", C.c_int32), ("nconcchip", C.c_int32),
         ("ymin", C.c_double), ("invdy", C.c_double),
         ("conc", _f64p), ("concops", _i32p), ("concchip", _f64p),
+        ("ngivals", C.c_int32), ("givals", _f64p),
     ]
 
 
@@ -76,7 +77,7 @@ class HshInfo(C.Structure):
 
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
-           "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
+           "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_get_givals", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
            "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_set_ravals",
            "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
@@ -152,7 +153,7 @@ def _check(rc, what):
     return rc
 
 
-def _hsh_model(model):
+def _hsh_model(model, givals: bool = False):
     """Build the hsh_model struct (and the arrays it points to) for a Model."""
     k = dict(ops=np.ascontiguousarray(model.ops, np.int32),
              chip=np.ascontiguousarray(model.chip, np.float64),
@@ -182,12 +183,15 @@ def _hsh_model(model):
     mm.conc = _p(k["conc"], C.c_double) if k["conc"].size else None
     mm.concops = _p(k["concops"], C.c_int32) if k["concops"].size else None
     mm.concchip = _p(k["concchip"], C.c_double) if k["concchip"].size else None
+    if givals and np.size(model.givals):
+        k["givals"] = np.ascontiguousarray(model.givals, np.float64)
+        mm.ngivals, mm.givals = k["givals"].size, _p(k["givals"], C.c_double)
     return mm, k
 
 
-def jit_source(model) -> str:
+def jit_source(model, givals: bool = False) -> str:
     """Device source generated for this model's channel programs."""
-    mm, keep = _hsh_model(model)
+    mm, keep = _hsh_model(model, givals)
     n = C.c_int64()
     _check(lib().hsh_jit_source(C.byref(mm), None, 0, C.byref(n)), "hsh_jit_source")
     buf = C.create_string_buffer(n.value + 1)
@@ -200,13 +204,14 @@ class HipSolver:
 
     def __init__(self, model, *, device: int = 0, batch_k: int = 0, batch_waves: int = 0,
                  flags: int = 0, host_only: bool = False, tile_max_rows: int = 0, pack_cap: int = 0,
-                 pack_waves: int = 0, pack_stage: int = 0):
+                 pack_waves: int = 0, pack_stage: int = 0, givals: bool = False):
         """Options as in hsh_options: 0 = autotuned (or the default under
         F_NO_AUTOTUNE); ``tile_max_rows < 0`` puts every tree in packs;
-        ``pack_stage`` 1 = schedule read from memory, 2 = staged in LDS."""
+        ``pack_stage`` 1 = schedule read from memory, 2 = staged in LDS;
+        ``givals``: pass the model's givals so the device writes that stream."""
         L = lib()
         self.m = model
-        mm, self._keep = _hsh_model(model)
+        mm, self._keep = _hsh_model(model, givals)
         if host_only:
             flags |= F_HOST_ONLY
         opt = HshOptions(device, batch_k, batch_waves, flags, tile_max_rows, pack_cap, pack_waves,
@@ -327,6 +332,12 @@ class HipSolver:
         _check(lib().hsh_get_state(self.h, _p(vm, C.c_double), _p(chip, C.c_double),
                                    _p(res, C.c_double), _p(ops, C.c_int32)), "hsh_get_state")
         return dict(vm=vm, chip=chip, results=res, ops=ops)
+
+    def givals(self) -> np.ndarray:
+        """The chanmode-4/5 output stream givals[] as the last step wrote it."""
+        out = np.empty(np.size(self.m.givals))
+        _check(lib().hsh_get_givals(self.h, _p(out, C.c_double)), "hsh_get_givals")
+        return out
 
     def conc(self) -> np.ndarray:
         """Pool concentrations conc[nconcs] (reference indices)."""

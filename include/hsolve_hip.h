@@ -98,6 +98,13 @@ typedef struct hsh_model {
     const int32_t *concops;  /* [nconcops]: (CACONC_OP k)* LCACONC_OP; other pool
                                 ops (shells, buffers, pumps) are refused */
     const double  *concchip; /* [nconcchip]: Ca_base, 1 + dt/(2 tau), dt B per pool */
+    /* chanmode-4/5 output stream (hines_chip.c:164-165, :536-564, :598): when
+     * given, the device writes it every step -- per compartment the leak
+     * current, {Gk, Ik, Ek} of every current (chanmode 5: Gk and Ik per gmax)
+     * and the total membrane current; the Rm entries the reference reads are
+     * taken from here.  NULL / 0: not produced (no cost). */
+    int32_t ngivals;
+    const double  *givals;   /* [ngivals] */
 } hsh_model;
 
 typedef struct hsh_options {
@@ -203,6 +210,9 @@ hsh_status hsh_put_state(struct hsh_handle *h, const double *vm, const double *c
  * pool (do_h4_conc_chip_update, hines.c:239-240, hines_conc_chip.c:196-207). */
 hsh_status hsh_get_conc(struct hsh_handle *h, double *conc);
 hsh_status hsh_put_conc(struct hsh_handle *h, const double *conc);
+/* givals[ngivals] as the last step wrote it (HSH_ERR_STATE if the model was
+ * created without givals). */
+hsh_status hsh_get_givals(struct hsh_handle *h, double *givals);
 /* Overwrite n chip[] values (reference indices), e.g. INJECT messages. */
 hsh_status hsh_set_chip(struct hsh_handle *h, int32_t n, const int32_t *idx, const double *val);
 /* Spike events raised by SPIKE_OP since the last drain: ops[] index of the

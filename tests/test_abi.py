@@ -199,3 +199,28 @@ def test_header_is_clean_c(tmp_path):
         r = subprocess.run([cc, std, "-Wall", "-Wextra", "-Werror", "-fsyntax-only", "-I", inc,
                             "-x", "c" if cc == "gcc" else "c++", str(src)], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_givals_plan_and_source():
+    """With givals the programs carry the leak-current op and its Rm slot; the
+    generated source (with the stream writes) compiles for gfx950; a givals
+    array shorter than the ops[] walk writes is rejected."""
+    m = random_tree_model(70, 16, seed=5)
+    with D.HipSolver(m, host_only=True, givals=True) as h:
+        with_giv = h.info()["bytes_per_step"]
+    with D.HipSolver(m, host_only=True) as h:
+        assert h.info()["bytes_per_step"] < with_giv        # the Rm slots are read
+    src = D.jit_source(m, givals=True)
+    # the generated templates (not the shared interpreter text) carry the writes
+    assert "Im = s0 - Vm / " in src and "if (giv) d.giv[gi] = Im;" in src
+    assert "Im = s0 - Vm / " not in D.jit_source(m)
+    L = C.CDLL("/opt/rocm/lib/libhiprtc.so")
+    prog = C.c_void_p()
+    assert L.hiprtcCreateProgram(C.byref(prog), src.encode(), b"hsh_jit.hip", 0, None, None) == 0
+    opts = [b"--offload-arch=gfx950", b"-O3", b"-ffp-contract=off", b"-fno-fast-math", b"-std=c++17",
+            b"-Xclang", b"-target-feature", b"-Xclang", b"-packed-fp32-ops"]
+    assert L.hiprtcCompileProgram(prog, len(opts), (C.c_char_p * len(opts))(*opts)) == 0
+    L.hiprtcDestroyProgram(C.byref(prog))
+    short = M.Model(**{**m.__dict__, "givals": m.givals[:-5]})
+    with pytest.raises(D.HshError):
+        D.HipSolver(short, host_only=True, givals=True)
