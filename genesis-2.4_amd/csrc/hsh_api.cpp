@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -64,8 +65,17 @@ struct KClass {
 
 }  // namespace
 
+// The merged arrays of a group (hsh_create_group), owned by its handle.
+struct GroupArrays {
+    std::vector<int32_t> ops, concops;
+    std::vector<double> chip, tab, xvals, stab, ravals, vm, conc, concchip, giv;
+    hsh::GroupSpec spec;
+};
+
 struct hsh_handle {
     hsh_model m{};
+    std::unique_ptr<GroupArrays> garr;   // a group's merged arrays (h->m points into them)
+    std::vector<hsh_member> members;     // where each solver's values sit in the arrays
     hsh_options opt{};
     Plan plan;
     bool refused = false;
@@ -864,10 +874,12 @@ extern "C" {
 
 const char* hsh_last_error(void) { return g_err.c_str(); }
 
-hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** out)
+}  // extern "C"
+
+namespace {
+
+hsh_status check_model(const hsh_model* m)
 {
-    if (!m || !out) return fail(HSH_ERR_ARG, "hsh_create: null argument");
-    *out = nullptr;
     if (m->ncompts <= 0 || m->nops <= 0 || m->nchips <= 0 || !m->ops || !m->chip || !m->vm ||
         !m->funcs || m->nfuncs <= 0)
         return fail(HSH_ERR_ARG, "hsh_create: empty model or missing ops/chip/vm/funcs");
@@ -883,11 +895,20 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         return fail(HSH_ERR_ARG, "hsh_create: inconsistent concentration arrays");
     if (m->ngivals < 0 || (m->ngivals > 0 && !m->givals))
         return fail(HSH_ERR_ARG, "hsh_create: ngivals > 0 but no givals");
+    return HSH_OK;
+}
+
+hsh_status create_impl(const hsh_model* m, const hsh_options* opt, hsh_handle** out, std::unique_ptr<GroupArrays> ga,
+                       std::vector<hsh_member> members)
+{
+    STCHK(check_model(m));
     if (opt && (opt->pack_waves < 0 || opt->pack_waves > 8 || opt->pack_stage < 0 || opt->pack_stage > 2 ||
                 opt->pack_cap < 0 || opt->batch_k < 0 || opt->batch_waves < 0))
         return fail(HSH_ERR_ARG, "hsh_create: option out of range");
     hsh_handle* h = new hsh_handle();
     h->m = *m;
+    h->garr = std::move(ga);
+    h->members = std::move(members);
     if (opt) h->opt = *opt;
     h->ops.assign(m->ops, m->ops + m->nops);
     h->chip.assign(m->chip, m->chip + m->nchips);
@@ -895,7 +916,7 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     if (m->nconcchip > 0) h->chip.insert(h->chip.end(), m->concchip, m->concchip + m->nconcchip);
     if (m->ngivals > 0 && m->givals) h->chip.insert(h->chip.end(), m->givals, m->givals + m->ngivals);
     std::string err;
-    if (!analyse(*m, h->plan, err)) {
+    if (!analyse(*m, h->plan, err, h->garr ? &h->garr->spec : nullptr)) {
         delete h;
         return fail(HSH_ERR_ARG, "hsh_create: " + err);
     }
@@ -1007,6 +1028,194 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
         }
     }
     *out = h;
+    return HSH_OK;
+}
+
+bool same_bits(const double* a, const double* b, size_t n)
+{
+    return n == 0 || std::memcmp(a, b, n * sizeof(double)) == 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** out)
+{
+    if (!m || !out) return fail(HSH_ERR_ARG, "hsh_create: null argument");
+    *out = nullptr;
+    std::vector<hsh_member> one(1);
+    one[0] = hsh_member{0, m->ncompts, 0, m->nchips, 0, std::max(0, m->nops - 1), 0, std::max(0, m->nconcs), 0,
+                        std::max(0, m->ngivals)};
+    return create_impl(m, opt, out, nullptr, std::move(one));
+}
+
+hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_options* opt, hsh_handle** out)
+{
+    if (!ms || !out || n <= 0) return fail(HSH_ERR_ARG, "hsh_create_group: null argument or no members");
+    *out = nullptr;
+    for (int32_t i = 0; i < n; i++) {
+        if (!ms[i]) return fail(HSH_ERR_ARG, "hsh_create_group: null member");
+        STCHK(check_model(ms[i]));
+    }
+    const hsh_model& m0 = *ms[0];
+    const bool giv = m0.ngivals > 0;
+    const hsh_model* my = nullptr;               // a member with concentration tables
+    const hsh_model* mx = nullptr;               // a member with a gate table
+    for (int32_t i = 0; i < n; i++) {
+        if (ms[i]->ydivs > 0 && !my) my = ms[i];
+        if (ms[i]->ncols > 0 && !mx) mx = ms[i];
+    }
+    for (int32_t i = 1; i < n; i++) {
+        const hsh_model& m = *ms[i];
+        // one step, one integration method for all members
+        if (m.dt != m0.dt || m.method != m0.method || m.chanmode != m0.chanmode || m.calcmode != m0.calcmode ||
+            (m.ngivals > 0) != giv)
+            return fail(HSH_ERR_ARG, "hsh_create_group: members differ in dt, method, chanmode, calcmode or givals");
+    }
+    for (int32_t i = 0; i < n; i++) {
+        const hsh_model& m = *ms[i];
+        // one voltage axis for every member with tables
+        if (m.ncols > 0 && (m.xdivs != mx->xdivs || m.xmin != mx->xmin || m.invdx != mx->invdx ||
+                            !same_bits(m.xvals, mx->xvals, (size_t)m.xdivs + 2)))
+            return fail(HSH_ERR_ARG, "hsh_create_group: members differ in the voltage table axis");
+    }
+    for (int32_t i = 0; i < n; i++)
+        if (ms[i]->ydivs > 0 && (ms[i]->ydivs != my->ydivs || ms[i]->ymin != my->ymin || ms[i]->invdy != my->invdy))
+            return fail(HSH_ERR_ARG, "hsh_create_group: members differ in the concentration table axis");
+    auto ga = std::make_unique<GroupArrays>();
+    GroupArrays& A = *ga;
+    hsh::GroupSpec& G = A.spec;
+    std::vector<hsh_member> mem(n);
+    // tables: members with the same columns share them
+    const int32_t ydivs = my ? my->ydivs : 0;
+    const int32_t xdivs = mx ? mx->xdivs : m0.xdivs;
+    const size_t trows = (size_t)std::max(xdivs, ydivs) + 2;
+    auto rows_of = [](const hsh_model& m) { return (size_t)std::max(m.xdivs, m.ydivs) + 2; };
+    std::vector<std::pair<const hsh_model*, int32_t>> tab_seen, stab_seen;
+    std::vector<int32_t> col0(n, 0), stab0(n, 0);
+    int32_t ncols = 0, nstab = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const hsh_model& m = *ms[i];
+        if (m.ncols > 0) {
+            col0[i] = -1;
+            for (auto& t : tab_seen)
+                if (t.first->ncols == m.ncols && rows_of(*t.first) == rows_of(m) &&
+                    same_bits(t.first->tablist, m.tablist, rows_of(m) * m.ncols)) {
+                    col0[i] = t.second;
+                    break;
+                }
+            if (col0[i] < 0) {
+                col0[i] = ncols;
+                tab_seen.emplace_back(&m, ncols);
+                ncols += m.ncols;
+            }
+        }
+        if (m.nstab > 0 && m.stablist) {
+            stab0[i] = -1;
+            for (auto& t : stab_seen)
+                if (t.first->nstab == m.nstab && same_bits(t.first->stablist, m.stablist, 4 * (size_t)m.nstab)) {
+                    stab0[i] = t.second;
+                    break;
+                }
+            if (stab0[i] < 0) {
+                stab0[i] = 4 * nstab;
+                stab_seen.emplace_back(&m, 4 * nstab);
+                nstab += m.nstab;
+            }
+        }
+    }
+    if (ncols > 0) {
+        A.tab.assign(trows * ncols, 0.0);
+        for (auto& t : tab_seen) {
+            // a member with fewer rows repeats its last (guard) row
+            const size_t mr = rows_of(*t.first);
+            for (size_t r = 0; r < trows; r++)
+                std::memcpy(&A.tab[r * ncols + t.second], t.first->tablist + std::min(r, mr - 1) * t.first->ncols,
+                            (size_t)t.first->ncols * sizeof(double));
+        }
+        A.xvals.assign(mx->xvals, mx->xvals + xdivs + 2);
+    }
+    for (auto& t : stab_seen) A.stab.insert(A.stab.end(), t.first->stablist, t.first->stablist + 4 * t.first->nstab);
+    int32_t rows = 0, nconcs = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const hsh_model& m = *ms[i];
+        if (m.ops[m.nops - 1] != 102) return fail(HSH_ERR_ARG, "hsh_create_group: a member's ops[] lacks LCOMPT_OP");
+        hsh_member& e = mem[i];
+        e.row0 = rows; e.nrows = m.ncompts;
+        e.chip0 = (int32_t)A.chip.size(); e.nchips = m.nchips;
+        e.op0 = (int32_t)A.ops.size(); e.nops = m.nops - 1;
+        e.conc0 = nconcs; e.nconcs = std::max(0, m.nconcs);
+        e.giv0 = (int32_t)A.giv.size(); e.ngivals = giv ? m.ngivals : 0;
+        A.ops.insert(A.ops.end(), m.ops, m.ops + m.nops - 1);
+        A.chip.insert(A.chip.end(), m.chip, m.chip + m.nchips);
+        A.vm.insert(A.vm.end(), m.vm, m.vm + m.ncompts);
+        if (m.nconcs > 0) A.conc.insert(A.conc.end(), m.conc, m.conc + m.nconcs);
+        if (m.nconcchip > 0) A.concchip.insert(A.concchip.end(), m.concchip, m.concchip + m.nconcchip);
+        if (giv) A.giv.insert(A.giv.end(), m.givals, m.givals + m.ngivals);
+        // Ca_concen pools of every member in one program, pool indices offset
+        for (int32_t q = 0; q + 1 < m.nconcops; q += 2) {
+            if (m.concops[q] == 10202) break;          // LCACONC_OP
+            A.concops.push_back(m.concops[q]);
+            A.concops.push_back(m.concops[q] == 10200 ? m.concops[q + 1] + nconcs : m.concops[q + 1]);
+        }
+        G.row0.push_back(rows);
+        G.col0.push_back(col0[i]);
+        G.stab0.push_back(stab0[i]);
+        G.conc0.push_back(nconcs);
+        G.giv0.push_back(e.giv0);
+        G.ra0.push_back((int32_t)A.ravals.size());
+        G.funcs.push_back(m.funcs);
+        G.nfuncs.push_back(m.nfuncs);
+        G.ravals.push_back(m.ravals);
+        G.nravals.push_back(m.nravals);
+        G.nrows.push_back(m.ncompts);
+        if (m.nravals > 0) A.ravals.insert(A.ravals.end(), m.ravals, m.ravals + m.nravals);
+        G.member_of.insert(G.member_of.end(), (size_t)m.ncompts, i);
+        rows += m.ncompts;
+        nconcs += std::max(0, m.nconcs);
+    }
+    A.ops.push_back(102);                               // LCOMPT_OP
+    if (!A.concops.empty()) A.concops.push_back(10202);
+    hsh_model mm = m0;
+    if (mx) {
+        mm.xdivs = mx->xdivs;
+        mm.xmin = mx->xmin;
+        mm.invdx = mx->invdx;
+    }
+    mm.ydivs = ydivs;
+    mm.ymin = my ? my->ymin : 0.0;
+    mm.invdy = my ? my->invdy : 0.0;
+    mm.ncompts = rows;
+    mm.nchips = (int32_t)A.chip.size();
+    mm.nops = (int32_t)A.ops.size();
+    mm.ncols = ncols;
+    mm.nstab = nstab;
+    mm.nconcs = nconcs;
+    mm.nconcops = (int32_t)A.concops.size();
+    mm.nconcchip = (int32_t)A.concchip.size();
+    mm.ngivals = (int32_t)A.giv.size();
+    mm.nravals = (int32_t)A.ravals.size();
+    mm.ops = A.ops.data();
+    mm.chip = A.chip.data();
+    mm.vm = A.vm.data();
+    mm.tablist = ncols > 0 ? A.tab.data() : nullptr;
+    mm.xvals = ncols > 0 ? A.xvals.data() : nullptr;
+    mm.stablist = nstab > 0 ? A.stab.data() : nullptr;
+    mm.ravals = A.ravals.empty() ? nullptr : A.ravals.data();
+    mm.conc = nconcs > 0 ? A.conc.data() : nullptr;
+    mm.concops = A.concops.empty() ? nullptr : A.concops.data();
+    mm.concchip = A.concchip.empty() ? nullptr : A.concchip.data();
+    mm.givals = A.giv.empty() ? nullptr : A.giv.data();
+    // funcs[] are decoded per member; the merged model keeps the first's as a placeholder
+    return create_impl(&mm, opt, out, std::move(ga), std::move(mem));
+}
+
+hsh_status hsh_group_member(hsh_handle* h, int32_t i, hsh_member* out)
+{
+    if (!h || !out) return fail(HSH_ERR_ARG, "hsh_group_member: null argument");
+    if (i < 0 || i >= (int32_t)h->members.size()) return fail(HSH_ERR_ARG, "hsh_group_member: no such member");
+    *out = h->members[i];
     return HSH_OK;
 }
 

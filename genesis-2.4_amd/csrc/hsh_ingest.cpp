@@ -36,7 +36,7 @@ enum EkKind { EK_NONE, EK_CONST, EK_DYN };
 
 }  // namespace
 
-bool analyse(const hsh_model& m, Plan& P, std::string& err)
+bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
 {
     const int n = m.ncompts;
     const int32_t* ops = m.ops;
@@ -93,6 +93,11 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     P.chanmode5 = (m.chanmode == 5);
     const int64_t gsrc0 = nchips + nconcs + std::max(0, m.nconcchip);
     int64_t gcur = 1;
+    // member offsets of a group (all zero for one solver)
+    int32_t ocol = 0, ostab = 0, oconc = 0;
+    auto member_start = [&](int r) {
+        return g && (r == 0 || g->member_of[r] != g->member_of[r - 1]);
+    };
     for (;;) {
         if (p >= m.nops) return bad("missing LCOMPT_OP");
         int32_t head = ops[p++];
@@ -100,6 +105,18 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
         if (head != COMPT_OP && head != FCOMPT_OP) return bad("expected COMPT_OP");
         comp++;
         if (comp >= n) return bad("more compartments than ncompts");
+        if (member_start(comp)) {
+            // a new solver: its chip update starts with fresh registers
+            // (hines_chip.c:106-150)
+            const int mi = g->member_of[comp];
+            ek = EK_NONE;
+            eksrc = -1;
+            creg = -1;
+            gcur = g->giv0[mi] + 1;
+            ocol = g->col0[mi];
+            ostab = g->stab0[mi];
+            oconc = g->conc0[mi];
+        }
         std::vector<int32_t> code;
         std::vector<int64_t> dsrc;
         std::vector<int64_t> isrc;
@@ -183,7 +200,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 break;
             case CONC_VAL_OP: {
                 if (p >= m.nops) return bad("CONC_VAL_OP operand");
-                const int32_t k = ops[p++];
+                const int32_t k = ops[p++] + oconc;
                 if (k < 0 || k >= nconcs) return bad("CONC_VAL_OP pool index out of range");
                 creg = k;
                 creg_intra = true;
@@ -220,7 +237,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 // the C1 row becomes the table register, then the row of
                 // conc[k] (hines_chip.c:224-232)
                 if (p >= m.nops) return bad("NEWCONC2_OP operand");
-                const int32_t k = ops[p++];
+                const int32_t k = ops[p++] + oconc;
                 if (k < 0 || k >= nconcs) return bad("NEWCONC2_OP pool index out of range");
                 if (m.ydivs <= 0) return bad("NEWCONC2_OP without a concentration table");
                 if (pool_chip[k] >= 0 && pool_owner[k] != comp) {
@@ -336,7 +353,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 int32_t k = ops[p++];
                 if (k == IPOL1V_OP || k == IPOL1C_OP || k == IPOL2V_OP || k == IPOL2C_OP) {
                     if (p + 2 > m.nops) return bad("IPOL operands");
-                    int32_t col = ops[p], pw = ops[p + 1];
+                    int32_t col = ops[p] + ocol, pw = ops[p + 1];
                     p += 2;
                     if (pw != 0 && !need_chip(1)) return bad("chip[] too short");
                     if (pw < -6 || pw > 6) return bad("gate power out of range");
@@ -394,7 +411,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 } else if (k == SYN2_OP || k == SYN3_OP) {
                     // [SYN2_OP, stablist index, event counter, child] (hines_chip.c:511-550)
                     if (p + 3 > m.nops) return bad("SYN2_OP operands");
-                    const int32_t st = ops[p];
+                    const int32_t st = ops[p] + ostab;
                     const int64_t cntidx = p + 1;
                     const bool mg = (p + 3 < m.nops && ops[p + 3] == MGBLOCK_OP);
                     if (k == SYN3_OP) {
@@ -508,6 +525,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     {
         int last = -1;
         for (int i = 0; i < n; i++) {
+            if (member_start(i)) last = -1;    // each solver's row cache starts empty
             if (has_volt[i]) { P.pred[i] = last; last = i; }
         }
     }
@@ -520,29 +538,30 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
     std::vector<char> got_bwd(n, 0);
     std::vector<int32_t> lastkid(n, -1);
     {
-        const int32_t* f = m.funcs;
-        const double* ra = m.ravals;
-        int64_t fi = 0, ri = 0;
-        bool ok = true;
-        if (n > 1) {
+        // One solver's funcs[]/ravals[] (hines_solve.c:223-296) for rows
+        // r0 .. r0+nn-1, its ravals at q0.. of the merged ravals.
+        auto decode = [&](const int32_t* f, int64_t nf, const double* ra, int64_t nra, int nn, int r0,
+                          int64_t q0) -> int {   // 1 ok, 0 refused, -1 malformed
+            int64_t fi = 0, ri = 0;
+            if (nn <= 1) return 1;
             int cur = 1;
             for (;;) {
-                if (fi >= m.nfuncs) { err = "funcs[] missing forward FINISH"; return false; }
+                if (fi >= nf) { err = "funcs[] missing forward FINISH"; return -1; }
                 int32_t op = f[fi++];
                 if (op == F_ELIM) {
-                    if (fi >= m.nfuncs || ri + 2 > m.nravals) { err = "funcs[] truncated"; return false; }
+                    if (fi >= nf || ri + 2 > nra) { err = "funcs[] truncated"; return -1; }
                     int32_t kr = f[fi++];
-                    if (kr < 0 || (kr & 1) || kr / 2 >= n || kr / 2 >= cur) {
-                        err = "FORWARD_ELIM row out of range"; return false;
+                    if (kr < 0 || (kr & 1) || kr / 2 >= nn || kr / 2 >= cur) {
+                        err = "FORWARD_ELIM row out of range"; return -1;
                     }
                     int kid = kr / 2;
-                    if (P.parent[kid] != -1) { ok = false; refuse(P, kid, op, (int)(fi - 2), HSH_R_TREE); }
-                    if (kid <= lastkid[cur]) { ok = false; refuse(P, cur, op, (int)(fi - 2), HSH_R_TREE); }
-                    lastkid[cur] = kid;
-                    P.parent[kid] = cur;
-                    P.ra_fwd[kid] = (int32_t)ri;
-                    P.ta[kid] = ra[ri++];
-                    P.tcf[kid] = ra[ri++];
+                    if (P.parent[r0 + kid] != -1) { refuse(P, r0 + kid, op, (int)(fi - 2), HSH_R_TREE); return 0; }
+                    if (kid <= lastkid[r0 + cur]) { refuse(P, r0 + cur, op, (int)(fi - 2), HSH_R_TREE); return 0; }
+                    lastkid[r0 + cur] = kid;
+                    P.parent[r0 + kid] = r0 + cur;
+                    P.ra_fwd[r0 + kid] = (int32_t)(q0 + ri);
+                    P.ta[r0 + kid] = ra[ri++];
+                    P.tcf[r0 + kid] = ra[ri++];
                 } else if (op == F_SET_DIAG) {
                     cur += 1;
                 } else if (op == F_SKIP_DIAG) {
@@ -550,44 +569,53 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err)
                 } else if (op == F_FINISH) {
                     break;
                 } else {
-                    refuse(P, cur, op, (int)(fi - 1), HSH_R_FUNCS);
-                    ok = false;
-                    break;
+                    refuse(P, r0 + cur, op, (int)(fi - 1), HSH_R_FUNCS);
+                    return 0;
                 }
-                if (cur >= n) { err = "funcs[] forward pass runs past the last row"; return false; }
+                if (cur >= nn) { err = "funcs[] forward pass runs past the last row"; return -1; }
             }
-            if (ok) {
-                int row = n - 2;
-                for (;;) {
-                    if (fi >= m.nfuncs) { err = "funcs[] missing backward FINISH"; return false; }
-                    int32_t op = f[fi++];
-                    if (op == F_ELIM) {
-                        if (fi >= m.nfuncs || ri + 1 > m.nravals || row < 0) { err = "funcs[] truncated"; return false; }
-                        int32_t pr = f[fi++];
-                        if (pr != 2 * P.parent[row] || got_bwd[row]) {
-                            refuse(P, row, op, (int)(fi - 2), HSH_R_TREE);
-                            ok = false;
-                            break;
-                        }
-                        got_bwd[row] = 1;
-                        P.ra_bwd[row] = (int32_t)ri;
-                        P.tcb[row] = ra[ri++];
-                    } else if (op == F_CALC) {
-                        row--;
-                    } else if (op == F_FINISH) {
-                        break;
-                    } else {
-                        refuse(P, row, op, (int)(fi - 1), HSH_R_FUNCS);
-                        ok = false;
-                        break;
+            int row = nn - 2;
+            for (;;) {
+                if (fi >= nf) { err = "funcs[] missing backward FINISH"; return -1; }
+                int32_t op = f[fi++];
+                if (op == F_ELIM) {
+                    if (fi >= nf || ri + 1 > nra || row < 0) { err = "funcs[] truncated"; return -1; }
+                    int32_t pr = f[fi++];
+                    if (pr != 2 * (P.parent[r0 + row] - r0) || got_bwd[r0 + row]) {
+                        refuse(P, r0 + row, op, (int)(fi - 2), HSH_R_TREE);
+                        return 0;
                     }
+                    got_bwd[r0 + row] = 1;
+                    P.ra_bwd[r0 + row] = (int32_t)(q0 + ri);
+                    P.tcb[r0 + row] = ra[ri++];
+                } else if (op == F_CALC) {
+                    row--;
+                } else if (op == F_FINISH) {
+                    break;
+                } else {
+                    refuse(P, r0 + row, op, (int)(fi - 1), HSH_R_FUNCS);
+                    return 0;
                 }
-                if (ok && row != -1) { err = "funcs[] backward pass does not cover every row"; return false; }
-                for (int r = 0; ok && r < n - 1; r++)
-                    if (P.parent[r] >= 0 && !got_bwd[r]) {
-                        refuse(P, r, 0, -1, HSH_R_TREE);
-                        ok = false;
-                    }
+            }
+            if (row != -1) { err = "funcs[] backward pass does not cover every row"; return -1; }
+            for (int r = 0; r < nn - 1; r++)
+                if (P.parent[r0 + r] >= 0 && !got_bwd[r0 + r]) {
+                    refuse(P, r0 + r, 0, -1, HSH_R_TREE);
+                    return 0;
+                }
+            return 1;
+        };
+        bool ok = true;
+        if (!g) {
+            const int rc = decode(m.funcs, m.nfuncs, m.ravals, m.nravals, n, 0, 0);
+            if (rc < 0) return false;
+            ok = rc == 1;
+        } else {
+            for (size_t mi = 0; mi < g->row0.size() && ok; mi++) {
+                const int rc = decode(g->funcs[mi], g->nfuncs[mi], g->ravals[mi], g->nravals[mi], g->nrows[mi],
+                                      g->row0[mi], g->ra0[mi]);
+                if (rc < 0) return false;
+                ok = rc == 1;
             }
         }
         if (ok) {

@@ -86,9 +86,22 @@ struct Plan {
     int64_t pack_lds = 0, pack_lds_stage = 0;   // LDS bytes of the pack kernels
 };
 
+// Several solvers in one handle (hsh_create_group): the analysed model is
+// their arrays concatenated, and each member keeps its own semantics -- the
+// interpreter registers start afresh at its first compartment, its operands
+// (table columns, synapse-table entries, pools) are offset into the merged
+// arrays, and its funcs[]/ravals[] are decoded on their own.
+struct GroupSpec {
+    std::vector<int32_t> member_of;                   // merged row -> member
+    std::vector<int32_t> row0, col0, stab0, conc0, giv0, ra0;
+    std::vector<const int32_t*> funcs;
+    std::vector<int32_t> nfuncs, nravals, nrows;
+    std::vector<const double*> ravals;
+};
+
 // Analyse ops[] / funcs[] and build the device layout.  Returns false on a
 // malformed model (message in err); refusals go into plan.refusals.
-bool analyse(const hsh_model& m, Plan& plan, std::string& err);
+bool analyse(const hsh_model& m, Plan& plan, std::string& err, const GroupSpec* g = nullptr);
 // (Re)build the device layout of an analysed model.
 bool build_layout(Plan& plan, const LayoutOpts& o, std::string& err);
 

@@ -54,6 +54,11 @@ class HshOptions(C.Structure):
                 ("pack_waves", C.c_int32), ("pack_stage", C.c_int32)]
 
 
+class HshMember(C.Structure):
+    _fields_ = [(f, C.c_int32) for f in ("row0", "nrows", "chip0", "nchips", "op0", "nops", "conc0", "nconcs",
+                                         "giv0", "ngivals")]
+
+
 class HshRefusal(C.Structure):
     _fields_ = [("comp", C.c_int32), ("opcode", C.c_int32), ("op_index", C.c_int32),
                 ("reason", C.c_int32)]
@@ -77,7 +82,7 @@ class HshInfo(C.Structure):
 
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
-           "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_get_givals", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
+           "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_create_group", "hsh_group_member", "hsh_get_givals", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
            "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_set_ravals",
            "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
@@ -92,6 +97,11 @@ def lib():
         L = C.CDLL(lib_path("libhsolve_hip.so"))
         L.hsh_create.argtypes = [C.POINTER(HshModel), C.POINTER(HshOptions), C.POINTER(C.c_void_p)]
         L.hsh_create.restype = C.c_int32
+        L.hsh_create_group.argtypes = [C.POINTER(C.POINTER(HshModel)), C.c_int32, C.POINTER(HshOptions),
+                                       C.POINTER(C.c_void_p)]
+        L.hsh_create_group.restype = C.c_int32
+        L.hsh_group_member.argtypes = [C.c_void_p, C.c_int32, C.POINTER(HshMember)]
+        L.hsh_group_member.restype = C.c_int32
         L.hsh_refusals.argtypes = [C.c_void_p, C.POINTER(HshRefusal), C.c_int32, _i32p]
         L.hsh_refusals.restype = C.c_int32
         L.hsh_step.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(HshStepOut)]
@@ -199,6 +209,17 @@ def jit_source(model, givals: bool = False) -> str:
     return buf.value.decode()
 
 
+class _GroupShape:
+    """Array shapes of a group handle (the members' arrays concatenated)."""
+
+    def __init__(self, models, givals):
+        self.ncompts = sum(int(m.ncompts) for m in models)
+        self.chip = np.concatenate([np.asarray(m.chip) for m in models])
+        self.ops = np.concatenate([np.asarray(m.ops)[:-1] for m in models] + [np.array([102], np.int32)])
+        self.conc = np.concatenate([np.asarray(m.conc) for m in models])
+        self.givals = np.concatenate([np.asarray(m.givals) for m in models]) if givals else np.zeros(0)
+
+
 class HipSolver:
     """One hsolve on the GPU.  ``model`` is a :class:`hsolve_hip.model.Model`."""
 
@@ -210,14 +231,23 @@ class HipSolver:
         ``pack_stage`` 1 = schedule read from memory, 2 = staged in LDS;
         ``givals``: pass the model's givals so the device writes that stream."""
         L = lib()
-        self.m = model
-        mm, self._keep = _hsh_model(model, givals)
         if host_only:
             flags |= F_HOST_ONLY
         opt = HshOptions(device, batch_k, batch_waves, flags, tile_max_rows, pack_cap, pack_waves,
                          pack_stage)
         h = C.c_void_p()
-        rc = L.hsh_create(C.byref(mm), C.byref(opt), C.byref(h))
+        if isinstance(model, (list, tuple)):
+            # a group: several solvers in one handle (hsh_create_group); the
+            # state arrays are the members' concatenated in member order
+            built = [_hsh_model(mi, givals) for mi in model]
+            self._keep = built
+            arr = (C.POINTER(HshModel) * len(built))(*[C.pointer(b[0]) for b in built])
+            self.m = _GroupShape(model, givals)
+            rc = L.hsh_create_group(arr, len(built), C.byref(opt), C.byref(h))
+        else:
+            self.m = model
+            mm, self._keep = _hsh_model(model, givals)
+            rc = L.hsh_create(C.byref(mm), C.byref(opt), C.byref(h))
         self.h = h
         if rc == HSH_REFUSED:
             refs = self.refusals()
@@ -332,6 +362,12 @@ class HipSolver:
         _check(lib().hsh_get_state(self.h, _p(vm, C.c_double), _p(chip, C.c_double),
                                    _p(res, C.c_double), _p(ops, C.c_int32)), "hsh_get_state")
         return dict(vm=vm, chip=chip, results=res, ops=ops)
+
+    def member(self, i: int) -> dict:
+        """Where member i of a group sits in the state arrays (hsh_group_member)."""
+        e = HshMember()
+        _check(lib().hsh_group_member(self.h, i, C.byref(e)), "hsh_group_member")
+        return {f: getattr(e, f) for f, _ in HshMember._fields_}
 
     def givals(self) -> np.ndarray:
         """The chanmode-4/5 output stream givals[] as the last step wrote it."""

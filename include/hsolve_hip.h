@@ -195,7 +195,33 @@ typedef struct hsh_info {
 /* opaque solver handle */
 struct hsh_handle;
 
+/* Where one solver of a group sits in the group's arrays (hsh_get_state and
+ * friends return the members' arrays concatenated in member order). */
+typedef struct hsh_member {
+    int32_t row0, nrows;     /* rows in vm[] (and results[], two per row) */
+    int32_t chip0, nchips;   /* chip[] */
+    int32_t op0, nops;       /* ops[], without the member's LCOMPT_OP */
+    int32_t conc0, nconcs;   /* conc[] */
+    int32_t giv0, ngivals;   /* givals[] */
+} hsh_member;
+
 hsh_status hsh_create(const hsh_model *m, const hsh_options *opt, struct hsh_handle **out);
+/* Several solvers (e.g. one hsolve per cell, the reference's VAnet2 layout)
+ * advanced by one handle: one launch per step for all of them, and synaptic
+ * connections between them (hsh_set_synapses with group ops indices).  Each
+ * member keeps the semantics of its own solver -- interpreter registers and
+ * row caches start afresh at its first compartment, its funcs[]/ravals[] are
+ * its own -- so vm, chip, ops, conc and givals equal the members' separate
+ * runs.  results[] follow the one-solver convention: the last row and
+ * single-compartment solvers of all but the last member hold their solved
+ * values.  Members must share dt, method, chanmode, calcmode, table axes
+ * (xdivs/xmin/invdx, ydivs/ymin/invdy) and whether they carry givals; equal
+ * tables and synapse tables are stored once.  Refusal rows are group rows.
+ * No reference counterpart: the reference batches one-cell solvers on the
+ * host each step (cuda_hsolve.c:65-145) and not with spikegens. */
+hsh_status hsh_create_group(const hsh_model *const *models, int32_t n, const hsh_options *opt,
+                            struct hsh_handle **out);
+hsh_status hsh_group_member(struct hsh_handle *h, int32_t i, hsh_member *out);
 hsh_status hsh_refusals(struct hsh_handle *h, hsh_refusal *out, int32_t cap, int32_t *n);
 hsh_status hsh_step(struct hsh_handle *h, int32_t nsteps, int32_t mode, hsh_step_out *out);
 hsh_status hsh_sync(struct hsh_handle *h);

@@ -70,6 +70,8 @@ struct ohs {
     ohs_arrays a;
     int32_t *ops;
     double *chip, *vm, *results, *givals, *conc, *flux, *codiag, *itotal;
+    double *results_base;
+    int32_t bad_funcs;         /* a funcs[] program that left the row range (not run further) */
     double *ravals;
     int32_t *spike_op, *spike_step;
     int32_t nspk, capspk;
@@ -102,7 +104,10 @@ ohs *ohs_create(const ohs_arrays *a)
     h->ops = (int32_t *)dupmem(a->ops, sizeof(int32_t) * a->nops);
     h->chip = (double *)dupmem(a->chip, sizeof(double) * a->nchips);
     h->vm = (double *)dupmem(a->vm, sizeof(double) * a->ncompts);
-    h->results = (double *)calloc(2 * (size_t)a->ncompts + 2, sizeof(double));
+    /* two leading pad values: after row 0 the reference's backward pass loads
+     * the entry before results[] (hines_solve.c:284-286), a value it never uses */
+    h->results_base = (double *)calloc(2 * (size_t)a->ncompts + 4, sizeof(double));
+    h->results = h->results_base + 2;
     h->ravals = (double *)dupmem(a->ravals, sizeof(double) * a->nravals);
     if (a->givals)
         h->givals = (double *)dupmem(a->givals, sizeof(double) * a->ngivals);
@@ -124,7 +129,7 @@ ohs *ohs_create(const ohs_arrays *a)
 void ohs_destroy(ohs *h)
 {
     if (!h) return;
-    free(h->ops); free(h->chip); free(h->vm); free(h->results); free(h->ravals);
+    free(h->ops); free(h->chip); free(h->vm); free(h->results_base); free(h->ravals);
     free(h->givals); free(h->conc); free(h->flux); free(h->codiag); free(h->itotal);
     free(h->spike_op); free(h->spike_step); free(h->trace);
     free(h->syn_of_op); free(h->conn_off); free(h->conn_dst); free(h->conn_dly); free(h->conn_w);
@@ -573,12 +578,14 @@ static size_t fwd_sweep(ohs *h, size_t *rowp, double *rv, double *dv, size_t *ra
             diaval -= raval[ra++] * temp;
             resultval -= res[funcs[f++]] * temp;
         } else if (op == SET_DIAG) {
+            if (row + 3 >= 2 * (size_t)h->a.ncompts) { h->bad_funcs = 1; break; }
             res[row] = resultval;
             res[row + 1] = diaval;
             row += 2;
             resultval = res[row];
             diaval = res[row + 1];
         } else if (op == SKIP_DIAG) {
+            if (row + 5 >= 2 * (size_t)h->a.ncompts) { h->bad_funcs = 1; break; }
             res[row] = resultval;
             res[row + 1] = diaval;
             row += 4;
@@ -618,6 +625,7 @@ void ohs_euler_solve(ohs *h)
     double resultval, diaval;
     size_t f = fwd_sweep(h, &row, &resultval, &diaval, &ra);
     size_t vi = (size_t)n - 1;
+    if (h->bad_funcs || row != 2 * vi) { h->bad_funcs = 1; return; }
     vm[vi] = res[row] = resultval / diaval;
     row -= 2;
     resultval = res[row];
@@ -626,6 +634,7 @@ void ohs_euler_solve(ohs *h)
         if (op == BACKWARD_ELIM) {
             resultval -= raval[ra++] * res[funcs[f++]];
         } else if (op == CALC_RESULTS) {
+            if (vi == 0) { h->bad_funcs = 1; break; }
             vi--;
             vm[vi] = res[row] = resultval / res[row + 1];
             row -= 2;
@@ -655,6 +664,7 @@ void ohs_crank_solve(ohs *h)
     double resultval, diaval;
     size_t f = fwd_sweep(h, &row, &resultval, &diaval, &ra);
     size_t vi = (size_t)n - 1;
+    if (h->bad_funcs || row != 2 * vi) { h->bad_funcs = 1; return; }
     resultval = resultval / diaval;
     vm[vi] = resultval + resultval - vm[vi];
     res[row] = resultval;
@@ -665,6 +675,7 @@ void ohs_crank_solve(ohs *h)
         if (op == BACKWARD_ELIM) {
             resultval -= raval[ra++] * res[funcs[f++]];
         } else if (op == CALC_RESULTS) {
+            if (vi == 0) { h->bad_funcs = 1; break; }
             resultval = resultval / res[row + 1];
             vi--;
             vm[vi] = resultval + resultval - vm[vi];
@@ -718,6 +729,7 @@ int32_t ohs_step(ohs *h, int32_t nsteps)
         if (ohs_conc_chip_update(h)) return -1;
         if (h->a.method == CRANK_INT) ohs_crank_solve(h);
         else ohs_euler_solve(h);
+        if (h->bad_funcs) return -1;           /* malformed funcs[]: stopped inside the rows */
         h->step++;
     }
     return 0;

@@ -167,7 +167,7 @@ class Oracle:
 
     def step(self, n: int = 1):
         if oracle_lib().ohs_step(self.h, n) != 0:
-            raise RuntimeError("oracle: undefined chip operation")
+            raise RuntimeError("oracle: undefined chip operation or a funcs[] program outside the rows")
 
     def chip_update(self):
         if oracle_lib().ohs_chip_update(self.h) != 0:

@@ -224,3 +224,30 @@ def test_givals_plan_and_source():
     short = M.Model(**{**m.__dict__, "givals": m.givals[:-5]})
     with pytest.raises(D.HshError):
         D.HipSolver(short, host_only=True, givals=True)
+
+
+def test_group_offsets_refusals_and_compatibility():
+    a, b = random_tree_model(3, 8, seed=1), M.isopotential_hh(4, seed=2)
+    with D.HipSolver([a, b, a], host_only=True) as h:
+        info = h.info()
+        assert info["ntrees"] == 3 + 4 + 3
+        e = [h.member(i) for i in range(3)]
+        assert [x["row0"] for x in e] == [0, 24, 28]
+        assert e[1]["op0"] == a.ops.size - 1 and e[2]["chip0"] == a.chip.size + b.chip.size
+        with pytest.raises(D.HshError):
+            h.member(3)
+    # members must share dt
+    c = random_tree_model(2, 8, seed=3, dt=10e-6)
+    with pytest.raises(D.HshError):
+        D.HipSolver([a, c], host_only=True)
+    # a refused member refuses the group; refusal rows are group rows
+    a4 = random_tree_model(3, 8, seed=1, dt=1e-4)
+    syn = synchan_model()
+    ops = np.array(syn.ops)
+    ops[int(syn.syn_op[0]) + 2] = 5            # a pending event
+    syn.ops = ops
+    with pytest.raises(D.Refused) as e:
+        D.HipSolver([a4, syn], host_only=True)
+    (comp, opcode, opidx, why), = e.value.refusals
+    assert why == "pending synaptic event" and comp == 24 + syn.hnum[1]
+    assert opidx == a4.ops.size - 1 + int(syn.syn_op[0])

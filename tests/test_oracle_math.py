@@ -156,3 +156,21 @@ def test_gate_tables_closed_form():
         assert np.allclose(tab[:m.xdivs + 1, 2 * g + 1][ok], Ap[ok], rtol=1e-12)
     # guard row repeats the top row
     assert np.array_equal(tab[m.xdivs + 1], tab[m.xdivs])
+
+
+def test_malformed_funcs_stops_inside_the_rows():
+    """A funcs[] program whose passes leave the row range (here: no SET_DIAG
+    to reach the last of 5 rows) is reported, not run past results[]."""
+    import dataclasses
+    import pytest
+    from hsolve_hip import model as M
+    from oracle import Oracle
+    m = M.population(5, (1,), (1.0,), seed=1)
+    assert m.funcs.tolist() == [1, 1, 1, 7, 6, 6, 6, 6, 7]
+    bad = dataclasses.replace(m, funcs=np.array([7, 6, 6, 6, 6, 7], np.int32))
+    o = Oracle(bad, givals=False)
+    with pytest.raises(RuntimeError):
+        o.step(1)
+    bad2 = dataclasses.replace(m, funcs=np.array([1, 1, 1, 7, 6, 6, 6, 6, 6, 6, 7], np.int32))
+    with pytest.raises(RuntimeError):
+        Oracle(bad2, givals=False).step(1)
