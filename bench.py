@@ -72,7 +72,7 @@ def parse():
     p.add_argument("--net-steps", type=int, default=50000)
     p.add_argument("--no-calcium", action="store_true",
                    help="skip the calcium section (Ca pools, Ca-gated K, GHK, table currents)")
-    p.add_argument("--ca-neurons", type=int, default=20000)
+    p.add_argument("--ca-neurons", type=int, default=50000)
     p.add_argument("--ca-steps", type=int, default=2000)
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
@@ -339,6 +339,34 @@ def network_section(a, D, M, Oracle):
                          "max_abs_dv_V": float(np.max(np.abs(o.vm - vm_dev)))}
         res["cpu_baseline"] = {"value": a.net_steps * m.ncompts / cpu, "unit": "compartment-steps/s", "cores": 1,
                                "kind": "port", "sample": f"the whole run ({cpu:.1f} s)"}
+    # the same kind of network in the reference's VAnet2 layout: one solver per
+    # cell, all of them in one group handle, events between solvers
+    t0 = time.perf_counter()
+    cells, merged, ctable = M.network_solvers(a.net_cells, seed=a.seed, inject=(0.3e-9, 0.8e-9))
+    print(f"[bench] network: {a.net_cells} one-cell solvers as a group (built {time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    with D.HipSolver(cells) as g:
+        g.set_synapses(*ctable)
+        g.autotune()
+        g.sync()
+        t1 = time.perf_counter()
+        g.step(a.net_steps, D.BATCHED)
+        g.sync()
+        wall = time.perf_counter() - t1
+        ops_, steps_ = g.drain_spikes()
+        vm_dev = g.vm
+        res["one_solver_per_cell"] = {"solvers": a.net_cells, "connections": int(ctable[0].size),
+                                      "value": a.net_steps * a.net_cells / wall, "unit": "compartment-steps/s",
+                                      "ms_per_step": wall * 1e3 / a.net_steps, "spikes": int(ops_.size),
+                                      "batch_k": g.info()["batch_k"]}
+    if not a.no_cpu:
+        o = Oracle(merged, givals=False)
+        o.set_synapses(*ctable)
+        o.step(a.net_steps)
+        w_op, w_step = o.spikes()
+        res["one_solver_per_cell"]["raster_identical"] = bool(np.array_equal(w_op, ops_) and
+                                                              np.array_equal(w_step, steps_))
+        res["one_solver_per_cell"]["max_abs_dv_V"] = float(np.max(np.abs(o.vm - vm_dev)))
     return res
 
 

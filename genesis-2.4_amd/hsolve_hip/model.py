@@ -357,6 +357,50 @@ def network(n_cells: int = 4000, *, seed: int = 0, dt: float = 1e-4, frac_exc: f
     return m, table
 
 
+def network_solvers(n_cells: int = 4000, *, seed: int = 0, dt: float = 1e-4, frac_exc: float = 0.8,
+                    p_conn: float = 0.02, g_exc: float = 6e-9, g_inh: float = 67e-9, tau_exc=(5e-3, 1e-3),
+                    tau_inh=(10e-3, 1e-3), delay: float = 1.5e-3, vm0=(-0.070, -0.050), inject=(0.0, 0.0),
+                    thresh: float = 0.0, abs_refract: float = 5e-3):
+    """The network of :func:`network` in the reference's VAnet2 layout: one
+    solver per cell.  Returns (models, merged, table): the per-cell solvers,
+    the same cells as one solver (arrays concatenated; the funcs[] program of
+    n isolated rows, SET_DIAG to the last row, then CALC_RESULTS back), and the
+    connection table in group ops indices (which are also the merged solver's
+    ops indices)."""
+    import dataclasses
+    rng = np.random.default_rng(seed)
+    L = D = 30e-6
+    area = np.pi * D * L
+    na, k = hh_chantypes(-0.065)
+    syntypes = [dict(Ek=0.0, tau1=tau_exc[0], tau2=tau_exc[1]), dict(Ek=-0.08, tau1=tau_inh[0], tau2=tau_inh[1])]
+    models = []
+    for i in range(n_cells):
+        comps = dict(parent=[-1], Cm=[0.01 * area], Rm=[0.33 / area], Ra=[1e6], Em=[-0.065],
+                     initVm=[rng.uniform(vm0[0], vm0[1])], inject=[rng.uniform(inject[0], inject[1])])
+        models.append(build(comps, [na, k], [(0, 0, 1200.0 * area), (0, 1, 360.0 * area)], dt=dt,
+                            spikes=[(0, thresh, abs_refract)], syntypes=syntypes,
+                            syns=[(0, 0, g_exc), (0, 1, g_inh)]))
+    n = n_cells
+    ops = np.concatenate([m.ops[:-1] for m in models] + [np.array([102], np.int32)]).astype(np.int32)
+    funcs = np.array([1] * (n - 2) + [7] + [6] * (n - 1) + [7] if n > 1 else [7, 7], np.int32)
+    merged = dataclasses.replace(models[0], ncompts=n, ops=ops, chip=np.concatenate([m.chip for m in models]),
+                                 vm=np.concatenate([m.vm for m in models]), funcs=funcs, ravals=np.zeros(0),
+                                 givals=np.zeros(0))
+    op0 = np.cumsum([0] + [m.ops.size - 1 for m in models[:-1]])
+    n_exc = int(round(frac_exc * n))
+    conn = rng.random((n, n)) < p_conn
+    np.fill_diagonal(conn, False)
+    src, dst = [], []
+    for i in range(n):
+        js = np.nonzero(conn[i])[0]
+        kind = 0 if i < n_exc else 1
+        src.extend([int(op0[i] + models[i].spike_op[0])] * js.size)
+        dst.extend(int(op0[j] + models[j].syn_op[kind]) for j in js)
+    kk = len(src)
+    table = (np.array(src, np.int32), np.array(dst, np.int32), np.ones(kk), np.full(kk, delay))
+    return models, merged, table
+
+
 def tabinterp(table: np.ndarray, xdivs: int, xmin: float, invdx: float, x: float) -> float:
     t = np.ascontiguousarray(table, dtype=np.float64)
     return lib().hsm_tabinterp(_p(t, C.c_double), xdivs, xmin, invdx, x)

@@ -59,48 +59,13 @@ def test_group_equals_separate_solvers(gpu, mode, kw):
             assert np.array_equal(conc[e["conc0"]:e["conc0"] + e["nconcs"]], o.conc()), i
 
 
-def one_cell(rng, exc_tau=(5e-3, 1e-3), inh_tau=(10e-3, 1e-3)):
-    L = D_ = 30e-6
-    area = np.pi * D_ * L
-    comps = dict(parent=[-1], Cm=[0.01 * area], Rm=[0.33 / area], Ra=[1e6], Em=[-0.065],
-                 initVm=[rng.uniform(-0.07, -0.05)], inject=[rng.uniform(0.3e-9, 0.8e-9)])
-    na, k = M.hh_chantypes(-0.065)
-    return M.build(comps, [na, k], [(0, 0, 1200.0 * area), (0, 1, 360.0 * area)], dt=1e-4,
-                   spikes=[(0, 0.0, 5e-3)],
-                   syntypes=[dict(Ek=0.0, tau1=exc_tau[0], tau2=exc_tau[1]),
-                             dict(Ek=-0.08, tau1=inh_tau[0], tau2=inh_tau[1])],
-                   syns=[(0, 0, 6e-9), (0, 1, 67e-9)])
-
-
-def merged_one_cell_model(ms):
-    """The same cells as one solver (single-compartment cells with equal
-    tables and synapse types): arrays concatenated, funcs of n roots."""
-    import dataclasses
-    ops = np.concatenate([m.ops[:-1] for m in ms] + [np.array([102], np.int32)]).astype(np.int32)
-    n = len(ms)
-    # SET_DIAG to the last row, FINISH; CALC_RESULTS back to row 0, FINISH
-    funcs = np.array([1] * (n - 2) + [7] + [6] * (n - 1) + [7], np.int32)
-    return dataclasses.replace(ms[0], ncompts=len(ms), ops=ops, chip=np.concatenate([m.chip for m in ms]),
-                               vm=np.concatenate([m.vm for m in ms]), funcs=funcs, ravals=np.zeros(0),
-                               givals=np.zeros(0))
-
-
 def test_network_of_one_cell_solvers(gpu):
     """400 one-cell solvers, 2% connectivity across solvers: the group's raster
     and state are those of the same cells held by one solver."""
-    rng = np.random.default_rng(8)
     n = 400
-    ms = [one_cell(rng) for _ in range(n)]
-    big = merged_one_cell_model(ms)
+    ms, big, table = M.network_solvers(n, seed=8, inject=(0.3e-9, 0.8e-9))
     op0 = np.cumsum([0] + [m.ops.size - 1 for m in ms[:-1]])
-    src, dst = [], []
-    conn = rng.random((n, n)) < 0.02
-    np.fill_diagonal(conn, False)
-    for i in range(n):
-        for j in np.nonzero(conn[i])[0]:
-            src.append(op0[i] + int(ms[i].spike_op[0]))
-            dst.append(op0[j] + int(ms[j].syn_op[0 if i < 0.8 * n else 1]))
-    table = (np.array(src, np.int32), np.array(dst, np.int32), np.ones(len(src)), np.full(len(src), 1.5e-3))
+    src = table[0]
     T = 1500
     print(f"[case] {n} one-cell solvers, {len(src)} connections", flush=True)
     o = Oracle(big, givals=False)
