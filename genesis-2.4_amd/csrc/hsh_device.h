@@ -798,11 +798,36 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
                 const int p = f0 & 0xffff;
                 const int ke = kb + (f0 >> 16);
                 double dp = LD[p], bp = LB[p];
-                for (int j = kb; j < ke; j++) {
-                    const int c = KI[j];
-                    const double tt = KTA[j] / LD[c];
-                    dp -= KTF[j] * tt;
-                    bp -= LB[c] * tt;
+                if (ke - kb <= 3) {
+                    // up to three kids: all loads and the independent divides
+                    // issue first, then the updates in the reference's order
+                    const int nk = ke - kb;
+                    double kd[3], kbv[3], kta[3], ktf[3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        const int j = q < nk ? kb + q : kb;
+                        const int c = KI[j];
+                        kd[q] = LD[c];
+                        kbv[q] = LB[c];
+                        kta[q] = KTA[j];
+                        ktf[q] = KTF[j];
+                    }
+                    double tt[3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) tt[q] = kta[q] / kd[q];
+#pragma unroll
+                    for (int q = 0; q < 3; q++)
+                        if (q < nk) {
+                            dp -= ktf[q] * tt[q];
+                            bp -= kbv[q] * tt[q];
+                        }
+                } else {
+                    for (int j = kb; j < ke; j++) {
+                        const int c = KI[j];
+                        const double tt = KTA[j] / LD[c];
+                        dp -= KTF[j] * tt;
+                        bp -= LB[c] * tt;
+                    }
                 }
                 LD[p] = dp;
                 LB[p] = bp;
