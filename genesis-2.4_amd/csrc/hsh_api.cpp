@@ -1140,7 +1140,7 @@ hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_opt
     int32_t rows = 0, nconcs = 0;
     for (int32_t i = 0; i < n; i++) {
         const hsh_model& m = *ms[i];
-        if (m.ops[m.nops - 1] != 102) return fail(HSH_ERR_ARG, "hsh_create_group: a member's ops[] lacks LCOMPT_OP");
+        if (m.ops[m.nops - 1] != LCOMPT_OP) return fail(HSH_ERR_ARG, "hsh_create_group: a member's ops[] lacks LCOMPT_OP");
         hsh_member& e = mem[i];
         e.row0 = rows; e.nrows = m.ncompts;
         e.chip0 = (int32_t)A.chip.size(); e.nchips = m.nchips;
@@ -1155,9 +1155,9 @@ hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_opt
         if (giv) A.giv.insert(A.giv.end(), m.givals, m.givals + m.ngivals);
         // Ca_concen pools of every member in one program, pool indices offset
         for (int32_t q = 0; q + 1 < m.nconcops; q += 2) {
-            if (m.concops[q] == 10202) break;          // LCACONC_OP
+            if (m.concops[q] == LCACONC_OP) break;
             A.concops.push_back(m.concops[q]);
-            A.concops.push_back(m.concops[q] == 10200 ? m.concops[q + 1] + nconcs : m.concops[q + 1]);
+            A.concops.push_back(m.concops[q] == CACONC_OP ? m.concops[q + 1] + nconcs : m.concops[q + 1]);
         }
         G.row0.push_back(rows);
         G.col0.push_back(col0[i]);
@@ -1175,8 +1175,8 @@ hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_opt
         rows += m.ncompts;
         nconcs += std::max(0, m.nconcs);
     }
-    A.ops.push_back(102);                               // LCOMPT_OP
-    if (!A.concops.empty()) A.concops.push_back(10202);
+    A.ops.push_back(LCOMPT_OP);
+    if (!A.concops.empty()) A.concops.push_back(LCACONC_OP);
     hsh_model mm = m0;
     if (mx) {
         mm.xdivs = mx->xdivs;

@@ -187,8 +187,10 @@ __device__ __forceinline__ void ipol_xy(const DevModel& d, int which, int col, c
 
 // GHK_OP (hines_chip.c:334-365): slope conductance and the Ek that gives the
 // GHK current at Vm.  exp() is the device library's.
-__device__ __noinline__ void ghk_op(double Vm, double Cin, double Cout, double zFRT, double zF, double& Ek,
-                                    double& slope)
+struct GhkOut {
+    double Ek, slope;
+};
+__device__ __forceinline__ GhkOut ghk_eval(double Vm, double Cin, double Cout, double zFRT, double zF)
 {
     double GHK = zFRT;
     const double X = Vm * GHK;
@@ -205,8 +207,13 @@ __device__ __noinline__ void ghk_op(double Vm, double Cin, double Cout, double z
         GHK = GHK * (Ik / G - E * GHK * (Cin - Cout) * Vm * Y / (G * G));
         Ik = -X * Ik / G;
     }
-    Ek = (Ik + GHK * Vm) / GHK;
-    slope = GHK;
+    return GhkOut{(Ik + GHK * Vm) / GHK, GHK};
+}
+// The interpreter calls it out of line (its exp would raise the register
+// count of every interpreted program); results come back in registers.
+__device__ __noinline__ GhkOut ghk_call(double Vm, double Cin, double Cout, double zFRT, double zF)
+{
+    return ghk_eval(Vm, Cin, Cout, zFRT, zF);
 }
 
 // Pool flux registers with a run-time pool number (interpreter): selects over
@@ -436,11 +443,11 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
         } else if (op == U_GHK) {
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
-            double slope;
-            ghk_op(Vm, SGET(ci), SGET(co), SGET(c), SGET(c + 1), Ek, slope);
-            ghk = slope;
+            const GhkOut g = ghk_call(Vm, SGET(ci), SGET(co), SGET(c), SGET(c + 1));
+            Ek = g.Ek;
+            ghk = g.slope;
             gmax = SGET(c + 2);
-            Gk = gmax * slope;
+            Gk = gmax * g.slope;
             c += 3;
         } else if (op == U_POOLCA) {
             const int p = code[pc], cs = code[pc + 1], ks = code[pc + 2];

@@ -192,9 +192,9 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
         case U_GHK: {
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
-            o += "    { double sl; ghk_op(Vm, " + slot(ci) + ", " + slot(co) + ", " + slot(c) + ", " + slot(c + 1) +
-                 ", Ek, sl); ghk = sl; " + (giv ? "gmax = " + slot(c + 2) + "; " : std::string()) + "Gk = " +
-                 slot(c + 2) + " * sl; }\n";
+            o += "    { const GhkOut g = ghk_call(Vm, " + slot(ci) + ", " + slot(co) + ", " + slot(c) + ", " +
+                 slot(c + 1) + "); Ek = g.Ek; ghk = g.slope; " +
+                 (giv ? "gmax = " + slot(c + 2) + "; " : std::string()) + "Gk = " + slot(c + 2) + " * g.slope; }\n";
             c += 3;
             break;
         }

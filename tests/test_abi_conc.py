@@ -47,7 +47,7 @@ def test_ca_generated_source_compiles_for_gfx950():
     L = C.CDLL("/opt/rocm/lib/libhiprtc.so")
     m = M.ca_population(20, (1, 8), (0.5, 0.5), features=M.CA_FEATURES[:6] + ("ghk",), seed=1)
     src = D.jit_source(m)
-    for frag in ("caconc(", "newconc(", "ghk_op(", "ipol_xy(", "flux[0] += "):
+    for frag in ("caconc(", "newconc(", "ghk_call(", "ipol_xy(", "flux[0] += "):
         assert frag in src, frag
     prog = C.c_void_p()
     assert L.hiprtcCreateProgram(C.byref(prog), src.encode(), b"hsh_jit.hip", 0, None, None) == 0
