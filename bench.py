@@ -496,6 +496,17 @@ def main():
     print(f"[bench] batched K={K}: {wall_b*1e3:.1f} ms wall, {dev_b_ms:.1f} ms device, "
           f"{r['launches']} graph/direct launches", file=sys.stderr, flush=True)
 
+    # ---- one step per call, as a per-step caller (the GENESIS glue) makes
+    # them: each hsh_step(h, 1, HSH_BATCHED) is one fused launch
+    ncall = min(a.steps, 2000)
+    gb.sync()
+    t0 = time.perf_counter()
+    for _ in range(ncall):
+        gb.step(1, D.BATCHED)
+    gb.sync()
+    wall_c = time.perf_counter() - t0
+    print(f"[bench] one step per call: {wall_c * 1e6 / ncall:.1f} us per call", file=sys.stderr, flush=True)
+
     # ---- plain path
     gp = D.HipSolver(m)
     gp.step(a.warmup, D.PLAIN)
@@ -546,6 +557,9 @@ def main():
         "plain": {"value": plain_value, "ms_per_step": wall_p * 1e3 / a.steps,
                   "channel_kernel_us": ch_ms * 1e3, "elim_kernel_us": el_ms * 1e3,
                   "batched_over_plain": value / plain_value},
+        "one_step_per_call": {"value": ncall * n / wall_c, "unit": "compartment-steps/s",
+                              "us_per_call": wall_c * 1e6 / ncall, "calls": ncall,
+                              "path": "hsh_step(h, 1, HSH_BATCHED) from Python: one fused launch per call"},
         "roofline": {"bound": "hbm",
                      "kernel": batched_kernel_name(info),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

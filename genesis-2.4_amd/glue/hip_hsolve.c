@@ -106,6 +106,8 @@ static int hip_init(Hsolve *hsolve)
         m.xvals = xv;
     }
     memset(&opt, 0, sizeof opt);
+    /* one step per PROCESS call: no K to tune, default layout */
+    opt.flags = HSH_F_NO_AUTOTUNE;
     st = (HipState *)calloc(1, sizeof(HipState));
     rc = hsh_create(&m, &opt, &st->h);
     free(tab);
@@ -155,7 +157,9 @@ int hip_chip_update(Hsolve *hsolve)
     /* h_in_msgs (hines_msgs.c:351-405) has just written message values into
      * chip[]; a GENESIS build would keep the list of chip indices its ininfo
      * entries target and push them with hsh_set_chip here. */
-    if (hsh_step(st->h, 1, HSH_PLAIN, &o) != HSH_OK) {
+    /* one step, channels and elimination in one fused launch (the batched
+     * kernel with K = 1; HSH_PLAIN would make two launches) */
+    if (hsh_step(st->h, 1, HSH_BATCHED, &o) != HSH_OK) {
         fprintf(stderr, "HIP: step failed: %s\n", hsh_last_error());
         return -1;
     }
