@@ -78,6 +78,8 @@ def parse():
                    help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
     p.add_argument("--adj-neurons", type=int, default=1024)
     p.add_argument("--adj-steps", type=int, default=2000)
+    p.add_argument("--only", choices=("heterogeneous", "calcium", "network", "adjoint"), default=None,
+                   help="run one section alone and print its JSON (for profiling it)")
     p.add_argument("--traffic-probe", action="store_true",
                    help="(child of the traffic measurement) copy calibration, then a few batched "
                         "launches, and exit")
@@ -466,6 +468,20 @@ def main():
             g.step(4 * g.info()["batch_k"], D.BATCHED)
             g.sync()
         print(json.dumps({"traffic_probe": "done"}))
+        return
+
+    if a.only:
+        from oracle import Oracle as _O
+        if a.only == "heterogeneous":
+            r = hetero_section(a, D, M, _O)
+        elif a.only == "calcium":
+            r = calcium_section(a, D, M, _O)
+        elif a.only == "network":
+            r = network_section(a, D, M, _O)
+        else:
+            from oracle import TangentLinear as _TL
+            r = adjoint_section(a, torch, _O, _TL)
+        print(json.dumps({a.only: r}))
         return
 
     t_e2e = time.perf_counter()
