@@ -74,6 +74,8 @@ def parse():
                    help="skip the calcium section (Ca pools, Ca-gated K, GHK, table currents)")
     p.add_argument("--ca-neurons", type=int, default=50000)
     p.add_argument("--ca-steps", type=int, default=2000)
+    p.add_argument("--no-percall", action="store_true",
+                   help="skip the one-step-per-call leg (keeps a profile of the headline to K-step launches)")
     p.add_argument("--no-adjoint", action="store_true",
                    help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
     p.add_argument("--adj-neurons", type=int, default=1024)
@@ -514,14 +516,19 @@ def main():
 
     # ---- one step per call, as a per-step caller (the GENESIS glue) makes
     # them: each hsh_step(h, 1, HSH_BATCHED) is one fused launch
-    ncall = min(a.steps, 2000)
-    gb.sync()
-    t0 = time.perf_counter()
-    for _ in range(ncall):
-        gb.step(1, D.BATCHED)
-    gb.sync()
-    wall_c = time.perf_counter() - t0
-    print(f"[bench] one step per call: {wall_c * 1e6 / ncall:.1f} us per call", file=sys.stderr, flush=True)
+    percall = None
+    if not a.no_percall:
+        ncall = min(a.steps, 2000)
+        gb.sync()
+        t0 = time.perf_counter()
+        for _ in range(ncall):
+            gb.step(1, D.BATCHED)
+        gb.sync()
+        wall_c = time.perf_counter() - t0
+        print(f"[bench] one step per call: {wall_c * 1e6 / ncall:.1f} us per call", file=sys.stderr, flush=True)
+        percall = {"value": ncall * n / wall_c, "unit": "compartment-steps/s",
+                   "us_per_call": wall_c * 1e6 / ncall, "calls": ncall,
+                   "path": "hsh_step(h, 1, HSH_BATCHED) from Python: one fused launch per call"}
 
     # ---- plain path
     gp = D.HipSolver(m)
@@ -573,9 +580,7 @@ def main():
         "plain": {"value": plain_value, "ms_per_step": wall_p * 1e3 / a.steps,
                   "channel_kernel_us": ch_ms * 1e3, "elim_kernel_us": el_ms * 1e3,
                   "batched_over_plain": value / plain_value},
-        "one_step_per_call": {"value": ncall * n / wall_c, "unit": "compartment-steps/s",
-                              "us_per_call": wall_c * 1e6 / ncall, "calls": ncall,
-                              "path": "hsh_step(h, 1, HSH_BATCHED) from Python: one fused launch per call"},
+        "one_step_per_call": percall,
         "roofline": {"bound": "hbm",
                      "kernel": batched_kernel_name(info),
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -355,6 +355,30 @@ void compute_classes(hsh_handle* h)
     if (!P.packs.empty()) h->classes.push_back(KClass{C_PACKS, 0, (int)P.packs.size(), 0});
 }
 
+// Does any program hold an op outside the interpreter's basic set (END,
+// NEWVOLT, CHAN, CHAN_EK, IPOL1V, ADDCURR, SPIKE, SYN2, GIV0)?  Operand
+// counts follow the micro-op table of hsh_devtypes.h; an op not named here
+// counts as extended, so the full interpreter runs it.
+bool programs_extended(const Plan& P)
+{
+    for (const Template& t : P.templates) {
+        const std::vector<int32_t>& code = t.code;
+        for (size_t pc = 0; pc < code.size();) {
+            const int op = code[pc++];
+            if (op == U_END) break;
+            if (op == U_IPOL1V || op == U_SYN2 || op == U_GIV0) {
+                pc += 2;
+            } else if (op == U_SPIKE) {
+                pc += 1;
+            } else if (op != U_NEWVOLT && op != U_CHAN && op != U_CHAN_EK && op != U_ADDCURR) {
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+
 void set_devmodel(hsh_handle* h)
 {
     const Plan& P = h->plan;
@@ -414,6 +438,7 @@ void set_devmodel(hsh_handle* h)
     d.single = (h->m.ncompts == 1);
     d.max_slots = 0;
     for (const Template& t : P.templates) d.max_slots = std::max(d.max_slots, t.nslots);
+    d.ext = programs_extended(P) ? 1 : 0;
 }
 
 // Allocate and fill the layout-dependent buffers from the plan and the given

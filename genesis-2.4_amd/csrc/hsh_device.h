@@ -272,9 +272,14 @@ __device__ __forceinline__ double caconc(double C, double base, double itau, dou
 // (s_set_gpr_idx).  A slot is never read after the program writes it (the
 // cursor only moves forward), so writes go to memory only and the register
 // copy stays read-only.  Without REG each slot is read when its micro-op runs.
+// Without EXT only the voltage-gated and synaptic ops are compiled (END,
+// NEWVOLT, CHAN, CHAN_EK, IPOL1V, ADDCURR, SPIKE, SYN2, GIV0): the
+// concentration, 2-D table, GHK and Mg-block cases would otherwise set the
+// register budget, and with it the occupancy, of every program; the host
+// picks EXT when any program holds another op.
 constexpr int MAXS = 12;
 
-template <bool REG>
+template <bool REG, bool EXT>
 __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __restrict__ code,
                                             double* __restrict__ Sb, int* __restrict__ Ib, int lane,
                                             double Vm, bool pred_ok, double Vp, bool active, int step)
@@ -343,12 +348,12 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             const int is = code[pc++];
             spike_op(d, Ib, is, lane, Vm, SGET(c), active, step);
             c += 1;
-        } else if (op == U_SYN2 || op == U_SYN2MG) {
+        } else if (op == U_SYN2 || (EXT && op == U_SYN2MG)) {
             const int k = code[pc], is = code[pc + 1];
             pc += 2;
             Gk *= syn2_op(d, Sb, Ib, c, k, is, lane, SGET(c), SGET(c + 1), active, step);
             c += 2;
-            if (op == U_SYN2MG) {
+            if (EXT && op == U_SYN2MG) {
                 if (giv) giv_put3(d, gi, Gk, Gk * (Ek - Vm), Ek, gmax);   // unblocked
                 Gk = mgblock(Gk, Vm, SGET(c), SGET(c + 1), SGET(c + 2));
                 c += 3;
@@ -360,9 +365,9 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
                 giv_put3(d, gi, Gk, Ik, Ek, gmax);
                 Im += Ik;
             }
-        } else if (op == U_CONCVAL) {
+        } else if (EXT && op == U_CONCVAL) {
             flux.set(code[pc++], 0.0);
-        } else if (op == U_NEWCONC1 || op == U_DOMAIN) {
+        } else if (EXT && (op == U_NEWCONC1 || op == U_DOMAIN)) {
             double concen;
             if (op == U_NEWCONC1) {
                 concen = SGET(code[pc++]);
@@ -372,12 +377,12 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             }
             newconc(d, concen, cfilo, cipol);
             ctab = d.tab + (long long)cfilo * ncols;
-        } else if (op == U_NEWCONC2) {
+        } else if (EXT && op == U_NEWCONC2) {
             tabr = ctab;
             ipol = cipol;
             newconc(d, SGET(code[pc++]), cfilo, cipol);
             ctab = d.tab + (long long)cfilo * ncols;
-        } else if (op == U_IPOL1C) {
+        } else if (EXT && op == U_IPOL1C) {
             const int col = code[pc];
             const int k = code[pc + 1];
             pc += 2;
@@ -386,7 +391,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             SPUT(c, X);
             c++;
             Gk = apply_power(Gk, X, k < 0 ? -k : k);
-        } else if (op == U_IPOLG) {
+        } else if (EXT && op == U_IPOLG) {
             const int which = code[pc], col = code[pc + 1], k = code[pc + 2];
             pc += 3;
             double X, Y;
@@ -395,7 +400,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             SPUT(c, X);
             c++;
             Gk = apply_power(Gk, X, k < 0 ? -k : k);
-        } else if (op == U_TABCURR || op == U_TABGHK) {
+        } else if (EXT && (op == U_TABCURR || op == U_TABGHK)) {
             const int col = code[pc];
             const int which = code[pc + 1];
             pc += 2;
@@ -414,7 +419,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
                     Im += Ik;
                 }
             }
-        } else if (op == U_SAMEGHK) {
+        } else if (EXT && op == U_SAMEGHK) {
             gmax = SGET(c);
             Gk = gmax * ghk;
             c++;
@@ -427,20 +432,20 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
                 d.giv[gi + 1] = Im;
                 gi += 2;
             }
-        } else if (op == U_FLUX) {
+        } else if (EXT && op == U_FLUX) {
             const int p = code[pc++];
             flux.set(p, flux.get(p) + Ik);
-        } else if (op == U_SCALEFLUX) {
+        } else if (EXT && op == U_SCALEFLUX) {
             const int p = code[pc++];
             flux.set(p, flux.get(p) + SGET(c) * Ik);
             c++;
-        } else if (op == U_NERNST) {
+        } else if (EXT && op == U_NERNST) {
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
             Ek = log(SGET(co) / SGET(ci)) * SGET(c);
             gmax = Gk = SGET(c + 1);
             c += 2;
-        } else if (op == U_GHK) {
+        } else if (EXT && op == U_GHK) {
             const int ci = code[pc], co = code[pc + 1];
             pc += 2;
             const GhkOut g = ghk_call(Vm, SGET(ci), SGET(co), SGET(c), SGET(c + 1));
@@ -449,7 +454,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             gmax = SGET(c + 2);
             Gk = gmax * g.slope;
             c += 3;
-        } else if (op == U_POOLCA) {
+        } else if (EXT && op == U_POOLCA) {
             const int p = code[pc], cs = code[pc + 1], ks = code[pc + 2];
             pc += 3;
             SPUT(cs, caconc(SGET(cs), SGET(ks), SGET(ks + 1), flux.get(p), SGET(ks + 2)));
@@ -468,13 +473,13 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
 
 // Runner policy: the interpreter.  (Generated modules supply their own runner
 // with one straight-line function per template.)
-template <bool REG>
+template <bool REG, bool EXT>
 struct InterpRun {
     static __device__ __forceinline__ CompOut run(const DevModel& d, const BlockMeta& bm, int lane,
                                                   double Vm, bool pok, double Vp, bool active,
                                                   int step)
     {
-        return run_prog<REG>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane, Vm, pok, Vp,
+        return run_prog<REG, EXT>(d, d.prog + bm.prog, d.S + bm.dslot, d.I + bm.islot, lane, Vm, pok, Vp,
                              active, step);
     }
 };

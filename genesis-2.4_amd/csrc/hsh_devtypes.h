@@ -145,7 +145,7 @@ struct DevModel {
     int last_pos;         // device position of the last row (ncompts-1)
     int single;           // ncompts == 1
     int max_slots;        // largest program slot count
-    int pad;
+    int ext;              // some program holds an op beyond the voltage-gated/synaptic set
 };
 
 }  // namespace hsh

@@ -18,43 +18,43 @@ namespace hsh {
 
 namespace {
 
-template <bool REG>
+template <bool REG, bool EXT>
 __global__ __launch_bounds__(256) void k_channel(DevModel d)
 {
-    body_channel<InterpRun<REG>>(d);
+    body_channel<InterpRun<REG, EXT>>(d);
 }
 
-template <int RPW>
+template <int RPW, bool EXT>
 __global__ __launch_bounds__(256) void k_batched_r(DevModel d, int K)
 {
     extern __shared__ double lds[];
-    body_batched_r<RPW, InterpRun<true>>(d, K, lds);
+    body_batched_r<RPW, InterpRun<true, EXT>>(d, K, lds);
 }
 
-template <bool REG>
+template <bool REG, bool EXT>
 __global__ __launch_bounds__(1024) void k_batched(DevModel d, int K)
 {
     extern __shared__ double lds[];
-    body_batched<InterpRun<REG>>(d, K, lds);
+    body_batched<InterpRun<REG, EXT>>(d, K, lds);
 }
 
-template <bool REG>
+template <bool REG, bool EXT>
 __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
 {
-    body_iso<InterpRun<REG>>(d, K);
+    body_iso<InterpRun<REG, EXT>>(d, K);
 }
 
-template <bool REG, bool STAGE>
+template <bool REG, bool STAGE, bool EXT>
 __global__ __launch_bounds__(512) void k_forest(DevModel d, int K)
 {
     extern __shared__ double lds[];
-    body_forest<InterpRun<REG>, true, STAGE>(d, K, lds);
+    body_forest<InterpRun<REG, EXT>, true, STAGE>(d, K, lds);
 }
 
 __global__ __launch_bounds__(256) void k_forest_elim(DevModel d)
 {
     extern __shared__ double lds[];
-    body_forest<InterpRun<false>, false, false>(d, 1, lds);
+    body_forest<InterpRun<false, false>, false, false>(d, 1, lds);
 }
 
 // Plain drop-in, part (b): one lane per neuron, one wave per tile.  Forward
@@ -198,14 +198,26 @@ hipError_t module_launch(hipFunction_t f, unsigned gx, unsigned bx, unsigned lds
 
 int batched_lds_bytes(int max_rows) { return 3 * max_rows * W * (int)sizeof(double); }
 
+// Interpreter variant of a launch: REG when every program's slots fit the
+// register copy; the basic op set (no EXT) only with REG, and only when no
+// program needs more.
+enum class Variant { RegBasic, RegExt, Mem };
+
+static Variant variant_of(const DevModel& d)
+{
+    if (d.max_slots > MAXS) return Variant::Mem;
+    return d.ext ? Variant::RegExt : Variant::RegBasic;
+}
+
 hipError_t launch_channel(const DevModel& d, const JitModule* jm, hipStream_t s)
 {
     const unsigned grid = (unsigned)((d.nblocks + 3) / 4);
     if (jm && jm->ok) return module_launch(jm->channel, grid, 256, 0, s, d, nullptr);
-    if (d.max_slots <= MAXS)
-        hipLaunchKernelGGL(k_channel<true>, dim3(grid), dim3(256), 0, s, d);
-    else
-        hipLaunchKernelGGL(k_channel<false>, dim3(grid), dim3(256), 0, s, d);
+    switch (variant_of(d)) {
+    case Variant::RegBasic: hipLaunchKernelGGL((k_channel<true, false>), dim3(grid), dim3(256), 0, s, d); break;
+    case Variant::RegExt: hipLaunchKernelGGL((k_channel<true, true>), dim3(grid), dim3(256), 0, s, d); break;
+    case Variant::Mem: hipLaunchKernelGGL((k_channel<false, true>), dim3(grid), dim3(256), 0, s, d); break;
+    }
     return hipGetLastError();
 }
 
@@ -220,31 +232,42 @@ hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int RPW>
+void launch_batched_r(const DevModel& d, int ksteps, unsigned lds, hipStream_t s)
+{
+    if (d.ext)
+        hipLaunchKernelGGL((k_batched_r<RPW, true>), dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+    else
+        hipLaunchKernelGGL((k_batched_r<RPW, false>), dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+}
+
 hipError_t launch_batched(const DevModel& d, const JitModule* jm, int ksteps, int waves, int max_rows,
                           hipStream_t s)
 {
     if (max_rows > BATCH_MAX_ROWS) return hipErrorInvalidValue;
-    const bool reg = d.max_slots <= MAXS;
+    const Variant v = variant_of(d);
     const unsigned lds = (unsigned)batched_lds_bytes(max_rows);
     const bool jit = jm && jm->ok;
-    if (waves == 0 && max_rows <= 16 && (reg || jit)) {
+    if (waves == 0 && max_rows <= 16 && (v != Variant::Mem || jit)) {
         const int which = max_rows <= 4 ? 0 : (max_rows <= 8 ? 1 : 2);
         if (jit) return module_launch(jm->batched_r[which], (unsigned)d.ntiles, 256, lds, s, d, &ksteps);
         if (which == 0)
-            hipLaunchKernelGGL(k_batched_r<1>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+            launch_batched_r<1>(d, ksteps, lds, s);
         else if (which == 1)
-            hipLaunchKernelGGL(k_batched_r<2>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+            launch_batched_r<2>(d, ksteps, lds, s);
         else
-            hipLaunchKernelGGL(k_batched_r<4>, dim3(d.ntiles), dim3(256), lds, s, d, ksteps);
+            launch_batched_r<4>(d, ksteps, lds, s);
         return hipGetLastError();
     }
     if (waves < 1) waves = 1;
     if (waves > 16) waves = 16;
+    const dim3 grid(d.ntiles), block(waves * W);
     if (jit) return module_launch(jm->batched, (unsigned)d.ntiles, (unsigned)(waves * W), lds, s, d, &ksteps);
-    if (reg)
-        hipLaunchKernelGGL(k_batched<true>, dim3(d.ntiles), dim3(waves * W), lds, s, d, ksteps);
-    else
-        hipLaunchKernelGGL(k_batched<false>, dim3(d.ntiles), dim3(waves * W), lds, s, d, ksteps);
+    switch (v) {
+    case Variant::RegBasic: hipLaunchKernelGGL((k_batched<true, false>), grid, block, lds, s, d, ksteps); break;
+    case Variant::RegExt: hipLaunchKernelGGL((k_batched<true, true>), grid, block, lds, s, d, ksteps); break;
+    case Variant::Mem: hipLaunchKernelGGL((k_batched<false, true>), grid, block, lds, s, d, ksteps); break;
+    }
     return hipGetLastError();
 }
 
@@ -252,11 +275,23 @@ hipError_t launch_iso(const DevModel& d, const JitModule* jm, int ksteps, hipStr
 {
     const unsigned grid = (unsigned)((d.ntiles + 3) / 4);
     if (jm && jm->ok) return module_launch(jm->iso, grid, 256, 0, s, d, &ksteps);
-    if (d.max_slots <= MAXS)
-        hipLaunchKernelGGL(k_iso<true>, dim3(grid), dim3(256), 0, s, d, ksteps);
-    else
-        hipLaunchKernelGGL(k_iso<false>, dim3(grid), dim3(256), 0, s, d, ksteps);
+    switch (variant_of(d)) {
+    case Variant::RegBasic: hipLaunchKernelGGL((k_iso<true, false>), dim3(grid), dim3(256), 0, s, d, ksteps); break;
+    case Variant::RegExt: hipLaunchKernelGGL((k_iso<true, true>), dim3(grid), dim3(256), 0, s, d, ksteps); break;
+    case Variant::Mem: hipLaunchKernelGGL((k_iso<false, true>), dim3(grid), dim3(256), 0, s, d, ksteps); break;
+    }
     return hipGetLastError();
+}
+
+template <bool STAGE>
+void launch_forest_v(const DevModel& d, Variant v, int ksteps, unsigned bx, int lds, hipStream_t s)
+{
+    const dim3 grid(d.npacks), block(bx);
+    switch (v) {
+    case Variant::RegBasic: hipLaunchKernelGGL((k_forest<true, STAGE, false>), grid, block, lds, s, d, ksteps); break;
+    case Variant::RegExt: hipLaunchKernelGGL((k_forest<true, STAGE, true>), grid, block, lds, s, d, ksteps); break;
+    case Variant::Mem: hipLaunchKernelGGL((k_forest<false, STAGE, true>), grid, block, lds, s, d, ksteps); break;
+    }
 }
 
 hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, bool stage, int lds,
@@ -266,14 +301,10 @@ hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int
     waves = waves < 1 ? 1 : (waves > 8 ? 8 : waves);
     const unsigned bx = (unsigned)(waves * W);
     if (jm && jm->ok) return module_launch(jm->forest[stage ? 1 : 0], (unsigned)d.npacks, bx, (unsigned)lds, s, d, &ksteps);
-    const bool reg = d.max_slots <= MAXS;
-    if (stage) {
-        if (reg) hipLaunchKernelGGL((k_forest<true, true>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
-        else hipLaunchKernelGGL((k_forest<false, true>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
-    } else {
-        if (reg) hipLaunchKernelGGL((k_forest<true, false>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
-        else hipLaunchKernelGGL((k_forest<false, false>), dim3(d.npacks), dim3(bx), lds, s, d, ksteps);
-    }
+    if (stage)
+        launch_forest_v<true>(d, variant_of(d), ksteps, bx, lds, s);
+    else
+        launch_forest_v<false>(d, variant_of(d), ksteps, bx, lds, s);
     return hipGetLastError();
 }
 
