@@ -26,6 +26,7 @@ namespace hsh {
 namespace {
 
 constexpr int MAXG = 3;     // gates per channel the reverse program keeps
+constexpr int ADJ_LDS_ROWS = 64;   // k_adj_elim_l: 4 x 64 x 64 doubles = 128 KiB of LDS
 
 struct Interp {
     int filo;
@@ -302,6 +303,172 @@ __global__ __launch_bounds__(256) void k_adj_elim(AdjModel a)
 #undef Q
 }
 
+// The same reverse sweep with the tile's working rows in LDS (one wave per
+// tile, 4 x rows x 64 doubles): the row-to-row chain of the sweeps then waits
+// on LDS instead of global memory, and what does not depend on the chain
+// (tree coefficients, parent indices, the gradient accumulators' old values)
+// is loaded ACH rows at a time into registers before those rows run.
+// Arrays: eliminated diagonal, eliminated rhs, the solution (overwritten by
+// the adjoint of diag once a row's reverse back-substitution is done: no
+// later row reads it), the adjoint of the solution (overwritten by the
+// adjoint of rhs, likewise).  Same operations in the same order as
+// k_adj_elim; a zero-initialised accumulator of k_adj_elim is written here
+// as 0.0 + v or 0.0 - v, so signed zeros agree as well.
+constexpr int ACH = 8;
+
+__global__ __launch_bounds__(64) void k_adj_elim_l(AdjModel a)
+{
+    extern __shared__ double lds[];
+    const DevModel& d = a.d;
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows;
+    const int M = a.max_rows;
+    double* sD = lds;
+    double* sB = lds + (size_t)M * W;
+    double* sX = lds + (size_t)2 * M * W;
+    double* sY = lds + (size_t)3 * M * W;
+    const long long p0 = (long long)tm.block0 * W + lane;
+#define Q(r) (p0 + (long long)(r) * W)
+#define L(r) ((r) * W + lane)
+    double* Gta = a.GT;
+    double* Gtf = a.GT + a.npos;
+    double* Gtb = a.GT + 2 * a.npos;
+    const double xs = d.crank ? 2.0 : 1.0;
+#pragma unroll 8
+    for (int r = 0; r < R; r++) {
+        sD[L(r)] = a.RD[Q(r)];
+        sB[L(r)] = a.RB[Q(r)];
+        sY[L(r)] = xs * a.LV[Q(r)];
+    }
+    // forward elimination and back-substitution, as the step ran them
+    for (int r0 = 0; r0 < R - 1; r0 += ACH) {
+        int pp[ACH];
+        double ca[ACH], cf[ACH];
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r0 + i;
+            pp[i] = -1;
+            ca[i] = cf[i] = 0.0;
+            if (r < R - 1) {
+                pp[i] = d.par[Q(r)];
+                ca[i] = d.ta[Q(r)];
+                cf[i] = d.tcf[Q(r)];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r0 + i;
+            const int p = pp[i];
+            if (p >= 0) {
+                const double tt = ca[i] / sD[L(r)];
+                sD[L(p)] -= cf[i] * tt;
+                sB[L(p)] -= sB[L(r)] * tt;
+            }
+        }
+    }
+    for (int r1 = R - 1; r1 >= 0; r1 -= ACH) {
+        int pp[ACH];
+        double cb[ACH];
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r1 - i;
+            pp[i] = -1;
+            cb[i] = 0.0;
+            if (r >= 0) {
+                pp[i] = d.par[Q(r)];
+                cb[i] = d.tcb[Q(r)];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r1 - i;
+            if (r >= 0) {
+                const int p = pp[i];
+                double bv = sB[L(r)];
+                if (p >= 0) bv -= cb[i] * sX[L(p)];
+                sX[L(r)] = bv / sD[L(r)];
+            }
+        }
+    }
+    // back-substitution in reverse: rows ascending (children before parents)
+    for (int r0 = 0; r0 < R; r0 += ACH) {
+        int pp[ACH];
+        double cb[ACH], gb[ACH];
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r0 + i;
+            pp[i] = -1;
+            cb[i] = gb[i] = 0.0;
+            if (r < R) {
+                pp[i] = d.par[Q(r)];
+                cb[i] = d.tcb[Q(r)];
+                if (pp[i] >= 0) gb[i] = Gtb[Q(r)];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r0 + i;
+            if (r < R) {
+                const int p = pp[i];
+                const double xb = sY[L(r)];
+                const double dk = sD[L(r)];
+                const double bb = 0.0 + xb / dk;
+                const double db = 0.0 - xb * sX[L(r)] / dk;
+                if (p >= 0) {
+                    Gtb[Q(r)] = gb[i] - xb * sX[L(p)] / dk;
+                    sY[L(p)] -= xb * cb[i] / dk;
+                }
+                sY[L(r)] = bb;
+                sX[L(r)] = db;
+            }
+        }
+    }
+    // forward elimination in reverse: rows descending
+    for (int r1 = R - 2; r1 >= 0; r1 -= ACH) {
+        int pp[ACH];
+        double ca[ACH], cf[ACH], ga[ACH], gf[ACH];
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r1 - i;
+            pp[i] = -1;
+            ca[i] = cf[i] = ga[i] = gf[i] = 0.0;
+            if (r >= 0) {
+                pp[i] = d.par[Q(r)];
+                if (pp[i] >= 0) {
+                    ca[i] = d.ta[Q(r)];
+                    cf[i] = d.tcf[Q(r)];
+                    ga[i] = Gta[Q(r)];
+                    gf[i] = Gtf[Q(r)];
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ACH; i++) {
+            const int r = r1 - i;
+            const int p = pp[i];
+            if (p >= 0) {
+                const double dk = sD[L(r)];
+                const double tt = ca[i] / dk;
+                const double bbp = sY[L(p)], dbp = sX[L(p)];
+                sY[L(r)] -= bbp * tt;
+                const double tbar = -bbp * sB[L(r)] - dbp * cf[i];
+                Gtf[Q(r)] = gf[i] - dbp * tt;
+                Gta[Q(r)] = ga[i] + tbar / dk;
+                sX[L(r)] -= tbar * tt / dk;
+            }
+        }
+    }
+#pragma unroll 8
+    for (int r = 0; r < R; r++) {
+        a.BB[Q(r)] = sY[L(r)];
+        a.DB[Q(r)] = sX[L(r)];
+    }
+#undef L
+#undef Q
+}
+
 __global__ void k_gather(const double* __restrict__ V, const int* __restrict__ pos, int n, double* __restrict__ out)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -326,6 +493,11 @@ hipError_t launch_adj_rhs(const AdjModel& a, hipStream_t s)
 
 hipError_t launch_adj_elim(const AdjModel& a, hipStream_t s)
 {
+    if (a.max_rows >= 1 && a.max_rows <= ADJ_LDS_ROWS) {
+        const unsigned lds = (unsigned)(4 * a.max_rows * W * sizeof(double));
+        hipLaunchKernelGGL(k_adj_elim_l, dim3(a.d.ntiles), dim3(64), lds, s, a);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(k_adj_elim, dim3((a.d.ntiles + 3) / 4), dim3(256), 0, s, a);
     return hipGetLastError();
 }

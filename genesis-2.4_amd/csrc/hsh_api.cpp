@@ -1666,6 +1666,7 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     am.X = A.X;
     am.XB = A.XB;
     am.npos = (long long)npos;
+    am.max_rows = P.max_rows;
     const int nseg = (A.nsteps + A.ckpt - 1) / A.ckpt;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int t0 = sg * A.ckpt, t1 = std::min(A.nsteps, t0 + A.ckpt);

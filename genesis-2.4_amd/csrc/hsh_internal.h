@@ -151,6 +151,7 @@ struct AdjModel {
     double* X;
     double* XB;
     long long npos;
+    int max_rows;         // rows of the largest tile
 };
 hipError_t launch_adj_rhs(const AdjModel& a, hipStream_t s);
 hipError_t launch_adj_elim(const AdjModel& a, hipStream_t s);

@@ -176,6 +176,97 @@ __global__ __launch_bounds__(64) void k_elim_r(DevModel d)
     }
 }
 
+// Plain drop-in elimination for trees of 17..ELIM_LDS_ROWS rows: one wave
+// per tile with the tile's rhs/diag in LDS (M = rows of the largest tile sets
+// the stride), so the parent updates of the sweep wait on LDS rather than on
+// global memory.  The tree coefficients do not depend on the sweep: they are
+// loaded ECH rows at a time into registers ahead of the rows that use them,
+// so one global-memory latency is paid per chunk, not per row.  Same
+// operations in the same order as k_elim.
+constexpr int ECH = 16;
+
+__global__ __launch_bounds__(64) void k_elim_l(DevModel d, int M)
+{
+    extern __shared__ double lds[];
+    double* sB = lds;
+    double* sD = lds + (size_t)M * W;
+    const int t = blockIdx.x;
+    const int lane = threadIdx.x;
+    const TileMeta tm = d.tiles[t];
+    const int R = tm.rows, b0 = tm.block0;
+    if (lane == 0) d.step[t] += 1;
+    double* Rr = d.R + (size_t)b0 * 2 * W + lane;
+    const size_t p0 = (size_t)b0 * W + lane;
+#pragma unroll 8
+    for (int r = 0; r < R; r++) {
+        sB[r * W + lane] = Rr[(size_t)r * 2 * W];
+        sD[r * W + lane] = Rr[(size_t)r * 2 * W + W];
+    }
+    const bool is_last = (p0 + (size_t)(R - 1) * W) == (size_t)d.last_pos;
+    const double droot0 = sD[(R - 1) * W + lane];
+    for (int r0 = 0; r0 < R - 1; r0 += ECH) {
+        int pp[ECH];
+        double ca[ECH], cf[ECH];
+#pragma unroll
+        for (int i = 0; i < ECH; i++) {
+            const int r = r0 + i;
+            pp[i] = -1;
+            ca[i] = cf[i] = 0.0;
+            if (r < R - 1) {
+                const size_t q = p0 + (size_t)r * W;
+                pp[i] = d.par[q];
+                ca[i] = d.ta[q];
+                cf[i] = d.tcf[q];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ECH; i++) {
+            const int r = r0 + i;
+            const int p = pp[i];
+            if (p >= 0) {
+                const double tt = ca[i] / sD[r * W + lane];
+                sD[p * W + lane] -= cf[i] * tt;
+                sB[p * W + lane] -= sB[r * W + lane] * tt;
+            }
+        }
+    }
+    for (int r1 = R - 1; r1 >= 0; r1 -= ECH) {
+        int pp[ECH];
+        double cb[ECH], vo[ECH];
+#pragma unroll
+        for (int i = 0; i < ECH; i++) {
+            const int r = r1 - i;
+            pp[i] = -1;
+            cb[i] = vo[i] = 0.0;
+            if (r >= 0) {
+                const size_t q = p0 + (size_t)r * W;
+                pp[i] = d.par[q];
+                cb[i] = d.tcb[q];
+                if (d.crank) vo[i] = d.V[q];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < ECH; i++) {
+            const int r = r1 - i;
+            if (r >= 0) {
+                const size_t q = p0 + (size_t)r * W;
+                double bv = sB[r * W + lane];
+                const int p = pp[i];
+                if (p >= 0) bv -= cb[i] * sB[p * W + lane];
+                const double x = bv / sD[r * W + lane];
+                if (!d.single) sB[r * W + lane] = x;
+                d.V[q] = d.crank ? x + x - vo[i] : x;
+            }
+        }
+    }
+    if (is_last) sD[(R - 1) * W + lane] = droot0;
+#pragma unroll 8
+    for (int r = 0; r < R; r++) {
+        Rr[(size_t)r * 2 * W] = sB[r * W + lane];
+        Rr[(size_t)r * 2 * W + W] = sD[r * W + lane];
+    }
+}
+
 __global__ void k_step_advance(int32_t* step, int32_t n, int32_t by)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -183,6 +274,7 @@ __global__ void k_step_advance(int32_t* step, int32_t n, int32_t by)
 }
 
 constexpr int BATCH_MAX_ROWS = 64;
+constexpr int ELIM_LDS_ROWS = 128;   // k_elim_l: 2 x 128 x 64 doubles = 128 KiB of LDS
 
 hipError_t module_launch(hipFunction_t f, unsigned gx, unsigned bx, unsigned lds, hipStream_t s,
                          const DevModel& d, const int* K)
@@ -225,6 +317,9 @@ hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s)
 {
     if (max_rows <= 16) {
         hipLaunchKernelGGL(k_elim_r<16>, dim3(d.ntiles), dim3(64), 0, s, d);
+    } else if (max_rows <= ELIM_LDS_ROWS) {
+        const unsigned lds = (unsigned)(2 * max_rows * W * sizeof(double));
+        hipLaunchKernelGGL(k_elim_l, dim3(d.ntiles), dim3(64), lds, s, d, max_rows);
     } else {
         dim3 grid((d.ntiles + 3) / 4);
         hipLaunchKernelGGL(k_elim, grid, dim3(256), 0, s, d);

@@ -39,12 +39,16 @@ def oracle_trace(m, nsteps, obs):
 
 
 @pytest.mark.parametrize("method", [M.BEULER, M.CRANK])
-def test_adjoint_dot_product_with_tangent_linear(gpu, method):
-    pm = F.ParamMap(6, (8, 16), (0.5, 0.5), seed=5, method=method)
+@pytest.mark.parametrize("sizes,rows", [((8, 16), 16), ((40,), 40), ((100,), 128)],
+                         ids=["rows16", "rows40-lds", "rows100-global"])
+def test_adjoint_dot_product_with_tangent_linear(gpu, method, sizes, rows):
+    """Tiles of up to 64 rows run the reverse sweep with the tile in LDS,
+    larger tiles in global memory."""
+    pm = F.ParamMap(6, sizes, (1.0 / len(sizes),) * len(sizes), seed=5, method=method)
     m, obs, T = pm.base, pm.soma, 200
-    print(f"[case] dot-product method={method} ncompts={m.ncompts}", flush=True)
+    print(f"[case] dot-product method={method} sizes={sizes} ncompts={m.ncompts}", flush=True)
     rng = np.random.default_rng(11)
-    with solver_for(m, 16) as s:
+    with solver_for(m, rows) as s:
         tr = s.adj_forward(T, 64, obs)
         assert np.array_equal(tr, oracle_trace(m, T, obs))
         w = rng.standard_normal(tr.shape)

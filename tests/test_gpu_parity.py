@@ -95,6 +95,17 @@ def test_batched_kernel_variants(gpu, ncomp, waves):
     assert_same(os_, gs)
 
 
+@pytest.mark.parametrize("ncomp", [17, 40, 64, 128, 150])
+@pytest.mark.parametrize("method", [M.BEULER, M.CRANK])
+def test_plain_tile_elimination(gpu, ncomp, method):
+    """Plain steps on tiles of larger trees (tile_max_rows 256): 17..128
+    rows eliminate with the tile's rhs/diag in LDS, 150 rows in global
+    memory."""
+    m = random_tree_model(100, ncomp, seed=ncomp + 1, method=method)
+    os_, gs = run_pair(m, 50, D.PLAIN, flags=D.F_NO_AUTOTUNE, tile_max_rows=256)
+    assert_same(os_, gs)
+
+
 @pytest.mark.parametrize("mode", [D.PLAIN, D.BATCHED])
 @pytest.mark.parametrize("method", [M.BEULER, M.CRANK])
 def test_heterogeneous_population(gpu, mode, method):
