@@ -190,12 +190,12 @@ def hetero_section(a, D, M, Oracle):
     m = M.population(a.hetero_neurons, sizes, tuple(p), seed=a.seed)
     t_build = time.perf_counter() - t0
     n = m.ncompts
-    opts = dict(batch_k=32, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=1024, pack_waves=8, pack_stage=2)
+    opts = dict(batch_k=32, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=1024, pack_waves=4, pack_stage=1)
     print(f"[bench] hetero: {a.hetero_neurons} neurons, {n} compartments (built {t_build:.1f}s)",
           file=sys.stderr, flush=True)
     res = {"workload": f"{a.hetero_neurons} random Hines trees, sizes {list(sizes)} with P ~ size^-0.67",
            "compartments": n, "steps": a.hetero_steps, "layout": "packs of ~1024 compartments, level-scheduled "
-           "elimination, 8 waves, schedule in LDS, K=32"}
+           "elimination, 4 waves, schedule read from global memory one level ahead, K=32"}
     with D.HipSolver(m, **opts) as g:
         info = g.info()
         g.step(64, D.BATCHED)
@@ -212,6 +212,19 @@ def hetero_section(a, D, M, Oracle):
                           "hbm_frac_algorithmic": info["bytes_per_step"] * a.hetero_steps / (dev_ms * 1e-3) / 1e9
                           / HBM_PEAK_GBS,
                           "packs": info["npacks"], "lane_fill": info["lane_fill"]}
+    # steps per launch: the same packs at K = 1, 8, 32, 128 (1024 steps each)
+    sweep = []
+    for K in (1, 8, 32, 128):
+        with D.HipSolver(m, **dict(opts, batch_k=K)) as g:
+            g.step(2 * K, D.BATCHED)
+            g.sync()
+            t0 = time.perf_counter()
+            g.step(1024, D.BATCHED)
+            g.sync()
+            wall = time.perf_counter() - t0
+        sweep.append({"batch_k": K, "ms_per_step": wall * 1e3 / 1024, "value": 1024 * n / wall})
+        print(f"[bench] hetero K={K}: {wall * 1e6 / 1024:.1f} us/step", file=sys.stderr, flush=True)
+    res["batch_k_sweep"] = sweep
     with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
         g.step(16, D.PLAIN)
         g.sync()

@@ -708,6 +708,14 @@ __device__ __forceinline__ void body_batched(const DevModel& d, int K, double* l
 // (:275-295) -- with a barrier between levels.  CHAN = false is the plain
 // drop-in's elimination (rhs/diag read from results, one step).  STAGE copies
 // the pack's level schedule into LDS once per launch.
+// Orders one wave's LDS writes before its later LDS reads, across lanes.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class RUN, bool CHAN, bool STAGE>
 __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* lds)
 {
@@ -783,6 +791,185 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
         droot0 = LD[pm.last_lp];
     }
     __syncthreads();
+    // Forward entry: a parent folds its kids in the reference's order.  The
+    // load part reads only what no other entry of the sweep writes (the
+    // schedule, the parent's own rhs/diag), so it is issued one level ahead;
+    // the run part reads the kids' values the previous level produced.
+    struct FwdE {
+        int p, kb, nk;
+        int c[3];
+        double ta[3], tf[3];
+        double dp, bp;
+    };
+    auto fwd_load = [&](int e, FwdE& x) {
+        const int f0 = FE[2 * e];
+        x.kb = FE[2 * e + 1];
+        x.p = f0 & 0xffff;
+        x.nk = f0 >> 16;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int j = q < x.nk ? x.kb + q : x.kb;
+            x.c[q] = KI[j];
+            x.ta[q] = KTA[j];
+            x.tf[q] = KTF[j];
+        }
+        x.dp = LD[x.p];
+        x.bp = LB[x.p];
+    };
+    // The kids' values (up to three), produced by the previous level.
+    struct FwdK {
+        double kd[3], kb[3];
+    };
+    auto fwd_kids = [&](const FwdE& x, FwdK& k) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            k.kd[q] = LD[x.c[q]];
+            k.kb[q] = LB[x.c[q]];
+        }
+    };
+    auto fwd_run = [&](const FwdE& x, const FwdK& k) {
+        double dp = x.dp, bp = x.bp;
+        if (x.nk <= 3) {
+            // up to three kids: the independent divides first, then the
+            // updates in the reference's order
+            double tt[3];
+#pragma unroll
+            for (int q = 0; q < 3; q++) tt[q] = x.ta[q] / k.kd[q];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+                if (q < x.nk) {
+                    dp -= x.tf[q] * tt[q];
+                    bp -= k.kb[q] * tt[q];
+                }
+        } else {
+            for (int j = x.kb; j < x.kb + x.nk; j++) {
+                const int c = KI[j];
+                const double tt = KTA[j] / LD[c];
+                dp -= KTF[j] * tt;
+                bp -= LB[c] * tt;
+            }
+        }
+        LD[x.p] = dp;
+        LB[x.p] = bp;
+    };
+    // Backward entry: a row's solution from its parent's.  Only the parent's
+    // solution comes from the previous level; the rest is loaded ahead.
+    struct BwdE {
+        int c, p;
+        double tcb, bc, dc, v;
+    };
+    auto bwd_load = [&](int e, BwdE& x) {
+        const int kp = BKP[e];
+        x.c = kp & 0xffff;
+        x.p = (kp >> 16) - 1;
+        x.tcb = BTC[e];
+        x.bc = LB[x.c];
+        x.dc = LD[x.c];
+        x.v = d.crank ? LV[x.c] : 0.0;
+    };
+    auto bwd_par = [&](const BwdE& x) { return x.p >= 0 ? LB[x.p] : 0.0; };
+    auto bwd_run = [&](const BwdE& x, double xp) {
+        double bv = x.bc;
+        if (x.p >= 0) bv -= x.tcb * xp;
+        const double xs = bv / x.dc;
+        // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
+        if (!d.single) LB[x.c] = xs;
+        LV[x.c] = d.crank ? xs + xs - x.v : xs;
+    };
+    // Levels [l0, l1) of a sweep (level l: entries lv[l] .. lv[l+1]-1).
+    // ALL: every thread, a workgroup barrier after each level.  Otherwise
+    // wave 0 alone, whose LDS accesses complete in order, so its levels need
+    // no barrier.  With the schedule in global memory (no STAGE) each
+    // thread's first entry of the next level is loaded before the current
+    // level runs, taking the schedule's memory latency off the level chain;
+    // with the schedule in LDS that would only queue the next level's LDS
+    // reads ahead of this level's (one wave's LDS reads complete in order).
+    auto fwd_levels = [&](int l0, int l1, bool all) {
+        const int t0 = all ? tid : lane, st = all ? T : W;
+        FwdE nx;
+        bool vn = false;
+        if (!STAGE && l0 < l1) {
+            const int e = lvl[l0] + t0;
+            vn = e < lvl[l0 + 1];
+            if (vn) fwd_load(e, nx);
+        }
+        for (int l = l0; l < l1; l++) {
+            int e1 = lvl[l] + t0;
+            if (!STAGE) {
+                const FwdE cu = nx;
+                const bool vc = vn;
+                if (l + 1 < l1) {
+                    const int e = lvl[l + 1] + t0;
+                    vn = e < lvl[l + 2];
+                    if (vn) fwd_load(e, nx);
+                }
+                if (vc) {
+                    FwdK k;
+                    fwd_kids(cu, k);
+                    fwd_run(cu, k);
+                }
+                e1 += st;
+            }
+            for (int e = e1; e < lvl[l + 1]; e += st) {
+                FwdE x;
+                FwdK k;
+                fwd_load(e, x);
+                fwd_kids(x, k);
+                fwd_run(x, k);
+            }
+            if (all)
+                __syncthreads();
+            else
+                wave_lds_sync();
+        }
+    };
+    auto bwd_levels = [&](const int* lv, int l0, int l1, bool all) {
+        const int t0 = all ? tid : lane, st = all ? T : W;
+        BwdE nx;
+        bool vn = false;
+        if (!STAGE && l0 < l1) {
+            const int e = lv[l0] + t0;
+            vn = e < lv[l0 + 1];
+            if (vn) bwd_load(e, nx);
+        }
+        for (int l = l0; l < l1; l++) {
+            int e1 = lv[l] + t0;
+            if (!STAGE) {
+                const BwdE cu = nx;
+                const bool vc = vn;
+                if (l + 1 < l1) {
+                    const int e = lv[l + 1] + t0;
+                    vn = e < lv[l + 2];
+                    if (vn) bwd_load(e, nx);
+                }
+                if (vc) bwd_run(cu, bwd_par(cu));
+                e1 += st;
+            }
+            for (int e = e1; e < lv[l + 1]; e += st) {
+                BwdE x;
+                bwd_load(e, x);
+                bwd_run(x, bwd_par(x));
+            }
+            if (all)
+                __syncthreads();
+            else
+                wave_lds_sync();
+        }
+    };
+    // Levels of at most one wave's width at the root end of the forward
+    // sweep and at either end of the backward sweep run in wave 0 alone:
+    // one wave's LDS accesses complete in order, so those levels need no
+    // workgroup barrier between them.  Wider levels use every wave.
+    const int* blv = lvl + pm.nfl + 1;
+    int fwide = pm.nfl;
+    while (fwide > 0 && lvl[fwide] - lvl[fwide - 1] <= W) fwide--;
+    int bhead = 0;
+    while (bhead < pm.nbl && blv[bhead + 1] - blv[bhead] <= W) bhead++;
+    int btail = pm.nbl;
+    while (btail > bhead && blv[btail] - blv[btail - 1] <= W) btail--;
+    fwide = __builtin_amdgcn_readfirstlane(fwide);
+    bhead = __builtin_amdgcn_readfirstlane(bhead);
+    btail = __builtin_amdgcn_readfirstlane(btail);
     for (int k = 0; k < K; k++) {
         if (CHAN) {
             for (int blk = wave; blk < pm.nblk; blk += nw) {
@@ -802,69 +989,19 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
             }
             __syncthreads();
         }
-        for (int l = 0; l < pm.nfl; l++) {
-            const int hi = lvl[l + 1];
-            for (int e = lvl[l] + tid; e < hi; e += T) {
-                const int f0 = FE[2 * e];
-                const int kb = FE[2 * e + 1];
-                const int p = f0 & 0xffff;
-                const int ke = kb + (f0 >> 16);
-                double dp = LD[p], bp = LB[p];
-                if (ke - kb <= 3) {
-                    // up to three kids: all loads and the independent divides
-                    // issue first, then the updates in the reference's order
-                    const int nk = ke - kb;
-                    double kd[3], kbv[3], kta[3], ktf[3];
-#pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        const int j = q < nk ? kb + q : kb;
-                        const int c = KI[j];
-                        kd[q] = LD[c];
-                        kbv[q] = LB[c];
-                        kta[q] = KTA[j];
-                        ktf[q] = KTF[j];
-                    }
-                    double tt[3];
-#pragma unroll
-                    for (int q = 0; q < 3; q++) tt[q] = kta[q] / kd[q];
-#pragma unroll
-                    for (int q = 0; q < 3; q++)
-                        if (q < nk) {
-                            dp -= ktf[q] * tt[q];
-                            bp -= kbv[q] * tt[q];
-                        }
-                } else {
-                    for (int j = kb; j < ke; j++) {
-                        const int c = KI[j];
-                        const double tt = KTA[j] / LD[c];
-                        dp -= KTF[j] * tt;
-                        bp -= LB[c] * tt;
-                    }
-                }
-                LD[p] = dp;
-                LB[p] = bp;
+        // forward levels that need every wave, then the narrow tail of the
+        // forward sweep and the narrow head of the backward sweep in wave 0
+        fwd_levels(0, fwide, true);
+        if (fwide < pm.nfl || bhead > 0) {
+            if (wave == 0) {
+                fwd_levels(fwide, pm.nfl, false);
+                bwd_levels(blv, 0, bhead, false);
             }
             __syncthreads();
         }
-        const int* blv = lvl + pm.nfl + 1;
-        for (int l = 0; l < pm.nbl; l++) {
-            const int hi = blv[l + 1];
-            for (int e = blv[l] + tid; e < hi; e += T) {
-                const int kp = BKP[e];
-                const int c = kp & 0xffff;
-                const int p = (kp >> 16) - 1;
-                double bv = LB[c];
-                if (p >= 0) bv -= BTC[e] * LB[p];
-                const double x = bv / LD[c];
-                // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
-                if (!d.single) LB[c] = x;
-                if (d.crank) {
-                    const double v = LV[c];
-                    LV[c] = x + x - v;
-                } else {
-                    LV[c] = x;
-                }
-            }
+        bwd_levels(blv, bhead, btail, true);
+        if (btail < pm.nbl) {
+            if (wave == 0) bwd_levels(blv, btail, pm.nbl, false);
             __syncthreads();
         }
     }
