@@ -278,6 +278,8 @@ __device__ __forceinline__ double caconc(double C, double base, double itau, dou
 // register budget, and with it the occupancy, of every program; the host
 // picks EXT when any program holds another op.
 constexpr int MAXS = 12;
+// rows of tree coefficients body_batched's one-wave elimination loads ahead
+constexpr int BCH = 16;
 
 template <bool REG, bool EXT>
 __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __restrict__ code,
@@ -528,6 +530,14 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
     double* LD = lds + 2 * R * W;
     int par[RPW], pidx[RPW];
     double ta[RPW], tcf[RPW], tcb[RPW];
+    // the rows' block records are the same every step: read once (scalar
+    // loads, the row is wave-uniform) instead of once per row per step
+    BlockMeta bms[RPW];
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int r = wave + NW * i;
+        if (r < R) bms[i] = d.blocks[b0 + r];
+    }
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
         const int r = wave + NW * i;
@@ -554,12 +564,16 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
         for (int i = 0; i < RPW; i++) {
             const int r = wave + NW * i;
             if (r < R) {
-                const int b = b0 + r;
-                const BlockMeta bm = d.blocks[b];
+                // i is wave-uniform: pick the record with selects (indexing
+                // the array with the loop counter would put it in scratch)
+                BlockMeta bm = bms[0];
+#pragma unroll
+                for (int q = 1; q < RPW; q++)
+                    if (i == q) bm = bms[q];
                 const double Vm = LV[r * W + lane];
                 const bool pok = pidx[i] >= 0;
                 const double Vp = pok ? LV[pidx[i]] : 0.0;
-                CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, active, step0 + k);
+                const CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, active, step0 + k);
                 LB[r * W + lane] = o.rhs;
                 LD[r * W + lane] = o.diag;
             }
@@ -574,9 +588,13 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                     const int i = r / NW;
                     const int p = par[i];
                     if (p >= 0) {
-                        const double tt = ta[i] / LD[r * W + lane];
-                        LD[p * W + lane] -= tcf[i] * tt;
-                        LB[p * W + lane] -= LB[r * W + lane] * tt;
+                        // all four LDS reads issue before the first write
+                        // (the compiler cannot tell LB from LD apart)
+                        const double dr = LD[r * W + lane], br = LB[r * W + lane];
+                        const double dp = LD[p * W + lane], bp = LB[p * W + lane];
+                        const double tt = ta[i] / dr;
+                        LD[p * W + lane] = dp - tcf[i] * tt;
+                        LB[p * W + lane] = bp - br * tt;
                     }
                 }
                 __syncthreads();
@@ -588,9 +606,11 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                 if (wave == (r & (NW - 1))) {
                     const int i = r / NW;
                     const int p = par[i];
+                    const double dr = LD[r * W + lane];
+                    const double xp = p >= 0 ? LB[p * W + lane] : 0.0;
                     double bv = LB[r * W + lane];
-                    if (p >= 0) bv -= tcb[i] * LB[p * W + lane];
-                    const double x = bv / LD[r * W + lane];
+                    if (p >= 0) bv -= tcb[i] * xp;
+                    const double x = bv / dr;
                     LB[r * W + lane] = x;
                     if (d.crank) {
                         const double v = LV[r * W + lane];
@@ -659,29 +679,68 @@ __device__ __forceinline__ void body_batched(const DevModel& d, int K, double* l
         }
         __syncthreads();
         if (wave == 0) {
+            // one lane per neuron; the tree coefficients do not depend on the
+            // sweep, so they are loaded BCH rows at a time ahead of the rows
+            // that use them, and each row's LDS reads issue before its writes
             const long long p0 = (long long)b0 * W + lane;
             const bool is_last = (p0 + (long long)(R - 1) * W) == (long long)d.last_pos;
             const double droot0 = LD[(R - 1) * W + lane];
-            for (int r = 0; r < R - 1; r++) {
-                const int p = d.par[p0 + (long long)r * W];
-                if (p >= 0) {
-                    const double tt = d.ta[p0 + (long long)r * W] / LD[r * W + lane];
-                    LD[p * W + lane] -= d.tcf[p0 + (long long)r * W] * tt;
-                    LB[p * W + lane] -= LB[r * W + lane] * tt;
+            for (int r0 = 0; r0 < R - 1; r0 += BCH) {
+                int pp[BCH];
+                double ca[BCH], cf[BCH];
+#pragma unroll
+                for (int i = 0; i < BCH; i++) {
+                    const int r = r0 + i;
+                    pp[i] = -1;
+                    ca[i] = cf[i] = 0.0;
+                    if (r < R - 1) {
+                        const long long q = p0 + (long long)r * W;
+                        pp[i] = d.par[q];
+                        ca[i] = d.ta[q];
+                        cf[i] = d.tcf[q];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < BCH; i++) {
+                    const int r = r0 + i;
+                    const int p = pp[i];
+                    if (p >= 0) {
+                        const double dr = LD[r * W + lane], br = LB[r * W + lane];
+                        const double dp = LD[p * W + lane], bp = LB[p * W + lane];
+                        const double tt = ca[i] / dr;
+                        LD[p * W + lane] = dp - cf[i] * tt;
+                        LB[p * W + lane] = bp - br * tt;
+                    }
                 }
             }
-            for (int r = R - 1; r >= 0; r--) {
-                const long long q = p0 + (long long)r * W;
-                double bv = LB[r * W + lane];
-                const int p = d.par[q];
-                if (p >= 0) bv -= d.tcb[q] * LB[p * W + lane];
-                const double x = bv / LD[r * W + lane];
-                LB[r * W + lane] = x;
-                if (d.crank) {
-                    const double v = LV[r * W + lane];
-                    LV[r * W + lane] = x + x - v;
-                } else {
-                    LV[r * W + lane] = x;
+            for (int r1 = R - 1; r1 >= 0; r1 -= BCH) {
+                int pp[BCH];
+                double cb[BCH];
+#pragma unroll
+                for (int i = 0; i < BCH; i++) {
+                    const int r = r1 - i;
+                    pp[i] = -1;
+                    cb[i] = 0.0;
+                    if (r >= 0) {
+                        const long long q = p0 + (long long)r * W;
+                        pp[i] = d.par[q];
+                        cb[i] = d.tcb[q];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < BCH; i++) {
+                    const int r = r1 - i;
+                    if (r >= 0) {
+                        const int p = pp[i];
+                        const double dr = LD[r * W + lane];
+                        const double xp = p >= 0 ? LB[p * W + lane] : 0.0;
+                        const double v = LV[r * W + lane];
+                        double bv = LB[r * W + lane];
+                        if (p >= 0) bv -= cb[i] * xp;
+                        const double x = bv / dr;
+                        LB[r * W + lane] = x;
+                        LV[r * W + lane] = d.crank ? x + x - v : x;
+                    }
                 }
             }
             if (is_last) LD[(R - 1) * W + lane] = droot0;
@@ -697,6 +756,14 @@ __device__ __forceinline__ void body_batched(const DevModel& d, int K, double* l
     if (threadIdx.x == 0) d.step[t] = step0 + K;
 }
 
+// Orders one wave's LDS writes before its later LDS reads, across lanes.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Packs (lane = compartment): whole neurons of any size, one workgroup per
 // pack.  V, rhs and diag of every pack-local position live in LDS for the
 // whole launch.  Per step: the waves run the channel programs of the pack's
@@ -708,14 +775,6 @@ __device__ __forceinline__ void body_batched(const DevModel& d, int K, double* l
 // (:275-295) -- with a barrier between levels.  CHAN = false is the plain
 // drop-in's elimination (rhs/diag read from results, one step).  STAGE copies
 // the pack's level schedule into LDS once per launch.
-// Orders one wave's LDS writes before its later LDS reads, across lanes.
-__device__ __forceinline__ void wave_lds_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 template <class RUN, bool CHAN, bool STAGE>
 __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* lds)
 {

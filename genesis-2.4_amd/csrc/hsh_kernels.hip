@@ -224,9 +224,13 @@ __global__ __launch_bounds__(64) void k_elim_l(DevModel d, int M)
             const int r = r0 + i;
             const int p = pp[i];
             if (p >= 0) {
-                const double tt = ca[i] / sD[r * W + lane];
-                sD[p * W + lane] -= cf[i] * tt;
-                sB[p * W + lane] -= sB[r * W + lane] * tt;
+                // all four LDS reads before the writes (sB and sD share one
+                // allocation, so the compiler cannot reorder them itself)
+                const double dr = sD[r * W + lane], br = sB[r * W + lane];
+                const double dp = sD[p * W + lane], bp = sB[p * W + lane];
+                const double tt = ca[i] / dr;
+                sD[p * W + lane] = dp - cf[i] * tt;
+                sB[p * W + lane] = bp - br * tt;
             }
         }
     }
@@ -250,10 +254,12 @@ __global__ __launch_bounds__(64) void k_elim_l(DevModel d, int M)
             const int r = r1 - i;
             if (r >= 0) {
                 const size_t q = p0 + (size_t)r * W;
-                double bv = sB[r * W + lane];
                 const int p = pp[i];
-                if (p >= 0) bv -= cb[i] * sB[p * W + lane];
-                const double x = bv / sD[r * W + lane];
+                const double dr = sD[r * W + lane];
+                const double xp = p >= 0 ? sB[p * W + lane] : 0.0;
+                double bv = sB[r * W + lane];
+                if (p >= 0) bv -= cb[i] * xp;
+                const double x = bv / dr;
                 if (!d.single) sB[r * W + lane] = x;
                 d.V[q] = d.crank ? x + x - vo[i] : x;
             }
