@@ -535,12 +535,12 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
     BlockMeta bms[RPW];
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
-        const int r = wave + NW * i;
+        const int r = wave * RPW + i;
         if (r < R) bms[i] = d.blocks[b0 + r];
     }
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
-        const int r = wave + NW * i;
+        const int r = wave * RPW + i;
         par[i] = -1;
         pidx[i] = -1;
         ta[i] = tcf[i] = tcb[i] = 0.0;
@@ -560,9 +560,13 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
     const int step0 = d.step[t];
     __syncthreads();
     for (int k = 0; k < K; k++) {
+        // the last row's diagonal as the channel program leaves it: its
+        // owner keeps it (other waves may fold kids into it right after the
+        // barrier) and puts it back after the sweep
+        double droot0 = 0.0;
 #pragma unroll 1
         for (int i = 0; i < RPW; i++) {
-            const int r = wave + NW * i;
+            const int r = wave * RPW + i;
             if (r < R) {
                 // i is wave-uniform: pick the record with selects (indexing
                 // the array with the loop counter would put it in scratch)
@@ -576,16 +580,15 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                 const CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, active, step0 + k);
                 LB[r * W + lane] = o.rhs;
                 LD[r * W + lane] = o.diag;
+                if (r == R - 1) droot0 = o.diag;
             }
         }
         __syncthreads();
-        double droot0 = 0.0;
-        if (wave == ((R - 1) & (NW - 1))) droot0 = LD[(R - 1) * W + lane];
 #pragma unroll
         for (int r = 0; r < RMAX - 1; r++) {
             if (r < R - 1) {
-                if (wave == (r & (NW - 1))) {
-                    const int i = r / NW;
+                if (wave == r / RPW) {
+                    const int i = r % RPW;
                     const int p = par[i];
                     if (p >= 0) {
                         // all four LDS reads issue before the first write
@@ -597,14 +600,16 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                         LB[p * W + lane] = bp - br * tt;
                     }
                 }
-                __syncthreads();
+                // a wave's consecutive rows need no barrier between them
+                // (its LDS accesses complete in order); the next wave's do
+                if (r % RPW == RPW - 1 || r == R - 2) __syncthreads();
             }
         }
 #pragma unroll
         for (int r = RMAX - 1; r >= 0; r--) {
             if (r < R) {
-                if (wave == (r & (NW - 1))) {
-                    const int i = r / NW;
+                if (wave == r / RPW) {
+                    const int i = r % RPW;
                     const int p = par[i];
                     const double dr = LD[r * W + lane];
                     const double xp = p >= 0 ? LB[p * W + lane] : 0.0;
@@ -622,13 +627,13 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                     // diagonal (hines_solve.c:275-276)
                     if (r == R - 1 && is_last) LD[r * W + lane] = droot0;
                 }
-                __syncthreads();
+                if (r % RPW == 0) __syncthreads();
             }
         }
     }
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
-        const int r = wave + NW * i;
+        const int r = wave * RPW + i;
         if (r < R) {
             const long long b = b0 + r;
             d.V[b * W + lane] = LV[r * W + lane];
