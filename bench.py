@@ -38,7 +38,7 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 NEURONS, NCOMP = 50000, 16
-CAL_MB = 256                   # device copy used to scale FETCH_SIZE / WRITE_SIZE
+CAL_MB = 256                   # device copies used to scale FETCH_SIZE / WRITE_SIZE
 
 
 def parse():
@@ -91,11 +91,17 @@ def parse():
 
 
 def copy_calibration(torch, mb):
-    """Four device-to-device copies of ``mb`` MiB: the counter reading of a
-    copy of known size gives the scale of FETCH_SIZE and WRITE_SIZE."""
+    """Copies of ``mb`` MiB whose counter readings scale FETCH_SIZE and
+    WRITE_SIZE: four by an elementwise kernel (``torch.mul`` by 1.0, one
+    8-byte load and store per element: the access kind of this library's
+    kernels) and four by the runtime's blit (``copy_``), kept for
+    comparison -- the blit's reads and writes count at half the rate of
+    ordinary kernels' on this GPU."""
     nel = mb * (1 << 20) // 8
     src = torch.ones(nel, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
+    for _ in range(4):
+        torch.mul(src, 1.0, out=dst)
     for _ in range(4):
         dst.copy_(src)
     torch.cuda.synchronize()
@@ -103,28 +109,35 @@ def copy_calibration(torch, mb):
 
 
 def read_counter(db, counter, kernel):
-    """Mean per-dispatch value of ``counter`` for the batched kernel and for
-    the calibration copy, from a rocprofv3 --pmc database."""
+    """Mean per-dispatch value of ``counter`` for the batched kernel, for the
+    elementwise calibration copy and for the blit copy, from a rocprofv3
+    --pmc database."""
+    # a built-in kernel's name ends in its first template arguments
+    # ("k_batched_r<4>"); the trace also lists the rest ("k_batched_r<4, false>")
+    pat = kernel[:-1] if kernel.endswith(">") else kernel
     con = sqlite3.connect(db)
-    kern, copy = [], []
+    kern, elem, blit = [], [], []
     for name, cn, val in con.execute("select kernel_name, counter_name, value from counters_collection"):
         if cn != counter:
             continue
-        if kernel in name:
+        if pat in name:
             kern.append(val)
+        elif "MulFunctor" in name:
+            elem.append(val)
         elif "copyBuffer" in name:
-            copy.append(val)
+            blit.append(val)
     con.close()
-    if not kern or not copy:
-        raise RuntimeError(f"{counter}: no dispatches of the batched kernel or the copy in {db}")
-    return sum(kern) / len(kern), sum(copy) / len(copy)
+    if not kern or not elem or not blit:
+        raise RuntimeError(f"{counter}: no dispatches of the batched kernel or a calibration copy in {db}")
+    return sum(kern) / len(kern), sum(elem) / len(elem), sum(blit) / len(blit)
 
 
 def measure_traffic(a, K, info, kernel):
     """HBM bytes per batched launch: two child runs of this script under
     ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE).  Each counter is
-    scaled by what it reads for a copy of CAL_MB MiB in the same run
-    (true bytes / counter), so the unit of the counter does not matter."""
+    scaled by what it reads for an elementwise copy of CAL_MB MiB in the
+    same run (true bytes / counter), so the unit of the counter does not
+    matter."""
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None, "rocprofv3 not found"
@@ -158,8 +171,12 @@ def measure_traffic(a, K, info, kernel):
     t = {"bytes_per_launch": (res["FETCH_SIZE"][0] * f_fetch + res["WRITE_SIZE"][0] * f_write) * 1024.0,
          "fetch_kib_per_launch": res["FETCH_SIZE"][0], "write_kib_per_launch": res["WRITE_SIZE"][0],
          "fetch_scale": f_fetch, "write_scale": f_write,
-         "calibration": f"{CAL_MB} MiB device copy read as {res['FETCH_SIZE'][1]:.0f} KiB FETCH_SIZE, "
-                        f"{res['WRITE_SIZE'][1]:.0f} KiB WRITE_SIZE"}
+         "calibration": f"{CAL_MB} MiB elementwise copy read as {res['FETCH_SIZE'][1]:.0f} KiB FETCH_SIZE, "
+                        f"{res['WRITE_SIZE'][1]:.0f} KiB WRITE_SIZE",
+         "blit_calibration": f"{CAL_MB} MiB runtime blit copy read as {res['FETCH_SIZE'][2]:.0f} KiB "
+                             f"FETCH_SIZE, {res['WRITE_SIZE'][2]:.0f} KiB WRITE_SIZE",
+         "bytes_per_launch_blit_scale": (res["FETCH_SIZE"][0] * cal_kib / res["FETCH_SIZE"][2] +
+                                         res["WRITE_SIZE"][0] * cal_kib / res["WRITE_SIZE"][2]) * 1024.0}
     return t, None
 
 
