@@ -336,7 +336,7 @@ __global__ __launch_bounds__(64) void k_adj_elim_l(AdjModel a)
     double* Gtf = a.GT + a.npos;
     double* Gtb = a.GT + 2 * a.npos;
     const double xs = d.crank ? 2.0 : 1.0;
-#pragma unroll 8
+#pragma unroll 16
     for (int r = 0; r < R; r++) {
         sD[L(r)] = a.RD[Q(r)];
         sB[L(r)] = a.RB[Q(r)];

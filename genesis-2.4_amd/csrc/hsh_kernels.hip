@@ -120,7 +120,9 @@ __global__ __launch_bounds__(64) void k_elim_r(DevModel d)
     double* Rr = d.R + (size_t)b0 * 2 * W + lane;
     const size_t p0 = (size_t)b0 * W + lane;
     int par[RMAX];
-    double ta[RMAX], tcf[RMAX], tcb[RMAX];
+    double ta[RMAX], tcf[RMAX], tcb[RMAX], rb[RMAX], rd[RMAX];
+    // every load first, then the LDS stores: a store right after its row's
+    // loads would wait for them, one memory latency per row
 #pragma unroll
     for (int r = 0; r < RMAX; r++) {
         if (r < R) {
@@ -129,11 +131,18 @@ __global__ __launch_bounds__(64) void k_elim_r(DevModel d)
             ta[r] = d.ta[q];
             tcf[r] = d.tcf[q];
             tcb[r] = d.tcb[q];
-            sB[r * W + lane] = Rr[(size_t)r * 2 * W];
-            sD[r * W + lane] = Rr[(size_t)r * 2 * W + W];
+            rb[r] = Rr[(size_t)r * 2 * W];
+            rd[r] = Rr[(size_t)r * 2 * W + W];
         } else {
             par[r] = -1;
-            ta[r] = tcf[r] = tcb[r] = 0.0;
+            ta[r] = tcf[r] = tcb[r] = rb[r] = rd[r] = 0.0;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < RMAX; r++) {
+        if (r < R) {
+            sB[r * W + lane] = rb[r];
+            sD[r * W + lane] = rd[r];
         }
     }
     const bool is_last = (p0 + (size_t)(R - 1) * W) == (size_t)d.last_pos;
@@ -197,7 +206,8 @@ __global__ __launch_bounds__(64) void k_elim_l(DevModel d, int M)
     if (lane == 0) d.step[t] += 1;
     double* Rr = d.R + (size_t)b0 * 2 * W + lane;
     const size_t p0 = (size_t)b0 * W + lane;
-#pragma unroll 8
+    // 16 rows' loads in flight before their LDS stores wait for them
+#pragma unroll 16
     for (int r = 0; r < R; r++) {
         sB[r * W + lane] = Rr[(size_t)r * 2 * W];
         sD[r * W + lane] = Rr[(size_t)r * 2 * W + W];
