@@ -538,22 +538,34 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
         const int r = wave * RPW + i;
         if (r < R) bms[i] = d.blocks[b0 + r];
     }
+    // every row's loads first, then the LDS stores of V (a store right
+    // after its row's loads would wait for them, one latency per row)
+    double v0[RPW];
+    int pp[RPW];
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
         const int r = wave * RPW + i;
         par[i] = -1;
-        pidx[i] = -1;
-        ta[i] = tcf[i] = tcb[i] = 0.0;
+        pp[i] = -1;
+        ta[i] = tcf[i] = tcb[i] = v0[i] = 0.0;
         if (r < R) {
             const long long q = (long long)(b0 + r) * W + lane;
             par[i] = d.par[q];
             ta[i] = d.ta[q];
             tcf[i] = d.tcf[q];
             tcb[i] = d.tcb[q];
-            LV[r * W + lane] = d.V[q];
-            const int pp = d.pred[q];
-            if (pp >= 0 && (pp >> 6) >= b0 && (pp >> 6) < b0 + R)
-                pidx[i] = ((pp >> 6) - b0) * W + (pp & (W - 1));
+            v0[i] = d.V[q];
+            pp[i] = d.pred[q];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < RPW; i++) {
+        const int r = wave * RPW + i;
+        pidx[i] = -1;
+        if (r < R) {
+            LV[r * W + lane] = v0[i];
+            if (pp[i] >= 0 && (pp[i] >> 6) >= b0 && (pp[i] >> 6) < b0 + R)
+                pidx[i] = ((pp[i] >> 6) - b0) * W + (pp[i] & (W - 1));
         }
     }
     const bool is_last = ((long long)(b0 + R - 1) * W + lane) == (long long)d.last_pos;
@@ -833,6 +845,8 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
         BKP = sBKP;
         KI = sKI;
     }
+    // several positions' loads in flight before their LDS stores wait
+#pragma unroll 4
     for (int i = tid; i < NP; i += T) {
         LV[i] = d.V[base + i];
         if (CHAN) {
