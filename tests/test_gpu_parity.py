@@ -170,3 +170,22 @@ def test_steps_split_across_calls(gpu):
         g.step(40, D.BATCHED)
         gs = g.state()
     assert_same(o.state(), gs)
+
+
+@pytest.mark.parametrize("layout", [
+    dict(tile_max_rows=16), dict(tile_max_rows=64), dict(tile_max_rows=-1, pack_cap=512, pack_waves=4, pack_stage=1),
+    dict(tile_max_rows=-1, pack_cap=1024, pack_waves=8, pack_stage=2)], ids=["tiles16", "tiles64", "packs", "packs-staged"])
+@pytest.mark.parametrize("method", [M.BEULER, M.CRANK])
+def test_one_step_per_call(gpu, layout, method):
+    """The glue's per-PROCESS path: hsh_step(h, 1, HSH_BATCHED) once per step
+    (each call one K = 1 launch of the fused kernel) equals the oracle."""
+    m = M.population(120, (1, 8, 16, 64, 256), (0.3, 0.3, 0.2, 0.15, 0.05), seed=21, method=method)
+    T = 60
+    print(f"[case] one step per call {layout} method={method}", flush=True)
+    o = Oracle(m, givals=False)
+    o.step(T)
+    with D.HipSolver(m, batch_k=128, flags=D.F_NO_AUTOTUNE, **layout) as g:
+        for _ in range(T):
+            g.step(1, D.BATCHED)
+        gs = g.state()
+    assert_same(o.state(), gs)
