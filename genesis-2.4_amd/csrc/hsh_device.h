@@ -509,11 +509,12 @@ __device__ __forceinline__ void body_channel(const DevModel& d)
 }
 
 // Batched persistent kernel for small trees (rows <= 4*RPW): four waves per
-// tile, wave w owns rows w, w+4, ...  Each owner keeps its rows' tree
-// coefficients in registers for the whole launch, and the elimination walks
-// the rows in order -- the owner of row r folds it into its parent, then all
-// four waves meet at a barrier -- so the tile needs only 24 bytes of LDS per
-// compartment and every tile of the workload is resident at once.
+// tile, wave w owns rows w*RPW .. w*RPW+RPW-1.  Each owner keeps its rows'
+// tree coefficients in registers for the whole launch, and the elimination
+// walks the rows in order -- the owner of row r folds it into its parent; the
+// waves meet at a barrier where the next row belongs to another wave -- so
+// the tile needs only 24 bytes of LDS per compartment and every tile of the
+// workload is resident at once.
 template <int RPW, class RUN>
 __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double* lds)
 {
