@@ -90,6 +90,18 @@ def parse():
     return p.parse_args()
 
 
+def cpu_model():
+    """The host CPU's model name (the first "model name" of /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def copy_calibration(torch, mb):
     """Copies of ``mb`` MiB whose counter readings scale FETCH_SIZE and
     WRITE_SIZE: four by an elementwise kernel (``torch.mul`` by 1.0, one
@@ -668,14 +680,17 @@ def main():
         step_parallel(subs, S, nth)
         cpuN = time.perf_counter() - t0
         cpuN_rate = S * n / cpuN
+        host = cpu_model()
         out["cpu_baseline"] = {"value": cpu1_rate, "unit": "compartment-steps/s", "cores": 1,
                                "kind": "port",
                                "sample": f"{rest} steps of the full {n}-compartment population "
-                                         f"({cpu1:.1f} s), oracle compiled -O2 -ffp-contract=off"}
+                                         f"({cpu1:.1f} s), oracle compiled -O2 -ffp-contract=off",
+                               "cpu": host}
         out["cpu_baseline_all_cores"] = {"value": cpuN_rate, "unit": "compartment-steps/s",
                                          "cores": nth, "kind": "port",
                                          "sample": f"{S} steps, neurons split into {len(subs)} "
-                                                   f"independent blocks ({cpuN:.1f} s)"}
+                                                   f"independent blocks ({cpuN:.1f} s)",
+                                         "cpu": host, "nproc": os.cpu_count()}
         out["parity"] = {"steps": S, "max_abs_dv_batched_V": par["batched"],
                          "max_abs_dv_plain_V": par["plain"],
                          "bitexact_batched": par["batched_bitexact"],
