@@ -644,6 +644,9 @@ def main():
             out["roofline"]["traffic"] = t["bytes_per_launch"]
             out["roofline"]["traffic_detail"] = t
             out["roofline"]["traffic_over_algorithmic"] = t["bytes_per_launch"] / (bytes_step * K)
+            # the measured bytes over the timed launch duration (counters of the
+            # same layout and K in the child runs)
+            out["roofline"]["traffic_gbs"] = t["bytes_per_launch"] / launch_s / 1e9
 
     # ---- CPU baseline + parity (oracle)
     if not a.no_cpu:
