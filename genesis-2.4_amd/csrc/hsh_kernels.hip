@@ -24,8 +24,10 @@ __global__ __launch_bounds__(256) void k_channel(DevModel d)
     body_channel<InterpRun<REG, EXT>>(d);
 }
 
+// The basic variant is held to 128 VGPRs (four waves per SIMD): the C2
+// population's 782 tiles then fit the chip in one round of workgroups.
 template <int RPW, bool EXT>
-__global__ __launch_bounds__(256) void k_batched_r(DevModel d, int K)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(EXT ? 1 : 4))) void k_batched_r(DevModel d, int K)
 {
     extern __shared__ double lds[];
     body_batched_r<RPW, InterpRun<true, EXT>>(d, K, lds);
