@@ -790,9 +790,11 @@ __device__ __forceinline__ void wave_lds_sync()
 // reference's funcs[] program builds a row before storing it
 // (hines_solve.c:246-273), so every parent sees the same sequence of
 // roundings; backward, roots solve and children follow their parents
-// (:275-295) -- with a barrier between levels.  CHAN = false is the plain
-// drop-in's elimination (rhs/diag read from results, one step).  STAGE copies
-// the pack's level schedule into LDS once per launch.
+// (:275-295) -- with a barrier between levels wider than one wave; the
+// narrower levels run in wave 0 alone.  CHAN = false is the plain drop-in's
+// elimination (rhs/diag read from results, one step).  STAGE copies the
+// pack's level schedule into LDS once per launch; without it each level's
+// entries are loaded from global memory one level ahead.
 template <class RUN, bool CHAN, bool STAGE>
 __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* lds)
 {
