@@ -251,3 +251,20 @@ def test_group_offsets_refusals_and_compatibility():
     (comp, opcode, opidx, why), = e.value.refusals
     assert why == "pending synaptic event" and comp == 24 + syn.hnum[1]
     assert opidx == a4.ops.size - 1 + int(syn.syn_op[0])
+
+
+@pytest.mark.parametrize("op", [3, 4, 5], ids=["FASTSIB", "COPY", "SIBARRAY"])
+@pytest.mark.parametrize("where", ["forward", "backward"])
+def test_refuses_symcompartment_elimination_ops(op, where):
+    """The symcompartment elimination ops of funcs[] (FASTSIBARRAY_ELIM,
+    COPY_ARRAY, SIBARRAY_ELIM; hines_solve.c:262-268, :289-291) are refused
+    with their reason: the solver keeps its CPU path."""
+    m = random_tree_model(2, 4, seed=3)
+    f = np.array(m.funcs, dtype=np.int32)
+    i = 0 if where == "forward" else int(np.nonzero(f == 7)[0][0]) + 1
+    f[i] = op
+    m.funcs = f
+    with pytest.raises(D.Refused) as e:
+        D.HipSolver(m, host_only=True)
+    refs = e.value.refusals
+    assert refs and all(r[3] == "funcs (sibling ops)" and r[1] == op for r in refs)
