@@ -192,6 +192,33 @@ def measure_traffic(a, K, info, kernel):
     return t, None
 
 
+def launch_plan(steps, K):
+    """The kernel launches ``hsh_step(h, steps, HSH_BATCHED)`` issues at K
+    steps per launch: ``steps // K`` launches of K steps (replayed from
+    captured graphs of up to 64 launches) and one direct launch of the
+    remaining ``steps % K`` steps."""
+    if K < 1:
+        raise ValueError("K must be at least 1")
+    full, rem = divmod(int(steps), int(K))
+    launches = full + (1 if rem else 0)
+    return {"batch_k": int(K), "full_launches": full, "remainder_steps": rem, "kernel_launches": launches,
+            "steps_per_launch": steps / launches if launches else 0.0,
+            # the launch shape the traffic passes repeat: a full launch when
+            # there is one, else the single short launch
+            "probe_k": int(K) if full > 0 else rem,
+            "graph_replayed": full > 0}
+
+
+def roofline_fields(bytes_step, steps, dev_ms, lp):
+    """Achieved algorithmic GB/s, its fraction of the HBM peak and the mean
+    launch duration, from the device time ``dev_ms`` of ``steps`` batched
+    steps issued as ``lp`` (launch_plan) describes."""
+    achieved = bytes_step * steps / (dev_ms * 1e-3) / 1e9
+    return {"achieved": achieved, "frac": achieved / HBM_PEAK_GBS,
+            "launch_us": dev_ms * 1e3 / max(1, lp["kernel_launches"]),
+            "steps_per_launch": lp["steps_per_launch"], "device_us_per_step": dev_ms * 1e3 / steps}
+
+
 def batched_kernel_name(info):
     """Name of the kernel a single-class batched launch runs."""
     jit = bool(info["jit"])
@@ -551,7 +578,7 @@ def main():
     wall_b = time.perf_counter() - t0
     dev_b_ms = gb.last_time(D.T_TOTAL)
     gb.set_timing(False)
-    launches_b = -(-a.steps // K)
+    lp = launch_plan(a.steps, K)
     t_e2e_total = time.perf_counter() - t_e2e
     print(f"[bench] batched K={K}: {wall_b*1e3:.1f} ms wall, {dev_b_ms:.1f} ms device, "
           f"{r['launches']} graph/direct launches", file=sys.stderr, flush=True)
@@ -596,8 +623,9 @@ def main():
     bytes_step = info["bytes_per_step"]
     value = a.steps * n / wall_b
     plain_value = a.steps * n / wall_p
-    launch_s = dev_b_ms * 1e-3 / launches_b
-    achieved = bytes_step * K / launch_s / 1e9
+    # algorithmic bytes of every step the timed launches ran over their
+    # device time (HIP events on the handle's stream around the timed call)
+    rf = roofline_fields(bytes_step, a.steps, dev_b_ms, lp)
     out = {
         "metric": "compartment-steps/s + %HBM peak, 50k x 16 Hines trees, plain vs batched @1e-7 V",
         "value": value,
@@ -617,7 +645,10 @@ def main():
                    "batch_k": K, "batch_waves": info["batch_waves"], "tiles": info["ntiles"],
                    "packs": info["npacks"], "autotuned": bool(info["autotuned"]),
                    "autotune_s": info["autotune_s"],
-                   "path": "batched persistent kernel in a captured hipGraph",
+                   "path": ("batched persistent kernel, K steps per launch, replayed from a captured hipGraph"
+                            if lp["graph_replayed"] else
+                            "batched persistent kernel, one direct launch of all timed steps (fewer than K)"),
+                   "launches": lp,
                    "specialised_kernels": bool(info["jit"]), "jit_compile_s": info["jit_compile_s"]},
         "plain": {"value": plain_value, "ms_per_step": wall_p * 1e3 / a.steps,
                   "channel_kernel_us": ch_ms * 1e3, "elim_kernel_us": el_ms * 1e3,
@@ -625,10 +656,11 @@ def main():
         "one_step_per_call": percall,
         "roofline": {"bound": "hbm",
                      "kernel": batched_kernel_name(info),
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "achieved": rf["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": rf["frac"], "traffic": None,
                      "algorithmic_bytes_per_compartment_step": bytes_step / n,
-                     "launch_us": launch_s * 1e6, "steps_per_launch": K},
+                     "launch_us": rf["launch_us"], "steps_per_launch": rf["steps_per_launch"],
+                     "device_us_per_step": rf["device_us_per_step"]},
         "end_to_end_s": {"build_create_upload": t_ready, "total_batched_incl_warmup": t_e2e_total},
     }
 
@@ -636,17 +668,18 @@ def main():
     gb.close()
     gp.close()
     if not a.no_traffic:
-        t, why = measure_traffic(a, K, info, out["roofline"]["kernel"])
+        Kp = lp["probe_k"]
+        t, why = measure_traffic(a, Kp, info, out["roofline"]["kernel"])
         if t is None:
             print(f"[bench] traffic not measured: {why}", file=sys.stderr, flush=True)
             out["roofline"]["traffic_note"] = why
         else:
+            # the child runs repeat launches of Kp steps with the same layout
             out["roofline"]["traffic"] = t["bytes_per_launch"]
-            out["roofline"]["traffic_detail"] = t
-            out["roofline"]["traffic_over_algorithmic"] = t["bytes_per_launch"] / (bytes_step * K)
-            # the measured bytes over the timed launch duration (counters of the
-            # same layout and K in the child runs)
-            out["roofline"]["traffic_gbs"] = t["bytes_per_launch"] / launch_s / 1e9
+            out["roofline"]["traffic_detail"] = dict(t, steps_per_probe_launch=Kp)
+            out["roofline"]["traffic_over_algorithmic"] = t["bytes_per_launch"] / (bytes_step * Kp)
+            # measured bytes per step over the timed device time per step
+            out["roofline"]["traffic_gbs"] = t["bytes_per_launch"] / Kp * a.steps / (dev_b_ms * 1e-3) / 1e9
 
     # ---- CPU baseline + parity (oracle)
     if not a.no_cpu:

@@ -160,10 +160,16 @@ class Trace(torch.autograd.Function):
     def forward(ctx, chip, ravals, runner):
         tr = runner.forward_arrays(chip.detach().cpu().numpy(), ravals.detach().cpu().numpy())
         ctx.runner = runner
+        ctx.generation = runner.generation
         return torch.from_numpy(tr)
 
     @staticmethod
     def backward(ctx, g):
+        # the solver keeps the checkpoints of its last forward run only
+        if ctx.runner.generation != ctx.generation:
+            raise RuntimeError("Trace.backward: the Runner ran another forward pass after this one; its "
+                               "checkpoints are gone (call backward before the next forward, or use a "
+                               "second Runner)")
         gc, gr, _ = ctx.runner.solver.adj_backward(g.detach().cpu().contiguous().numpy())
         return torch.from_numpy(gc), torch.from_numpy(gr), None
 
@@ -180,12 +186,15 @@ class Runner:
         b = pmap.base
         self.solver = D.HipSolver(b, device=device, flags=D.F_NO_AUTOTUNE,
                                   tile_max_rows=int(np.max(np.bincount(tree_of_row(b)))))
+        # forward runs so far: a backward pass must belong to the latest one
+        self.generation = 0
 
     def forward_arrays(self, chip, ravals):
         b = self.pmap.base
         s = self.solver
         s.put_state(vm=b.vm, chip=chip, ops=b.ops)
         s.set_ravals(ravals)
+        self.generation += 1
         return s.adj_forward(self.nsteps, self.ckpt, self.obs)
 
     def __call__(self, theta: torch.Tensor) -> torch.Tensor:

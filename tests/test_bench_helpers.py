@@ -71,3 +71,35 @@ def test_read_counter_needs_both_calibrations(bench, tmp_path):
 ])
 def test_batched_kernel_name(bench, info, name):
     assert bench.batched_kernel_name(info) == name
+
+
+@pytest.mark.parametrize("steps,K,full,rem,launches,probe,graph", [
+    (20, 128, 0, 20, 1, 20, False),       # fewer steps than K: one short direct launch
+    (300, 128, 2, 44, 3, 128, True),      # a remainder launch after the graph replays
+    (10000, 128, 78, 16, 79, 128, True),
+    (1280, 128, 10, 0, 10, 128, True),
+    (1, 1, 1, 0, 1, 1, True),
+])
+def test_launch_plan(bench, steps, K, full, rem, launches, probe, graph):
+    lp = bench.launch_plan(steps, K)
+    assert (lp["full_launches"], lp["remainder_steps"], lp["kernel_launches"]) == (full, rem, launches)
+    assert lp["probe_k"] == probe and lp["graph_replayed"] == graph
+    assert lp["full_launches"] * K + lp["remainder_steps"] == steps
+    assert lp["steps_per_launch"] == pytest.approx(steps / launches)
+
+
+def test_roofline_from_timed_steps(bench):
+    """The driver's short run (20 steps at K = 128: one 20-step launch of
+    256.4 us) must give the fraction its own device time implies, not one
+    scaled by K."""
+    bytes_step = 109.25 * 800000
+    rf = bench.roofline_fields(bytes_step, 20, 0.2564, bench.launch_plan(20, 128))
+    # 87.4 MB per step, 12.82 us per step: 6.82 TB/s
+    assert rf["frac"] == pytest.approx(0.852, abs=1e-3)
+    assert rf["launch_us"] == pytest.approx(256.4) and rf["steps_per_launch"] == 20
+    # a long run: 79 launches (78 of 128 steps, one of 16), 12 us per step
+    rf = bench.roofline_fields(bytes_step, 10000, 120.0, bench.launch_plan(10000, 128))
+    assert rf["frac"] == pytest.approx(109.25 * 800000 / 12e-6 / 8e12)
+    assert rf["launch_us"] == pytest.approx(120e3 / 79)
+    with pytest.raises(ValueError):
+        bench.launch_plan(10, 0)
