@@ -6,6 +6,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <unordered_map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -89,13 +90,27 @@ struct hsh_handle {
     // model-constant device buffers
     int32_t* d_prog = nullptr;
     double *d_tab = nullptr, *d_xvals = nullptr, *d_stab = nullptr;
-    // synaptic connections (hsh_set_synapses); rings are indexed by synchan id
-    double* d_ring = nullptr;
-    int32_t *d_out_off = nullptr, *d_out_dst = nullptr, *d_out_dly = nullptr;
-    double* d_out_w = nullptr;
-    int ring_len = 0;                  // 0: no connections
-    int max_k = 1 << 30;               // batched steps per launch allowed (min delay)
+    // synchans' pending event lists (hsh_set_synapses, hsh_post_events),
+    // indexed by synchan id (plan.syn_ops) and spike source id
+    // (plan.spike_ops)
+    bool ev_on = false;
+    EvCtl* d_evctl = nullptr;
+    int32_t *d_syn_op = nullptr, *d_in_off = nullptr, *d_in_src = nullptr, *d_in_srcop = nullptr,
+            *d_in_T = nullptr, *d_lst_off = nullptr, *d_lst_cap = nullptr, *d_lst_n = nullptr, *d_lst_t = nullptr,
+            *d_ipos = nullptr, *d_sflag = nullptr, *d_post_sid = nullptr, *d_post_T = nullptr,
+            *d_post_after = nullptr;
+    double *d_in_wdt = nullptr, *d_lst_w = nullptr, *d_tle = nullptr, *d_post_wdt = nullptr;
+    SpikeSrc* d_spksrc = nullptr;
+    size_t lst_total = 0;              // list nodes allocated
+    int post_cap = 0;                  // posted events the device buffers hold
+    struct Post {
+        int32_t sid, T, after, seq;
+        double wdt;
+    };
+    std::vector<Post> posts;           // posted since the last step, applied before the next
+    int max_k = 1 << 30;               // batched steps per launch allowed (shortest connection - 1)
     int nconn = 0;
+    double t_next = 0.0;               // SimulationTime() of the next step
     long long tab_max = 0;             // largest start offset of a register-based 2-D lookup
     int32_t *d_spk = nullptr, *d_nspk = nullptr;
     double* d_giv = nullptr;           // givals output (models created with givals)
@@ -305,17 +320,32 @@ void free_layout(hsh_handle* h)
     h->d_kidx = nullptr;
 }
 
+void free_events(hsh_handle* h)
+{
+    void* ptrs[] = {h->d_evctl, h->d_syn_op, h->d_in_off, h->d_in_src, h->d_in_srcop, h->d_in_T, h->d_lst_off,
+                    h->d_lst_cap, h->d_lst_n, h->d_lst_t, h->d_ipos, h->d_sflag, h->d_post_sid, h->d_post_T,
+                    h->d_post_after, h->d_in_wdt, h->d_lst_w, h->d_tle, h->d_post_wdt, h->d_spksrc};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    h->d_evctl = nullptr;
+    h->d_syn_op = h->d_in_off = h->d_in_src = h->d_in_srcop = h->d_in_T = h->d_lst_off = h->d_lst_cap = nullptr;
+    h->d_lst_n = h->d_lst_t = h->d_ipos = h->d_sflag = h->d_post_sid = h->d_post_T = h->d_post_after = nullptr;
+    h->d_in_wdt = h->d_lst_w = h->d_tle = h->d_post_wdt = nullptr;
+    h->d_spksrc = nullptr;
+    h->ev_on = false;
+    h->lst_total = 0;
+    h->post_cap = 0;
+}
+
 void free_device(hsh_handle* h)
 {
     free_layout(h);
-    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_stab, h->d_spk, h->d_nspk, h->d_ring, h->d_out_off,
-                    h->d_out_dst, h->d_out_dly, h->d_out_w, h->d_giv};
+    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_stab, h->d_spk, h->d_nspk, h->d_giv};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    free_events(h);
     h->d_prog = nullptr;
     h->d_tab = h->d_xvals = h->d_stab = nullptr;
-    h->d_ring = h->d_out_w = nullptr;
-    h->d_out_off = h->d_out_dst = h->d_out_dly = nullptr;
     h->d_spk = h->d_nspk = nullptr;
     h->d_giv = nullptr;
     jit_free(h->jit);
@@ -412,13 +442,22 @@ void set_devmodel(hsh_handle* h)
     d.btcb = h->d_btcb;
     d.npacks = (int32_t)P.packs.size();
     d.stab = h->d_stab;
-    d.ring = h->d_ring;
-    d.out_off = h->d_out_off;
-    d.out_dst = h->d_out_dst;
-    d.out_dly = h->d_out_dly;
-    d.out_w = h->d_out_w;
-    d.ring_mask = h->ring_len > 0 ? h->ring_len - 1 : 0;
-    d.nsrc = h->ring_len > 0 ? (int32_t)P.spike_ops.size() : 0;
+    d.ev = h->ev_on ? 1 : 0;
+    d.sflag = h->d_sflag;
+    d.in_off = h->d_in_off;
+    d.in_src = h->d_in_src;
+    d.in_srcop = h->d_in_srcop;
+    d.in_T = h->d_in_T;
+    d.in_wdt = h->d_in_wdt;
+    d.syn_op = h->d_syn_op;
+    d.lst_off = h->d_lst_off;
+    d.lst_cap = h->d_lst_cap;
+    d.lst_n = h->d_lst_n;
+    d.lst_t = h->d_lst_t;
+    d.lst_w = h->d_lst_w;
+    d.tle = h->d_tle;
+    d.tnow = h->d_evctl ? &h->d_evctl->tnow : nullptr;
+    d.everr = h->d_evctl ? &h->d_evctl->err : nullptr;
     d.dt = h->m.dt;
     d.spk_cap = SPIKE_CAP;
     d.giv = P.giv ? h->d_giv : nullptr;
@@ -439,6 +478,279 @@ void set_devmodel(hsh_handle* h)
     d.max_slots = 0;
     for (const Template& t : P.templates) d.max_slots = std::max(d.max_slots, t.nslots);
     d.ext = programs_extended(P) ? 1 : 0;
+}
+
+// ---------------------------------------------------------------------------
+// Synchans' pending event lists (hsh_events.hip, syn2_op in hsh_device.h)
+
+// Position in the layout's int slots of each synchan's SYN2_OP counter, and
+// where the spike pass finds each spike source's values.
+void ev_positions(const hsh_handle* h, std::vector<int32_t>& ipos, std::vector<SpikeSrc>& src)
+{
+    const Plan& P = h->plan;
+    std::unordered_map<int64_t, int32_t> sid_of_cnt, cnt_of_spk, rel_of_spk;
+    for (size_t i = 0; i < P.syn_ops.size(); i++) sid_of_cnt[(int64_t)P.syn_ops[i] + 2] = (int32_t)i;
+    for (size_t i = 0; i < P.spike_ops.size(); i++) {
+        cnt_of_spk[(int64_t)P.spike_ops[i] + 1] = (int32_t)i;
+        rel_of_spk[(int64_t)P.spike_ops[i] + 2] = (int32_t)i;
+    }
+    ipos.assign(P.syn_ops.size(), -1);
+    src.assign(P.spike_ops.size(), SpikeSrc{-1, -1, -1, -1, -1, 0});
+    for (int g = 0; g < h->m.ncompts; g++) {
+        const Template& t = P.templates[P.comp_tmpl[g]];
+        const int pos = P.comp_pos[g];
+        const BlockMeta& bm = P.blocks[pos / W];
+        for (int k = 0; k < t.nislots; k++) {
+            const int64_t srcix = P.isrc[P.comp_isrc_off[g] + k];
+            if (srcix < 0) continue;
+            const int ip = bm.islot + k * W + pos % W;
+            auto a = sid_of_cnt.find(srcix);
+            if (a != sid_of_cnt.end()) ipos[a->second] = ip;
+            auto b = cnt_of_spk.find(srcix);
+            if (b != cnt_of_spk.end()) src[b->second].cpos = ip;
+            auto c = rel_of_spk.find(srcix);
+            if (c != rel_of_spk.end()) src[c->second].rpos = ip;
+        }
+    }
+    for (size_t i = 0; i < P.spike_ops.size(); i++) {
+        const int64_t ci = P.spike_thresh[i];
+        src[i].vpos = P.comp_pos[P.spike_comp[i]];
+        if (ci >= 0 && ci + 1 < (int64_t)h->c2s_off.size() && h->c2s_off[ci] < h->c2s_off[ci + 1])
+            src[i].spos = (int)h->c2s[h->c2s_off[ci]];
+        src[i].opidx = P.spike_ops[i];
+    }
+}
+
+// After a (re)layout: the positions the spike and post passes use.
+hsh_status ev_layout(hsh_handle* h)
+{
+    if (!h->ev_on) return HSH_OK;
+    std::vector<int32_t> ipos;
+    std::vector<SpikeSrc> src;
+    ev_positions(h, ipos, src);
+    for (const SpikeSrc& q : src)
+        if (q.vpos < 0 || q.spos < 0 || q.cpos < 0 || q.rpos < 0)
+            return fail(HSH_ERR_ARG, "hsh: a SPIKE_OP's values are not in the device layout");
+    for (int32_t q : ipos)
+        if (q < 0) return fail(HSH_ERR_ARG, "hsh: a SYN2_OP counter is not in the device layout");
+    HIPCHK(dput(h->d_ipos, ipos));
+    HIPCHK(dput(h->d_spksrc, src));
+    return HSH_OK;
+}
+
+// Sticky errors of the lists.
+hsh_status ev_check(hsh_handle* h)
+{
+    if (!h->ev_on) return HSH_OK;
+    int32_t err = 0;
+    HIPCHK(hipMemcpy(&err, &h->d_evctl->err, sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (err & EV_ERR_FULL)
+        return fail(HSH_ERR_LIMIT, "hsh: a synchan's pending event list ran out of room; events were lost");
+    return HSH_OK;
+}
+
+// The lists' host image: what ev_rebuild keeps across a new connection table
+// and what the autotuner saves and restores (counters live in ops[]).
+struct EvImage {
+    std::vector<int32_t> n, off, t;
+    std::vector<double> w, tle;
+    EvCtl ctl{};
+};
+
+hsh_status ev_download(hsh_handle* h, EvImage& im)
+{
+    const size_t ns = h->plan.syn_ops.size();
+    HIPCHK(hipStreamSynchronize(h->stream));
+    im.n.resize(ns);
+    im.off.resize(ns);
+    im.tle.resize(ns);
+    im.t.resize(h->lst_total);
+    im.w.resize(h->lst_total);
+    HIPCHK(hipMemcpy(im.n.data(), h->d_lst_n, ns * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(im.off.data(), h->d_lst_off, ns * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(im.tle.data(), h->d_tle, ns * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(im.t.data(), h->d_lst_t, h->lst_total * sizeof(int32_t), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(im.w.data(), h->d_lst_w, h->lst_total * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&im.ctl, h->d_evctl, sizeof(EvCtl), hipMemcpyDeviceToHost));
+    return HSH_OK;
+}
+
+// Put back an image taken with the same list offsets.
+hsh_status ev_upload(hsh_handle* h, const EvImage& im)
+{
+    const size_t ns = h->plan.syn_ops.size();
+    HIPCHK(hipMemcpy(h->d_lst_n, im.n.data(), ns * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_tle, im.tle.data(), ns * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_lst_t, im.t.data(), h->lst_total * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_lst_w, im.w.data(), h->lst_total * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(h->d_evctl, &im.ctl, sizeof(EvCtl), hipMemcpyHostToDevice));
+    return HSH_OK;
+}
+
+// (Re)build the list buffers for a connection table: in-connections of each
+// synchan sorted by the source's ops index (the order a step's EVENTs reach
+// it) then connection order; list capacities from the most events a
+// connection can hold pending (one per reload interval of its source over
+// its delay, plus two) with room for 32 more (posted events, early fires
+// shifting nothing later).  Pending events and time_last_event carry over; a
+// first call starts every list empty.  Counters stay in ops[].
+hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std::vector<int32_t>& cdst,
+                      const std::vector<int32_t>& cT, const std::vector<double>& cwdt)
+{
+    const Plan& P = h->plan;
+    const size_t ns = P.syn_ops.size(), nsrc = P.spike_ops.size();
+    const size_t n = csrc.size();
+    EvImage old;
+    const bool had = h->ev_on;
+    if (had) STCHK(ev_download(h, old));
+    STCHK(download_chip_ops(h));
+    std::vector<int32_t> order(n);
+    for (size_t i = 0; i < n; i++) order[i] = (int32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        if (cdst[a] != cdst[b]) return cdst[a] < cdst[b];
+        return P.spike_ops[csrc[a]] < P.spike_ops[csrc[b]];
+    });
+    std::vector<int32_t> in_off(ns + 1, 0), in_src(std::max<size_t>(1, n)), in_srcop(std::max<size_t>(1, n)),
+        in_T(std::max<size_t>(1, n));
+    std::vector<double> in_wdt(std::max<size_t>(1, n));
+    std::vector<int64_t> cap(ns, 32);
+    for (size_t q = 0; q < n; q++) {
+        const int32_t i = order[q];
+        in_off[cdst[i] + 1]++;
+        in_src[q] = csrc[i];
+        in_srcop[q] = P.spike_ops[csrc[i]];
+        in_T[q] = cT[i];
+        in_wdt[q] = cwdt[i];
+        const int32_t reload = std::max(1, h->ops[P.spike_ops[csrc[i]] + 2]);
+        cap[cdst[i]] += (cT[i] + 2 + reload - 1) / reload;
+    }
+    for (size_t i = 0; i < ns; i++) in_off[i + 1] += in_off[i];
+    std::vector<int32_t> lst_off(ns), lst_cap(ns), lst_n(ns, 0), syn_op(ns);
+    std::vector<double> tle(ns, 0.0);
+    int64_t total = 0;
+    for (size_t i = 0; i < ns; i++) {
+        if (had) cap[i] = std::max<int64_t>(cap[i], old.n[i] + 32);
+        lst_off[i] = (int32_t)total;
+        lst_cap[i] = (int32_t)cap[i];
+        total += cap[i];
+        if (total > ((int64_t)1 << 30))
+            return fail(HSH_ERR_LIMIT, "hsh_set_synapses: pending-list capacity too large");
+        syn_op[i] = P.syn_ops[i];
+    }
+    std::vector<int32_t> lt((size_t)std::max<int64_t>(1, total), 0);
+    std::vector<double> lw((size_t)std::max<int64_t>(1, total), 0.0);
+    EvCtl ctl{};
+    if (had) {
+        for (size_t i = 0; i < ns; i++) {
+            lst_n[i] = old.n[i];
+            tle[i] = old.tle[i];
+            for (int k = 0; k < old.n[i]; k++) {
+                lt[lst_off[i] + k] = old.t[old.off[i] + k];
+                lw[lst_off[i] + k] = old.w[old.off[i] + k];
+            }
+        }
+        ctl = old.ctl;
+    } else {
+        ctl.tnext = h->t_next;
+        ctl.tnow = h->t_next - h->m.dt;
+        ctl.tlast = ctl.tnow - h->m.dt;
+    }
+    free_events(h);
+    const size_t ntot = lt.size();
+    HIPCHK(dalloc(&h->d_evctl, 1));
+    HIPCHK(dalloc(&h->d_syn_op, ns));
+    HIPCHK(dalloc(&h->d_in_off, ns + 1));
+    HIPCHK(dalloc(&h->d_in_src, in_src.size()));
+    HIPCHK(dalloc(&h->d_in_srcop, in_srcop.size()));
+    HIPCHK(dalloc(&h->d_in_T, in_T.size()));
+    HIPCHK(dalloc(&h->d_in_wdt, in_wdt.size()));
+    HIPCHK(dalloc(&h->d_lst_off, ns));
+    HIPCHK(dalloc(&h->d_lst_cap, ns));
+    HIPCHK(dalloc(&h->d_lst_n, ns));
+    HIPCHK(dalloc(&h->d_lst_t, ntot));
+    HIPCHK(dalloc(&h->d_lst_w, ntot));
+    HIPCHK(dalloc(&h->d_tle, ns));
+    HIPCHK(dalloc(&h->d_ipos, ns));
+    HIPCHK(dalloc(&h->d_sflag, std::max<size_t>(1, nsrc)));
+    HIPCHK(dalloc(&h->d_spksrc, std::max<size_t>(1, nsrc)));
+    HIPCHK(hipMemcpy(h->d_evctl, &ctl, sizeof(EvCtl), hipMemcpyHostToDevice));
+    HIPCHK(dput(h->d_syn_op, syn_op));
+    HIPCHK(dput(h->d_in_off, in_off));
+    HIPCHK(dput(h->d_in_src, in_src));
+    HIPCHK(dput(h->d_in_srcop, in_srcop));
+    HIPCHK(dput(h->d_in_T, in_T));
+    HIPCHK(dput(h->d_in_wdt, in_wdt));
+    HIPCHK(dput(h->d_lst_off, lst_off));
+    HIPCHK(dput(h->d_lst_cap, lst_cap));
+    HIPCHK(dput(h->d_lst_n, lst_n));
+    HIPCHK(dput(h->d_lst_t, lt));
+    HIPCHK(dput(h->d_lst_w, lw));
+    HIPCHK(dput(h->d_tle, tle));
+    HIPCHK(hipMemset(h->d_sflag, 0, std::max<size_t>(1, nsrc) * sizeof(int32_t)));
+    h->lst_total = ntot;
+    h->ev_on = true;
+    STCHK(ev_layout(h));
+    set_devmodel(h);
+    return HSH_OK;
+}
+
+// Room for n posted events in the device buffers.
+hsh_status ev_post_room(hsh_handle* h, int n)
+{
+    if (n <= h->post_cap) return HSH_OK;
+    int cap = std::max(1024, h->post_cap);
+    while (cap < n) cap *= 2;
+    void* ptrs[] = {h->d_post_sid, h->d_post_T, h->d_post_after, h->d_post_wdt};
+    for (void* p : ptrs)
+        if (p) HIPCHK(hipFree(p));
+    HIPCHK(dalloc(&h->d_post_sid, (size_t)cap));
+    HIPCHK(dalloc(&h->d_post_T, (size_t)cap));
+    HIPCHK(dalloc(&h->d_post_after, (size_t)cap));
+    HIPCHK(dalloc(&h->d_post_wdt, (size_t)cap));
+    h->post_cap = cap;
+    return HSH_OK;
+}
+
+// Apply the events posted since the last step.
+hsh_status ev_flush(hsh_handle* h)
+{
+    if (!h->ev_on || h->posts.empty()) return HSH_OK;
+    std::vector<hsh_handle::Post>& ps = h->posts;
+    std::stable_sort(ps.begin(), ps.end(), [](const hsh_handle::Post& a, const hsh_handle::Post& b) {
+        return a.sid != b.sid ? a.sid < b.sid : a.seq < b.seq;
+    });
+    const int n = (int)ps.size();
+    STCHK(ev_post_room(h, n));
+    std::vector<int32_t> sid(n), T(n), after(n);
+    std::vector<double> w(n);
+    for (int i = 0; i < n; i++) {
+        sid[i] = ps[i].sid;
+        T[i] = ps[i].T;
+        after[i] = ps[i].after;
+        w[i] = ps[i].wdt;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(dput(h->d_post_sid, sid));
+    HIPCHK(dput(h->d_post_T, T));
+    HIPCHK(dput(h->d_post_after, after));
+    HIPCHK(dput(h->d_post_wdt, w));
+    HIPCHK(hipMemcpy(&h->d_evctl->npost, &n, sizeof(int32_t), hipMemcpyHostToDevice));
+    const EvPost p{h->d_post_sid, h->d_post_T, h->d_post_wdt, h->d_post_after, h->d_ipos,
+                   (int)h->plan.syn_ops.size()};
+    HIPCHK(launch_ev_post(h->dm, p, h->d_evctl, h->stream));
+    const int zero = 0;
+    HIPCHK(hipMemcpyAsync(&h->d_evctl->npost, &zero, sizeof(int32_t), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    ps.clear();
+    return HSH_OK;
+}
+
+// The step's spike pass (lists on): every SPIKE_OP before the step's kernels.
+hsh_status ev_spikes(hsh_handle* h)
+{
+    if (!h->ev_on) return HSH_OK;
+    HIPCHK(launch_ev_spikes(h->dm, h->d_spksrc, (int)h->plan.spike_ops.size(), h->d_evctl, h->d_sflag, h->stream));
+    return HSH_OK;
 }
 
 // Allocate and fill the layout-dependent buffers from the plan and the given
@@ -540,7 +852,7 @@ hsh_status alloc_layout(hsh_handle* h, const double* vm, const double* results)
     }
     set_devmodel(h);
     compute_classes(h);
-    return HSH_OK;
+    return ev_layout(h);
 }
 
 // Move the device state to a new layout.
@@ -590,9 +902,10 @@ hsh_status launch_class(hsh_handle* h, const KClass& c, int K, hipStream_t s)
 }
 
 // One batched launch of K steps: the kernel of every class, in turn, on the
-// handle's stream.
+// handle's stream (K = 1 after the spike pass when the lists are on).
 hsh_status batched_launch(hsh_handle* h, int K)
 {
+    STCHK(ev_spikes(h));
     for (const KClass& c : h->classes) STCHK(launch_class(h, c, K, h->stream));
     return HSH_OK;
 }
@@ -641,6 +954,7 @@ hsh_status plain_step(hsh_handle* h, bool channels_only)
         }
         HIPCHK(hipEventRecord(h->evs[h->nev_used], h->stream));
     }
+    STCHK(ev_spikes(h));
     HIPCHK(launch_channel(h->dm, &h->jit, h->stream));
     if (h->timing) HIPCHK(hipEventRecord(h->evs[h->nev_used + 1], h->stream));
     if (channels_only) {
@@ -692,10 +1006,10 @@ hsh_status autotune(hsh_handle* h)
     HIPCHK(hipMemcpy(&nspk0, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
     std::vector<double> giv0;
     STCHK(save_givals(h, giv0));
-    // pending synaptic events: trials consume and add them
-    std::vector<double> ring0((size_t)h->ring_len * h->plan.syn_ops.size());
-    if (!ring0.empty())
-        HIPCHK(hipMemcpy(ring0.data(), h->d_ring, ring0.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // pending synaptic events: trials fire and add them
+    STCHK(ev_flush(h));
+    EvImage ev0;
+    if (h->ev_on) STCHK(ev_download(h, ev0));
     const hsh_options& o = h->opt;
     const Plan& P = h->plan;
     const int K0 = std::min(o.batch_k > 0 ? o.batch_k : 32, h->max_k);
@@ -782,8 +1096,7 @@ hsh_status autotune(hsh_handle* h)
     STCHK(alloc_layout(h, vm0.data(), res0.data()));
     HIPCHK(hipMemcpy(h->d_nspk, &nspk0, sizeof(int32_t), hipMemcpyHostToDevice));
     STCHK(load_givals(h, giv0));
-    if (!ring0.empty())
-        HIPCHK(hipMemcpy(h->d_ring, ring0.data(), ring0.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (h->ev_on) STCHK(ev_upload(h, ev0));
     h->tuned = true;
     h->autotune_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     return HSH_OK;
@@ -1273,6 +1586,7 @@ hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* o
     if (mode == HSH_BATCHED && !h->tuned && !(h->opt.flags & HSH_F_NO_AUTOTUNE) && nsteps > 0)
         STCHK(autotune(h));
     hsh_step_out o{};
+    STCHK(ev_flush(h));
     if (h->timing) HIPCHK(hipEventRecord(h->ev_a, h->stream));
     h->nev_used = 0;
     if (mode == HSH_PLAIN || mode == HSH_CHANNELS || h->plain_loop) {
@@ -1288,6 +1602,7 @@ hsh_status hsh_step(hsh_handle* h, int32_t nsteps, int32_t mode, hsh_step_out* o
     if (h->timing) HIPCHK(hipEventRecord(h->ev_b, h->stream));
     h->total_valid = h->timing;
     h->host_step += nsteps;
+    for (int s = 0; s < nsteps; s++) h->t_next += h->m.dt;
     if (out) *out = o;
     return HSH_OK;
 }
@@ -1297,7 +1612,7 @@ hsh_status hsh_sync(hsh_handle* h)
     if (!h) return fail(HSH_ERR_ARG, "hsh_sync: null handle");
     if (h->refused || h->host_only) return HSH_OK;
     HIPCHK(hipStreamSynchronize(h->stream));
-    return HSH_OK;
+    return ev_check(h);
 }
 
 hsh_status hsh_get_state(hsh_handle* h, double* vm, double* chip, double* results, int32_t* ops)
@@ -1305,7 +1620,9 @@ hsh_status hsh_get_state(hsh_handle* h, double* vm, double* chip, double* result
     if (!h) return fail(HSH_ERR_ARG, "hsh_get_state: null handle");
     if (h->host_only) return fail(HSH_ERR_STATE, "hsh_get_state: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_get_state: handle refused the model");
+    STCHK(ev_flush(h));
     HIPCHK(hipStreamSynchronize(h->stream));
+    STCHK(ev_check(h));
     if (vm || results) STCHK(download_vm_results(h, vm, results));
     if (chip || ops) {
         STCHK(download_chip_ops(h));
@@ -1320,6 +1637,7 @@ hsh_status hsh_put_state(hsh_handle* h, const double* vm, const double* chip, co
     if (!h) return fail(HSH_ERR_ARG, "hsh_put_state: null handle");
     if (h->host_only) return fail(HSH_ERR_STATE, "hsh_put_state: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_put_state: handle refused the model");
+    STCHK(ev_flush(h));                 // posted events apply to the state being replaced
     HIPCHK(hipStreamSynchronize(h->stream));
     if (chip || ops) {
         // refresh the host copies with the device's mutable state first, so
@@ -1394,6 +1712,7 @@ hsh_status hsh_drain_spikes(hsh_handle* h, int32_t* opidx, int32_t* step, int32_
     if (h->host_only) return fail(HSH_ERR_STATE, "hsh_drain_spikes: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_drain_spikes: handle refused the model");
     HIPCHK(hipStreamSynchronize(h->stream));
+    STCHK(ev_check(h));
     int32_t cnt = 0;
     HIPCHK(hipMemcpy(&cnt, h->d_nspk, sizeof(int32_t), hipMemcpyDeviceToHost));
     if (cnt > SPIKE_CAP) return fail(HSH_ERR_LIMIT, "hsh_drain_spikes: device spike buffer overflowed; drain more often");
@@ -1500,75 +1819,94 @@ hsh_status hsh_set_synapses(hsh_handle* h, int32_t n, const int32_t* src_op, con
     for (size_t i = 0; i < P.spike_ops.size(); i++) src_id[P.spike_ops[i]] = (int32_t)i;
     for (size_t i = 0; i < P.syn_ops.size(); i++) dst_id[P.syn_ops[i]] = (int32_t)i;
     const double dt = h->m.dt;
-    std::vector<std::vector<int32_t>> by_src(P.spike_ops.size());
-    std::vector<int32_t> cdst(n), cdly(n);
-    std::vector<double> cw(n);
-    int min_d = 1 << 30, max_d = 0;
+    std::vector<int32_t> csrc(n), cdst(n), cT(n);
+    std::vector<double> cwdt(n);
+    int min_T = 1 << 30;
     for (int32_t i = 0; i < n; i++) {
         auto s = src_id.find(src_op[i]);
         auto t = dst_id.find(dst_op[i]);
         if (s == src_id.end()) return fail(HSH_ERR_ARG, "hsh_set_synapses: src_op " + std::to_string(src_op[i]) + " is not a SPIKE_OP");
         if (t == dst_id.end()) return fail(HSH_ERR_ARG, "hsh_set_synapses: dst_op " + std::to_string(dst_op[i]) + " is not a SYN2_OP");
-        // weight and delay are float fields in GENESIS
-        const double w = (double)(float)weight[i];
+        // weight and delay are float fields of the synapse
+        // (newconn_struct.h:158-159); an hsolve spike is sent at the current
+        // time, so it hits (int)(delay/dt + 0.9) steps later
+        // (newconn/synchan.c:193-194)
         const double dl = (double)(float)delay[i];
         if (!(dl >= 0.0) || dl / dt > 1e6) return fail(HSH_ERR_ARG, "hsh_set_synapses: delay out of range");
-        // newconn/synchan.c:193-194; an op earlier in ops[] than the spike
-        // has already run this step, so the event reaches it one step later
-        const int steps = (int)(dl / dt + 0.9) + (dst_op[i] < src_op[i] ? 1 : 0);
-        if (steps < 1) return fail(HSH_ERR_ARG, "hsh_set_synapses: a connection shorter than one step");
+        const int T = (int)((dl - 0.0) / dt + 0.9);
+        if (T < 1) return fail(HSH_ERR_ARG, "hsh_set_synapses: a connection shorter than one step");
+        csrc[i] = s->second;
         cdst[i] = t->second;
-        cdly[i] = steps;
-        cw[i] = w / dt;
-        min_d = std::min(min_d, steps);
-        max_d = std::max(max_d, steps);
-        by_src[s->second].push_back(i);
+        cT[i] = T;
+        cwdt[i] = (double)(float)weight[i] / dt;
+        min_T = std::min(min_T, T);
     }
+    // An EVENT can hit as early as the next step (the head's re-timing
+    // truncates, so its time field drifts upward and later events fire
+    // early), so with lists on every step is its own launch after its spike
+    // pass.
+    (void)min_T;
+    const int max_k = 1;
     if (h->host_only) {
-        h->max_k = n > 0 ? min_d : (1 << 30);
+        h->max_k = max_k;
         h->nconn = n;
         return HSH_OK;
     }
     HIPCHK(hipStreamSynchronize(h->stream));
-    void* ptrs[] = {h->d_ring, h->d_out_off, h->d_out_dst, h->d_out_dly, h->d_out_w};
-    for (void* p : ptrs)
-        if (p) HIPCHK(hipFree(p));
-    h->d_ring = h->d_out_w = nullptr;
-    h->d_out_off = h->d_out_dst = h->d_out_dly = nullptr;
-    h->ring_len = 0;
-    h->max_k = 1 << 30;
+    STCHK(ev_flush(h));
+    STCHK(ev_rebuild(h, csrc, cdst, cT, cwdt));
+    h->max_k = max_k;
     h->nconn = n;
-    if (n > 0) {
-        // arrivals of one launch (at most min_d steps) stay ahead of the
-        // slots that launch reads
-        int R = 1;
-        while (R < max_d + min_d + 1) R <<= 1;
-        std::vector<int32_t> off(P.spike_ops.size() + 1, 0), odst, odly;
-        std::vector<double> ow;
-        for (size_t sid = 0; sid < by_src.size(); sid++) {
-            for (int32_t i : by_src[sid]) {
-                odst.push_back(cdst[i]);
-                odly.push_back(cdly[i]);
-                ow.push_back(cw[i]);
-            }
-            off[sid + 1] = (int32_t)odst.size();
-        }
-        const size_t nring = (size_t)R * std::max<size_t>(1, P.syn_ops.size());
-        HIPCHK(dalloc(&h->d_ring, nring));
-        HIPCHK(hipMemset(h->d_ring, 0, nring * sizeof(double)));
-        HIPCHK(dalloc(&h->d_out_off, off.size()));
-        HIPCHK(dalloc(&h->d_out_dst, odst.size()));
-        HIPCHK(dalloc(&h->d_out_dly, odly.size()));
-        HIPCHK(dalloc(&h->d_out_w, ow.size()));
-        HIPCHK(dput(h->d_out_off, off));
-        HIPCHK(dput(h->d_out_dst, odst));
-        HIPCHK(dput(h->d_out_dly, odly));
-        HIPCHK(dput(h->d_out_w, ow));
-        h->ring_len = R;
-        h->max_k = min_d;
-    }
     free_graphs(h);
     set_devmodel(h);
+    return HSH_OK;
+}
+
+hsh_status hsh_post_events(hsh_handle* h, int32_t n, const int32_t* syn_op, const int32_t* steps,
+                           const double* weight, int32_t after)
+{
+    if (!h) return fail(HSH_ERR_ARG, "hsh_post_events: null handle");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_post_events: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_post_events: handle refused the model");
+    if (n < 0 || (n > 0 && (!syn_op || !steps || !weight)) || (after != 0 && after != 1))
+        return fail(HSH_ERR_ARG, "hsh_post_events: bad arguments");
+    if (after && h->host_step == 0)
+        return fail(HSH_ERR_ARG, "hsh_post_events: no step has run to post after");
+    const Plan& P = h->plan;
+    std::map<int32_t, int32_t> dst_id;
+    for (size_t i = 0; i < P.syn_ops.size(); i++) dst_id[P.syn_ops[i]] = (int32_t)i;
+    std::vector<hsh_handle::Post> add;
+    for (int32_t i = 0; i < n; i++) {
+        auto t = dst_id.find(syn_op[i]);
+        if (t == dst_id.end())
+            return fail(HSH_ERR_ARG, "hsh_post_events: " + std::to_string(syn_op[i]) + " is not a SYN2_OP");
+        if (steps[i] < 0) return fail(HSH_ERR_ARG, "hsh_post_events: negative step count");
+        add.push_back(hsh_handle::Post{t->second, steps[i], after, (int32_t)(h->posts.size() + add.size()),
+                                       (double)(float)weight[i] / h->m.dt});
+    }
+    if (n == 0) return HSH_OK;
+    if (!h->ev_on) {
+        // lists for every synchan, no connections
+        HIPCHK(hipStreamSynchronize(h->stream));
+        STCHK(ev_rebuild(h, {}, {}, {}, {}));
+        h->max_k = 1;
+        free_graphs(h);
+        set_devmodel(h);
+    }
+    h->posts.insert(h->posts.end(), add.begin(), add.end());
+    return HSH_OK;
+}
+
+hsh_status hsh_set_time(hsh_handle* h, double t)
+{
+    if (!h) return fail(HSH_ERR_ARG, "hsh_set_time: null handle");
+    if (!(t == t)) return fail(HSH_ERR_ARG, "hsh_set_time: not a number");
+    h->t_next = t;
+    if (h->ev_on) {
+        STCHK(ev_flush(h));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipMemcpy(&h->d_evctl->tnext, &t, sizeof(double), hipMemcpyHostToDevice));
+    }
     return HSH_OK;
 }
 

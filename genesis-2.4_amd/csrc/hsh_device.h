@@ -85,14 +85,14 @@ __device__ __forceinline__ double apply_power(double Gk, double X, int k)
 }
 
 // SPIKE_OP (hines_chip.c:266-276): counter-1 every step; when Vm > thresh and
-// the counter is spent, record an event, deliver it to the spike's synaptic
-// connections and reload the counter.  Delivery adds weight/dt to the target
-// synchan's activation of the arrival step (its ring slot); every connection
-// is at least one step long, so no synchan reads a slot another wave is
-// adding to in the same step.
+// the counter is spent, record an event and reload the counter.  With the
+// synchans' lists on, the spike pass of the step has already run every
+// SPIKE_OP (it reads only Vm, thresh and the counter), so here it does
+// nothing.
 __device__ __forceinline__ void spike_op(const DevModel& d, int* __restrict__ Ib, int is, int lane,
                                          double Vm, double thresh, bool active, int step)
 {
+    if (d.ev) return;
     int cnt = Ib[is * W + lane] - 1;
     if (Vm > thresh) {
         if (cnt <= 0) {
@@ -102,13 +102,6 @@ __device__ __forceinline__ void spike_op(const DevModel& d, int* __restrict__ Ib
                     d.spk[2 * q] = Ib[(is + 2) * W + lane];
                     d.spk[2 * q + 1] = step;
                 }
-                const int src = Ib[(is + 3) * W + lane];
-                if (d.ring_mask != 0 && src >= 0 && src < d.nsrc) {
-                    const long long rl = (long long)d.ring_mask + 1;
-                    for (int j = d.out_off[src]; j < d.out_off[src + 1]; j++)
-                        atomicAdd(d.ring + (long long)d.out_dst[j] * rl + ((step + d.out_dly[j]) & d.ring_mask),
-                                  d.out_w[j]);
-                }
             }
             cnt = Ib[(is + 1) * W + lane];
         }
@@ -116,25 +109,106 @@ __device__ __forceinline__ void spike_op(const DevModel& d, int* __restrict__ Ib
     Ib[is * W + lane] = cnt;
 }
 
-// SYN2_OP (hines_chip.c:511-550): X *= xconst2; an event arriving this step
-// adds dt * activation * stablist[k+3] (h_dosynchan, hines_conn.c:60-89);
-// the op counter decrements; Y = X*yconst1 + Y*yconst2.  Returns Y (the
+// EVENT of an hsolved synchan at simulation time `now`
+// (newconn/synchan.c:232-300): a list that is not empty first re-times its
+// head by the whole steps (int)((now - tle)/dt) since time_last_event; the
+// event goes before the first event whose time field is >= its own (time
+// fields relative to the event before); a new head sets the SYN2_OP counter
+// (HSEVENT, hines.c:352-357); time_last_event = now.  Returns false when the
+// list is full.
+__device__ __forceinline__ bool ev_insert(int* __restrict__ lt, double* __restrict__ lw, int& n, int cap,
+                                          int& cnt, double& tle, int T, double wdt, double now, double dt)
+{
+    if (n >= cap) return false;
+    if (n == 0) {
+        lt[0] = T;
+        lw[0] = wdt;
+        n = 1;
+        cnt = T;
+    } else {
+        lt[0] -= (int)((now - tle) / dt);
+        int i = 0;
+        for (; i < n; i++) {
+            if (T <= lt[i]) break;
+            T -= lt[i];
+        }
+        if (i < n) {
+            lt[i] -= T;
+            for (int j = n; j > i; j--) {
+                lt[j] = lt[j - 1];
+                lw[j] = lw[j - 1];
+            }
+        }
+        lt[i] = T;
+        lw[i] = wdt;
+        n++;
+        if (i == 0) cnt = T;
+    }
+    tle = now;
+    return true;
+}
+
+// SYN2_OP (hines_chip.c:511-550): X *= xconst2; if the counter is 0 the
+// synchan fires (h_dosynchan, hines_conn.c:60-89; HPROCESS,
+// newconn/synchan.c:305-358): the events at the front whose time field is 0
+// (the head's forced to 0) are popped, weight/dt summed into the float
+// activation, X += dt*activation*stablist[k+3], and the counter takes the new
+// head's time or -1; the counter decrements; Y = X*yconst1 + Y*yconst2.
+// With the lists on, the EVENTs this step's spikes send to the synchan run
+// around it in ops[] order: those of SPIKE_OPs before the SYN2_OP first (they
+// may re-arm the counter), the rest after the decrement.  Returns Y (the
 // caller multiplies it into Gk and adds the current).  Padding lanes repeat
-// lane 0's synchan and must not take its events.
+// lane 0's synchan and must not touch its list.
 __device__ __forceinline__ double syn2_op(const DevModel& d, double* __restrict__ Sb, int* __restrict__ Ib, int c,
                                           int k, int is, int lane, double X0, double Y0, bool active, int step)
 {
     double X = X0 * d.stab[k];
-    if (d.ring_mask != 0 && active) {
+    if (d.ev && active) {
         const int sid = Ib[(is + 1) * W + lane];
-        double* slot = d.ring + (long long)sid * ((long long)d.ring_mask + 1) + (step & d.ring_mask);
-        const double act = *slot;
-        if (act != 0.0) {
-            X += d.dt * act * d.stab[k + 3];
-            *slot = 0.0;
+        const int synop = d.syn_op[sid];
+        const int j0 = d.in_off[sid], j1 = d.in_off[sid + 1];
+        const int off = d.lst_off[sid], cap = d.lst_cap[sid];
+        int* __restrict__ lt = d.lst_t + off;
+        double* __restrict__ lw = d.lst_w + off;
+        int n = d.lst_n[sid];
+        int cnt = Ib[is * W + lane];
+        double tle = d.tle[sid];
+        const double now = *d.tnow;
+        bool full = false;
+        int j = j0;
+        for (; j < j1 && d.in_srcop[j] < synop; j++)
+            if (d.sflag[d.in_src[j]])
+                full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[j], d.in_wdt[j], now, d.dt);
+        if (cnt == 0) {
+            float act = 0.0f;
+            if (n > 0) {
+                lt[0] = 0;
+                int i = 0;
+                while (i < n && lt[i] == 0) {
+                    act = (float)((double)act + lw[i]);
+                    i++;
+                }
+                for (int q = i; q < n; q++) {
+                    lt[q - i] = lt[q];
+                    lw[q - i] = lw[q];
+                }
+                n -= i;
+                if (n > 0) tle = now;
+            }
+            X += d.dt * (double)act * d.stab[k + 3];
+            cnt = n > 0 ? lt[0] : -1;
         }
+        cnt -= 1;
+        for (; j < j1; j++)
+            if (d.sflag[d.in_src[j]])
+                full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[j], d.in_wdt[j], now, d.dt);
+        d.lst_n[sid] = n;
+        d.tle[sid] = tle;
+        Ib[is * W + lane] = cnt;
+        if (full) atomicOr(d.everr, 2);
+    } else {
+        Ib[is * W + lane] -= 1;
     }
-    Ib[is * W + lane] -= 1;
     Sb[c * W + lane] = X;
     const double Y = X * d.stab[k + 1] + Y0 * d.stab[k + 2];
     Sb[(c + 1) * W + lane] = Y;

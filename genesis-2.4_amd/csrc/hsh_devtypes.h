@@ -120,13 +120,26 @@ struct DevModel {
     int npacks;
     // synaptic events
     const double* stab;   // stablist: xconst2, yconst1, yconst2, xconst1(/dt) per type
-    double* ring;         // per synchan: activation of steps (step & ring_mask)
-    const int* out_off;   // per spike source: first outgoing connection
-    const int* out_dst;   // connection: target synchan id
-    const int* out_dly;   // connection: delay in steps
-    const double* out_w;  // connection: weight / dt
-    int ring_mask;        // ring length - 1 (0: no connections)
-    int nsrc;             // spike sources with a connection table
+    // synchans' pending event lists (set when the model has them): the
+    // SYN2_OP of each synchan keeps its own list, in the order of
+    // newconn/synchan.c:161-358, from the spikes the spike pass of the step
+    // decided before the step ran
+    int ev;               // 1: lists on; SPIKE_OPs run in the spike pass, not in programs
+    const int* sflag;     // per spike source: 1 if it spiked this step
+    const int* in_off;    // per synchan: in-connections, by source ops index then connection order
+    const int* in_src;    // spike source id
+    const int* in_srcop;  // ops index of the source's SPIKE_OP
+    const int* in_T;      // steps to the hit, (int)(delay/dt + 0.9)
+    const double* in_wdt; // (float)weight / dt
+    const int* syn_op;    // per synchan: ops index of its SYN2_OP
+    const int* lst_off;   // per synchan: first node of its list
+    const int* lst_cap;   // nodes it may hold
+    int* lst_n;           // pending events
+    int* lst_t;           // node time fields
+    double* lst_w;        // node weight / dt
+    double* tle;          // per synchan: time_last_event
+    const double* tnow;   // SimulationTime() of the step being run
+    int* everr;           // error bits (a full list)
     double dt;
     // concentration-indexed rows of tab
     double ymin;

@@ -1,0 +1,98 @@
+// hsh_events.hip -- the step's spike pass and the posting of outside events
+// into the synchans' pending lists.
+//
+// The reference delivers a spike while it walks ops[]: SPIKE_OP calls
+// h_dospike_event (hines_conn.c:44-56), each connected synchan's EVENT puts
+// the spike into its pending list (newconn/synchan.c:161-302), and a SYN2_OP
+// later in the same walk may already see the list changed.  A spike can hit
+// as early as the next step: every EVENT re-times the list head by the whole
+// steps elapsed since time_last_event, truncating the floating interval, so
+// the head's time field drifts upward and events queued behind it fire
+// early.  So the device decides all of a step's spikes first -- a SPIKE_OP
+// reads only Vm (the last step's solution), its threshold and its counter --
+// and every SYN2_OP then runs its synchan's EVENTs and fire in ops[] order
+// (syn2_op, hsh_device.h).
+#include "hsh_internal.h"
+#include "hsh_device.h"
+
+namespace hsh {
+namespace {
+
+// SPIKE_OP (hines_chip.c:266-276) for every spike source, and the clock:
+// SimulationTime() of this step, advanced by dt (sim/sim_step.c:112).
+__global__ __launch_bounds__(256) void k_ev_spikes(DevModel d, const SpikeSrc* __restrict__ src, int nsrc,
+                                                   EvCtl* ctl, int* __restrict__ sflag)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        const double t = ctl->tnext;
+        ctl->tlast = ctl->tnow;
+        ctl->tnow = t;
+        ctl->tnext = t + d.dt;
+    }
+    if (i >= nsrc) return;
+    const SpikeSrc s = src[i];
+    const int step = d.step[d.blocks[s.vpos / W].tile];
+    int cnt = d.I[s.cpos] - 1;
+    int fired = 0;
+    if (d.V[s.vpos] > d.S[s.spos] && cnt <= 0) {
+        const int q = atomicAdd(d.nspk, 1);
+        if (q < d.spk_cap) {
+            d.spk[2 * q] = s.opidx;
+            d.spk[2 * q + 1] = step;
+        }
+        cnt = d.I[s.rpos];
+        fired = 1;
+    }
+    d.I[s.cpos] = cnt;
+    sflag[i] = fired;
+}
+
+// Events posted from the host (hsh_post_events), one thread per synchan:
+// those timed after the last step's ops first, then those before the next
+// step's, each an EVENT at that step's SimulationTime().
+__global__ __launch_bounds__(256) void k_ev_post(DevModel d, EvPost p, EvCtl* ctl)
+{
+    const int sid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sid >= p.nsyn) return;
+    const int np = ctl->npost;
+    int lo = 0, hi = np;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (p.sid[mid] < sid) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= np || p.sid[lo] != sid) return;
+    const int off = d.lst_off[sid], cap = d.lst_cap[sid];
+    int n = d.lst_n[sid];
+    int cnt = d.I[p.ipos[sid]];
+    double tle = d.tle[sid];
+    bool full = false;
+    for (int pass = 1; pass >= 0; pass--)
+        for (int i = lo; i < np && p.sid[i] == sid; i++)
+            if (p.after[i] == pass)
+                full |= !ev_insert(d.lst_t + off, d.lst_w + off, n, cap, cnt, tle, p.T[i], p.wdt[i],
+                                   pass ? ctl->tnow : ctl->tnext, d.dt);
+    d.lst_n[sid] = n;
+    d.tle[sid] = tle;
+    d.I[p.ipos[sid]] = cnt;
+    if (full) atomicOr(&ctl->err, EV_ERR_FULL);
+}
+
+}  // namespace
+
+hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, int* sflag, hipStream_t s)
+{
+    const int n = nsrc > 0 ? nsrc : 1;
+    hipLaunchKernelGGL(k_ev_spikes, dim3((n + 255) / 256), dim3(256), 0, s, d, src, nsrc, ctl, sflag);
+    return hipGetLastError();
+}
+
+hipError_t launch_ev_post(const DevModel& d, const EvPost& p, EvCtl* ctl, hipStream_t s)
+{
+    if (p.nsyn <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ev_post, dim3((p.nsyn + 255) / 256), dim3(256), 0, s, d, p, ctl);
+    return hipGetLastError();
+}
+
+}  // namespace hsh

@@ -194,6 +194,8 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
                 isrc.push_back(-(1 + opidx));        // ops index, literal
                 isrc.push_back(-(1 + (int64_t)P.spike_ops.size()));   // spike source id, literal
                 P.spike_ops.push_back((int32_t)opidx);
+                P.spike_comp.push_back(comp);
+                P.spike_thresh.push_back(c);
                 dsrc.push_back(c);                   // threshold
                 c += 1;
                 p += 2;

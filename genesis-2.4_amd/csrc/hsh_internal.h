@@ -65,6 +65,8 @@ struct Plan {
     std::vector<int32_t> tree_hi;
     std::vector<int32_t> pred;            // previous NEWVOLT compartment (global row), -1
     std::vector<int32_t> spike_ops;       // spike source id -> ops index of its SPIKE_OP
+    std::vector<int32_t> spike_comp;      // spike source id -> its compartment (row)
+    std::vector<int64_t> spike_thresh;    // spike source id -> chip index of its threshold
     std::vector<int32_t> syn_ops;         // synchan id -> ops index of its SYN2_OP
     std::vector<int32_t> prog;            // programs of all templates
     std::vector<int32_t> tprog;           // template -> offset in prog
@@ -117,6 +119,44 @@ struct JitModule {
 std::string jit_source(const Plan& P, int ncols);
 bool jit_build(const Plan& P, int ncols, JitModule& jm, std::string& err);
 void jit_free(JitModule& jm);
+
+// Synchans' pending event lists on the device (hsh_events.hip).  Each step
+// first runs the spike pass: every SPIKE_OP of the model (it reads only Vm,
+// its threshold and its counter, so it can run before the step's other
+// ops), recording spikes and setting the sources' flags; then the step's
+// kernels run, and each SYN2_OP keeps its own synchan's list from the flags
+// (syn2_op, hsh_device.h).
+constexpr int EV_ERR_FULL = 2;    // a pending list ran out of room
+
+struct EvCtl {
+    double tnext;         // SimulationTime() of the next step
+    double tnow;          // of the step being run (set by its spike pass)
+    double tlast;         // of the last step run
+    int err;              // EV_ERR_* bits
+    int npost;            // posted events the next post pass applies
+};
+
+// Where the spike pass finds a spike source's values in the layout.
+struct SpikeSrc {
+    int vpos;             // V position of its compartment
+    int spos;             // S position of its threshold
+    int cpos;             // I positions of its counter and reload
+    int rpos;
+    int opidx;            // ops index of the SPIKE_OP
+    int pad;
+};
+
+struct EvPost {
+    const int* sid;       // sorted by synchan, then post order
+    const int* T;
+    const double* wdt;
+    const int* after;
+    const int* ipos;      // per synchan: I position of its SYN2_OP counter
+    int nsyn;
+};
+
+hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, int* sflag, hipStream_t s);
+hipError_t launch_ev_post(const DevModel& d, const EvPost& p, EvCtl* ctl, hipStream_t s);
 
 // kernel launchers (hsh_kernels.hip); jm may be null or not ok
 hipError_t launch_channel(const DevModel& d, const JitModule* jm, hipStream_t s);

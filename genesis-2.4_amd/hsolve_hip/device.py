@@ -83,7 +83,8 @@ class HshInfo(C.Structure):
 # every symbol include/hsolve_hip.h declares
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
            "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_create_group", "hsh_group_member", "hsh_get_givals", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
-           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_set_ravals",
+           "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_post_events",
+           "hsh_set_time", "hsh_set_ravals",
            "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
 
@@ -122,6 +123,10 @@ def lib():
         L.hsh_autotune.restype = C.c_int32
         L.hsh_set_synapses.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, _f64p]
         L.hsh_set_synapses.restype = C.c_int32
+        L.hsh_post_events.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, C.c_int32]
+        L.hsh_post_events.restype = C.c_int32
+        L.hsh_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.hsh_set_time.restype = C.c_int32
         L.hsh_set_ravals.argtypes = [C.c_void_p, _f64p]
         L.hsh_set_ravals.restype = C.c_int32
         L.hsh_adj_forward.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _i32p, _f64p]
@@ -290,12 +295,26 @@ class HipSolver:
         return {f: getattr(i, f) for f, _ in HshInfo._fields_}
 
     def set_synapses(self, src_op, dst_op, weight, delay):
-        """Synaptic connections delivered on the device (hsh_set_synapses):
-        SPIKE_OP ops index -> SYN2_OP ops index, weight, delay (s)."""
+        """Synaptic connections and the synchans' pending lists on the device
+        (hsh_set_synapses): SPIKE_OP ops index -> SYN2_OP ops index, weight,
+        delay (s)."""
         a = [np.ascontiguousarray(x, t) for x, t in ((src_op, np.int32), (dst_op, np.int32),
                                                        (weight, np.float64), (delay, np.float64))]
         _check(lib().hsh_set_synapses(self.h, a[0].size, _p(a[0], C.c_int32), _p(a[1], C.c_int32),
                                       _p(a[2], C.c_double), _p(a[3], C.c_double)), "hsh_set_synapses")
+
+    def post_events(self, syn_op, steps, weight, after=False):
+        """EVENTs from outside the handle (hsh_post_events): SYN2_OP ops
+        index, steps to the hit, weight; after the last step's ops or before
+        the next step's."""
+        a = [np.ascontiguousarray(np.atleast_1d(x), t) for x, t in ((syn_op, np.int32), (steps, np.int32),
+                                                                     (weight, np.float64))]
+        _check(lib().hsh_post_events(self.h, a[0].size, _p(a[0], C.c_int32), _p(a[1], C.c_int32),
+                                     _p(a[2], C.c_double), 1 if after else 0), "hsh_post_events")
+
+    def set_time(self, t):
+        """SimulationTime() of the next step (hsh_set_time)."""
+        _check(lib().hsh_set_time(self.h, float(t)), "hsh_set_time")
 
     def set_ravals(self, ravals):
         r = np.ascontiguousarray(ravals, np.float64)

@@ -251,21 +251,44 @@ hsh_status hsh_info_get(struct hsh_handle *h, hsh_info *info);
  * K on the handle's own model and keeps the fastest.  The simulation state
  * (vm, chip, ops, results, step count, pending spikes) is unchanged. */
 hsh_status hsh_autotune(struct hsh_handle *h);
-/* Synaptic connections, delivered on the device.  Each connection takes a
- * spike raised by the SPIKE_OP at ops index src_op[i] to the SYN2_OP synchan
- * at ops index dst_op[i] with weight[i] and delay[i] (s); weight and delay are
- * rounded to float, as GENESIS stores them (newconn_struct.h:158-159).  A
- * spike at step s adds weight/dt to the synchan's activation of step
- * s + (int)(delay/dt + 0.9), one step later when the synchan's op precedes the
- * spike's op in ops[] -- the arrival the reference's EVENT / HSEVENT / SYN2_OP
- * counter produce for an event reaching an empty pending list
- * (newconn/synchan.c:161-275, hines.c:352-357, hines_chip.c:514-528); SYN2_OP
- * then adds dt*activation*stablist[k+3] to X (hines_conn.c:60-89).  Replaces
- * the reference's per-spike host messaging (h_dospike_event, h_dosynchan).
- * Batched launches run at most min(delay steps) steps, so every event a
- * launch raises lands after it.  n = 0 removes all connections. */
+/* Synaptic connections and the synchans' pending event lists, on the device.
+ * Each connection takes a spike raised by the SPIKE_OP at ops index
+ * src_op[i] to the SYN2_OP synchan at ops index dst_op[i] with weight[i] and
+ * delay[i] (s), rounded to float as GENESIS stores them
+ * (newconn_struct.h:158-159).  Every synchan keeps the reference's pending
+ * list (newconn/synchan.c:161-358): a spike is an EVENT that hits
+ * (int)(delay/dt + 0.9) steps later, timed relative to the event before it;
+ * a non-empty list first re-times its head by (int)((t - time_last_event)/dt)
+ * of the floating simulation times; a new head sets the SYN2_OP counter
+ * (HSEVENT, hines.c:352-357); at 0 the synchan sums weight/dt of the events
+ * of that step into its float activation and adds dt*activation*
+ * stablist[k+3] to X (h_dosynchan, hines_conn.c:60-89).  EVENTs within a
+ * step follow ops[] order (a synchan whose op comes before the spike's has
+ * already counted that step).  Replaces the reference's per-spike host
+ * messaging (h_dospike_event, hines_conn.c:44-56).  Batched launches run at
+ * most (shortest connection in steps) - 1 steps, and at most 64.  n = 0
+ * removes the connections; pending events stay.  HSH_ERR_LIMIT later (from
+ * hsh_sync / hsh_get_state) if a list overflows its capacity (the events one
+ * connection can hold pending over its delay, plus 32). */
 hsh_status hsh_set_synapses(struct hsh_handle *h, int32_t n, const int32_t *src_op,
                             const int32_t *dst_op, const double *weight, const double *delay);
+/* EVENTs that reach hsolved synchans from outside the handle: spikegens and
+ * other solvers connected through GENESIS messages (hines_conn.c:91-105 ->
+ * synchan EVENT -> HSEVENT, hines.c:352-357).  steps[i] is the caller's
+ * (int)((delay - (SimulationTime() - eventtime))/dt + 0.9); weight[i] is the
+ * synapse weight (rounded to float).  after = 0: the events arrive at the
+ * start of the next step, before the solver's ops (their source ran earlier
+ * in the step); after = 1: at the end of the last step run, after its ops.
+ * Within one call, and across calls before the next step, events keep the
+ * order they were posted in.  They are applied by the next hsh_step (or
+ * hsh_get_state).  Replaces the reference's synchan EVENT path for hsolved
+ * synchans; no counterpart in the reference's GPU backend (which has none). */
+hsh_status hsh_post_events(struct hsh_handle *h, int32_t n, const int32_t *syn_op,
+                           const int32_t *steps, const double *weight, int32_t after);
+/* SimulationTime() of the next step (0 at hsh_create); it then grows by dt
+ * per step as simulation_time += ClockValue(0) does (sim/sim_step.c:112).
+ * The pending lists' re-timing reads it. */
+hsh_status hsh_set_time(struct hsh_handle *h, double t);
 /* Replace the elimination coefficients with ravals[nravals] in the layout of
  * the model's funcs[] program (the values h_funcs_init computes from Cm and
  * Ra, genesis/src/hines/hines_init.c:604-742, :1011-1172); the state is kept. */

@@ -66,6 +66,22 @@
 #define OHS_TINY 1.0e-6   /* hines_defs.h:114 */
 #define CRANK_INT 11
 
+/* One synchan's pending synaptic events (newconn/synchan.c:161-316): the
+ * list in order, each event's time field (steps after the event before it;
+ * for the head, steps counted from time_last_event) and weight/dt of its
+ * synapse; time_last_event; the ops index of the synchan's SYN2_OP. */
+typedef struct {
+    int32_t time;
+    double wdt;
+} ohs_ev;
+
+typedef struct {
+    ohs_ev *ev;
+    int32_t n, cap;
+    double tle;
+    int32_t synop;
+} ohs_pend;
+
 struct ohs {
     ohs_arrays a;
     int32_t *ops;
@@ -81,13 +97,16 @@ struct ohs {
     void *ctx;
     int32_t trace_on;
     uint64_t *trace;
-    /* synaptic connections (ohs_set_synapses) */
-    int32_t ring_len;          /* 0: none */
+    /* synaptic events: the synchans' pending lists (ohs_set_synapses,
+     * ohs_post_event) and the simulation time they are timed with */
+    int32_t nsyn;              /* synchans with a pending list; 0: none */
     int32_t *syn_of_op;        /* ops index -> synchan id, -1 */
+    ohs_pend *pend;            /* [nsyn] */
     int32_t *conn_off;         /* per ops index of a SPIKE_OP: first connection (CSR over ops indices) */
-    int32_t *conn_dst, *conn_dly;
-    double *conn_w;
-    double *ring;              /* per synchan id: ring_len activations */
+    int32_t *conn_dst, *conn_T;
+    double *conn_wdt;
+    double simtime;            /* SimulationTime() of the next step (sim_step.c:106-112) */
+    double t_last;             /* SimulationTime() of the last step run */
 };
 
 static void *dupmem(const void *p, size_t n)
@@ -132,8 +151,10 @@ void ohs_destroy(ohs *h)
     free(h->ops); free(h->chip); free(h->vm); free(h->results_base); free(h->ravals);
     free(h->givals); free(h->conc); free(h->flux); free(h->codiag); free(h->itotal);
     free(h->spike_op); free(h->spike_step); free(h->trace);
-    free(h->syn_of_op); free(h->conn_off); free(h->conn_dst); free(h->conn_dly); free(h->conn_w);
-    free(h->ring);
+    free(h->conn_off); free(h->conn_dst); free(h->conn_T); free(h->conn_wdt);
+    if (h->pend)
+        for (int32_t i = 0; i < h->nsyn; i++) free(h->pend[i].ev);
+    free(h->pend); free(h->syn_of_op);
     free(h);
 }
 
@@ -167,16 +188,96 @@ static void record_spike(ohs *h, int32_t opidx)
     h->nspk++;
 }
 
-/* A spike of the SPIKE_OP at ops index op adds weight/dt to the activation
- * of each connected synchan at its arrival step, in connection order
- * (newconn/synchan.c:340-351 sums weight/dt over the events of a step). */
+/* EVENT of an hsolved synchan (newconn/synchan.c:161-302): insert an event
+ * that hits `time` steps from now into the pending list at simulation time
+ * t.  A non-empty list first has its head's time field re-timed by the whole
+ * steps elapsed since time_last_event, (int)((t - tle)/dt) of the two
+ * floating simulation times (:249-250); the new event goes before the first
+ * event whose (relative) time is >= its own, becoming relative to the event
+ * before it, and the event after it becomes relative to the new one
+ * (:251-294).  A new head sets the SYN2_OP counter (h_eventaction ->
+ * HSEVENT, hines.c:352-357).  time_last_event = t (:300). */
+static void pend_insert(ohs *h, int32_t sid, int32_t time, double wdt, double t)
+{
+    ohs_pend *q = &h->pend[sid];
+    if (q->n == q->cap) {
+        q->cap = q->cap ? 2 * q->cap : 8;
+        q->ev = (ohs_ev *)realloc(q->ev, sizeof(ohs_ev) * q->cap);
+    }
+    if (q->n == 0) {
+        q->ev[0].time = time;
+        q->ev[0].wdt = wdt;
+        q->n = 1;
+        h->ops[q->synop + 2] = time;
+    } else {
+        q->ev[0].time -= (int32_t)((t - q->tle) / h->a.dt);
+        int32_t i = 0;
+        for (; i < q->n; i++) {
+            if (time <= q->ev[i].time) break;
+            time -= q->ev[i].time;
+        }
+        if (i < q->n) {
+            q->ev[i].time -= time;
+            memmove(&q->ev[i + 1], &q->ev[i], sizeof(ohs_ev) * (size_t)(q->n - i));
+        }
+        q->ev[i].time = time;
+        q->ev[i].wdt = wdt;
+        q->n++;
+        if (i == 0) h->ops[q->synop + 2] = time;
+    }
+    q->tle = t;
+}
+
+/* HPROCESS of a synchan whose SYN2_OP counter reached 0 (h_dosynchan,
+ * hines_conn.c:60-89; newconn/synchan.c:305-358): the head's time is forced
+ * to 0, the activation (a float field, struct_defs.h:112) restarts at 0 and
+ * takes weight/dt of every event at the front whose time is 0, in list
+ * order; a list left non-empty sets time_last_event = t.  Returns the
+ * activation; the caller adds dt*activation*stablist[k+3] to X and sets the
+ * counter to the new head's time or -1. */
+static float pend_fire(ohs *h, int32_t sid, double t)
+{
+    ohs_pend *q = &h->pend[sid];
+    float act = 0.0f;
+    if (q->n == 0) return act;
+    q->ev[0].time = 0;
+    int32_t i = 0;
+    while (i < q->n && q->ev[i].time == 0) {
+        act = (float)((double)act + q->ev[i].wdt);
+        i++;
+    }
+    memmove(&q->ev[0], &q->ev[i], sizeof(ohs_ev) * (size_t)(q->n - i));
+    q->n -= i;
+    if (q->n > 0) q->tle = t;
+    return act;
+}
+
+/* A spike of the SPIKE_OP at ops index op: the spikegen's messages in
+ * connection order, each an EVENT at its synchan (h_dospike_event,
+ * hines_conn.c:44-56). */
 static void deliver(ohs *h, int32_t op)
 {
-    const int32_t mask = h->ring_len - 1;
-    for (int32_t j = h->conn_off[op]; j < h->conn_off[op + 1]; j++) {
-        double *slot = h->ring + (size_t)h->conn_dst[j] * h->ring_len + ((h->step + h->conn_dly[j]) & mask);
-        *slot += h->conn_w[j];
+    for (int32_t j = h->conn_off[op]; j < h->conn_off[op + 1]; j++)
+        pend_insert(h, h->conn_dst[j], h->conn_T[j], h->conn_wdt[j], h->simtime);
+}
+
+/* The pending list of the synchan whose SYN2_OP is at ops index op, made on
+ * first use; -1 if ops[op] is not a SYN2_OP. */
+static int32_t syn_id(ohs *h, int32_t op)
+{
+    const int32_t nops = h->a.nops;
+    if (!h->syn_of_op) {
+        h->syn_of_op = (int32_t *)malloc(sizeof(int32_t) * (size_t)nops);
+        for (int32_t i = 0; i < nops; i++) h->syn_of_op[i] = -1;
     }
+    if (op < 0 || op + 3 >= nops || h->ops[op] != SYN2_OP) return -1;
+    if (h->syn_of_op[op] < 0) {
+        h->pend = (ohs_pend *)realloc(h->pend, sizeof(ohs_pend) * (size_t)(h->nsyn + 1));
+        memset(&h->pend[h->nsyn], 0, sizeof(ohs_pend));
+        h->pend[h->nsyn].synop = op;
+        h->syn_of_op[op] = h->nsyn++;
+    }
+    return h->syn_of_op[op];
 }
 
 int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t *dst_op,
@@ -184,36 +285,27 @@ int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t
 {
     const int32_t nops = h->a.nops;
     const double dt = h->a.dt;
-    free(h->syn_of_op); free(h->conn_off); free(h->conn_dst); free(h->conn_dly); free(h->conn_w);
-    free(h->ring);
-    h->syn_of_op = NULL; h->conn_off = NULL; h->conn_dst = h->conn_dly = NULL; h->conn_w = NULL;
-    h->ring = NULL;
-    h->ring_len = 0;
+    free(h->conn_off); free(h->conn_dst); free(h->conn_T); free(h->conn_wdt);
+    h->conn_off = NULL; h->conn_dst = h->conn_T = NULL; h->conn_wdt = NULL;
     if (n <= 0) return 0;
-    h->syn_of_op = (int32_t *)malloc(sizeof(int32_t) * nops);
-    int32_t nsyn = 0;
-    for (int32_t i = 0; i < nops; i++) h->syn_of_op[i] = -1;
-    int32_t mind = 1 << 30, maxd = 0;
-    int32_t *dly = (int32_t *)malloc(sizeof(int32_t) * n);
+    int32_t mind = 1 << 30;
+    int32_t *T = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
     for (int32_t i = 0; i < n; i++) {
-        if (src_op[i] < 0 || src_op[i] >= nops || h->ops[src_op[i]] != SPIKE_OP) { free(dly); return -1; }
-        if (dst_op[i] < 0 || dst_op[i] >= nops || h->ops[dst_op[i]] != SYN2_OP) { free(dly); return -1; }
-        if (h->syn_of_op[dst_op[i]] < 0) h->syn_of_op[dst_op[i]] = nsyn++;
-        /* weight and delay are float fields of the synapse; arrival
-         * (int)(delay/dt + 0.9) steps later (newconn/synchan.c:193-194), one
-         * more when the synchan's op comes before the spike's in ops[] */
+        if (src_op[i] < 0 || src_op[i] >= nops || h->ops[src_op[i]] != SPIKE_OP || syn_id(h, dst_op[i]) < 0) {
+            free(T);
+            return -1;
+        }
+        /* delay is a float field of the synapse; the event hits
+         * (int)((delay - (t - eventtime))/dt + 0.9) steps later, and an hsolve
+         * spike is sent at the current time (newconn/synchan.c:193-194) */
         const double dl = (double)(float)delay[i];
-        dly[i] = (int32_t)(dl / dt + 0.9) + (dst_op[i] < src_op[i] ? 1 : 0);
-        if (dly[i] < 1) { free(dly); return -1; }
-        if (dly[i] < mind) mind = dly[i];
-        if (dly[i] > maxd) maxd = dly[i];
+        T[i] = (int32_t)((dl - 0.0) / dt + 0.9);
+        if (T[i] < mind) mind = T[i];
     }
-    int32_t R = 1;
-    while (R < maxd + mind + 1) R <<= 1;
     h->conn_off = (int32_t *)calloc((size_t)nops + 1, sizeof(int32_t));
-    h->conn_dst = (int32_t *)malloc(sizeof(int32_t) * n);
-    h->conn_dly = (int32_t *)malloc(sizeof(int32_t) * n);
-    h->conn_w = (double *)malloc(sizeof(double) * n);
+    h->conn_dst = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    h->conn_T = (int32_t *)malloc(sizeof(int32_t) * (size_t)n);
+    h->conn_wdt = (double *)malloc(sizeof(double) * (size_t)n);
     for (int32_t i = 0; i < n; i++) h->conn_off[src_op[i] + 1]++;
     for (int32_t i = 0; i < nops; i++) h->conn_off[i + 1] += h->conn_off[i];
     int32_t *cur = (int32_t *)malloc(sizeof(int32_t) * ((size_t)nops + 1));
@@ -221,14 +313,37 @@ int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t
     for (int32_t i = 0; i < n; i++) {
         const int32_t q = cur[src_op[i]]++;
         h->conn_dst[q] = h->syn_of_op[dst_op[i]];
-        h->conn_dly[q] = dly[i];
-        h->conn_w[q] = (double)(float)weight[i] / dt;
+        h->conn_T[q] = T[i];
+        /* synapse weight (float) / dt (newconn/synchan.c:351) */
+        h->conn_wdt[q] = (double)(float)weight[i] / dt;
     }
     free(cur);
-    free(dly);
-    h->ring = (double *)calloc((size_t)R * (nsyn > 0 ? nsyn : 1), sizeof(double));
-    h->ring_len = R;
+    free(T);
     return mind;
+}
+
+int32_t ohs_post_event(ohs *h, int32_t syn_op, int32_t time, double weight, int32_t after)
+{
+    const int32_t sid = syn_id(h, syn_op);
+    if (sid < 0 || time < 0) return -1;
+    if (after && h->step == 0) return -1;
+    pend_insert(h, sid, time, (double)(float)weight / h->a.dt, after ? h->t_last : h->simtime);
+    return 0;
+}
+
+void ohs_set_time(ohs *h, double t) { h->simtime = t; }
+double ohs_time(ohs *h) { return h->simtime; }
+
+int32_t ohs_pending(ohs *h, int32_t syn_op, int32_t *time, double *wdt, int32_t cap, double *tle)
+{
+    if (!h->syn_of_op || syn_op < 0 || syn_op >= h->a.nops || h->syn_of_op[syn_op] < 0) return -1;
+    const ohs_pend *q = &h->pend[h->syn_of_op[syn_op]];
+    for (int32_t i = 0; i < q->n && i < cap; i++) {
+        if (time) time[i] = q->ev[i].time;
+        if (wdt) wdt[i] = q->ev[i].wdt;
+    }
+    if (tle) *tle = q->tle;
+    return q->n;
 }
 
 /* do_chip_hh4_update with SOLVEINTERPOL (hines_chip.c:78-602).  The op
@@ -341,7 +456,7 @@ int32_t ohs_chip_update(ohs *h)
                 if (Vm > chip[c++]) {
                     if (ops[p] <= 0) {
                         record_spike(h, (int32_t)p - 1);
-                        if (h->ring_len > 0) deliver(h, (int32_t)p - 1);
+                        if (h->conn_off) deliver(h, (int32_t)p - 1);
                         ops[p] = ops[p + 1];
                     }
                 }
@@ -492,20 +607,17 @@ int32_t ohs_chip_update(ohs *h)
                     k = ops[p++];
                     if (h->trace_on) tr = fnv(tr, k);
                     chip[c] *= stablist[k];
-                    if (h->ring_len > 0) {
-                        /* events from the connection table (ohs_set_synapses) */
-                        const int32_t sid = h->syn_of_op[synop];
-                        if (sid >= 0) {
-                            double *slot = h->ring + (size_t)sid * h->ring_len + (h->step & (h->ring_len - 1));
-                            if (*slot != 0.0) {
-                                chip[c] += A->dt * *slot * stablist[k + 3];
-                                *slot = 0.0;
-                            }
-                        }
-                    }
                     if (ops[p] == 0) {
                         int32_t nxt = -1;
-                        if (h->syn) nxt = h->syn(h->ctx, k, ops[p + 1], (int32_t)c, h->step);
+                        const int32_t sid = h->syn_of_op ? h->syn_of_op[synop] : -1;
+                        if (sid >= 0) {
+                            /* h_dosynchan (hines_conn.c:60-89) */
+                            const float act = pend_fire(h, sid, h->simtime);
+                            chip[c] += A->dt * (double)act * stablist[k + 3];
+                            nxt = h->pend[sid].n > 0 ? h->pend[sid].ev[0].time : -1;
+                        } else if (h->syn) {
+                            nxt = h->syn(h->ctx, k, ops[p + 1], (int32_t)c, h->step);
+                        }
                         ops[p] = nxt;
                     }
                     ops[p] -= 1;
@@ -731,6 +843,9 @@ int32_t ohs_step(ohs *h, int32_t nsteps)
         else ohs_euler_solve(h);
         if (h->bad_funcs) return -1;           /* malformed funcs[]: stopped inside the rows */
         h->step++;
+        /* simulation_time += ClockValue(0) after every step (sim_step.c:112) */
+        h->t_last = h->simtime;
+        h->simtime += h->a.dt;
     }
     return 0;
 }

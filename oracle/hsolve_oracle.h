@@ -94,16 +94,35 @@ void    ohs_trace(ohs *h, uint64_t *per_comp);
  * Used for the multi-core CPU baseline. */
 int32_t ohs_step_parallel(ohs **hs, int32_t nh, int32_t nsteps, int32_t nthreads);
 
-/* Synaptic connections: a spike of the SPIKE_OP at ops index src_op[i] adds
- * (float)weight[i]/dt to the activation of the SYN2_OP synchan at dst_op[i]
- * at step s + (int)((float)delay[i]/dt + 0.9) (+1 when dst_op < src_op), and
- * SYN2_OP adds dt*activation*stablist[k+3] to X at that step -- the arrival
- * of an event that reaches an empty pending list in the reference
- * (newconn/synchan.c:161-275, hines.c:352-357, hines_chip.c:514-528,
- * hines_conn.c:60-89).  Returns the shortest connection in steps, or -1 on
- * bad arguments. */
+/* Synaptic connections and the synchans' pending event lists, as hsolved
+ * synchans keep them (newconn/synchan.c:161-358, hines.c:352-357,
+ * hines_conn.c:44-89).  A spike of the SPIKE_OP at ops index src_op[i] is an
+ * EVENT at the synchan whose SYN2_OP is at dst_op[i], in connection order:
+ * it hits time = (int)((float)delay[i]/dt + 0.9) steps later; a non-empty
+ * list first re-times its head by (int)((t - time_last_event)/dt) of the
+ * floating simulation times, the event goes before the first one whose time
+ * is >= its own (times relative to the event before), and a new head sets
+ * the SYN2_OP counter.  When the counter reaches 0 the synchan pops the
+ * front events whose time is 0, sums (float)weight/dt into its float
+ * activation, adds dt*activation*stablist[k+3] to X and sets the counter to
+ * the new head's time or -1.  Returns the shortest connection in steps
+ * (`time` of the shortest delay), or -1 on bad arguments.  The pending
+ * lists persist across calls; n = 0 removes the connections. */
 int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t *dst_op,
                          const double *weight, const double *delay);
+/* An EVENT from outside the solver: `time` steps to the hit (the caller's
+ * (int)((delay - (t - eventtime))/dt + 0.9)), a float weight.  after = 0: at
+ * the simulation time of the next step, before its ops; after = 1: at the
+ * time of the last step run, after its ops.  Returns 0 or -1. */
+int32_t ohs_post_event(ohs *h, int32_t syn_op, int32_t time, double weight, int32_t after);
+/* SimulationTime() of the next step (0 at creation); it grows by dt per
+ * step (simulation_time += ClockValue(0), sim/sim_step.c:112). */
+void    ohs_set_time(ohs *h, double t);
+double  ohs_time(ohs *h);
+/* The pending list of the synchan at ops index syn_op: its events' time
+ * fields and weight/dt (up to cap), time_last_event; returns the list
+ * length, -1 if the synchan has none. */
+int32_t ohs_pending(ohs *h, int32_t syn_op, int32_t *time, double *wdt, int32_t cap, double *tle);
 
 /* Tangent-linear model (hsolve_tl.c): the same step carrying the derivative
  * of every value along one direction of (vm0, chip, ravals).  Tabchannel

@@ -70,6 +70,13 @@ def oracle_lib():
         L.ohs_step_parallel.restype = C.c_int32
         L.ohs_set_synapses.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, _f64p]
         L.ohs_set_synapses.restype = C.c_int32
+        L.ohs_post_event.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32]
+        L.ohs_post_event.restype = C.c_int32
+        L.ohs_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.ohs_time.argtypes = [C.c_void_p]
+        L.ohs_time.restype = C.c_double
+        L.ohs_pending.argtypes = [C.c_void_p, C.c_int32, _i32p, _f64p, C.c_int32, _f64p]
+        L.ohs_pending.restype = C.c_int32
         L.ohs_tl_create.argtypes = [C.POINTER(Arrays)]
         L.ohs_tl_create.restype = C.c_void_p
         L.ohs_tl_destroy.argtypes = [C.c_void_p]
@@ -230,6 +237,31 @@ class Oracle:
         if r < 0:
             raise ValueError("oracle: bad synaptic connection")
         return r
+
+    def post_event(self, syn_op, time, weight, after=False):
+        """An event from outside the solver (see ohs_post_event)."""
+        if oracle_lib().ohs_post_event(self.h, int(syn_op), int(time), float(weight), 1 if after else 0) != 0:
+            raise ValueError("oracle: bad posted event")
+
+    def set_time(self, t):
+        oracle_lib().ohs_set_time(self.h, float(t))
+
+    @property
+    def time(self):
+        return oracle_lib().ohs_time(self.h)
+
+    def pending(self, syn_op, cap=4096):
+        """(time fields, weight/dt, time_last_event) of a synchan's pending
+        list, or None when it has none."""
+        t = np.empty(cap, np.int32)
+        w = np.empty(cap, np.float64)
+        tle = C.c_double(0.0)
+        n = oracle_lib().ohs_pending(self.h, int(syn_op), _p(t, C.c_int32), _p(w, C.c_double), cap,
+                                     C.byref(tle))
+        if n < 0:
+            return None
+        n = min(n, cap)
+        return t[:n].copy(), w[:n].copy(), tle.value
 
     def spikes(self):
         L = oracle_lib()

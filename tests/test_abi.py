@@ -104,7 +104,9 @@ def test_accepts_synchan_and_checks_connections():
         assert h.refusals() == []
         src, dst = int(m.spike_op[0]), int(m.syn_op[0])
         h.set_synapses([src], [dst], [1.0], [1.5e-3])
-        assert h.info()["batch_k"] == (16 if dst < src else 15)   # K <= shortest connection
+        # with pending lists every step is a launch of its own: an EVENT can
+        # hit in the next step (the head's re-timing drifts)
+        assert h.info()["batch_k"] == 1
         with pytest.raises(D.HshError):          # not a SPIKE_OP
             h.set_synapses([dst], [dst], [1.0], [1.5e-3])
         with pytest.raises(D.HshError):          # not a SYN2_OP

@@ -1,11 +1,12 @@
-"""Event-driven networks on the device: SPIKE_OP, device-side delivery of
-synaptic events (hsh_set_synapses) and SYN2_OP, against the oracle's event
-mode on the same connection table.
+"""Event-driven networks on the device: SPIKE_OP, the synchans' pending
+event lists kept by the device's event pass (hsh_set_synapses,
+hsh_post_events) and SYN2_OP, against the oracle's restatement of the
+reference's lists (newconn/synchan.c:161-358) on the same connections.
 
 The bar: the spike raster (ops index, step) is identical and vm, chip,
-results and ops are bit-exact -- events of one step reaching one synchan
-all carry the same weight here, so the order in which the device adds them
-cannot change the sum.
+results and ops (the SYN2_OP counters included) are bit-exact -- with
+unequal weights and delays too: the device sums a step's events in the
+list's order, in single precision, as the reference's float activation does.
 """
 import os
 
@@ -36,10 +37,10 @@ def check_same(o, g):
 
 @pytest.mark.parametrize("mode,kw", [
     (D.PLAIN, dict(flags=D.F_NO_AUTOTUNE)),
-    (D.BATCHED, dict(batch_k=15, flags=D.F_NO_AUTOTUNE)),
+    (D.BATCHED, dict(batch_k=14, flags=D.F_NO_AUTOTUNE)),
     (D.BATCHED, dict(batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=256)),
     (D.BATCHED, dict()),
-], ids=["plain", "iso-K15", "packs-K8", "autotuned"])
+], ids=["plain", "iso", "packs", "autotuned"])
 def test_network_raster_and_state(gpu, mode, kw):
     m, table = M.network(400, seed=3, inject=(0.3e-9, 0.8e-9))
     T = 1500
@@ -51,7 +52,7 @@ def test_network_raster_and_state(gpu, mode, kw):
     assert want_op.size > 100           # the network is active
     with D.HipSolver(m, **kw) as g:
         g.set_synapses(*table)
-        assert g.info()["batch_k"] <= 15
+        assert g.info()["batch_k"] == 1           # one step per launch with pending lists
         g.step(T, mode)
         got_op, got_step = g.drain_spikes()
         check_same(o, g)
@@ -86,9 +87,9 @@ def test_event_arrival_both_op_orders(gpu, mode):
 
 @pytest.mark.parametrize("mode,kw", [
     (D.PLAIN, dict(flags=D.F_NO_AUTOTUNE)),
-    (D.BATCHED, dict(batch_k=15, flags=D.F_NO_AUTOTUNE)),
+    (D.BATCHED, dict(batch_k=14, flags=D.F_NO_AUTOTUNE)),
     (D.BATCHED, dict(batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=256)),
-], ids=["plain", "iso-K15", "packs-K8"])
+], ids=["plain", "iso", "packs"])
 def test_network_with_mg_block(gpu, mode, kw):
     """Excitatory synchans with a Mg block (NMDA-like): the block calls the
     device's exp, so vm is held to 1e-7 V; the raster must still be the
@@ -113,3 +114,94 @@ def test_network_with_mg_block(gpu, mode, kw):
     assert float(np.max(np.abs(os_["vm"] - gs["vm"]))) <= 1e-7
     assert np.array_equal(os_["ops"], gs["ops"])
     assert np.array_equal(got_step, want_step) and np.array_equal(got_op, want_op)
+
+
+def unequal_table(table, seed):
+    """The same connections with unequal weights (0.5..2) and delays
+    (1.5..4 ms): events queue behind and ahead of one another, and a step's
+    events carry different weights."""
+    rng = np.random.default_rng(seed)
+    n = table[0].size
+    return (table[0], table[1], rng.uniform(0.5, 2.0, n), rng.uniform(1.5e-3, 4e-3, n))
+
+
+@pytest.mark.parametrize("mode,kw", [
+    (D.PLAIN, dict(flags=D.F_NO_AUTOTUNE)),
+    (D.BATCHED, dict(batch_k=14, flags=D.F_NO_AUTOTUNE)),
+    (D.BATCHED, dict(batch_k=5, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=256)),
+], ids=["plain", "iso", "packs"])
+def test_network_unequal_weights_and_delays(gpu, mode, kw):
+    m, table0 = M.network(400, seed=5, inject=(0.3e-9, 0.8e-9))
+    table = unequal_table(table0, 9)
+    T = 2000
+    print(f"[case] network unequal weights/delays, mode {mode} {kw}", flush=True)
+    o = Oracle(m, givals=False)
+    o.set_synapses(*table)
+    o.step(T)
+    want_op, want_step = o.spikes()
+    assert want_op.size > 100
+    with D.HipSolver(m, **kw) as g:
+        g.set_synapses(*table)
+        g.step(T // 2, mode)
+        g.step(T - T // 2, mode)              # a call boundary inside the run
+        got_op, got_step = g.drain_spikes()
+        check_same(o, g)
+    assert np.array_equal(got_step, want_step)
+    assert np.array_equal(got_op, want_op)
+
+
+def one_synchan_model():
+    comps = dict(parent=[-1], Cm=[1e-11], Rm=[1e8], Ra=[1e6], Em=[-0.065], initVm=[-0.065], inject=[0.0])
+    return M.build(comps, [], [], dt=1e-4, syntypes=[dict(Ek=0.0, tau1=5e-3, tau2=1e-3)],
+                   syns=[(0, 0, 1e-9)], float_fields=False)
+
+
+@pytest.mark.parametrize("mode", [D.PLAIN, D.BATCHED])
+def test_posted_events_queue_like_the_reference(gpu, mode):
+    """hsh_post_events against the oracle: events before and after a step's
+    ops, queued behind the head, ahead of it, several in one step, and the
+    re-timing interval (t_23 - t_8)/dt that truncates to 14."""
+    m = one_synchan_model()
+    syn = int(m.syn_op[0])
+    # (steps already run when posting, steps to the hit, weight, after)
+    plan = [(0, 15, 2e-9, False), (3, 15, 5e-9, False), (3, 4, 1.3e-9, True), (8, 30, 1e-9, False),
+            (23, 20, 2e-9, False), (23, 20, 7.53e-9, False), (23, 20, 1.02e-9, False), (30, 0, 3e-9, False),
+            (41, 6, 8.48e-9, True)]
+    o = Oracle(m, givals=False)
+    with D.HipSolver(m, flags=D.F_NO_AUTOTUNE, batch_k=7) as g:
+        done = 0
+        for at, T, w, after in plan:
+            if at > done:
+                o.step(at - done)
+                g.step(at - done, mode)
+                done = at
+            o.post_event(syn, T, w, after=after)
+            g.post_events([syn], [T], [w], after=after)
+            check_same(o, g)                 # HSEVENT counters right after posting
+        for _ in range(6):
+            o.step(9)
+            g.step(9, mode)
+            check_same(o, g)
+
+
+def test_event_state_survives_autotune_and_put(gpu):
+    """The autotuner's trial launches fire and add events; it puts the lists
+    back.  hsh_put_state with ops sets the counters the event pass counts."""
+    m, table0 = M.network(200, seed=7, inject=(0.3e-9, 0.8e-9))
+    table = unequal_table(table0, 3)
+    o = Oracle(m, givals=False)
+    o.set_synapses(*table)
+    o.step(300)
+    with D.HipSolver(m) as g:
+        g.set_synapses(*table)
+        g.step(300, D.PLAIN)
+        g.autotune()
+        check_same(o, g)
+        o.step(400)
+        g.step(400, D.BATCHED)
+        check_same(o, g)
+        st = o.state()
+        g.put_state(ops=st["ops"])
+        o.step(100)
+        g.step(100, D.BATCHED)
+        check_same(o, g)
