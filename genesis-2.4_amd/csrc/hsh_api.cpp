@@ -95,10 +95,11 @@ struct hsh_handle {
     // (plan.spike_ops)
     bool ev_on = false;
     EvCtl* d_evctl = nullptr;
-    int32_t *d_syn_op = nullptr, *d_in_off = nullptr, *d_in_src = nullptr, *d_in_srcop = nullptr,
-            *d_in_T = nullptr, *d_lst_off = nullptr, *d_lst_cap = nullptr, *d_lst_n = nullptr, *d_lst_t = nullptr,
-            *d_ipos = nullptr, *d_sflag = nullptr, *d_post_sid = nullptr, *d_post_T = nullptr,
-            *d_post_after = nullptr;
+    int32_t *d_syn_op = nullptr, *d_in_off = nullptr, *d_in_mid = nullptr, *d_in_T = nullptr,
+            *d_lst_off = nullptr, *d_lst_cap = nullptr, *d_lst_n = nullptr, *d_lst_t = nullptr, *d_ipos = nullptr,
+            *d_narr = nullptr, *d_arr = nullptr, *d_out_slot = nullptr, *d_out_sid = nullptr,
+            *d_post_sid = nullptr, *d_post_T = nullptr, *d_post_after = nullptr;
+    std::vector<int32_t> out_off;      // per spike source: its out-connections in d_out_slot / d_out_sid
     double *d_in_wdt = nullptr, *d_lst_w = nullptr, *d_tle = nullptr, *d_post_wdt = nullptr;
     SpikeSrc* d_spksrc = nullptr;
     size_t lst_total = 0;              // list nodes allocated
@@ -322,14 +323,17 @@ void free_layout(hsh_handle* h)
 
 void free_events(hsh_handle* h)
 {
-    void* ptrs[] = {h->d_evctl, h->d_syn_op, h->d_in_off, h->d_in_src, h->d_in_srcop, h->d_in_T, h->d_lst_off,
-                    h->d_lst_cap, h->d_lst_n, h->d_lst_t, h->d_ipos, h->d_sflag, h->d_post_sid, h->d_post_T,
-                    h->d_post_after, h->d_in_wdt, h->d_lst_w, h->d_tle, h->d_post_wdt, h->d_spksrc};
+    void* ptrs[] = {h->d_evctl, h->d_syn_op, h->d_in_off, h->d_in_mid, h->d_in_T, h->d_lst_off, h->d_lst_cap,
+                    h->d_lst_n, h->d_lst_t, h->d_ipos, h->d_narr, h->d_arr, h->d_out_slot, h->d_out_sid,
+                    h->d_post_sid, h->d_post_T, h->d_post_after, h->d_in_wdt, h->d_lst_w, h->d_tle,
+                    h->d_post_wdt, h->d_spksrc};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_evctl = nullptr;
-    h->d_syn_op = h->d_in_off = h->d_in_src = h->d_in_srcop = h->d_in_T = h->d_lst_off = h->d_lst_cap = nullptr;
-    h->d_lst_n = h->d_lst_t = h->d_ipos = h->d_sflag = h->d_post_sid = h->d_post_T = h->d_post_after = nullptr;
+    h->d_syn_op = h->d_in_off = h->d_in_mid = h->d_in_T = h->d_lst_off = h->d_lst_cap = nullptr;
+    h->d_lst_n = h->d_lst_t = h->d_ipos = h->d_narr = h->d_arr = h->d_out_slot = h->d_out_sid = nullptr;
+    h->d_post_sid = h->d_post_T = h->d_post_after = nullptr;
+    h->out_off.clear();
     h->d_in_wdt = h->d_lst_w = h->d_tle = h->d_post_wdt = nullptr;
     h->d_spksrc = nullptr;
     h->ev_on = false;
@@ -443,10 +447,10 @@ void set_devmodel(hsh_handle* h)
     d.npacks = (int32_t)P.packs.size();
     d.stab = h->d_stab;
     d.ev = h->ev_on ? 1 : 0;
-    d.sflag = h->d_sflag;
+    d.narr = h->d_narr;
+    d.arr = h->d_arr;
     d.in_off = h->d_in_off;
-    d.in_src = h->d_in_src;
-    d.in_srcop = h->d_in_srcop;
+    d.in_mid = h->d_in_mid;
     d.in_T = h->d_in_T;
     d.in_wdt = h->d_in_wdt;
     d.syn_op = h->d_syn_op;
@@ -495,7 +499,7 @@ void ev_positions(const hsh_handle* h, std::vector<int32_t>& ipos, std::vector<S
         rel_of_spk[(int64_t)P.spike_ops[i] + 2] = (int32_t)i;
     }
     ipos.assign(P.syn_ops.size(), -1);
-    src.assign(P.spike_ops.size(), SpikeSrc{-1, -1, -1, -1, -1, 0});
+    src.assign(P.spike_ops.size(), SpikeSrc{-1, -1, -1, -1, -1, 0, 0, 0});
     for (int g = 0; g < h->m.ncompts; g++) {
         const Template& t = P.templates[P.comp_tmpl[g]];
         const int pos = P.comp_pos[g];
@@ -518,6 +522,8 @@ void ev_positions(const hsh_handle* h, std::vector<int32_t>& ipos, std::vector<S
         if (ci >= 0 && ci + 1 < (int64_t)h->c2s_off.size() && h->c2s_off[ci] < h->c2s_off[ci + 1])
             src[i].spos = (int)h->c2s[h->c2s_off[ci]];
         src[i].opidx = P.spike_ops[i];
+        src[i].out0 = h->out_off.empty() ? 0 : h->out_off[i];
+        src[i].out1 = h->out_off.empty() ? 0 : h->out_off[i + 1];
     }
 }
 
@@ -610,14 +616,14 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
         if (cdst[a] != cdst[b]) return cdst[a] < cdst[b];
         return P.spike_ops[csrc[a]] < P.spike_ops[csrc[b]];
     });
-    std::vector<int32_t> in_off(ns + 1, 0), in_src(std::max<size_t>(1, n)), in_srcop(std::max<size_t>(1, n)),
-        in_T(std::max<size_t>(1, n));
+    std::vector<int32_t> in_off(ns + 1, 0), in_srcop(std::max<size_t>(1, n)), in_T(std::max<size_t>(1, n)),
+        slot_of(n), in_mid(ns);
     std::vector<double> in_wdt(std::max<size_t>(1, n));
     std::vector<int64_t> cap(ns, 32);
     for (size_t q = 0; q < n; q++) {
         const int32_t i = order[q];
         in_off[cdst[i] + 1]++;
-        in_src[q] = csrc[i];
+        slot_of[i] = (int32_t)q;
         in_srcop[q] = P.spike_ops[csrc[i]];
         in_T[q] = cT[i];
         in_wdt[q] = cwdt[i];
@@ -625,6 +631,23 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
         cap[cdst[i]] += (cT[i] + 2 + reload - 1) / reload;
     }
     for (size_t i = 0; i < ns; i++) in_off[i + 1] += in_off[i];
+    for (size_t i = 0; i < ns; i++) {
+        int32_t q = in_off[i];
+        while (q < in_off[i + 1] && in_srcop[q] < P.syn_ops[i]) q++;
+        in_mid[i] = q;
+    }
+    // out-connections by source: the in-connection slot and synchan a spike flags
+    std::vector<int32_t> out_off(nsrc + 1, 0), out_slot(std::max<size_t>(1, n)), out_sid(std::max<size_t>(1, n));
+    for (size_t i = 0; i < n; i++) out_off[csrc[i] + 1]++;
+    for (size_t i = 0; i < nsrc; i++) out_off[i + 1] += out_off[i];
+    {
+        std::vector<int32_t> cur(out_off.begin(), out_off.end() - 1);
+        for (size_t i = 0; i < n; i++) {
+            const int32_t o = cur[csrc[i]]++;
+            out_slot[o] = slot_of[i];
+            out_sid[o] = cdst[i];
+        }
+    }
     std::vector<int32_t> lst_off(ns), lst_cap(ns), lst_n(ns, 0), syn_op(ns);
     std::vector<double> tle(ns, 0.0);
     int64_t total = 0;
@@ -660,8 +683,11 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
     HIPCHK(dalloc(&h->d_evctl, 1));
     HIPCHK(dalloc(&h->d_syn_op, ns));
     HIPCHK(dalloc(&h->d_in_off, ns + 1));
-    HIPCHK(dalloc(&h->d_in_src, in_src.size()));
-    HIPCHK(dalloc(&h->d_in_srcop, in_srcop.size()));
+    HIPCHK(dalloc(&h->d_in_mid, ns));
+    HIPCHK(dalloc(&h->d_out_slot, out_slot.size()));
+    HIPCHK(dalloc(&h->d_out_sid, out_sid.size()));
+    HIPCHK(dalloc(&h->d_arr, std::max<size_t>(1, n)));
+    HIPCHK(dalloc(&h->d_narr, ns));
     HIPCHK(dalloc(&h->d_in_T, in_T.size()));
     HIPCHK(dalloc(&h->d_in_wdt, in_wdt.size()));
     HIPCHK(dalloc(&h->d_lst_off, ns));
@@ -671,13 +697,14 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
     HIPCHK(dalloc(&h->d_lst_w, ntot));
     HIPCHK(dalloc(&h->d_tle, ns));
     HIPCHK(dalloc(&h->d_ipos, ns));
-    HIPCHK(dalloc(&h->d_sflag, std::max<size_t>(1, nsrc)));
     HIPCHK(dalloc(&h->d_spksrc, std::max<size_t>(1, nsrc)));
     HIPCHK(hipMemcpy(h->d_evctl, &ctl, sizeof(EvCtl), hipMemcpyHostToDevice));
     HIPCHK(dput(h->d_syn_op, syn_op));
     HIPCHK(dput(h->d_in_off, in_off));
-    HIPCHK(dput(h->d_in_src, in_src));
-    HIPCHK(dput(h->d_in_srcop, in_srcop));
+    HIPCHK(dput(h->d_in_mid, in_mid));
+    HIPCHK(dput(h->d_out_slot, out_slot));
+    HIPCHK(dput(h->d_out_sid, out_sid));
+    HIPCHK(hipMemset(h->d_narr, 0, std::max<size_t>(1, ns) * sizeof(int32_t)));
     HIPCHK(dput(h->d_in_T, in_T));
     HIPCHK(dput(h->d_in_wdt, in_wdt));
     HIPCHK(dput(h->d_lst_off, lst_off));
@@ -686,7 +713,7 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
     HIPCHK(dput(h->d_lst_t, lt));
     HIPCHK(dput(h->d_lst_w, lw));
     HIPCHK(dput(h->d_tle, tle));
-    HIPCHK(hipMemset(h->d_sflag, 0, std::max<size_t>(1, nsrc) * sizeof(int32_t)));
+    h->out_off = out_off;
     h->lst_total = ntot;
     h->ev_on = true;
     STCHK(ev_layout(h));
@@ -749,7 +776,8 @@ hsh_status ev_flush(hsh_handle* h)
 hsh_status ev_spikes(hsh_handle* h)
 {
     if (!h->ev_on) return HSH_OK;
-    HIPCHK(launch_ev_spikes(h->dm, h->d_spksrc, (int)h->plan.spike_ops.size(), h->d_evctl, h->d_sflag, h->stream));
+    HIPCHK(launch_ev_spikes(h->dm, h->d_spksrc, (int)h->plan.spike_ops.size(), h->d_evctl, h->d_out_slot,
+                            h->d_out_sid, h->stream));
     return HSH_OK;
 }
 

@@ -165,47 +165,61 @@ __device__ __forceinline__ double syn2_op(const DevModel& d, double* __restrict_
     double X = X0 * d.stab[k];
     if (d.ev && active) {
         const int sid = Ib[(is + 1) * W + lane];
-        const int synop = d.syn_op[sid];
-        const int j0 = d.in_off[sid], j1 = d.in_off[sid + 1];
-        const int off = d.lst_off[sid], cap = d.lst_cap[sid];
-        int* __restrict__ lt = d.lst_t + off;
-        double* __restrict__ lw = d.lst_w + off;
-        int n = d.lst_n[sid];
         int cnt = Ib[is * W + lane];
-        double tle = d.tle[sid];
-        const double now = *d.tnow;
-        bool full = false;
-        int j = j0;
-        for (; j < j1 && d.in_srcop[j] < synop; j++)
-            if (d.sflag[d.in_src[j]])
-                full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[j], d.in_wdt[j], now, d.dt);
-        if (cnt == 0) {
-            float act = 0.0f;
-            if (n > 0) {
-                lt[0] = 0;
-                int i = 0;
-                while (i < n && lt[i] == 0) {
-                    act = (float)((double)act + lw[i]);
-                    i++;
+        const int na = d.narr[sid];
+        if (na > 0 || cnt == 0) {
+            const int j0 = d.in_off[sid], mid = d.in_mid[sid];
+            const int off = d.lst_off[sid], cap = d.lst_cap[sid];
+            int* __restrict__ lt = d.lst_t + off;
+            double* __restrict__ lw = d.lst_w + off;
+            int* __restrict__ ar = d.arr + j0;
+            int n = d.lst_n[sid];
+            double tle = d.tle[sid];
+            const double now = *d.tnow;
+            bool full = false;
+            // the arrivals in ops[] order of their sources (slot order)
+            for (int a = 1; a < na; a++) {
+                const int v = ar[a];
+                int b = a - 1;
+                while (b >= 0 && ar[b] > v) {
+                    ar[b + 1] = ar[b];
+                    b--;
                 }
-                for (int q = i; q < n; q++) {
-                    lt[q - i] = lt[q];
-                    lw[q - i] = lw[q];
-                }
-                n -= i;
-                if (n > 0) tle = now;
+                ar[b + 1] = v;
             }
-            X += d.dt * (double)act * d.stab[k + 3];
-            cnt = n > 0 ? lt[0] : -1;
+            int a = 0;
+            for (; a < na && ar[a] < mid; a++)
+                full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[ar[a]], d.in_wdt[ar[a]], now, d.dt);
+            if (cnt == 0) {
+                float act = 0.0f;
+                if (n > 0) {
+                    lt[0] = 0;
+                    int i = 0;
+                    while (i < n && lt[i] == 0) {
+                        act = (float)((double)act + lw[i]);
+                        i++;
+                    }
+                    for (int q = i; q < n; q++) {
+                        lt[q - i] = lt[q];
+                        lw[q - i] = lw[q];
+                    }
+                    n -= i;
+                    if (n > 0) tle = now;
+                }
+                X += d.dt * (double)act * d.stab[k + 3];
+                cnt = n > 0 ? lt[0] : -1;
+            }
+            cnt -= 1;
+            for (; a < na; a++)
+                full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[ar[a]], d.in_wdt[ar[a]], now, d.dt);
+            if (na > 0) d.narr[sid] = 0;
+            d.lst_n[sid] = n;
+            d.tle[sid] = tle;
+            if (full) atomicOr(d.everr, 2);
+        } else {
+            cnt -= 1;
         }
-        cnt -= 1;
-        for (; j < j1; j++)
-            if (d.sflag[d.in_src[j]])
-                full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[j], d.in_wdt[j], now, d.dt);
-        d.lst_n[sid] = n;
-        d.tle[sid] = tle;
         Ib[is * W + lane] = cnt;
-        if (full) atomicOr(d.everr, 2);
     } else {
         Ib[is * W + lane] -= 1;
     }

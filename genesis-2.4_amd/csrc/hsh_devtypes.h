@@ -125,10 +125,10 @@ struct DevModel {
     // newconn/synchan.c:161-358, from the spikes the spike pass of the step
     // decided before the step ran
     int ev;               // 1: lists on; SPIKE_OPs run in the spike pass, not in programs
-    const int* sflag;     // per spike source: 1 if it spiked this step
+    int* narr;            // per synchan: EVENTs arriving this step (the spike pass counts, the synchan clears)
+    int* arr;             // their in-connection slots, at in_off[synchan] + 0..narr-1, unordered
     const int* in_off;    // per synchan: in-connections, by source ops index then connection order
-    const int* in_src;    // spike source id
-    const int* in_srcop;  // ops index of the source's SPIKE_OP
+    const int* in_mid;    // per synchan: first in-connection whose source comes after its SYN2_OP
     const int* in_T;      // steps to the hit, (int)(delay/dt + 0.9)
     const double* in_wdt; // (float)weight / dt
     const int* syn_op;    // per synchan: ops index of its SYN2_OP

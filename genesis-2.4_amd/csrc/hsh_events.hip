@@ -19,9 +19,13 @@ namespace hsh {
 namespace {
 
 // SPIKE_OP (hines_chip.c:266-276) for every spike source, and the clock:
-// SimulationTime() of this step, advanced by dt (sim/sim_step.c:112).
+// SimulationTime() of this step, advanced by dt (sim/sim_step.c:112).  A
+// spike adds each of its connections (the in-connection slot) to its target
+// synchan's arrivals, which the synchan takes in slot order when its SYN2_OP
+// runs.
 __global__ __launch_bounds__(256) void k_ev_spikes(DevModel d, const SpikeSrc* __restrict__ src, int nsrc,
-                                                   EvCtl* ctl, int* __restrict__ sflag)
+                                                   EvCtl* ctl, const int* __restrict__ out_slot,
+                                                   const int* __restrict__ out_sid)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) {
@@ -30,22 +34,38 @@ __global__ __launch_bounds__(256) void k_ev_spikes(DevModel d, const SpikeSrc* _
         ctl->tnow = t;
         ctl->tnext = t + d.dt;
     }
-    if (i >= nsrc) return;
-    const SpikeSrc s = src[i];
-    const int step = d.step[d.blocks[s.vpos / W].tile];
-    int cnt = d.I[s.cpos] - 1;
-    int fired = 0;
-    if (d.V[s.vpos] > d.S[s.spos] && cnt <= 0) {
-        const int q = atomicAdd(d.nspk, 1);
-        if (q < d.spk_cap) {
-            d.spk[2 * q] = s.opidx;
-            d.spk[2 * q + 1] = step;
+    // every lane of the wave takes part in the scatter below
+    const bool live = i < nsrc;
+    SpikeSrc s{};
+    bool fired = false;
+    if (live) {
+        s = src[i];
+        const int step = d.step[d.blocks[s.vpos / W].tile];
+        int cnt = d.I[s.cpos] - 1;
+        if (d.V[s.vpos] > d.S[s.spos] && cnt <= 0) {
+            const int q = atomicAdd(d.nspk, 1);
+            if (q < d.spk_cap) {
+                d.spk[2 * q] = s.opidx;
+                d.spk[2 * q + 1] = step;
+            }
+            cnt = d.I[s.rpos];
+            fired = true;
         }
-        cnt = d.I[s.rpos];
-        fired = 1;
+        d.I[s.cpos] = cnt;
     }
-    d.I[s.cpos] = cnt;
-    sflag[i] = fired;
+    // the wave scatters each spiking lane's connections together
+    const int lane = threadIdx.x & (W - 1);
+    unsigned long long m = __ballot(fired);
+    while (m) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const int o0 = __shfl(s.out0, l), o1 = __shfl(s.out1, l);
+        for (int o = o0 + lane; o < o1; o += W) {
+            const int sid = out_sid[o];
+            const int k = atomicAdd(d.narr + sid, 1);
+            d.arr[d.in_off[sid] + k] = out_slot[o];
+        }
+    }
 }
 
 // Events posted from the host (hsh_post_events), one thread per synchan:
@@ -81,10 +101,11 @@ __global__ __launch_bounds__(256) void k_ev_post(DevModel d, EvPost p, EvCtl* ct
 
 }  // namespace
 
-hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, int* sflag, hipStream_t s)
+hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, const int* out_slot,
+                            const int* out_sid, hipStream_t s)
 {
     const int n = nsrc > 0 ? nsrc : 1;
-    hipLaunchKernelGGL(k_ev_spikes, dim3((n + 255) / 256), dim3(256), 0, s, d, src, nsrc, ctl, sflag);
+    hipLaunchKernelGGL(k_ev_spikes, dim3((n + 255) / 256), dim3(256), 0, s, d, src, nsrc, ctl, out_slot, out_sid);
     return hipGetLastError();
 }
 

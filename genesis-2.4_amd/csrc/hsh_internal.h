@@ -143,6 +143,7 @@ struct SpikeSrc {
     int cpos;             // I positions of its counter and reload
     int rpos;
     int opidx;            // ops index of the SPIKE_OP
+    int out0, out1;       // its out-connections: out_slot / out_sid [out0, out1)
     int pad;
 };
 
@@ -155,7 +156,8 @@ struct EvPost {
     int nsyn;
 };
 
-hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, int* sflag, hipStream_t s);
+hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, const int* out_slot,
+                            const int* out_sid, hipStream_t s);
 hipError_t launch_ev_post(const DevModel& d, const EvPost& p, EvCtl* ctl, hipStream_t s);
 
 // kernel launchers (hsh_kernels.hip); jm may be null or not ok
