@@ -109,6 +109,9 @@ struct hsh_handle {
         double wdt;
     };
     std::vector<Post> posts;           // posted since the last step, applied before the next
+    std::vector<double> stab_host;     // stablist (message activations are scaled by it)
+    InEntry* d_inputs = nullptr;       // staging of hsh_set_inputs / hsh_set_chip entries
+    int inputs_cap = 0;
     int max_k = 1 << 30;               // batched steps per launch allowed (shortest connection - 1)
     int nconn = 0;
     double t_next = 0.0;               // SimulationTime() of the next step
@@ -344,9 +347,11 @@ void free_events(hsh_handle* h)
 void free_device(hsh_handle* h)
 {
     free_layout(h);
-    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_stab, h->d_spk, h->d_nspk, h->d_giv};
+    void* ptrs[] = {h->d_prog, h->d_tab, h->d_xvals, h->d_stab, h->d_spk, h->d_nspk, h->d_giv, h->d_inputs};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    h->d_inputs = nullptr;
+    h->inputs_cap = 0;
     free_events(h);
     h->d_prog = nullptr;
     h->d_tab = h->d_xvals = h->d_stab = nullptr;
@@ -1278,6 +1283,7 @@ hsh_status create_impl(const hsh_model* m, const hsh_options* opt, hsh_handle** 
     if (opt) h->opt = *opt;
     h->ops.assign(m->ops, m->ops + m->nops);
     h->chip.assign(m->chip, m->chip + m->nchips);
+    if (m->nstab > 0 && m->stablist) h->stab_host.assign(m->stablist, m->stablist + 4 * (size_t)m->nstab);
     if (m->nconcs > 0) h->chip.insert(h->chip.end(), m->conc, m->conc + m->nconcs);
     if (m->nconcchip > 0) h->chip.insert(h->chip.end(), m->concchip, m->concchip + m->nconcchip);
     if (m->ngivals > 0 && m->givals) h->chip.insert(h->chip.end(), m->givals, m->givals + m->ngivals);
@@ -1716,22 +1722,92 @@ hsh_status hsh_put_conc(hsh_handle* h, const double* conc)
     return upload_state(h, nullptr);
 }
 
+}  // extern "C"
+
+namespace {
+
+// The device entries (position, set or add, value) of message inputs, in
+// position order and the caller's order within a position.
+hsh_status input_entries(hsh_handle* h, int32_t n, const hsh_input* in, std::vector<InEntry>& out, const char* who)
+{
+    struct Tagged {
+        InEntry e;
+        int32_t seq;
+    };
+    std::vector<Tagged> v;
+    const int32_t nchips = h->m.nchips;
+    for (int32_t i = 0; i < n; i++) {
+        const hsh_input& e = in[i];
+        if (e.op == HSH_IN_INJECT || e.op == HSH_IN_BASE || e.op == HSH_IN_THRESH) {
+            if (e.index < 0 || e.index >= nchips)
+                return fail(HSH_ERR_ARG, std::string(who) + ": chip index out of range");
+            h->chip[e.index] = e.value;
+            for (int64_t q = h->c2s_off[e.index]; q < h->c2s_off[e.index + 1]; q++)
+                v.push_back(Tagged{InEntry{(int)h->c2s[q], 0, e.value}, (int32_t)v.size()});
+        } else if (e.op == HSH_IN_ACTIVATION) {
+            if (e.index < 0 || e.index >= nchips || h->c2s_off[e.index + 1] - h->c2s_off[e.index] != 1)
+                return fail(HSH_ERR_ARG, std::string(who) + ": activation index is not a synchan's X");
+            if (e.stab < 0 || (size_t)e.stab + 3 >= h->stab_host.size())
+                return fail(HSH_ERR_ARG, std::string(who) + ": stablist index out of range");
+            if (!(e.value > 1.0e-10)) continue;          // VTINY (hines_defs.h:115)
+            const double a = e.per_dt ? e.value * h->stab_host[e.stab + 3] * h->m.dt
+                                      : e.value * h->stab_host[e.stab + 3];
+            v.push_back(Tagged{InEntry{(int)h->c2s[h->c2s_off[e.index]], 1, a}, (int32_t)v.size()});
+        } else if (e.op == HSH_IN_FLUX) {
+            // CONC_VAL_OP clears the pool's flux register at the start of
+            // every program that feeds it (hines_chip.c:205-212), and the
+            // device runs only pools such a program feeds: the value is
+            // overwritten before anything reads it
+            if (e.index < 0 || e.index >= std::max(0, h->m.nconcs))
+                return fail(HSH_ERR_ARG, std::string(who) + ": pool index out of range");
+        } else {
+            return fail(HSH_ERR_ARG, std::string(who) + ": unknown input op " + std::to_string(e.op));
+        }
+    }
+    std::stable_sort(v.begin(), v.end(), [](const Tagged& a, const Tagged& b) { return a.e.pos < b.e.pos; });
+    out.resize(v.size());
+    for (size_t i = 0; i < v.size(); i++) out[i] = v[i].e;
+    return HSH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hsh_status hsh_set_inputs(hsh_handle* h, int32_t n, const hsh_input* in)
+{
+    if (!h || (n > 0 && !in) || n < 0) return fail(HSH_ERR_ARG, "hsh_set_inputs: bad arguments");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_set_inputs: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_inputs: handle refused the model");
+    std::vector<InEntry> e;
+    STCHK(input_entries(h, n, in, e, "hsh_set_inputs"));
+    if (e.empty()) return HSH_OK;
+    if ((int)e.size() > h->inputs_cap) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (h->d_inputs) HIPCHK(hipFree(h->d_inputs));
+        h->d_inputs = nullptr;
+        h->inputs_cap = 0;
+        int cap = 256;
+        while (cap < (int)e.size()) cap *= 2;
+        HIPCHK(dalloc(&h->d_inputs, (size_t)cap));
+        h->inputs_cap = cap;
+    }
+    // the staging buffer is reused by the next call: the copy is ordered on
+    // the handle's stream after the last kernel that read it
+    HIPCHK(hipMemcpyAsync(h->d_inputs, e.data(), e.size() * sizeof(InEntry), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(launch_inputs(h->d_S, h->d_inputs, (int)e.size(), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return HSH_OK;
+}
+
 hsh_status hsh_set_chip(hsh_handle* h, int32_t n, const int32_t* idx, const double* val)
 {
     if (!h || (n > 0 && (!idx || !val))) return fail(HSH_ERR_ARG, "hsh_set_chip: null argument");
-    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_set_chip: host-only handle");
-    if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_chip: handle refused the model");
-    std::vector<std::pair<int64_t, double>> w;
-    for (int32_t i = 0; i < n; i++) {
-        int32_t c = idx[i];
-        if (c < 0 || c >= h->m.nchips) return fail(HSH_ERR_ARG, "hsh_set_chip: chip index out of range");
-        h->chip[c] = val[i];
-        for (int64_t q = h->c2s_off[c]; q < h->c2s_off[c + 1]; q++) w.emplace_back(h->c2s[q], val[i]);
-    }
-    HIPCHK(hipStreamSynchronize(h->stream));
-    for (auto& e : w)
-        HIPCHK(hipMemcpy(h->d_S + e.first, &e.second, sizeof(double), hipMemcpyHostToDevice));
-    return HSH_OK;
+    std::vector<hsh_input> in((size_t)std::max(0, n));
+    for (int32_t i = 0; i < n; i++) in[i] = hsh_input{HSH_IN_INJECT, idx[i], 0, 0, val[i]};
+    hsh_status st = hsh_set_inputs(h, n, in.data());
+    if (st != HSH_OK) g_err = "hsh_set_chip: " + g_err;
+    return st;
 }
 
 hsh_status hsh_drain_spikes(hsh_handle* h, int32_t* opidx, int32_t* step, int32_t cap, int32_t* n)

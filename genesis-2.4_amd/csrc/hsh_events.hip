@@ -99,7 +99,26 @@ __global__ __launch_bounds__(256) void k_ev_post(DevModel d, EvPost p, EvCtl* ct
     if (full) atomicOr(&ctl->err, EV_ERR_FULL);
 }
 
+// Message inputs: the first entry of each position applies that position's
+// entries in order (a later set overwrites, adds accumulate in order).
+__global__ __launch_bounds__(256) void k_inputs(double* __restrict__ S, const InEntry* __restrict__ e, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (i > 0 && e[i - 1].pos == e[i].pos)) return;
+    const int pos = e[i].pos;
+    double v = S[pos];
+    for (int j = i; j < n && e[j].pos == pos; j++) v = e[j].mode ? v + e[j].value : e[j].value;
+    S[pos] = v;
+}
+
 }  // namespace
+
+hipError_t launch_inputs(double* S, const InEntry* e, int n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_inputs, dim3((n + 255) / 256), dim3(256), 0, s, S, e, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, const int* out_slot,
                             const int* out_sid, hipStream_t s)

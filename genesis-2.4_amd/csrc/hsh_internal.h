@@ -160,6 +160,16 @@ hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, Ev
                             const int* out_sid, hipStream_t s);
 hipError_t launch_ev_post(const DevModel& d, const EvPost& p, EvCtl* ctl, hipStream_t s);
 
+// A message input applied to a device slot before a step (h_in_msgs,
+// hines_msgs.c:351-405): mode 0 sets S[pos] = value, mode 1 adds value.
+// Entries are sorted by position, in the caller's order within one.
+struct InEntry {
+    int pos;
+    int mode;
+    double value;
+};
+hipError_t launch_inputs(double* S, const InEntry* e, int n, hipStream_t s);
+
 // kernel launchers (hsh_kernels.hip); jm may be null or not ok
 hipError_t launch_channel(const DevModel& d, const JitModule* jm, hipStream_t s);
 hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s);

@@ -21,6 +21,19 @@ _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 
 HSH_OK, HSH_REFUSED = 0, 1
+# hsh_input (message inputs of hsh_set_inputs) and its op codes
+IN_INJECT, IN_BASE, IN_THRESH, IN_FLUX, IN_ACTIVATION = 1, 2, 3, 3300, 4100
+INPUT_DTYPE = np.dtype([("op", np.int32), ("index", np.int32), ("stab", np.int32), ("per_dt", np.int32),
+                        ("value", np.float64)])
+
+
+def input_array(inputs):
+    """(op, index, stab, per_dt, value) tuples as an hsh_input array."""
+    a = np.zeros(len(inputs), INPUT_DTYPE)
+    for i, (op, index, stab, per_dt, value) in enumerate(inputs):
+        a[i] = (op, index, stab, per_dt, value)
+    return a
+
 PLAIN, BATCHED, CHANNELS = 0, 1, 2
 T_TOTAL, T_CHANNEL, T_ELIM = 0, 1, 2
 F_NO_AUTOTUNE = 1
@@ -84,6 +97,7 @@ class HshInfo(C.Structure):
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
            "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_create_group", "hsh_group_member", "hsh_get_givals", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
            "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_post_events",
+           "hsh_set_inputs",
            "hsh_set_time", "hsh_set_ravals",
            "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
@@ -125,6 +139,8 @@ def lib():
         L.hsh_set_synapses.restype = C.c_int32
         L.hsh_post_events.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, C.c_int32]
         L.hsh_post_events.restype = C.c_int32
+        L.hsh_set_inputs.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.hsh_set_inputs.restype = C.c_int32
         L.hsh_set_time.argtypes = [C.c_void_p, C.c_double]
         L.hsh_set_time.restype = C.c_int32
         L.hsh_set_ravals.argtypes = [C.c_void_p, _f64p]
@@ -311,6 +327,12 @@ class HipSolver:
                                                                      (weight, np.float64))]
         _check(lib().hsh_post_events(self.h, a[0].size, _p(a[0], C.c_int32), _p(a[1], C.c_int32),
                                      _p(a[2], C.c_double), 1 if after else 0), "hsh_post_events")
+
+    def set_inputs(self, inputs):
+        """h_in_msgs inputs before the next step (hsh_set_inputs): a
+        sequence of (op, index, stab, per_dt, value)."""
+        a = input_array(inputs)
+        _check(lib().hsh_set_inputs(self.h, a.size, a.ctypes.data), "hsh_set_inputs")
 
     def set_time(self, t):
         """SimulationTime() of the next step (hsh_set_time)."""

@@ -20,7 +20,8 @@
  *   hsh_get_state    cuda_backend_sync_chip (cuda_backend.cu:529-538) /
  *                    cuda_sync_chip (cuda_hsolve.h:24)
  *   hsh_put_state    cuda_backend_upload_chip (cuda_backend.cu:318-336)
- *   hsh_set_chip     per-step chip writes of h_in_msgs (hines_msgs.c:351-405)
+ *   hsh_set_inputs   h_in_msgs (hines_msgs.c:351-405): message inputs before a step
+ *   hsh_set_chip     its chip overwrites alone
  *   hsh_destroy      cuda_backend_cleanup (cuda_backend.cu:540-568)
  *   hsh_autotune     no counterpart: the reference multiloop launches one fixed
  *                    shape (ceil(n_trees/64) x 64, cuda_backend.cu:461-527)
@@ -239,7 +240,34 @@ hsh_status hsh_put_conc(struct hsh_handle *h, const double *conc);
 /* givals[ngivals] as the last step wrote it (HSH_ERR_STATE if the model was
  * created without givals). */
 hsh_status hsh_get_givals(struct hsh_handle *h, double *givals);
-/* Overwrite n chip[] values (reference indices), e.g. INJECT messages. */
+/* Message inputs, the h_in_msgs step of PROCESS (hines_msgs.c:351-405),
+ * applied to the device state before the next step, in the order given (the
+ * reference's ininfo walk).  op and index as the reference's ininfo entry:
+ *   HSH_IN_INJECT / HSH_IN_BASE / HSH_IN_THRESH: chip[index] = value;
+ *   HSH_IN_ACTIVATION: an ACTIVATION message to the synchan whose X is
+ *     chip[index], of synapse type stab (the stablist index its SYN2_OP
+ *     holds): if value > VTINY (1e-10), X += value*stablist[stab+3], times
+ *     dt when per_dt (a synchan's ACTIVATION, not a channelC2/C3's);
+ *   HSH_IN_FLUX: flux[index] = value -- accepted and without effect: in
+ *     chanmodes 4 and 5 CONC_VAL_OP clears the pool's flux at the start of
+ *     every program that feeds it (hines_chip.c:205-212), and the device
+ *     runs only pools such a program feeds.
+ * Indices are those of the handle's arrays (a group's merged arrays). */
+#define HSH_IN_INJECT     1     /* INJECT_OP (hines_defs.h:225) */
+#define HSH_IN_BASE       2     /* BASE_OP */
+#define HSH_IN_THRESH     3     /* THRESH_OP */
+#define HSH_IN_FLUX       3300  /* FLUX_OP */
+#define HSH_IN_ACTIVATION 4100  /* SYN2_OP */
+typedef struct hsh_input {
+    int32_t op;
+    int32_t index;
+    int32_t stab;
+    int32_t per_dt;
+    double  value;
+} hsh_input;
+hsh_status hsh_set_inputs(struct hsh_handle *h, int32_t n, const hsh_input *in);
+/* Overwrite n chip[] values (reference indices), e.g. INJECT messages
+ * (hsh_set_inputs with HSH_IN_INJECT). */
 hsh_status hsh_set_chip(struct hsh_handle *h, int32_t n, const int32_t *idx, const double *val);
 /* Spike events raised by SPIKE_OP since the last drain: ops[] index of the
  * SPIKE_OP and the step number. */

@@ -331,6 +331,39 @@ int32_t ohs_post_event(ohs *h, int32_t syn_op, int32_t time, double weight, int3
     return 0;
 }
 
+/* h_in_msgs (hines_msgs.c:351-405).  The activation is scaled by the full
+ * dt even under Crank-Nicolson (the local dt = dt/2 is not used there). */
+int32_t ohs_in_msgs(ohs *h, int32_t n, const ohs_input *in)
+{
+    for (int32_t i = 0; i < n; i++) {
+        const ohs_input *e = &in[i];
+        switch (e->op) {
+        case 1:   /* INJECT_OP */
+        case 2:   /* BASE_OP */
+        case 3:   /* THRESH_OP */
+            if (e->index < 0 || e->index >= h->a.nchips) return -1;
+            h->chip[e->index] = e->value;
+            break;
+        case SYN2_OP:
+            if (e->index < 0 || e->index >= h->a.nchips || e->stab < 0 || e->stab + 3 >= 4 * h->a.nstab) return -1;
+            if (e->value > 1.0e-10) {            /* VTINY, hines_defs.h:115 */
+                if (e->per_dt)
+                    h->chip[e->index] += e->value * h->a.stablist[e->stab + 3] * h->a.dt;
+                else
+                    h->chip[e->index] += e->value * h->a.stablist[e->stab + 3];
+            }
+            break;
+        case FLUX_OP:
+            if (e->index < 0 || e->index >= h->a.nconcs || !h->flux) return -1;
+            h->flux[e->index] = e->value;
+            break;
+        default:
+            return -1;
+        }
+    }
+    return 0;
+}
+
 void ohs_set_time(ohs *h, double t) { h->simtime = t; }
 double ohs_time(ohs *h) { return h->simtime; }
 

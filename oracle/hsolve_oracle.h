@@ -115,6 +115,19 @@ int32_t ohs_set_synapses(ohs *h, int32_t n, const int32_t *src_op, const int32_t
  * the simulation time of the next step, before its ops; after = 1: at the
  * time of the last step run, after its ops.  Returns 0 or -1. */
 int32_t ohs_post_event(ohs *h, int32_t syn_op, int32_t time, double weight, int32_t after);
+/* One message input of h_in_msgs (hines_msgs.c:351-405): op is INJECT_OP
+ * (1), BASE_OP (2), THRESH_OP (3) -- chip[index] = value -- SYN2_OP (4100)
+ * -- an ACTIVATION to the synchan whose X is chip[index]: when value >
+ * VTINY, chip[index] += value*stablist[stab+3], times dt for a synchan
+ * (not a channelC2/C3) -- or FLUX_OP (3300): flux[index] = value. */
+typedef struct ohs_input {
+    int32_t op, index, stab, per_dt;
+    double value;
+} ohs_input;
+/* Apply inputs in order, as h_in_msgs walks its ininfo list before the
+ * chip update.  Returns 0 or -1 on an unknown op or index. */
+int32_t ohs_in_msgs(ohs *h, int32_t n, const ohs_input *in);
+
 /* SimulationTime() of the next step (0 at creation); it grows by dt per
  * step (simulation_time += ClockValue(0), sim/sim_step.c:112). */
 void    ohs_set_time(ohs *h, double t);

@@ -34,6 +34,18 @@ RANDFN = C.CFUNCTYPE(C.c_double, C.c_void_p)
 
 _lib = None
 
+INPUT_DTYPE = np.dtype([("op", np.int32), ("index", np.int32), ("stab", np.int32), ("per_dt", np.int32),
+                        ("value", np.float64)])
+
+
+def input_array(inputs):
+    """h_in_msgs inputs (op, index, stab, per_dt, value) as the C struct
+    array ohs_in_msgs and hsh_set_inputs take."""
+    a = np.zeros(len(inputs), INPUT_DTYPE)
+    for i, (op, index, stab, per_dt, value) in enumerate(inputs):
+        a[i] = (op, index, stab, per_dt, value)
+    return a
+
 
 def oracle_lib():
     global _lib
@@ -72,6 +84,8 @@ def oracle_lib():
         L.ohs_set_synapses.restype = C.c_int32
         L.ohs_post_event.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_int32]
         L.ohs_post_event.restype = C.c_int32
+        L.ohs_in_msgs.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.ohs_in_msgs.restype = C.c_int32
         L.ohs_set_time.argtypes = [C.c_void_p, C.c_double]
         L.ohs_time.argtypes = [C.c_void_p]
         L.ohs_time.restype = C.c_double
@@ -242,6 +256,13 @@ class Oracle:
         """An event from outside the solver (see ohs_post_event)."""
         if oracle_lib().ohs_post_event(self.h, int(syn_op), int(time), float(weight), 1 if after else 0) != 0:
             raise ValueError("oracle: bad posted event")
+
+    def in_msgs(self, inputs):
+        """Apply h_in_msgs inputs: a sequence of (op, index, stab, per_dt,
+        value) (see ohs_in_msgs)."""
+        a = input_array(inputs)
+        if oracle_lib().ohs_in_msgs(self.h, a.size, a.ctypes.data) != 0:
+            raise ValueError("oracle: bad input")
 
     def set_time(self, t):
         oracle_lib().ohs_set_time(self.h, float(t))
