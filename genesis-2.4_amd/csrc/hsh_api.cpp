@@ -112,6 +112,13 @@ struct hsh_handle {
     std::vector<double> stab_host;     // stablist (message activations are scaled by it)
     InEntry* d_inputs = nullptr;       // staging of hsh_set_inputs / hsh_set_chip entries
     int inputs_cap = 0;
+    // host staging of hsh_set_inputs_static, sized at creation
+    struct InTagged {
+        InEntry e;
+        int32_t seq;
+    };
+    std::vector<InTagged> in_tag;
+    std::vector<InEntry> in_host;
     int max_k = 1 << 30;               // batched steps per launch allowed (shortest connection - 1)
     int nconn = 0;
     double t_next = 0.0;               // SimulationTime() of the next step
@@ -1352,6 +1359,11 @@ hsh_status create_impl(const hsh_model* m, const hsh_options* opt, hsh_handle** 
                                                          " failed: " + hipGetErrorString(e_))); \
     } while (0)
     HIPCHK_C(hipSetDevice(h->device));
+    // hsh_set_inputs_static: the static staging array's address on this
+    // device, looked up once per device with the device current
+    HIPCHK_C(inputs_static_setup(h->device));
+    h->in_tag.resize(HSH_INPUTS_STATIC_MAX);
+    h->in_host.resize(HSH_INPUTS_STATIC_MAX);
     HIPCHK_C(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
     HIPCHK_C(hipEventCreate(&h->ev_a));
     HIPCHK_C(hipEventCreate(&h->ev_b));
@@ -1727,46 +1739,59 @@ hsh_status hsh_put_conc(hsh_handle* h, const double* conc)
 namespace {
 
 // The device entries (position, set or add, value) of message inputs, in
-// position order and the caller's order within a position.
-hsh_status input_entries(hsh_handle* h, int32_t n, const hsh_input* in, std::vector<InEntry>& out, const char* who)
+// position order and the caller's order within a position, into buf
+// (capacity cap; *cnt entries).  More than cap is HSH_ERR_LIMIT before
+// anything changes.
+hsh_status input_entries(hsh_handle* h, int32_t n, const hsh_input* in, hsh_handle::InTagged* buf, size_t cap,
+                         size_t* cnt, const char* who)
 {
-    struct Tagged {
-        InEntry e;
-        int32_t seq;
-    };
-    std::vector<Tagged> v;
     const int32_t nchips = h->m.nchips;
+    size_t need = 0;
     for (int32_t i = 0; i < n; i++) {
         const hsh_input& e = in[i];
         if (e.op == HSH_IN_INJECT || e.op == HSH_IN_BASE || e.op == HSH_IN_THRESH) {
             if (e.index < 0 || e.index >= nchips)
                 return fail(HSH_ERR_ARG, std::string(who) + ": chip index out of range");
-            h->chip[e.index] = e.value;
-            for (int64_t q = h->c2s_off[e.index]; q < h->c2s_off[e.index + 1]; q++)
-                v.push_back(Tagged{InEntry{(int)h->c2s[q], 0, e.value}, (int32_t)v.size()});
+            need += (size_t)(h->c2s_off[e.index + 1] - h->c2s_off[e.index]);
         } else if (e.op == HSH_IN_ACTIVATION) {
             if (e.index < 0 || e.index >= nchips || h->c2s_off[e.index + 1] - h->c2s_off[e.index] != 1)
                 return fail(HSH_ERR_ARG, std::string(who) + ": activation index is not a synchan's X");
             if (e.stab < 0 || (size_t)e.stab + 3 >= h->stab_host.size())
                 return fail(HSH_ERR_ARG, std::string(who) + ": stablist index out of range");
-            if (!(e.value > 1.0e-10)) continue;          // VTINY (hines_defs.h:115)
-            const double a = e.per_dt ? e.value * h->stab_host[e.stab + 3] * h->m.dt
-                                      : e.value * h->stab_host[e.stab + 3];
-            v.push_back(Tagged{InEntry{(int)h->c2s[h->c2s_off[e.index]], 1, a}, (int32_t)v.size()});
+            need += 1;
         } else if (e.op == HSH_IN_FLUX) {
-            // CONC_VAL_OP clears the pool's flux register at the start of
-            // every program that feeds it (hines_chip.c:205-212), and the
-            // device runs only pools such a program feeds: the value is
-            // overwritten before anything reads it
             if (e.index < 0 || e.index >= std::max(0, h->m.nconcs))
                 return fail(HSH_ERR_ARG, std::string(who) + ": pool index out of range");
         } else {
             return fail(HSH_ERR_ARG, std::string(who) + ": unknown input op " + std::to_string(e.op));
         }
     }
-    std::stable_sort(v.begin(), v.end(), [](const Tagged& a, const Tagged& b) { return a.e.pos < b.e.pos; });
-    out.resize(v.size());
-    for (size_t i = 0; i < v.size(); i++) out[i] = v[i].e;
+    if (need > cap)
+        return fail(HSH_ERR_LIMIT, std::string(who) + ": " + std::to_string(need) + " device entries, more than " +
+                                       std::to_string(cap));
+    size_t k = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const hsh_input& e = in[i];
+        if (e.op == HSH_IN_INJECT || e.op == HSH_IN_BASE || e.op == HSH_IN_THRESH) {
+            h->chip[e.index] = e.value;
+            for (int64_t q = h->c2s_off[e.index]; q < h->c2s_off[e.index + 1]; q++, k++)
+                buf[k] = hsh_handle::InTagged{InEntry{(int)h->c2s[q], 0, e.value}, (int32_t)k};
+        } else if (e.op == HSH_IN_ACTIVATION) {
+            if (!(e.value > 1.0e-10)) continue;          // VTINY (hines_defs.h:115)
+            const double a = e.per_dt ? e.value * h->stab_host[e.stab + 3] * h->m.dt
+                                      : e.value * h->stab_host[e.stab + 3];
+            buf[k] = hsh_handle::InTagged{InEntry{(int)h->c2s[h->c2s_off[e.index]], 1, a}, (int32_t)k};
+            k++;
+        }
+        // HSH_IN_FLUX: CONC_VAL_OP clears the pool's flux register at the
+        // start of every program that feeds it (hines_chip.c:205-212), and
+        // the device runs only pools such a program feeds: the value is
+        // overwritten before anything reads it
+    }
+    std::sort(buf, buf + k, [](const hsh_handle::InTagged& a, const hsh_handle::InTagged& b) {
+        return a.e.pos != b.e.pos ? a.e.pos < b.e.pos : a.seq < b.seq;
+    });
+    *cnt = k;
     return HSH_OK;
 }
 
@@ -1779,23 +1804,50 @@ hsh_status hsh_set_inputs(hsh_handle* h, int32_t n, const hsh_input* in)
     if (!h || (n > 0 && !in) || n < 0) return fail(HSH_ERR_ARG, "hsh_set_inputs: bad arguments");
     if (h->host_only) return fail(HSH_ERR_STATE, "hsh_set_inputs: host-only handle");
     if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_inputs: handle refused the model");
-    std::vector<InEntry> e;
-    STCHK(input_entries(h, n, in, e, "hsh_set_inputs"));
-    if (e.empty()) return HSH_OK;
-    if ((int)e.size() > h->inputs_cap) {
+    size_t need = 0;
+    for (int32_t i = 0; i < n; i++) {
+        const hsh_input& e = in[i];
+        const bool set = e.op == HSH_IN_INJECT || e.op == HSH_IN_BASE || e.op == HSH_IN_THRESH;
+        if (set && e.index >= 0 && e.index < h->m.nchips) need += (size_t)(h->c2s_off[e.index + 1] - h->c2s_off[e.index]);
+        else need += 1;
+    }
+    std::vector<hsh_handle::InTagged> tag(std::max<size_t>(1, need));
+    size_t cnt = 0;
+    STCHK(input_entries(h, n, in, tag.data(), tag.size(), &cnt, "hsh_set_inputs"));
+    if (cnt == 0) return HSH_OK;
+    std::vector<InEntry> e(cnt);
+    for (size_t i = 0; i < cnt; i++) e[i] = tag[i].e;
+    if ((int)cnt > h->inputs_cap) {
         HIPCHK(hipStreamSynchronize(h->stream));
         if (h->d_inputs) HIPCHK(hipFree(h->d_inputs));
         h->d_inputs = nullptr;
         h->inputs_cap = 0;
         int cap = 256;
-        while (cap < (int)e.size()) cap *= 2;
+        while (cap < (int)cnt) cap *= 2;
         HIPCHK(dalloc(&h->d_inputs, (size_t)cap));
         h->inputs_cap = cap;
     }
-    // the staging buffer is reused by the next call: the copy is ordered on
-    // the handle's stream after the last kernel that read it
-    HIPCHK(hipMemcpyAsync(h->d_inputs, e.data(), e.size() * sizeof(InEntry), hipMemcpyHostToDevice, h->stream));
-    HIPCHK(launch_inputs(h->d_S, h->d_inputs, (int)e.size(), h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_inputs, e.data(), cnt * sizeof(InEntry), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(launch_inputs(h->d_S, h->d_inputs, (int)cnt, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return HSH_OK;
+}
+
+hsh_status hsh_set_inputs_static(hsh_handle* h, int32_t n, const hsh_input* in)
+{
+    if (!h || (n > 0 && !in) || n < 0) return fail(HSH_ERR_ARG, "hsh_set_inputs_static: bad arguments");
+    if (h->host_only) return fail(HSH_ERR_STATE, "hsh_set_inputs_static: host-only handle");
+    if (h->refused) return fail(HSH_ERR_STATE, "hsh_set_inputs_static: handle refused the model");
+    if (n > HSH_INPUTS_STATIC_MAX)
+        return fail(HSH_ERR_LIMIT, "hsh_set_inputs_static: more than HSH_INPUTS_STATIC_MAX inputs");
+    InEntry* scr = inputs_static_ptr(h->device);
+    if (!scr) return fail(HSH_ERR_STATE, "hsh_set_inputs_static: static staging not set up on this device");
+    size_t cnt = 0;
+    STCHK(input_entries(h, n, in, h->in_tag.data(), h->in_tag.size(), &cnt, "hsh_set_inputs_static"));
+    if (cnt == 0) return HSH_OK;
+    for (size_t i = 0; i < cnt; i++) h->in_host[i] = h->in_tag[i].e;
+    HIPCHK(hipMemcpyAsync(scr, h->in_host.data(), cnt * sizeof(InEntry), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(launch_inputs(h->d_S, scr, (int)cnt, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return HSH_OK;
 }

@@ -15,8 +15,22 @@
 #include "hsh_internal.h"
 #include "hsh_device.h"
 
+#include <mutex>
+
 namespace hsh {
 namespace {
+
+// Staging of hsh_set_inputs_static: device memory of fixed size with static
+// storage, so that entry point allocates nothing.  Each GPU holds its own
+// copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (scratch_ptr).  Every
+// call on a GPU shares that GPU's array; a call reads only the entries it
+// has just written, so nothing in it needs clearing.
+static __device__ __align__(256) InEntry scratch[HSH_INPUTS_STATIC_MAX];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex scratch_mu;
+InEntry* scratch_ptr[MAX_DEVICES] = {};
 
 // SPIKE_OP (hines_chip.c:266-276) for every spike source, and the clock:
 // SimulationTime() of this step, advanced by dt (sim/sim_step.c:112).  A
@@ -112,6 +126,26 @@ __global__ __launch_bounds__(256) void k_inputs(double* __restrict__ S, const In
 }
 
 }  // namespace
+
+hipError_t inputs_static_setup(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (scratch_ptr[device]) return hipSuccess;
+    // the caller has made `device` current
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    scratch_ptr[device] = static_cast<InEntry*>(p);
+    return hipSuccess;
+}
+
+InEntry* inputs_static_ptr(int device)
+{
+    if (device < 0 || device >= MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    return scratch_ptr[device];
+}
 
 hipError_t launch_inputs(double* S, const InEntry* e, int n, hipStream_t s)
 {

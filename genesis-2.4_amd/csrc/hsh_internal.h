@@ -169,6 +169,10 @@ struct InEntry {
     double value;
 };
 hipError_t launch_inputs(double* S, const InEntry* e, int n, hipStream_t s);
+// The static staging array of hsh_set_inputs_static: look up its address on
+// a device (with that device current; once per device), and get it.
+hipError_t inputs_static_setup(int device);
+InEntry* inputs_static_ptr(int device);
 
 // kernel launchers (hsh_kernels.hip); jm may be null or not ok
 hipError_t launch_channel(const DevModel& d, const JitModule* jm, hipStream_t s);

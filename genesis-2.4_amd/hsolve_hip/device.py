@@ -97,7 +97,7 @@ class HshInfo(C.Structure):
 EXPORTS = ("hsh_create", "hsh_refusals", "hsh_step", "hsh_sync", "hsh_get_state",
            "hsh_put_state", "hsh_get_conc", "hsh_put_conc", "hsh_create_group", "hsh_group_member", "hsh_get_givals", "hsh_set_chip", "hsh_drain_spikes", "hsh_info_get", "hsh_autotune",
            "hsh_last_time", "hsh_set_timing", "hsh_jit_source", "hsh_set_synapses", "hsh_post_events",
-           "hsh_set_inputs",
+           "hsh_set_inputs", "hsh_set_inputs_static",
            "hsh_set_time", "hsh_set_ravals",
            "hsh_adj_forward",
            "hsh_adj_backward", "hsh_destroy", "hsh_last_error")
@@ -141,6 +141,8 @@ def lib():
         L.hsh_post_events.restype = C.c_int32
         L.hsh_set_inputs.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
         L.hsh_set_inputs.restype = C.c_int32
+        L.hsh_set_inputs_static.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.hsh_set_inputs_static.restype = C.c_int32
         L.hsh_set_time.argtypes = [C.c_void_p, C.c_double]
         L.hsh_set_time.restype = C.c_int32
         L.hsh_set_ravals.argtypes = [C.c_void_p, _f64p]

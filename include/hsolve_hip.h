@@ -266,6 +266,18 @@ typedef struct hsh_input {
     double  value;
 } hsh_input;
 hsh_status hsh_set_inputs(struct hsh_handle *h, int32_t n, const hsh_input *in);
+/* hsh_set_inputs that allocates nothing: its device staging is a
+ * fixed-size array of static storage in the library (one per GPU, kept
+ * until the process ends) and its host staging belongs to the handle.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid it).  At most
+ * HSH_INPUTS_STATIC_MAX entries after expansion (an overwrite counts once
+ * per device slot holding the chip value); more is HSH_ERR_LIMIT with
+ * nothing applied.  Calls that overlap on one GPU (other threads, other
+ * handles on that GPU) share the array and race; a call returns only after
+ * its inputs are applied. */
+#define HSH_INPUTS_STATIC_MAX 16384
+hsh_status hsh_set_inputs_static(struct hsh_handle *h, int32_t n, const hsh_input *in);
 /* Overwrite n chip[] values (reference indices), e.g. INJECT messages
  * (hsh_set_inputs with HSH_IN_INJECT). */
 hsh_status hsh_set_chip(struct hsh_handle *h, int32_t n, const int32_t *idx, const double *val);
