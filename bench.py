@@ -443,6 +443,69 @@ def network_section(a, D, M, Oracle):
     return res
 
 
+def row_crossings(run, n, j, h):
+    """Steps at which the runs theta[n, j] +- h interpolate a different gate
+    table row in some compartment of neuron n (rows from the device
+    voltages, which are the oracle's bit for bit)."""
+    from hsolve_hip import fit as F
+    N = run.pmap.n_neurons
+    rows = np.nonzero(F.tree_of_row(run.pmap.base) == n)[0]
+    tp = np.ones((N, 5))
+    tp[n, j] += h
+    tm = np.ones((N, 5))
+    tm[n, j] -= h
+    return int(np.sum(np.any(run.table_rows(tp, rows) != run.table_rows(tm, rows), axis=1)))
+
+
+def fd_sweep(run, tgt, g, j, h, chunk=128):
+    """Central differences of every neuron's part of the loss at once
+    (theta[:, j] +- h for all neurons: neurons are independent) against the
+    adjoint g[:, j], split by whether the neuron's +-h runs interpolate the
+    same table rows at every step.  The quotient's rounding noise is measured
+    per neuron as |FD(h) - FD(1.1 h)| (the secant itself moves by O(h^2)
+    between the two on a crossing-free neuron): its bar is 1e-5 |g| plus
+    three times that noise, or the summation floor 10 eps L_n / h."""
+    import torch
+    from hsolve_hip import fit as F
+    N = run.pmap.n_neurons
+    nrn = F.tree_of_row(run.pmap.base)
+
+    def quotient(hh):
+        tp = np.ones((N, 5))
+        tp[:, j] += hh
+        tm = np.ones((N, 5))
+        tm[:, j] -= hh
+        with torch.no_grad():
+            trp = run(torch.from_numpy(tp)).numpy()
+            trm = run(torch.from_numpy(tm)).numpy()
+        q = (((trp - tgt) ** 2).sum(axis=0) - ((trm - tgt) ** 2).sum(axis=0)) / trp.size / (2 * hh)
+        cross = np.zeros(N, np.int64)
+        for n0 in range(0, N, chunk):
+            rows = np.nonzero((nrn >= n0) & (nrn < n0 + chunk))[0]
+            d = run.table_rows(tp, rows) != run.table_rows(tm, rows)
+            for n in range(n0, min(N, n0 + chunk)):
+                cross[n] = int(np.sum(np.any(d[:, nrn[rows] == n], axis=1)))
+        return q, cross
+
+    with torch.no_grad():
+        tr0 = run(torch.ones(N, 5, dtype=torch.float64)).numpy()
+    L_n = ((tr0 - tgt) ** 2).sum(axis=0) / tr0.size
+    fd, cross = quotient(h)
+    fd2, cross2 = quotient(1.1 * h)
+    gj = g[:, j]
+    rel = np.abs(gj - fd) / np.abs(gj)
+    floor = np.maximum(10 * np.finfo(float).eps * L_n / h, 3 * np.abs(fd - fd2)) / np.abs(gj)
+    free = (cross == 0) & (cross2 == 0)
+    ok = rel <= 1e-5 + floor
+    return {"param": F.PARAMS[j], "neurons": int(N), "crossing_free": int(free.sum()),
+            "crossing_free_within_bar": int((ok & free).sum()),
+            "crossing_free_within_1e-5": int(((rel <= 1e-5) & free).sum()),
+            "median_rel_crossing_free": float(np.median(rel[free])) if free.any() else None,
+            "max_rel_crossing_free_quiet": float(np.max(rel[free & (floor < 1e-5)], initial=0.0)),
+            "with_crossings": int((~free).sum()),
+            "max_rel_with_crossings": float(np.max(rel[~free], initial=0.0))}
+
+
 def adjoint_section(a, torch, Oracle, TangentLinear):
     """Adjoint parameter fit: 64-compartment random trees, soma-Vm MSE over
     the run against CPU-solver targets at hidden per-neuron gbar/Ra/Cm
@@ -498,7 +561,13 @@ def adjoint_section(a, torch, Oracle, TangentLinear):
                        "rel_vs_tl": abs(g[nn, j] - g_tl) / abs(g_tl),
                        "rel_vs_fd_h1e-6": abs(g[nn, j] - fd[1e-6]) / abs(g[nn, j]),
                        "rel_vs_fd_h1e-8": abs(g[nn, j] - fd[1e-8]) / abs(g[nn, j]),
+                       "row_crossings_h1e-6": row_crossings(run, nn, j, 1e-6),
+                       "row_crossings_h1e-8": row_crossings(run, nn, j, 1e-8),
                        "fd_rounding_floor_rel": {f"h{h:.0e}": floor[h] / abs(g[nn, j]) for h in fd}})
+    # every neuron at once: neurons are independent, so one pair of runs with
+    # parameter j of all neurons at +-h gives each neuron's own difference
+    # quotient of its part of the loss, and its row crossings
+    sweep = [fd_sweep(run, tgt.numpy(), g, j, 1e-6) for j in range(5)]
     run.close()
     return {"workload": f"{N} random Hines trees x 64 compartments, {T} steps, dt 20 us, fp64, checkpoint "
                         f"every 100 steps; loss = soma-Vm MSE vs CPU targets at hidden +-30% gNa/gK/gKdr/Ra/Cm",
@@ -507,9 +576,12 @@ def adjoint_section(a, torch, Oracle, TangentLinear):
             "forward_cs_per_s": T * n / t_fwd, "backward_cs_per_s": T * n / t_bwd,
             "gradient_checks": checks,
             "max_rel_vs_tl": max(c["rel_vs_tl"] for c in checks),
-            "note": "central differences at h=1e-6 move spike upstrokes of gNa/Cm across gate-table intervals "
-                    "(piecewise-linear rates); at h=1e-8 they measure the tabulated model the adjoint "
-                    "differentiates"}
+            "fd_sweep_h1e-6": sweep,
+            "note": "gate tables make the model piecewise linear in Vm: a central difference measures its "
+                    "derivative only when the +h and -h runs interpolate the same table rows at every step "
+                    "(row_crossings = steps where some compartment of the neuron does not).  fd_sweep: every "
+                    "neuron's quotient at h=1e-6 per parameter; the 1e-5 bar (plus the quotient's rounding "
+                    "floor 10 eps L_n/h) holds on the neurons without crossings"}
 
 
 def main():

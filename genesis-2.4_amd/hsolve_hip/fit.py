@@ -197,6 +197,24 @@ class Runner:
         self.generation += 1
         return s.adj_forward(self.nsteps, self.ckpt, self.obs)
 
+    def table_rows(self, theta, rows):
+        """The gate-table row (filo, hines_chip.c:174-190) every compartment
+        in ``rows`` interpolates at each step of a run at theta: the row of
+        its voltage at the start of the step, clamp(int((Vm - xmin)*invdx),
+        0, xdivs).  Shape (nsteps, len(rows)).  Invalidates a pending
+        backward pass (the run replaces the checkpoints)."""
+        b = self.pmap.base
+        rows = np.ascontiguousarray(rows, np.int32)
+        with torch.no_grad():
+            chip, rav = self.pmap.arrays(torch.as_tensor(theta, dtype=torch.float64))
+        s = self.solver
+        s.put_state(vm=b.vm, chip=chip.numpy(), ops=b.ops)
+        s.set_ravals(rav.numpy())
+        self.generation += 1
+        tr = s.adj_forward(self.nsteps, self.ckpt, rows)
+        V = np.vstack([b.vm[rows][None, :], tr[:-1]])
+        return np.clip(np.floor((V - b.xmin) * b.invdx), 0, b.xdivs).astype(np.int32)
+
     def __call__(self, theta: torch.Tensor) -> torch.Tensor:
         chip, rav = self.pmap.arrays(theta)
         return Trace.apply(chip, rav, self)

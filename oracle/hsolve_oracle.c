@@ -97,6 +97,7 @@ struct ohs {
     void *ctx;
     int32_t trace_on;
     uint64_t *trace;
+    int32_t *rows;             /* NEWVOLT table row of each compartment in the last chip update, or NULL */
     /* synaptic events: the synchans' pending lists (ohs_set_synapses,
      * ohs_post_event) and the simulation time they are timed with */
     int32_t nsyn;              /* synchans with a pending list; 0: none */
@@ -150,7 +151,7 @@ void ohs_destroy(ohs *h)
     if (!h) return;
     free(h->ops); free(h->chip); free(h->vm); free(h->results_base); free(h->ravals);
     free(h->givals); free(h->conc); free(h->flux); free(h->codiag); free(h->itotal);
-    free(h->spike_op); free(h->spike_step); free(h->trace);
+    free(h->spike_op); free(h->spike_step); free(h->trace); free(h->rows);
     free(h->conn_off); free(h->conn_dst); free(h->conn_T); free(h->conn_wdt);
     if (h->pend)
         for (int32_t i = 0; i < h->nsyn; i++) free(h->pend[i].ev);
@@ -164,6 +165,21 @@ void ohs_set_hooks(ohs *h, ohs_syn_fn syn, ohs_rand_fn rnd, void *ctx)
 }
 
 void ohs_trace_enable(ohs *h, int32_t on) { h->trace_on = on; }
+
+void ohs_rows_enable(ohs *h, int32_t on)
+{
+    free(h->rows);
+    h->rows = NULL;
+    if (on) {
+        h->rows = (int32_t *)malloc(sizeof(int32_t) * (size_t)h->a.ncompts);
+        for (int32_t i = 0; i < h->a.ncompts; i++) h->rows[i] = -1;
+    }
+}
+
+void ohs_rows(ohs *h, int32_t *rows)
+{
+    if (h->rows) memcpy(rows, h->rows, sizeof(int32_t) * (size_t)h->a.ncompts);
+}
 void ohs_trace(ohs *h, uint64_t *per_comp)
 {
     memcpy(per_comp, h->trace, sizeof(uint64_t) * h->a.ncompts);
@@ -451,6 +467,7 @@ int32_t ohs_chip_update(ohs *h)
                 } else {
                     vipol = vipol - (double)filo;
                 }
+                if (h->rows) h->rows[comp] = filo;
                 continue;
             case CONC_VAL_OP:
                 k = ops[p++];

@@ -89,6 +89,14 @@ int32_t ohs_stepno(ohs *h);
 void    ohs_trace_enable(ohs *h, int32_t on);
 void    ohs_trace(ohs *h, uint64_t *per_comp);
 
+/* The gate-table row (filo) each compartment's NEWVOLT_OP used in the last
+ * ohs_chip_update (-1: no NEWVOLT), recorded once enabled.  Two runs whose
+ * rows agree at every step interpolate the same table intervals: their
+ * difference quotient measures the derivative of one piecewise-linear
+ * model. */
+void    ohs_rows_enable(ohs *h, int32_t on);
+void    ohs_rows(ohs *h, int32_t *rows);
+
 /* Step nh independent solvers (disjoint neuron blocks, each its own hsolve)
  * nsteps times on nthreads POSIX threads; solver i runs on thread i % nthreads.
  * Used for the multi-core CPU baseline. */

@@ -78,6 +78,8 @@ def oracle_lib():
         L.ohs_stepno.restype = C.c_int32
         L.ohs_trace_enable.argtypes = [C.c_void_p, C.c_int32]
         L.ohs_trace.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.ohs_rows_enable.argtypes = [C.c_void_p, C.c_int32]
+        L.ohs_rows.argtypes = [C.c_void_p, _i32p]
         L.ohs_step_parallel.argtypes = [C.POINTER(C.c_void_p), C.c_int32, C.c_int32, C.c_int32]
         L.ohs_step_parallel.restype = C.c_int32
         L.ohs_set_synapses.argtypes = [C.c_void_p, C.c_int32, _i32p, _i32p, _f64p, _f64p]
@@ -297,6 +299,21 @@ class Oracle:
         t = np.empty(self.m.ncompts, np.uint64)
         oracle_lib().ohs_trace(self.h, t.ctypes.data_as(C.POINTER(C.c_uint64)))
         return t
+
+    def row_trace(self, nsteps, rows=None):
+        """Step nsteps times and return the NEWVOLT table row of each
+        compartment (or of ``rows``) at every step, shape (nsteps, n)."""
+        L = oracle_lib()
+        L.ohs_rows_enable(self.h, 1)
+        buf = np.empty(self.m.ncompts, np.int32)
+        sel = np.arange(self.m.ncompts) if rows is None else np.asarray(rows)
+        out = np.empty((nsteps, sel.size), np.int32)
+        for t in range(nsteps):
+            self.step(1)
+            L.ohs_rows(self.h, _p(buf, C.c_int32))
+            out[t] = buf[sel]
+        L.ohs_rows_enable(self.h, 0)
+        return out
 
     def trace_enable(self, on=True):
         oracle_lib().ohs_trace_enable(self.h, 1 if on else 0)
