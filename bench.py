@@ -80,7 +80,12 @@ def parse():
                    help="skip the adjoint section (1024 x 64-compartment fit, 2000 steps)")
     p.add_argument("--adj-neurons", type=int, default=1024)
     p.add_argument("--adj-steps", type=int, default=2000)
-    p.add_argument("--only", choices=("heterogeneous", "calcium", "network", "adjoint"), default=None,
+    p.add_argument("--no-iso", action="store_true",
+                   help="skip the isopotential section (1000 single-compartment HH cells, 200 steps)")
+    p.add_argument("--iso-cells", type=int, default=1000)
+    p.add_argument("--iso-steps", type=int, default=200)
+    p.add_argument("--only", choices=("isopotential", "heterogeneous", "calcium", "network", "adjoint"),
+                   default=None,
                    help="run one section alone and print its JSON (for profiling it)")
     p.add_argument("--traffic-probe", action="store_true",
                    help="(child of the traffic measurement) copy calibration, then a few batched "
@@ -303,6 +308,55 @@ def hetero_section(a, D, M, Oracle):
                          "bitexact": bool(np.array_equal(v, ref))}
         res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{S} steps of the population ({cpu:.1f} s)"}
+    return res
+
+
+def iso_section(a, D, M, Oracle):
+    """Isopotential cells: 1000 single-compartment HH cells (30x30 um soma,
+    Na/K 1200/360 S/m^2, inject U(0.1, 0.5) nA from seed 0), dt 20 us, 200
+    steps, fp64 -- the persistent multi-step kernel with V in a register for
+    K steps.  Step phase (device time), end to end (build, create, upload,
+    steps, download), roofline by algorithmic bytes, parity against the
+    oracle over the whole run, and the oracle's time for the same run."""
+    n, T = a.iso_cells, a.iso_steps
+    t_e2e = time.perf_counter()
+    m = M.isopotential_hh(n, seed=0)
+    with D.HipSolver(m) as g:
+        g.step(T, D.BATCHED)               # autotunes, then runs: end to end as a caller sees it
+        v = g.vm
+        t_e2e = time.perf_counter() - t_e2e
+        info = g.info()
+        K = info["batch_k"]
+        # step phase on the same handle, from the state it reached
+        g.step(2 * T, D.BATCHED)
+        g.sync()
+        g.set_timing(True)
+        t0 = time.perf_counter()
+        g.step(T, D.BATCHED)
+        g.sync()
+        wall = time.perf_counter() - t0
+        dev_ms = g.last_time(D.T_TOTAL)
+        g.set_timing(False)
+    lp = launch_plan(T, K)
+    rf = roofline_fields(info["bytes_per_step"], T, dev_ms, lp)
+    res = {"workload": f"{n} isopotential HH cells, {T} steps, dt 20 us, fp64", "compartments": n, "steps": T,
+           "batched": {"value": T * n / wall, "unit": "compartment-steps/s", "us_per_step": wall * 1e6 / T,
+                       "device_us_per_step": dev_ms * 1e3 / T, "batch_k": K, "tiles": info["ntiles"],
+                       "kernel": batched_kernel_name(info), "launches": lp},
+           "end_to_end": {"value": T * n / t_e2e, "unit": "compartment-steps/s", "seconds": t_e2e,
+                          "includes": "model build, hsh_create (JIT), autotune, steps, vm download"},
+           "roofline": dict(rf, bound="hbm", peak=HBM_PEAK_GBS, unit="GB/s",
+                            algorithmic_bytes_per_compartment_step=info["bytes_per_step"] / n,
+                            note="16 tiles of 64 cells: a few waves on a 256-CU GPU, latency-bound")}
+    if not a.no_cpu:
+        o = Oracle(m, givals=False)
+        t0 = time.perf_counter()
+        o.step(T)
+        cpu = time.perf_counter() - t0
+        res["parity"] = {"steps": T, "bitexact": bool(np.array_equal(o.vm, v)),
+                         "max_abs_dv_V": float(np.max(np.abs(o.vm - v)))}
+        res["cpu_baseline"] = {"value": T * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole run ({cpu:.3f} s)"}
     return res
 
 
@@ -615,7 +669,9 @@ def main():
 
     if a.only:
         from oracle import Oracle as _O
-        if a.only == "heterogeneous":
+        if a.only == "isopotential":
+            r = iso_section(a, D, M, _O)
+        elif a.only == "heterogeneous":
             r = hetero_section(a, D, M, _O)
         elif a.only == "calcium":
             r = calcium_section(a, D, M, _O)
@@ -651,7 +707,10 @@ def main():
     dev_b_ms = gb.last_time(D.T_TOTAL)
     gb.set_timing(False)
     lp = launch_plan(a.steps, K)
+    t_vm = time.perf_counter()
+    _ = gb.vm                             # the caller's view: state back on the host
     t_e2e_total = time.perf_counter() - t_e2e
+    t_vm = time.perf_counter() - t_vm
     print(f"[bench] batched K={K}: {wall_b*1e3:.1f} ms wall, {dev_b_ms:.1f} ms device, "
           f"{r['launches']} graph/direct launches", file=sys.stderr, flush=True)
 
@@ -733,7 +792,10 @@ def main():
                      "algorithmic_bytes_per_compartment_step": bytes_step / n,
                      "launch_us": rf["launch_us"], "steps_per_launch": rf["steps_per_launch"],
                      "device_us_per_step": rf["device_us_per_step"]},
-        "end_to_end_s": {"build_create_upload": t_ready, "total_batched_incl_warmup": t_e2e_total},
+        "end_to_end_s": {"build_create_upload": t_ready, "total_batched_incl_warmup": t_e2e_total,
+                         "vm_download": t_vm},
+        "end_to_end": {"value": (a.warmup + a.steps) * n / t_e2e_total, "unit": "compartment-steps/s",
+                       "includes": "model build, hsh_create (JIT), autotune, warmup and timed steps, vm download"},
     }
 
     # ---- HBM traffic of the batched kernel (two rocprofv3 --pmc child runs)
@@ -804,6 +866,9 @@ def main():
                          "bitexact_batched": par["batched_bitexact"],
                          "bitexact_plain": par["plain_bitexact"], "tolerance_V": 1e-7}
         out["speedup_vs_cpu_1core"] = value / cpu1_rate
+    if not a.no_iso:
+        from oracle import Oracle as _O
+        out["isopotential"] = iso_section(a, D, M, _O)
     if not a.no_hetero:
         from oracle import Oracle as _O
         out["heterogeneous"] = hetero_section(a, D, M, _O)
