@@ -1430,7 +1430,7 @@ hsh_status hsh_create(const hsh_model* m, const hsh_options* opt, hsh_handle** o
     *out = nullptr;
     std::vector<hsh_member> one(1);
     one[0] = hsh_member{0, m->ncompts, 0, m->nchips, 0, std::max(0, m->nops - 1), 0, std::max(0, m->nconcs), 0,
-                        std::max(0, m->ngivals)};
+                        std::max(0, m->ngivals), 0, 0};
     return create_impl(m, opt, out, nullptr, std::move(one));
 }
 
@@ -1531,6 +1531,7 @@ hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_opt
         e.op0 = (int32_t)A.ops.size(); e.nops = m.nops - 1;
         e.conc0 = nconcs; e.nconcs = std::max(0, m.nconcs);
         e.giv0 = (int32_t)A.giv.size(); e.ngivals = giv ? m.ngivals : 0;
+        e.stab0 = stab0[i]; e.pad = 0;
         A.ops.insert(A.ops.end(), m.ops, m.ops + m.nops - 1);
         A.chip.insert(A.chip.end(), m.chip, m.chip + m.nchips);
         A.vm.insert(A.vm.end(), m.vm, m.vm + m.ncompts);

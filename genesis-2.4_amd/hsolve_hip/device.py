@@ -69,7 +69,7 @@ class HshOptions(C.Structure):
 
 class HshMember(C.Structure):
     _fields_ = [(f, C.c_int32) for f in ("row0", "nrows", "chip0", "nchips", "op0", "nops", "conc0", "nconcs",
-                                         "giv0", "ngivals")]
+                                         "giv0", "ngivals", "stab0", "pad")]
 
 
 class HshRefusal(C.Structure):

@@ -204,6 +204,10 @@ typedef struct hsh_member {
     int32_t op0, nops;       /* ops[], without the member's LCOMPT_OP */
     int32_t conc0, nconcs;   /* conc[] */
     int32_t giv0, ngivals;   /* givals[] */
+    int32_t stab0;           /* its stablist entries start here in the group's (the
+                                stab of an hsh_input for one of its synchans is its
+                                own index plus stab0) */
+    int32_t pad;
 } hsh_member;
 
 hsh_status hsh_create(const hsh_model *m, const hsh_options *opt, struct hsh_handle **out);

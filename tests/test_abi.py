@@ -270,3 +270,16 @@ def test_refuses_symcompartment_elimination_ops(op, where):
         D.HipSolver(m, host_only=True)
     refs = e.value.refusals
     assert refs and all(r[3] == "funcs (sibling ops)" and r[1] == op for r in refs)
+
+
+def test_group_member_stablist_offsets():
+    """hsh_group_member reports where a member's synapse types start in the
+    group's stablist: equal stablists are stored once, a different one
+    follows (a member's hsh_input stab is its own index plus stab0)."""
+    a = synchan_model()
+    comps = dict(parent=[-1, -1], Cm=[1e-11] * 2, Rm=[1e8] * 2, Ra=[1e7] * 2, Em=[-0.065] * 2,
+                 initVm=[-0.065] * 2, inject=[0.0] * 2)
+    b = M.build(comps, [], [], dt=1e-4, spikes=[(0, 0.0, 5e-3)],
+                syntypes=[dict(Ek=-0.08, tau1=10e-3, tau2=1e-3)], syns=[(1, 0, 1e-9)])
+    with D.HipSolver([a, a, b, a], host_only=True) as h:
+        assert [h.member(i)["stab0"] for i in range(4)] == [0, 0, 4, 0]

@@ -119,3 +119,41 @@ def test_channels_mode_results(gpu, maker):
         assert np.array_equal(os_["results"], gs["results"])
         assert np.array_equal(os_["chip"], gs["chip"]) and np.array_equal(os_["ops"], gs["ops"])
         assert np.array_equal(gs["vm"], vm0)
+
+
+def test_group_member_inputs_with_offsets(gpu):
+    """Inputs to members of a group handle take the member's chip and
+    stablist offsets (hsh_group_member): the group's state equals each
+    member run on its own with its own inputs."""
+    def cell(ek, tau1, inj):
+        comps = dict(parent=[-1], Cm=[1e-11], Rm=[1e8], Ra=[1e6], Em=[-0.065], initVm=[-0.065], inject=[inj])
+        return M.build(comps, [], [], dt=1e-4, syntypes=[dict(Ek=ek, tau1=tau1, tau2=1e-3)], syns=[(0, 0, 1e-9)],
+                       float_fields=False)
+    cells = [cell(0.0, 5e-3, 0.1e-9), cell(-0.08, 10e-3, 0.2e-9), cell(0.0, 5e-3, 0.3e-9)]
+
+    def own_inputs(m, v):
+        xs = int(m.syn_chip[0])
+        k = int(m.ops[int(m.syn_op[0]) + 1])
+        return [(INJECT, int(m.compchips[0]) + 1, 0, 0, 0.4e-9 * v), (ACT, xs, k, 1, 2.0 * v)]
+    with D.HipSolver(cells, flags=D.F_NO_AUTOTUNE) as g:
+        mem = [g.member(i) for i in range(3)]
+        singles = [D.HipSolver(c, flags=D.F_NO_AUTOTUNE) for c in cells]
+        for step in range(3):
+            grp_in = []
+            for i, c in enumerate(cells):
+                e = own_inputs(c, step + i + 1)
+                singles[i].set_inputs(e)
+                for (op, idx, stab, per_dt, val) in e:
+                    grp_in.append((op, idx + mem[i]["chip0"], stab + (mem[i]["stab0"] if op == ACT else 0),
+                                   per_dt, val))
+            g.set_inputs(grp_in)
+            g.step(7, D.BATCHED)
+            for s in singles:
+                s.step(7, D.BATCHED)
+        gs = g.state()
+        for i, s in enumerate(singles):
+            ss = s.state()
+            e = mem[i]
+            assert np.array_equal(gs["vm"][e["row0"]:e["row0"] + e["nrows"]], ss["vm"])
+            assert np.array_equal(gs["chip"][e["chip0"]:e["chip0"] + e["nchips"]], ss["chip"])
+            s.close()
