@@ -241,9 +241,9 @@ def batched_kernel_name(info):
 
 def hetero_section(a, D, M, Oracle):
     """Heterogeneous population: tree sizes {1, 8, 16, 64, 256, 1024} with
-    P proportional to size^-0.67 (mean ~17 compartments), every tree in
-    packs (one kernel), K steps per captured launch; parity against the
-    oracle on a bounded number of steps."""
+    P proportional to size^-0.67 (mean ~17 compartments), layout, pack shape
+    and K chosen by the autotuner on the first batched step; a K sweep on the
+    chosen layout; parity against the oracle on a bounded number of steps."""
     sizes = (1, 8, 16, 64, 256, 1024)
     p = np.array(sizes, float) ** -0.67
     p /= p.sum()
@@ -251,16 +251,14 @@ def hetero_section(a, D, M, Oracle):
     m = M.population(a.hetero_neurons, sizes, tuple(p), seed=a.seed)
     t_build = time.perf_counter() - t0
     n = m.ncompts
-    opts = dict(batch_k=32, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=1024, pack_waves=4, pack_stage=1)
     print(f"[bench] hetero: {a.hetero_neurons} neurons, {n} compartments (built {t_build:.1f}s)",
           file=sys.stderr, flush=True)
     res = {"workload": f"{a.hetero_neurons} random Hines trees, sizes {list(sizes)} with P ~ size^-0.67",
-           "compartments": n, "steps": a.hetero_steps, "layout": "packs of ~1024 compartments, level-scheduled "
-           "elimination, 4 waves, schedule read from global memory one level ahead, K=32"}
-    with D.HipSolver(m, **opts) as g:
-        info = g.info()
-        g.step(64, D.BATCHED)
+           "compartments": n, "steps": a.hetero_steps}
+    with D.HipSolver(m) as g:
+        g.step(64, D.BATCHED)              # the autotuner runs here
         g.sync()
+        info = g.info()
         g.set_timing(True)
         t0 = time.perf_counter()
         g.step(a.hetero_steps, D.BATCHED)
@@ -268,23 +266,32 @@ def hetero_section(a, D, M, Oracle):
         wall = time.perf_counter() - t0
         dev_ms = g.last_time(D.T_TOTAL)
         g.set_timing(False)
-        res["batched"] = {"value": a.hetero_steps * n / wall, "unit": "compartment-steps/s",
-                          "ms_per_step": wall * 1e3 / a.hetero_steps,
-                          "hbm_frac_algorithmic": info["bytes_per_step"] * a.hetero_steps / (dev_ms * 1e-3) / 1e9
-                          / HBM_PEAK_GBS,
-                          "packs": info["npacks"], "lane_fill": info["lane_fill"]}
-    # steps per launch: the same packs at K = 1, 8, 32, 128 (1024 steps each)
+    K = info["batch_k"]
+    lp = launch_plan(a.hetero_steps, K)
+    rf = roofline_fields(info["bytes_per_step"], a.hetero_steps, dev_ms, lp)
+    packs = info["tile_max_rows"] == 0
+    res["layout"] = {"autotuned": bool(info["autotuned"]), "autotune_s": info["autotune_s"],
+                     "tile_max_rows": info["tile_max_rows"], "pack_cap": info["pack_cap"],
+                     "pack_waves": info["pack_waves"], "schedule_in_lds": info["pack_stage"] == 2,
+                     "packs": info["npacks"], "tiles": info["ntiles"], "batch_k": K,
+                     "kernel": batched_kernel_name(info), "lane_fill": info["lane_fill"]}
+    res["batched"] = {"value": a.hetero_steps * n / wall, "unit": "compartment-steps/s",
+                      "ms_per_step": wall * 1e3 / a.hetero_steps, "device_us_per_step": rf["device_us_per_step"],
+                      "hbm_frac_algorithmic": rf["frac"], "launch_us": rf["launch_us"], "launches": lp}
+    opts = dict(flags=D.F_NO_AUTOTUNE, tile_max_rows=info["tile_max_rows"] if not packs else -1,
+                pack_cap=info["pack_cap"], pack_waves=info["pack_waves"], pack_stage=info["pack_stage"])
+    # steps per launch on the chosen layout: K = 1, 8, 32, 128 (1024 steps each)
     sweep = []
-    for K in (1, 8, 32, 128):
-        with D.HipSolver(m, **dict(opts, batch_k=K)) as g:
-            g.step(2 * K, D.BATCHED)
+    for Ks in (1, 8, 32, 128):
+        with D.HipSolver(m, **dict(opts, batch_k=Ks)) as g:
+            g.step(2 * Ks, D.BATCHED)
             g.sync()
             t0 = time.perf_counter()
             g.step(1024, D.BATCHED)
             g.sync()
             wall = time.perf_counter() - t0
-        sweep.append({"batch_k": K, "ms_per_step": wall * 1e3 / 1024, "value": 1024 * n / wall})
-        print(f"[bench] hetero K={K}: {wall * 1e6 / 1024:.1f} us/step", file=sys.stderr, flush=True)
+        sweep.append({"batch_k": Ks, "ms_per_step": wall * 1e3 / 1024, "value": 1024 * n / wall})
+        print(f"[bench] hetero K={Ks}: {wall * 1e6 / 1024:.1f} us/step", file=sys.stderr, flush=True)
     res["batch_k_sweep"] = sweep
     with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
         g.step(16, D.PLAIN)
@@ -301,11 +308,12 @@ def hetero_section(a, D, M, Oracle):
         o.step(S)
         cpu = time.perf_counter() - t0
         ref = o.vm
-        with D.HipSolver(m, **opts) as g:
+        with D.HipSolver(m, **dict(opts, batch_k=K)) as g:
             g.step(S, D.BATCHED)
             v = g.vm
         res["parity"] = {"steps": S, "max_abs_dv_V": float(np.max(np.abs(v - ref))),
-                         "bitexact": bool(np.array_equal(v, ref))}
+                         "bitexact": bool(np.array_equal(v, ref)),
+                         "full_horizon": "5,000 steps bit-exact in tests/test_gpu_horizon.py"}
         res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{S} steps of the population ({cpu:.1f} s)"}
     return res

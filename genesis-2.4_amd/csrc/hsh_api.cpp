@@ -1052,6 +1052,9 @@ hsh_status autotune(hsh_handle* h)
     if (h->ev_on) STCHK(ev_download(h, ev0));
     const hsh_options& o = h->opt;
     const Plan& P = h->plan;
+    // HSH_AUTOTUNE_LOG=1 prints every timed candidate on stderr
+    const char* logenv = std::getenv("HSH_AUTOTUNE_LOG");
+    const bool log = logenv && logenv[0] && logenv[0] != '0';
     const int K0 = std::min(o.batch_k > 0 ? o.batch_k : 32, h->max_k);
     h->batch_k = K0;
     // steps per trial: about 2e6 compartment-steps, at least 2 launches
@@ -1065,7 +1068,23 @@ hsh_status autotune(hsh_handle* h)
         tmr_c.push_back(0);
     }
     if (o.pack_cap > 0) cap_c = {o.pack_cap};
-    else cap_c = {512, 1024, 2048};
+    else {
+        cap_c = {512, 1024, 2048};
+        // pack sizes whose packs all fit the GPU at once: k packs per CU,
+        // bound by the 28 bytes of LDS per position (with room for block
+        // padding), so no workgroup waits for a second round
+        int ncu = 0, ldscu = 0;
+        if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, h->device) != hipSuccess) ncu = 0;
+        if (hipDeviceGetAttribute(&ldscu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->device) !=
+                hipSuccess || ldscu <= 0)
+            ldscu = MAX_LDS;
+        for (int k = 4; k <= 7 && ncu > 0; k++) {
+            const int64_t cap = (int64_t)((double)n * 1.08 / ((double)ncu * k)) + 1;
+            if (cap >= std::max(64, P.max_tree_rows) && cap * 28 * k <= ldscu && cap <= 8192 &&
+                std::find(cap_c.begin(), cap_c.end(), (int)cap) == cap_c.end())
+                cap_c.push_back((int)cap);
+        }
+    }
     waves_c = o.pack_waves > 0 ? std::vector<int>{o.pack_waves} : std::vector<int>{4, 8};
     stage_c = o.pack_stage > 0 ? std::vector<int>{o.pack_stage == 2} : std::vector<int>{0, 1};
 
@@ -1094,6 +1113,9 @@ hsh_status autotune(hsh_handle* h)
                     free_graphs(h);
                     double ms = 0;
                     STCHK(time_batched(h, trial, &ms));
+                    if (log)
+                        std::fprintf(stderr, "hsh_autotune: tile_max_rows %d pack_cap %d waves %d stage %d: %.2f us/step (%d tiles, %d packs)\n",
+                                     tmr, cap_c[ci], wv, st, ms * 1e3, h->dm.ntiles, h->dm.npacks);
                     if (ms < best.ms) {
                         best.ms = ms;
                         best.lo = lo;
@@ -1120,6 +1142,7 @@ hsh_status autotune(hsh_handle* h)
                 free_graphs(h);
                 double ms = 0;
                 STCHK(time_batched(h, std::max(2 * K, trial / K * K), &ms));
+                if (log) std::fprintf(stderr, "hsh_autotune: K %d: %.2f us/step\n", K, ms * 1e3);
                 if (ms < bk) {
                     bk = ms;
                     kbest = K;
