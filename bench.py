@@ -583,18 +583,38 @@ def adjoint_section(a, torch, Oracle, TangentLinear):
     tgt = torch.from_numpy(tgt_np)
     print(f"[bench] adjoint: {N} x 64 compartments, {T} steps (setup + targets {time.perf_counter() - t0:.1f}s)",
           file=sys.stderr, flush=True)
-    run = F.Runner(pm, T, ckpt=100)
     n = pm.base.ncompts
-    theta = torch.ones(N, 5, dtype=torch.float64, requires_grad=True)
-    run(theta.detach())                     # warm (JIT, layout)
-    t0 = time.perf_counter()
-    tr = run(theta)
-    t_fwd = time.perf_counter() - t0
-    loss = torch.mean((tr - tgt) ** 2)
-    t0 = time.perf_counter()
-    loss.backward()
-    t_bwd = time.perf_counter() - t0
-    g = theta.grad.numpy()
+
+    def timed(layout, cap):
+        """(runner, forward s, backward s, trace, loss, theta gradient) of one
+        fit step on a fresh Runner, after a warm-up pass"""
+        r = F.Runner(pm, T, ckpt=100, layout=layout, pack_cap=cap)
+        r(torch.ones(N, 5, dtype=torch.float64))        # warm (JIT, layout)
+        th = torch.ones(N, 5, dtype=torch.float64, requires_grad=True)
+        t0 = time.perf_counter()
+        tr = r(th)
+        t_f = time.perf_counter() - t0
+        ls = torch.mean((tr - tgt) ** 2)
+        t0 = time.perf_counter()
+        ls.backward()
+        t_b = time.perf_counter() - t0
+        return r, t_f, t_b, tr, ls, th.grad.numpy().copy()
+
+    # pack sizes for the pack layout: forward + backward of one fit step
+    sweep_caps = []
+    for cap in (64, 128, 256, 512):
+        r, t_f, t_b, _, _, _ = timed("packs", cap)
+        inf = r.solver.info()
+        sweep_caps.append({"pack_cap": cap, "packs": inf["npacks"], "forward_s": t_f, "backward_s": t_b})
+        r.close()
+    cap_best = min(sweep_caps, key=lambda x: x["forward_s"] + x["backward_s"])["pack_cap"]
+    rt, tf_t, tb_t, tr_t, _, g_t = timed("tiles", 0)
+    rt.close()
+    run, t_fwd, t_bwd, tr, loss, g = timed("packs", cap_best)
+    info = run.solver.info()
+    same_as_tiles = bool(np.array_equal(g, g_t) and np.array_equal(tr.detach().numpy(), tr_t.detach().numpy()))
+    print(f"[bench] adjoint packs of {cap_best}: forward {t_fwd * 1e3:.1f} ms, backward {t_bwd * 1e3:.1f} ms; "
+          f"tiles {tf_t * 1e3:.1f} / {tb_t * 1e3:.1f} ms; same gradients {same_as_tiles}", file=sys.stderr, flush=True)
     w = (2 * (tr.detach() - tgt) / tr.numel()).numpy()
     checks = []
     # per parameter type: the neuron with the largest gradient component (the
@@ -634,8 +654,13 @@ def adjoint_section(a, torch, Oracle, TangentLinear):
     return {"workload": f"{N} random Hines trees x 64 compartments, {T} steps, dt 20 us, fp64, checkpoint "
                         f"every 100 steps; loss = soma-Vm MSE vs CPU targets at hidden +-30% gNa/gK/gKdr/Ra/Cm",
             "parameters": N * 5, "loss": loss.item(),
+            "layout": {"kind": "packs", "pack_cap": cap_best, "packs": info["npacks"],
+                       "pack_waves": info["pack_waves"]},
             "forward_s": t_fwd, "backward_s": t_bwd,
             "forward_cs_per_s": T * n / t_fwd, "backward_cs_per_s": T * n / t_bwd,
+            "pack_cap_sweep": sweep_caps,
+            "tile_layout": {"forward_s": tf_t, "backward_s": tb_t, "forward_cs_per_s": T * n / tf_t,
+                            "backward_cs_per_s": T * n / tb_t, "gradients_bitexact_vs_packs": same_as_tiles},
             "gradient_checks": checks,
             "max_rel_vs_tl": max(c["rel_vs_tl"] for c in checks),
             "fd_sweep_h1e-6": sweep,

@@ -168,6 +168,10 @@ struct hsh_handle {
         double *segV = nullptr, *segS = nullptr;     // [ckpt][...]
         double *RB = nullptr, *RD = nullptr, *LV = nullptr, *LS = nullptr, *BB = nullptr, *DB = nullptr,
                *GT = nullptr, *Dp = nullptr, *Bp = nullptr, *X = nullptr, *XB = nullptr;
+        // pack layout: observed row per position, the tree per position
+        bool packs = false;
+        int32_t *obs_idx = nullptr, *ppar = nullptr;
+        double *pta = nullptr, *ptcf = nullptr, *ptcb = nullptr;
     } adj;
 };
 
@@ -308,7 +312,8 @@ void free_adj(hsh_handle* h)
 {
     auto& A = h->adj;
     void* ptrs[] = {A.obs_pos, A.trace, A.dtrace, A.ckV, A.ckS, A.ckI, A.ckStep, A.segV, A.segS, A.RB,
-                    A.RD, A.LV, A.LS, A.BB, A.DB, A.GT, A.Dp, A.Bp, A.X, A.XB};
+                    A.RD, A.LV, A.LS, A.BB, A.DB, A.GT, A.Dp, A.Bp, A.X, A.XB, A.obs_idx, A.ppar, A.pta,
+                    A.ptcf, A.ptcb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     A = hsh_handle::Adj();
@@ -1202,13 +1207,69 @@ bool adj_supported(const Plan& P, std::string& why)
     return true;
 }
 
-// The adjoint kernels run on the tile layout (one lane per neuron).
-hsh_status ensure_tiles(hsh_handle* h)
+// The adjoint runs on one layout class: all tiles (three launches per
+// reverse step) or all packs (one launch per checkpoint segment).  A layout
+// that mixes them moves every tree to packs; packs whose reverse-sweep LDS
+// does not fit a workgroup are made smaller.
+hsh_status ensure_adj_layout(hsh_handle* h)
 {
     if (h->plan.packs.empty()) return HSH_OK;
     LayoutOpts lo = h->plan.layout;
-    lo.tile_max_rows = std::max(1, h->plan.max_tree_rows);
-    return relayout(h, lo);
+    if (!h->plan.tiles.empty()) {
+        lo.tile_max_rows = 0;
+        STCHK(relayout(h, lo));
+    }
+    while ((int64_t)ADJ_PACK_LDS_PER_POS * h->plan.max_pack_pos + 16 > MAX_LDS) {
+        if (lo.pack_cap <= std::max(64, h->plan.max_tree_rows))
+            return fail(HSH_ERR_LIMIT, "hsh_adj_forward: a tree too large for the adjoint's LDS arrays");
+        lo.pack_cap = std::max(std::max(64, h->plan.max_tree_rows), lo.pack_cap / 2);
+        STCHK(relayout(h, lo));
+    }
+    return HSH_OK;
+}
+
+// Per-position arrays of the pack adjoint: observed row, pack-local parent
+// (-1 root, -2 padding) and the row's coefficients.
+hsh_status adj_pack_arrays(hsh_handle* h, int nobs, const int32_t* obs)
+{
+    auto& A = h->adj;
+    const Plan& P = h->plan;
+    const size_t npos = A.npos;
+    std::vector<int32_t> oi(npos, -1), par(npos, -2);
+    std::vector<double> ta(npos, 0.0), tf(npos, 0.0), tb(npos, 0.0);
+    for (int j = 0; j < nobs; j++) oi[P.comp_pos[obs[j]]] = j;
+    for (int g = 0; g < h->m.ncompts; g++) {
+        const int pos = P.comp_pos[g];
+        const int pg = P.parent[g];
+        if (pg < 0) {
+            par[pos] = -1;
+            continue;
+        }
+        par[pos] = P.comp_pos[pg];
+        ta[pos] = P.ta[g];
+        tf[pos] = P.tcf[g];
+        tb[pos] = P.tcb[g];
+    }
+    // parents as pack-local positions (whole neurons sit in one pack)
+    for (const PackMeta& pm : P.packs) {
+        const int lo = pm.block0 * W, hi = (pm.block0 + pm.nblk) * W;
+        for (int q = lo; q < hi; q++)
+            if (par[q] >= 0) {
+                if (par[q] < lo || par[q] >= hi) return fail(HSH_ERR_STATE, "hsh_adj_forward: a tree split over packs");
+                par[q] -= lo;
+            }
+    }
+    HIPCHK(dalloc(&A.obs_idx, npos));
+    HIPCHK(dalloc(&A.ppar, npos));
+    HIPCHK(dalloc(&A.pta, npos));
+    HIPCHK(dalloc(&A.ptcf, npos));
+    HIPCHK(dalloc(&A.ptcb, npos));
+    HIPCHK(dput(A.obs_idx, oi));
+    HIPCHK(dput(A.ppar, par));
+    HIPCHK(dput(A.pta, ta));
+    HIPCHK(dput(A.ptcf, tf));
+    HIPCHK(dput(A.ptcb, tb));
+    return HSH_OK;
 }
 
 hsh_status adj_alloc(hsh_handle* h, int nsteps, int ckpt, int nobs)
@@ -2123,21 +2184,37 @@ hsh_status hsh_adj_forward(hsh_handle* h, int32_t nsteps, int32_t ckpt, int32_t 
     }
     std::string why;
     if (!adj_supported(h->plan, why)) return fail(HSH_REFUSED, "hsh_adj_forward: the reverse program does not run " + why);
-    STCHK(ensure_tiles(h));
+    STCHK(ensure_adj_layout(h));
     if (ckpt > nsteps) ckpt = nsteps;
     STCHK(adj_alloc(h, nsteps, ckpt, nobs));
     auto& A = h->adj;
+    A.packs = !h->plan.packs.empty();
     std::vector<int32_t> pos(std::max(1, nobs));
     for (int j = 0; j < nobs; j++) pos[j] = h->plan.comp_pos[obs[j]];
     if (nobs > 0) HIPCHK(hipMemcpy(A.obs_pos, pos.data(), nobs * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (A.packs) STCHK(adj_pack_arrays(h, nobs, obs));
     const bool timing = h->timing;
     h->timing = false;
     HIPCHK(hipStreamSynchronize(h->stream));
     STCHK(adj_save(h, 0));
-    for (int t = 0; t < nsteps; t++) {
-        STCHK(plain_step(h, false));
-        HIPCHK(launch_gather(h->d_V, A.obs_pos, nobs, A.trace + (size_t)t * nobs, h->stream));
-        if ((t + 1) % ckpt == 0 && t + 1 < nsteps) STCHK(adj_save(h, (t + 1) / ckpt));
+    if (A.packs) {
+        // one batched launch per checkpoint segment, recording the trace
+        DevModel dm = h->dm;
+        dm.obs_idx = A.obs_idx;
+        dm.nobs = nobs;
+        for (int t0 = 0; t0 < nsteps; t0 += ckpt) {
+            const int n = std::min(ckpt, nsteps - t0);
+            dm.trace = nobs > 0 ? A.trace + (size_t)t0 * nobs : nullptr;
+            HIPCHK(launch_forest(dm, &h->jit, n, h->pack_waves, h->pack_stage, pack_lds_bytes(h, h->pack_stage),
+                                 h->stream));
+            if (t0 + n < nsteps) STCHK(adj_save(h, (t0 + n) / ckpt));
+        }
+    } else {
+        for (int t = 0; t < nsteps; t++) {
+            STCHK(plain_step(h, false));
+            HIPCHK(launch_gather(h->d_V, A.obs_pos, nobs, A.trace + (size_t)t * nobs, h->stream));
+            if ((t + 1) % ckpt == 0 && t + 1 < nsteps) STCHK(adj_save(h, (t + 1) / ckpt));
+        }
     }
     STCHK(adj_save(h, A.ncp - 1));
     h->timing = timing;
@@ -2186,9 +2263,40 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     am.npos = (long long)npos;
     am.max_rows = P.max_rows;
     const int nseg = (A.nsteps + A.ckpt - 1) / A.ckpt;
+    AdjPack ap{};
+    ap.d = h->dm;
+    ap.segV = A.segV;
+    ap.segS = A.segS;
+    ap.npos = (long long)npos;
+    ap.nS = (long long)A.nS;
+    ap.dtrace = A.dtrace;
+    ap.nobs = A.nobs;
+    ap.obs_idx = A.obs_idx;
+    ap.LV = A.LV;
+    ap.LS = A.LS;
+    ap.GT = A.GT;
+    ap.ppar = A.ppar;
+    ap.pta = A.pta;
+    ap.ptcf = A.ptcf;
+    ap.ptcb = A.ptcb;
+    DevModel rec = h->dm;
+    rec.recV = A.segV;
+    rec.recS = A.segS;
+    rec.rec_npos = (long long)npos;
+    rec.rec_nS = (long long)A.nS;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int t0 = sg * A.ckpt, t1 = std::min(A.nsteps, t0 + A.ckpt);
         STCHK(adj_load(h, sg));
+        if (A.packs) {
+            // recompute the segment keeping each step's state, then run it
+            // backwards in one launch
+            HIPCHK(launch_forest(rec, &h->jit, t1 - t0, h->pack_waves, h->pack_stage,
+                                 pack_lds_bytes(h, h->pack_stage), h->stream));
+            ap.t0 = t0;
+            ap.n = t1 - t0;
+            HIPCHK(launch_adj_forest(ap, P.max_pack_pos, h->stream));
+            continue;
+        }
         for (int t = t0; t < t1; t++) {
             const size_t j = (size_t)(t - t0);
             HIPCHK(hipMemcpyAsync(A.segV + j * npos, h->d_V, npos * sizeof(double), hipMemcpyDeviceToDevice, h->stream));

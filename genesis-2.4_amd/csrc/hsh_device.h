@@ -1141,9 +1141,18 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
     btail = __builtin_amdgcn_readfirstlane(btail);
     for (int k = 0; k < K; k++) {
         if (CHAN) {
+            // the adjoint's recomputation keeps the state each step starts from
+            if (d.recV)
+                for (int i = tid; i < NP; i += T) d.recV[k * d.rec_npos + base + i] = LV[i];
             for (int blk = wave; blk < pm.nblk; blk += nw) {
                 const BlockMeta bm = d.blocks[pm.block0 + blk];
                 const int i = blk * W + lane;
+                if (d.recS) {
+                    const int ns = d.prog[bm.prog];
+                    const double* src = d.S + bm.dslot + lane;
+                    double* dst = d.recS + k * d.rec_nS + bm.dslot + lane;
+                    for (int c = 0; c < ns; c++) dst[c * W] = src[c * W];
+                }
                 const double Vm = LV[i];
                 const int pi = LP[i];
                 const bool pok = pi >= 0;
@@ -1173,6 +1182,12 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
             if (wave == 0) bwd_levels(blv, btail, pm.nbl, false);
             __syncthreads();
         }
+        // the adjoint's forward run keeps the observed voltages of each step
+        if (CHAN && d.trace)
+            for (int i = tid; i < NP; i += T) {
+                const int j = d.obs_idx[base + i];
+                if (j >= 0) d.trace[(long long)k * d.nobs + j] = LV[i];
+            }
     }
     if (own_last) LD[pm.last_lp] = droot0;
     __syncthreads();

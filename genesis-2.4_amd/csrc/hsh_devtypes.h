@@ -159,6 +159,17 @@ struct DevModel {
     int single;           // ncompts == 1
     int max_slots;        // largest program slot count
     int ext;              // some program holds an op beyond the voltage-gated/synaptic set
+    // recording of a batched pack launch for the adjoint (null: off): the
+    // observed positions' voltages after each step k, trace[k*nobs + j], and
+    // the state each step starts from, recV[k*rec_npos + pos] and
+    // recS[k*rec_nS + slot]
+    double* trace;
+    const int* obs_idx;   // per position: j of an observed row, -1
+    int nobs;
+    double* recV;
+    double* recS;
+    long long rec_npos;
+    long long rec_nS;
 };
 
 }  // namespace hsh

@@ -176,16 +176,32 @@ class Trace(torch.autograd.Function):
 
 class Runner:
     """A device solver for a ParamMap's population that runs nsteps from the
-    initial state for given arrays."""
+    initial state for given arrays.
 
-    def __init__(self, pmap: ParamMap, nsteps: int, *, ckpt: int = 100, obs=None, device: int = 0):
+    layout "packs" (default): whole neurons in packs of about ``pack_cap``
+    compartments, one workgroup of ``pack_waves`` waves each (``pack_stage``
+    2: level schedule in LDS, 1: read from memory); the forward pass runs one
+    batched launch per checkpoint interval and the backward pass one
+    recording launch and one reverse launch per interval.  "tiles": 64 neurons per tile, one
+    lane each, three launches per step.  Both give the same gradients bit for
+    bit."""
+
+    def __init__(self, pmap: ParamMap, nsteps: int, *, ckpt: int = 100, obs=None, device: int = 0,
+                 layout: str = "packs", pack_cap: int = 256, pack_waves: int = 4, pack_stage: int = 2):
         self.pmap = pmap
         self.nsteps = nsteps
         self.ckpt = ckpt
         self.obs = pmap.soma if obs is None else np.ascontiguousarray(obs, np.int32)
         b = pmap.base
-        self.solver = D.HipSolver(b, device=device, flags=D.F_NO_AUTOTUNE,
-                                  tile_max_rows=int(np.max(np.bincount(tree_of_row(b)))))
+        rows = int(np.max(np.bincount(tree_of_row(b))))
+        if layout == "packs":
+            opts = dict(tile_max_rows=-1, pack_cap=max(pack_cap, rows), pack_waves=pack_waves,
+                        pack_stage=pack_stage)
+        elif layout == "tiles":
+            opts = dict(tile_max_rows=rows)
+        else:
+            raise ValueError(f"Runner: layout {layout!r} is not 'packs' or 'tiles'")
+        self.solver = D.HipSolver(b, device=device, flags=D.F_NO_AUTOTUNE, **opts)
         # forward runs so far: a backward pass must belong to the latest one
         self.generation = 0
 

@@ -25,7 +25,9 @@ from oracle import Oracle, TangentLinear
 pytestmark = pytest.mark.gpu
 
 
-def solver_for(m, rows):
+def solver_for(m, rows, layout="tiles", cap=256):
+    if layout == "packs":
+        return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=cap)
     return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=rows)
 
 
@@ -63,6 +65,52 @@ def test_adjoint_dot_product_with_tangent_linear(gpu, method, sizes, rows):
         rhs = float(np.sum(terms))
         scale = max(abs(lhs), float(np.sum(np.abs(terms))))
         assert abs(lhs - rhs) <= 1e-8 * scale, (trial, lhs, rhs)
+
+
+@pytest.mark.parametrize("method", [M.BEULER, M.CRANK])
+@pytest.mark.parametrize("sizes,cap", [((8, 16), 64), ((40,), 256), ((100, 7), 128), ((30, 1, 12), 1024)],
+                         ids=["rows16-cap64", "rows40-cap256", "rows100-cap128", "mixed-cap1024"])
+def test_pack_adjoint_equals_tile_adjoint(gpu, method, sizes, cap):
+    """The pack layout's adjoint (one recording launch and one reverse launch
+    per checkpoint interval, level-scheduled sweeps) gives the tile layout's
+    trace and gradients bit for bit, and leaves the forward run's state."""
+    pm = F.ParamMap(9, sizes, (1.0 / len(sizes),) * len(sizes), seed=7, method=method)
+    m, obs, T = pm.base, pm.soma, 160
+    rows = int(np.max(np.bincount(F.tree_of_row(m))))
+    print(f"[case] packs vs tiles method={method} sizes={sizes} cap={cap}", flush=True)
+    w = np.random.default_rng(3).standard_normal((T, obs.size))
+    out = {}
+    for layout in ("tiles", "packs"):
+        with solver_for(m, rows, layout, cap) as s:
+            tr = s.adj_forward(T, 50, obs)
+            after_fwd = s.state()
+            g = s.adj_backward(w)
+            after_bwd = s.state()
+            for k in ("vm", "chip", "ops"):
+                assert np.array_equal(after_fwd[k], after_bwd[k]), (layout, k)
+            out[layout] = (tr, g)
+    assert np.array_equal(out["tiles"][0], out["packs"][0])
+    assert np.array_equal(out["tiles"][0], oracle_trace(m, T, obs))
+    for a, b in zip(out["tiles"][1], out["packs"][1]):
+        assert np.array_equal(a, b)
+
+
+def test_mixed_layout_moves_to_packs(gpu):
+    """A handle whose layout mixes tiles and packs runs the adjoint with
+    every tree in packs: same gradients as the tile layout."""
+    pm = F.ParamMap(8, (4, 40), (0.5, 0.5), seed=2)
+    m, obs, T = pm.base, pm.soma, 80
+    w = np.random.default_rng(4).standard_normal((T, obs.size))
+    with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as s:
+        assert s.info()["ntiles"] > 0 and s.info()["npacks"] > 0
+        s.adj_forward(T, 40, obs)
+        assert s.info()["ntiles"] == 0
+        gm = s.adj_backward(w)
+    with solver_for(m, 40) as s:
+        s.adj_forward(T, 40, obs)
+        gt = s.adj_backward(w)
+    for a, b in zip(gm, gt):
+        assert np.array_equal(a, b)
 
 
 def oracle_row_crossings(pm, n, j, h, T):
@@ -160,14 +208,15 @@ def test_stale_backward_is_refused(gpu):
     run.close()
 
 
-def test_checkpoint_interval_and_state(gpu):
+@pytest.mark.parametrize("layout", ["tiles", "packs"])
+def test_checkpoint_interval_and_state(gpu, layout):
     pm = F.ParamMap(5, (16,), (1.0,), seed=9)
     m, obs, T = pm.base, pm.soma, 150
-    print("[case] checkpoint intervals", flush=True)
+    print(f"[case] checkpoint intervals {layout}", flush=True)
     w = np.random.default_rng(2).standard_normal((T, obs.size))
     outs = []
     for ck in (7, 50, 150):
-        with solver_for(m, 16) as s:
+        with solver_for(m, 16, layout, 64) as s:
             s.adj_forward(T, ck, obs)
             after_fwd = s.state()
             outs.append(s.adj_backward(w))
