@@ -166,6 +166,8 @@ struct hsh_handle {
         double *ckV = nullptr, *ckS = nullptr;       // [ncp][...]
         int32_t *ckI = nullptr, *ckStep = nullptr;
         double *segV = nullptr, *segS = nullptr;     // [ckpt][...]
+        double *segB = nullptr, *segD = nullptr, *segX = nullptr;   // pack layout: each step's
+                                                                    // elimination [ckpt][npos]
         double *RB = nullptr, *RD = nullptr, *LV = nullptr, *LS = nullptr, *BB = nullptr, *DB = nullptr,
                *GT = nullptr, *Dp = nullptr, *Bp = nullptr, *X = nullptr, *XB = nullptr;
         // pack layout: observed row per position, the tree per position
@@ -311,8 +313,8 @@ void free_graphs(hsh_handle* h)
 void free_adj(hsh_handle* h)
 {
     auto& A = h->adj;
-    void* ptrs[] = {A.obs_pos, A.trace, A.dtrace, A.ckV, A.ckS, A.ckI, A.ckStep, A.segV, A.segS, A.RB,
-                    A.RD, A.LV, A.LS, A.BB, A.DB, A.GT, A.Dp, A.Bp, A.X, A.XB, A.obs_idx, A.ppar, A.pta,
+    void* ptrs[] = {A.obs_pos, A.trace, A.dtrace, A.ckV, A.ckS, A.ckI, A.ckStep, A.segV, A.segS, A.segB,
+                    A.segD, A.segX, A.RB, A.RD, A.LV, A.LS, A.BB, A.DB, A.GT, A.Dp, A.Bp, A.X, A.XB, A.obs_idx, A.ppar, A.pta,
                     A.ptcf, A.ptcb};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -1297,6 +1299,13 @@ hsh_status adj_alloc(hsh_handle* h, int nsteps, int ckpt, int nobs)
     HIPCHK(dalloc(&A.ckStep, (size_t)ncp * A.nstep));
     HIPCHK(dalloc(&A.segV, (size_t)ckpt * npos));
     HIPCHK(dalloc(&A.segS, (size_t)ckpt * nS));
+    if (!P.packs.empty()) {
+        // padding positions are never written: zeros there
+        for (double** b : {&A.segB, &A.segD, &A.segX}) {
+            HIPCHK(dalloc(b, (size_t)ckpt * npos));
+            HIPCHK(hipMemset(*b, 0, (size_t)ckpt * npos * sizeof(double)));
+        }
+    }
     double** bufs[] = {&A.RB, &A.RD, &A.LV, &A.BB, &A.DB, &A.Dp, &A.Bp, &A.X, &A.XB};
     for (double** b : bufs) HIPCHK(dalloc(b, npos));
     HIPCHK(dalloc(&A.LS, nS));
@@ -2267,6 +2276,9 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     ap.d = h->dm;
     ap.segV = A.segV;
     ap.segS = A.segS;
+    ap.segB = A.segB;
+    ap.segD = A.segD;
+    ap.segX = A.segX;
     ap.npos = (long long)npos;
     ap.nS = (long long)A.nS;
     ap.dtrace = A.dtrace;
@@ -2282,6 +2294,9 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     DevModel rec = h->dm;
     rec.recV = A.segV;
     rec.recS = A.segS;
+    rec.recB = A.segB;
+    rec.recD = A.segD;
+    rec.recX = A.segX;
     rec.rec_npos = (long long)npos;
     rec.rec_nS = (long long)A.nS;
     for (int sg = nseg - 1; sg >= 0; sg--) {
@@ -2294,7 +2309,7 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
                                  pack_lds_bytes(h, h->pack_stage), h->stream));
             ap.t0 = t0;
             ap.n = t1 - t0;
-            HIPCHK(launch_adj_forest(ap, P.max_pack_pos, h->stream));
+            HIPCHK(launch_adj_forest(ap, &h->jit, P.max_pack_pos, h->stream));
             continue;
         }
         for (int t = t0; t < t1; t++) {

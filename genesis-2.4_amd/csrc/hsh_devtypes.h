@@ -160,16 +160,44 @@ struct DevModel {
     int max_slots;        // largest program slot count
     int ext;              // some program holds an op beyond the voltage-gated/synaptic set
     // recording of a batched pack launch for the adjoint (null: off): the
-    // observed positions' voltages after each step k, trace[k*nobs + j], and
-    // the state each step starts from, recV[k*rec_npos + pos] and
-    // recS[k*rec_nS + slot]
+    // observed positions' voltages after each step k, trace[k*nobs + j], the
+    // state each step starts from, recV[k*rec_npos + pos] and
+    // recS[k*rec_nS + slot], and its elimination
     double* trace;
     const int* obs_idx;   // per position: j of an observed row, -1
     int nobs;
     double* recV;
     double* recS;
+    double* recB;         // each step's eliminated rhs and diag and its solution,
+    double* recD;         // recB[k*rec_npos + pos]
+    double* recX;
     long long rec_npos;
     long long rec_nS;
+};
+
+// Adjoint on the pack layout: every reverse step of one checkpoint segment in
+// one launch, one workgroup per pack, from what a recording forest launch
+// kept (DevModel::recV / recS / recB / recD).  Arrays are [position] unless
+// noted.
+struct AdjPack {
+    DevModel d;
+    const double* segV;   // [n][npos] state each step of the segment starts from
+    const double* segS;   // [n][nS]
+    const double* segB;   // [n][npos] each step's eliminated rhs and diag, solution
+    const double* segD;
+    const double* segX;
+    long long npos, nS;
+    int t0, n;            // first step of the segment, its steps
+    const double* dtrace; // [nsteps][nobs] dL/d(observed V after step t)
+    int nobs;
+    const int* obs_idx;   // observed row index j, -1
+    double* LV;           // adjoint of V, carried from segment to segment
+    double* LS;           // slot adjoints [nS]
+    double* GT;           // accumulated gradients of ta, tcf, tcb [3][npos]
+    const int* ppar;      // pack-local parent position, -1 root, -2 padding
+    const double* pta;    // the row's coefficients (0 for roots)
+    const double* ptcf;
+    const double* ptcb;
 };
 
 }  // namespace hsh

@@ -112,7 +112,7 @@ bool build_layout(Plan& plan, const LayoutOpts& o, std::string& err);
 struct JitModule {
     hipModule_t mod = nullptr;
     hipFunction_t channel = nullptr, batched_r[3] = {nullptr, nullptr, nullptr}, batched = nullptr,
-                  iso = nullptr, forest[2] = {nullptr, nullptr};
+                  iso = nullptr, forest[2] = {nullptr, nullptr}, adj_forest = nullptr;
     bool ok = false;
     double compile_s = 0.0;
 };
@@ -212,32 +212,13 @@ struct AdjModel {
 hipError_t launch_adj_rhs(const AdjModel& a, hipStream_t s);
 hipError_t launch_adj_elim(const AdjModel& a, hipStream_t s);
 hipError_t launch_adj_local(const AdjModel& a, hipStream_t s);
-// Adjoint on the pack layout: every reverse step of one checkpoint segment in
-// one launch, one workgroup per pack, from the state a recording forest
-// launch kept (DevModel::recV / recS).  Arrays are [position] unless noted.
-struct AdjPack {
-    DevModel d;
-    const double* segV;   // [n][npos] state each step of the segment starts from
-    const double* segS;   // [n][nS]
-    long long npos, nS;
-    int t0, n;            // first step of the segment, its steps
-    const double* dtrace; // [nsteps][nobs] dL/d(observed V after step t)
-    int nobs;
-    const int* obs_idx;   // observed row index j, -1
-    double* LV;           // adjoint of V, carried from segment to segment
-    double* LS;           // slot adjoints [nS]
-    double* GT;           // accumulated gradients of ta, tcf, tcb [3][npos]
-    const int* ppar;      // pack-local parent position, -1 root, -2 padding
-    const double* pta;    // the row's coefficients (0 for roots)
-    const double* ptcf;
-    const double* ptcb;
-};
 // LDS bytes per pack position of k_adj_forest: twelve doubles (working
 // arrays, gradients, coefficients), parent and observed index, and at most
 // one forward entry (two ints), one kid (a short), one backward entry and
 // two level bounds per row
 constexpr int ADJ_PACK_LDS_PER_POS = 12 * 8 + 2 * 4 + 2 * 4 + 2 + 4 + 2 * 4;
-hipError_t launch_adj_forest(const AdjPack& a, int max_pack_pos, hipStream_t s);
+// the pack layout's reverse launch (AdjPack, hsh_devtypes.h); jm may be null
+hipError_t launch_adj_forest(const AdjPack& a, const JitModule* jm, int max_pack_pos, hipStream_t s);
 hipError_t launch_gather(const double* V, const int* pos, int n, double* out, hipStream_t s);
 hipError_t launch_inject(double* LV, const int* pos, int n, const double* g, hipStream_t s);
 int batched_lds_bytes(int max_rows);

@@ -25,7 +25,10 @@ namespace hsh {
 
 namespace {
 
-#include "hsh_jit_src.inc"   // kDevTypesSrc, kDeviceSrc: hsh_devtypes.h, hsh_device.h
+constexpr int MAXG_ADJ = 3;   // gates per channel the adjoint runs (hsh_adjoint_device.h MAXG)
+
+#include "hsh_jit_src.inc"   // kDevTypesSrc, kDeviceSrc, kAdjointSrc: hsh_devtypes.h, hsh_device.h,
+                             // hsh_adjoint_device.h
 
 std::mutex g_jit_mu;
 std::map<std::string, std::vector<char>> g_code_cache;   // source -> code object
@@ -217,6 +220,129 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
     return o;
 }
 
+// The reverse of one program for the adjoint's pack kernel: adj_reverse_local
+// (hsh_adjoint_device.h) unrolled for this program -- the same expressions in
+// the same order, so the gradients are the interpreter's bit for bit -- with
+// every slot and slot adjoint loaded at the start and the adjoints stored at
+// the end, so the loads of a program issue together.  Programs the adjoint
+// refuses (ops past the voltage-gated set) get a stub that is never run.
+std::string gen_adj_template(const Template& t, int id, int ncols)
+{
+    std::string o;
+    o += "__device__ __forceinline__ double adj_tmpl_" + std::to_string(id) +
+         "(const DevModel& d, const double* __restrict__ Sb, double* __restrict__ LSb, int lane, double Vm, "
+         "double bb, double db)\n{\n";
+    bool ok = true;
+    int ng_chan = 0;
+    for (size_t pc = 0; pc < t.code.size();) {
+        const int op = t.code[pc++];
+        if (op == U_IPOL1V) {
+            pc += 2;
+            if (++ng_chan > MAXG_ADJ) ok = false;
+        } else if (op == U_CHAN || op == U_CHAN_EK) {
+            ng_chan = 0;
+        } else if (op == U_SPIKE) {
+            pc += 1;
+        } else if (op == U_NEWVOLT || op == U_ADDCURR) {
+        } else if (op == U_END) {
+            break;
+        } else {
+            ok = false;
+            break;
+        }
+    }
+    if (!ok) {
+        o += "    (void)d; (void)Sb; (void)LSb; (void)lane; (void)Vm; (void)bb; (void)db;\n    return 0.0;\n}\n";
+        return o;
+    }
+    const int ns = t.nslots;
+    auto S = [](int c) { return "s" + std::to_string(c); };
+    auto L = [](int c) { return "l" + std::to_string(c); };
+    for (int c = 0; c < ns; c++) o += "    const double " + S(c) + " = Sb[" + std::to_string(c) + " * W + lane];\n";
+    for (int c = 0; c < ns; c++) o += "    double " + L(c) + " = LSb[" + std::to_string(c) + " * W + lane];\n";
+    const std::string NC = std::to_string(ncols);
+    o += "    const double tbyc = " + S(ns - 2) + ";\n";
+    o += "    const double lam_i = bb * tbyc;\n    const double lam_g = db * tbyc;\n";
+    o += "    double vbar = bb;\n    double sumg = 0.0;\n    double ichan = s0 + s1;\n";
+    o += "    l0 += lam_i;\n    l1 += lam_i;\n";
+    o += "    double Gk = 0.0, Ek = 0.0, gmax = 0.0;\n    Interp ip{0, 0.0, false};\n    const double* vtab = d.tab;\n";
+    o += "    (void)Gk; (void)Ek; (void)gmax; (void)ip; (void)vtab;\n";
+    int c = 2, ekslot = -1, gslot = -1, gid = 0;
+    std::vector<int> gs, gp, gn;      // gates of the current channel: slot, power, variable id
+    for (size_t pc = 0; pc < t.code.size();) {
+        const int op = t.code[pc++];
+        if (op == U_IPOL1V) {
+            const int col = t.code[pc], k = t.code[pc + 1];
+            pc += 2;
+            const std::string g = std::to_string(gid), C = std::to_string(col);
+            o += "    const double* tab" + g + " = vtab + " + C + ";\n";
+            o += "    const double B" + g + " = tab" + g + "[0] + ip.vipol * (tab" + g + "[" + NC + "] - tab" + g + "[0]);\n";
+            o += "    const double dB" + g + " = ip.inside ? (tab" + g + "[" + NC + "] - tab" + g + "[0]) * d.invdx : 0.0;\n";
+            o += "    const double dA" + g + " = ip.inside ? (tab" + g + "[" + NC + " + 1] - tab" + g +
+                 "[1]) * d.invdx : 0.0;\n";
+            if (k < 0) {
+                o += "    const double X" + g + " = gate_update(vtab, " + C + ", " + NC + ", ip.vipol, " + S(c) + ", true);\n";
+                o += "    const double gdx" + g + " = 0.0;\n";
+                o += "    const double gdv" + g + " = (dA" + g + " - X" + g + " * dB" + g + ") / B" + g + ";\n";
+            } else {
+                o += "    const double X" + g + " = gate_update(vtab, " + C + ", " + NC + ", ip.vipol, " + S(c) + ", false);\n";
+                o += "    const double gdx" + g + " = (2.0 - B" + g + ") / B" + g + ";\n";
+                o += "    const double gdv" + g + " = (dA" + g + " - " + S(c) + " * dB" + g + " - X" + g + " * dB" + g +
+                     ") / B" + g + ";\n";
+            }
+            const int pw = k < 0 ? -k : k;
+            o += "    Gk = apply_power(Gk, X" + g + ", " + std::to_string(pw) + ");\n";
+            gs.push_back(c);
+            gp.push_back(pw);
+            gn.push_back(gid);
+            gid++;
+            c++;
+        } else if (op == U_CHAN_EK) {
+            ekslot = c;
+            gslot = c + 1;
+            o += "    Ek = " + S(c) + ";\n    gmax = Gk = " + S(c + 1) + ";\n";
+            c += 2;
+            gs.clear(); gp.clear(); gn.clear();
+        } else if (op == U_CHAN) {
+            gslot = c;
+            o += "    gmax = Gk = " + S(c) + ";\n";
+            c += 1;
+            gs.clear(); gp.clear(); gn.clear();
+        } else if (op == U_ADDCURR) {
+            o += "    {\n        sumg += Gk;\n        ichan += Ek * Gk;\n        const double lamG = lam_i * Ek + lam_g;\n";
+            if (ekslot >= 0) o += "        " + L(ekslot) + " += lam_i * Gk;\n";
+            o += "        double prod = 1.0;\n";
+            for (size_t q = 0; q < gs.size(); q++)
+                o += "        prod *= ipow(X" + std::to_string(gn[q]) + ", " + std::to_string(gp[q]) + ");\n";
+            if (gslot >= 0) o += "        " + L(gslot) + " += lamG * prod;\n";
+            for (size_t q = 0; q < gs.size(); q++) {
+                o += "        {\n            double dG = gmax * (double)" + std::to_string(gp[q]) + " * ipow(X" +
+                     std::to_string(gn[q]) + ", " + std::to_string(gp[q] - 1) + ");\n";
+                for (size_t h = 0; h < gs.size(); h++)
+                    if (h != q)
+                        o += "            dG *= ipow(X" + std::to_string(gn[h]) + ", " + std::to_string(gp[h]) + ");\n";
+                o += "            const double lamX = " + L(gs[q]) + " + lamG * dG;\n";
+                o += "            " + L(gs[q]) + " = lamX * gdx" + std::to_string(gn[q]) + ";\n";
+                o += "            vbar += lamX * gdv" + std::to_string(gn[q]) + ";\n        }\n";
+            }
+            o += "    }\n";
+            gs.clear(); gp.clear(); gn.clear();
+        } else if (op == U_NEWVOLT) {
+            o += "    ip = table_row(d, Vm);\n    vtab = d.tab + (long long)ip.filo * " + NC + ";\n";
+        } else if (op == U_SPIKE) {
+            pc += 1;
+            c += 1;
+        } else {   // U_END
+            o += "    " + L(c) + " += bb * ichan + db * sumg;\n";
+            o += "    " + L(c + 1) + " += db;\n";
+            break;
+        }
+    }
+    for (int q = 0; q < ns; q++) o += "    LSb[" + std::to_string(q) + " * W + lane] = " + L(q) + ";\n";
+    o += "    return vbar;\n}\n";
+    return o;
+}
+
 std::string strip_pragma_once(const char* s)
 {
     std::string r(s);
@@ -233,8 +359,18 @@ std::string jit_source(const Plan& P, int ncols)
     std::string src;
     src += strip_pragma_once(kDevTypesSrc);
     src += strip_pragma_once(kDeviceSrc);
+    src += strip_pragma_once(kAdjointSrc);
     src += "namespace hsh {\n";
     for (size_t t = 0; t < P.templates.size(); t++) src += gen_template(P.templates[t], (int)t, ncols, P.giv);
+    for (size_t t = 0; t < P.templates.size(); t++) src += gen_adj_template(P.templates[t], (int)t, ncols);
+    src += "struct JitAdj {\n"
+           "    static __device__ __forceinline__ double rev(const DevModel& d, const BlockMeta& bm, const double* Sb, "
+           "double* LSb, int lane, double Vm, double bb, double db)\n    {\n"
+           "        switch (bm.tmpl) {\n";
+    for (size_t t = 1; t < P.templates.size(); t++)
+        src += "        case " + std::to_string(t) + ": return adj_tmpl_" + std::to_string(t) +
+               "(d, Sb, LSb, lane, Vm, bb, db);\n";
+    src += "        default: return adj_tmpl_0(d, Sb, LSb, lane, Vm, bb, db);\n        }\n    }\n};\n";
     src += "struct JitRun {\n"
            "    static __device__ __forceinline__ CompOut run(const DevModel& d, const BlockMeta& bm, "
            "int lane, double Vm, bool pok, double Vp, bool active, int step)\n    {\n"
@@ -255,6 +391,8 @@ std::string jit_source(const Plan& P, int ncols)
            "{ extern __shared__ double lds[]; hsh::body_batched<hsh::JitRun>(d, K, lds); }\n";
     src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_iso(hsh::DevModel d, int K)\n"
            "{ hsh::body_iso<hsh::JitRun>(d, K); }\n";
+    src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_adj_forest(hsh::AdjPack a)\n"
+           "{ extern __shared__ double lds[]; hsh::body_adj_forest<hsh::JitAdj>(a, lds); }\n";
     for (int st : {0, 1})
         src += "extern \"C\" __global__ __launch_bounds__(512) void hsh_jit_forest" + std::to_string(st) +
                "(hsh::DevModel d, int K)\n{ extern __shared__ double lds[]; hsh::body_forest<hsh::JitRun, true, " +
@@ -314,10 +452,10 @@ bool jit_build(const Plan& P, int ncols, JitModule& jm, std::string& err)
     }
     const char* names[] = {"hsh_jit_channel", "hsh_jit_batched_r1", "hsh_jit_batched_r2",
                            "hsh_jit_batched_r4", "hsh_jit_batched", "hsh_jit_iso",
-                           "hsh_jit_forest0", "hsh_jit_forest1"};
+                           "hsh_jit_forest0", "hsh_jit_forest1", "hsh_jit_adj_forest"};
     hipFunction_t* slots[] = {&jm.channel, &jm.batched_r[0], &jm.batched_r[1], &jm.batched_r[2],
-                              &jm.batched, &jm.iso, &jm.forest[0], &jm.forest[1]};
-    for (int i = 0; i < 8; i++) {
+                              &jm.batched, &jm.iso, &jm.forest[0], &jm.forest[1], &jm.adj_forest};
+    for (int i = 0; i < 9; i++) {
         if (hipModuleGetFunction(slots[i], jm.mod, names[i]) != hipSuccess) {
             err = std::string("hipModuleGetFunction failed: ") + names[i];
             jit_free(jm);

@@ -28,6 +28,8 @@ pytestmark = pytest.mark.gpu
 def solver_for(m, rows, layout="tiles", cap=256):
     if layout == "packs":
         return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=cap)
+    if layout == "packs-interp":
+        return D.HipSolver(m, flags=D.F_NO_AUTOTUNE | D.F_NO_JIT, tile_max_rows=-1, pack_cap=cap)
     return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=rows)
 
 
@@ -72,15 +74,16 @@ def test_adjoint_dot_product_with_tangent_linear(gpu, method, sizes, rows):
                          ids=["rows16-cap64", "rows40-cap256", "rows100-cap128", "mixed-cap1024"])
 def test_pack_adjoint_equals_tile_adjoint(gpu, method, sizes, cap):
     """The pack layout's adjoint (one recording launch and one reverse launch
-    per checkpoint interval, level-scheduled sweeps) gives the tile layout's
-    trace and gradients bit for bit, and leaves the forward run's state."""
+    per checkpoint interval, level-scheduled sweeps; generated reverse
+    programs, and the interpreter) gives the tile layout's trace and
+    gradients bit for bit, and leaves the forward run's state."""
     pm = F.ParamMap(9, sizes, (1.0 / len(sizes),) * len(sizes), seed=7, method=method)
     m, obs, T = pm.base, pm.soma, 160
     rows = int(np.max(np.bincount(F.tree_of_row(m))))
     print(f"[case] packs vs tiles method={method} sizes={sizes} cap={cap}", flush=True)
     w = np.random.default_rng(3).standard_normal((T, obs.size))
     out = {}
-    for layout in ("tiles", "packs"):
+    for layout in ("tiles", "packs", "packs-interp"):
         with solver_for(m, rows, layout, cap) as s:
             tr = s.adj_forward(T, 50, obs)
             after_fwd = s.state()
@@ -89,10 +92,11 @@ def test_pack_adjoint_equals_tile_adjoint(gpu, method, sizes, cap):
             for k in ("vm", "chip", "ops"):
                 assert np.array_equal(after_fwd[k], after_bwd[k]), (layout, k)
             out[layout] = (tr, g)
-    assert np.array_equal(out["tiles"][0], out["packs"][0])
     assert np.array_equal(out["tiles"][0], oracle_trace(m, T, obs))
-    for a, b in zip(out["tiles"][1], out["packs"][1]):
-        assert np.array_equal(a, b)
+    for layout in ("packs", "packs-interp"):
+        assert np.array_equal(out["tiles"][0], out[layout][0]), layout
+        for a, b in zip(out["tiles"][1], out[layout][1]):
+            assert np.array_equal(a, b), layout
 
 
 def test_mixed_layout_moves_to_packs(gpu):
