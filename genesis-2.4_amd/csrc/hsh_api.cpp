@@ -168,6 +168,8 @@ struct hsh_handle {
         double *segV = nullptr, *segS = nullptr;     // [ckpt][...]
         double *segB = nullptr, *segD = nullptr, *segX = nullptr;   // pack layout: each step's
                                                                     // elimination [ckpt][npos]
+        int nbuf = 1;                 // segment buffers (pack layout: two, one being recorded
+                                      // while the other is run backwards)
         double *RB = nullptr, *RD = nullptr, *LV = nullptr, *LS = nullptr, *BB = nullptr, *DB = nullptr,
                *GT = nullptr, *Dp = nullptr, *Bp = nullptr, *X = nullptr, *XB = nullptr;
         // pack layout: observed row per position, the tree per position
@@ -175,6 +177,10 @@ struct hsh_handle {
         int32_t *obs_idx = nullptr, *ppar = nullptr;
         double *pta = nullptr, *ptcf = nullptr, *ptcb = nullptr;
     } adj;
+    // the pack adjoint's reverse launches run on a stream of their own, beside
+    // the recording launches on `stream`
+    hipStream_t adj_stream = nullptr;
+    hipEvent_t adj_rec[2] = {nullptr, nullptr}, adj_rev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -377,6 +383,13 @@ void free_device(hsh_handle* h)
     if (h->ev_a) (void)hipEventDestroy(h->ev_a);
     if (h->ev_b) (void)hipEventDestroy(h->ev_b);
     h->ev_a = h->ev_b = nullptr;
+    for (int b = 0; b < 2; b++) {
+        if (h->adj_rec[b]) (void)hipEventDestroy(h->adj_rec[b]);
+        if (h->adj_rev[b]) (void)hipEventDestroy(h->adj_rev[b]);
+        h->adj_rec[b] = h->adj_rev[b] = nullptr;
+    }
+    if (h->adj_stream) (void)hipStreamDestroy(h->adj_stream);
+    h->adj_stream = nullptr;
     if (h->stream) (void)hipStreamDestroy(h->stream);
     h->stream = nullptr;
 }
@@ -1297,13 +1310,14 @@ hsh_status adj_alloc(hsh_handle* h, int nsteps, int ckpt, int nobs)
     HIPCHK(dalloc(&A.ckS, (size_t)ncp * nS));
     HIPCHK(dalloc(&A.ckI, (size_t)ncp * nI));
     HIPCHK(dalloc(&A.ckStep, (size_t)ncp * A.nstep));
-    HIPCHK(dalloc(&A.segV, (size_t)ckpt * npos));
-    HIPCHK(dalloc(&A.segS, (size_t)ckpt * nS));
+    A.nbuf = P.packs.empty() ? 1 : 2;
+    HIPCHK(dalloc(&A.segV, (size_t)A.nbuf * ckpt * npos));
+    HIPCHK(dalloc(&A.segS, (size_t)A.nbuf * ckpt * nS));
     if (!P.packs.empty()) {
         // padding positions are never written: zeros there
         for (double** b : {&A.segB, &A.segD, &A.segX}) {
-            HIPCHK(dalloc(b, (size_t)ckpt * npos));
-            HIPCHK(hipMemset(*b, 0, (size_t)ckpt * npos * sizeof(double)));
+            HIPCHK(dalloc(b, (size_t)A.nbuf * ckpt * npos));
+            HIPCHK(hipMemset(*b, 0, (size_t)A.nbuf * ckpt * npos * sizeof(double)));
         }
     }
     double** bufs[] = {&A.RB, &A.RD, &A.LV, &A.BB, &A.DB, &A.Dp, &A.Bp, &A.X, &A.XB};
@@ -2172,8 +2186,36 @@ hsh_status hsh_set_ravals(hsh_handle* h, const double* ravals)
         }
         if (P.ra_bwd[r] >= 0) P.tcb[r] = ravals[P.ra_bwd[r]];
     }
+    // the layout keeps its shape: only the coefficient arrays change
+    const size_t npos = P.blocks.size() * W;
+    std::vector<double> ta(npos), tcf(npos), tcb(npos);
+    for (size_t q = 0; q < npos; q++) {
+        const int g = P.pos_comp[q];
+        ta[q] = P.ta[g];
+        tcf[q] = P.tcf[g];
+        tcb[q] = P.tcb[g];
+    }
+    for (const PackMeta& pm : P.packs) {
+        const int base = pm.block0 * W;
+        for (int j = pm.ke0; j < pm.ke0 + pm.nke; j++) {
+            const int c = P.pos_comp[base + P.kidx[j]];
+            P.kta[j] = P.ta[c];
+            P.ktcf[j] = P.tcf[c];
+        }
+        for (int e = pm.be0; e < pm.be0 + pm.nbe; e++) {
+            const int r = P.pos_comp[base + (P.bkp[e] & 0xffff)];
+            P.btcb[e] = (P.bkp[e] >> 16) > 0 ? P.tcb[r] : 0.0;
+        }
+    }
     if (h->host_only) return HSH_OK;
-    return relayout(h, P.layout);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    HIPCHK(dput(h->d_ta, ta));
+    HIPCHK(dput(h->d_tcf, tcf));
+    HIPCHK(dput(h->d_tcb, tcb));
+    HIPCHK(dput(h->d_kta, P.kta));
+    HIPCHK(dput(h->d_ktcf, P.ktcf));
+    HIPCHK(dput(h->d_btcb, P.btcb));
+    return HSH_OK;
 }
 
 hsh_status hsh_adj_forward(hsh_handle* h, int32_t nsteps, int32_t ckpt, int32_t nobs, const int32_t* obs,
@@ -2299,19 +2341,57 @@ hsh_status hsh_adj_backward(hsh_handle* h, const double* dtrace, double* g_chip,
     rec.recX = A.segX;
     rec.rec_npos = (long long)npos;
     rec.rec_nS = (long long)A.nS;
-    for (int sg = nseg - 1; sg >= 0; sg--) {
+    if (A.packs) {
+        // Per segment, last first: load its checkpoint and recompute it,
+        // keeping every step's state and elimination (`stream`), then run
+        // it backwards in one launch (adj_stream).  Segments alternate
+        // between two buffers, so segment s-1 is recorded while segment s
+        // runs backwards; the reverse launches run in segment order on their
+        // own stream (the adjoints carry from one to the next).
+        if (!h->adj_stream) {
+            HIPCHK(hipStreamCreateWithFlags(&h->adj_stream, hipStreamNonBlocking));
+            for (int b = 0; b < 2; b++) {
+                HIPCHK(hipEventCreateWithFlags(&h->adj_rec[b], hipEventDisableTiming));
+                HIPCHK(hipEventCreateWithFlags(&h->adj_rev[b], hipEventDisableTiming));
+            }
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));       // the cleared adjoints
+        bool used[2] = {false, false};
+        int last = 0;
+        for (int sg = nseg - 1; sg >= 0; sg--) {
+            const int t0 = sg * A.ckpt, t1 = std::min(A.nsteps, t0 + A.ckpt);
+            const int b = sg & 1;
+            const size_t ov = (size_t)b * A.ckpt * npos, os = (size_t)b * A.ckpt * A.nS;
+            if (used[b]) HIPCHK(hipStreamWaitEvent(h->stream, h->adj_rev[b], 0));
+            STCHK(adj_load(h, sg));
+            DevModel rb = rec;
+            rb.recV = A.segV + ov;
+            rb.recS = A.segS + os;
+            rb.recB = A.segB + ov;
+            rb.recD = A.segD + ov;
+            rb.recX = A.segX + ov;
+            HIPCHK(launch_forest(rb, &h->jit, t1 - t0, h->pack_waves, h->pack_stage,
+                                 pack_lds_bytes(h, h->pack_stage), h->stream));
+            HIPCHK(hipEventRecord(h->adj_rec[b], h->stream));
+            AdjPack ab = ap;
+            ab.segV = A.segV + ov;
+            ab.segS = A.segS + os;
+            ab.segB = A.segB + ov;
+            ab.segD = A.segD + ov;
+            ab.segX = A.segX + ov;
+            ab.t0 = t0;
+            ab.n = t1 - t0;
+            HIPCHK(hipStreamWaitEvent(h->adj_stream, h->adj_rec[b], 0));
+            HIPCHK(launch_adj_forest(ab, &h->jit, P.max_pack_pos, h->adj_stream));
+            HIPCHK(hipEventRecord(h->adj_rev[b], h->adj_stream));
+            used[b] = true;
+            last = b;
+        }
+        HIPCHK(hipStreamWaitEvent(h->stream, h->adj_rev[last], 0));
+    }
+    for (int sg = nseg - 1; sg >= 0 && !A.packs; sg--) {
         const int t0 = sg * A.ckpt, t1 = std::min(A.nsteps, t0 + A.ckpt);
         STCHK(adj_load(h, sg));
-        if (A.packs) {
-            // recompute the segment keeping each step's state, then run it
-            // backwards in one launch
-            HIPCHK(launch_forest(rec, &h->jit, t1 - t0, h->pack_waves, h->pack_stage,
-                                 pack_lds_bytes(h, h->pack_stage), h->stream));
-            ap.t0 = t0;
-            ap.n = t1 - t0;
-            HIPCHK(launch_adj_forest(ap, &h->jit, P.max_pack_pos, h->stream));
-            continue;
-        }
         for (int t = t0; t < t1; t++) {
             const size_t j = (size_t)(t - t0);
             HIPCHK(hipMemcpyAsync(A.segV + j * npos, h->d_V, npos * sizeof(double), hipMemcpyDeviceToDevice, h->stream));

@@ -117,6 +117,28 @@ def test_mixed_layout_moves_to_packs(gpu):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("kw", [dict(tile_max_rows=64), dict(tile_max_rows=-1, pack_cap=128),
+                                dict(tile_max_rows=16)], ids=["tiles", "packs", "mixed"])
+def test_set_ravals_equals_fresh_model(gpu, kw):
+    """hsh_set_ravals updates the coefficient arrays of every layout in place:
+    a run after it equals the oracle's run of the model built with those
+    coefficients, plain and batched."""
+    pm = F.ParamMap(6, (8, 40), (0.5, 0.5), seed=3)
+    th = np.ones((6, 5))
+    th[:, 3] = np.linspace(0.7, 1.4, 6)
+    th[:, 4] = np.linspace(1.2, 0.8, 6)
+    m2 = pm.model_at(th)
+    assert not np.array_equal(m2.ravals, pm.base.ravals)
+    o = Oracle(m2, givals=False)
+    o.step(60)
+    with D.HipSolver(pm.base, flags=D.F_NO_AUTOTUNE, **kw) as s:
+        s.set_ravals(m2.ravals)
+        s.put_state(vm=m2.vm, chip=m2.chip, ops=m2.ops)
+        s.step(30, D.PLAIN)
+        s.step(30, D.BATCHED)
+        assert np.array_equal(s.state()["vm"], o.vm)
+
+
 def oracle_row_crossings(pm, n, j, h, T):
     """Steps at which the central-difference runs theta[n, j] +- h
     interpolate a different gate-table row in some compartment of neuron n
