@@ -187,7 +187,7 @@ class Runner:
     bit."""
 
     def __init__(self, pmap: ParamMap, nsteps: int, *, ckpt: int = 100, obs=None, device: int = 0,
-                 layout: str = "packs", pack_cap: int = 256, pack_waves: int = 4, pack_stage: int = 2):
+                 layout: str = "packs", pack_cap: int = 128, pack_waves: int = 4, pack_stage: int = 2):
         self.pmap = pmap
         self.nsteps = nsteps
         self.ckpt = ckpt
