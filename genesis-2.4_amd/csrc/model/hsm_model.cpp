@@ -25,7 +25,7 @@ enum {
 };
 // funcs codes (hines_defs.h:119-128)
 enum { FORWARD_ELIM = 0, BACKWARD_ELIM = 0, SET_DIAG = 1, SKIP_DIAG = 2,
-       CALC_RESULTS = 6, FINISH = 7 };
+       FASTSIBARRAY_ELIM = 3, COPY_ARRAY = 4, SIBARRAY_ELIM = 5, CALC_RESULTS = 6, FINISH = 7 };
 // child types used for ordering (hines_defs.h:130-164)
 enum { SPIKEGEN_T = 11, TABCHAN_T = 20, SYNCHANC_T = 32 };
 
@@ -116,6 +116,317 @@ int fail(const char* msg)
     return -1;
 }
 
+}  // namespace
+
+// Elimination program of a solver of symmetric compartments: the symmetric
+// branch of h_funcs_init (hines_init.c:361-1160, symflag set).  Every
+// compartment couples to its parent and to its siblings through the
+// junction of their half-resistances; coeff_head(i) = (1 + Ra_i/Ra_parent +
+// sum over siblings Ra_i/Ra_s)/2 and coeff_tail(i) = (1 + sum over kids
+// Ra_i/Ra_k)/2 (hines_init.c:449-484), and the off-diagonal of row m in
+// column n is -dt/(Cm_m Ra_n coeff_m) with the coefficient of the junction
+// that joins them.  Eliminating a row's kids and earlier siblings changes
+// the off-diagonals to its parent and later siblings: the program copies
+// those into working ravals slots at the start of the row (COPY_ARRAY),
+// updates them as each pivot row is eliminated (FASTSIBARRAY_ELIM,
+// SIBARRAY_ELIM), and FORWARD_ELIM / SIBARRAY_ELIM read the working slots
+// where the ravals stream reaches them.  A working slot is a placeholder
+// (-1) in funcs until the ravals position that the stream reads it from is
+// known; funcs entries holding -(placeholder index) are copies resolved at
+// the same time.  The slot bookkeeping follows the reference's ravptr table
+// entry for entry (per row: [0] the right off-diagonal to the parent,
+// [1..maxkids] the kids' left off-diagonals, [maxkids+1..2 maxkids] the
+// siblings' off-diagonals; -1 unset, 0 unused, > 0 a placeholder's funcs
+// index, < -1 minus a ravals index), so the program, its ravals and the
+// diagonal terms are the reference's for the same tree.  `prod` is a float
+// product when element fields are float (two float fields multiplied
+// without a double operand, as at hines_init.c:632, :718, :887).
+namespace {
+struct SymOut {
+    std::vector<int> funcs;
+    std::vector<double> ravals, diag;
+};
+
+int sym_funcs(int n, double cdt, const std::vector<int>& elmnum, const std::vector<int>& hnum,
+              const std::vector<int>& parents, const std::vector<int>& nkids, const std::vector<int>& kstart,
+              const std::vector<int>& kids, const std::vector<double>& Cmv, const std::vector<double>& Rmv,
+              const std::vector<double>& Rav, bool ff, SymOut& out)
+{
+    auto prod = [ff](double a, double b) { return ff ? (double)((float)a * (float)b) : a * b; };
+    auto kid = [&](int e, int j) { return kids[kstart[e] + j]; };
+    auto bug = [](int k) {
+        g_err = "hsm_build: symmetric elimination program inconsistent (check " + std::to_string(k) + ")";
+        return -1;
+    };
+    int maxkids = 1;
+    for (int i = 0; i < n; i++) maxkids = std::max(maxkids, nkids[i]);
+    const int W2 = 2 * maxkids + 1;
+    std::vector<int> ravptr((size_t)n * W2, -1);
+    auto RP = [&](int row, int l) -> int& { return ravptr[(size_t)row * W2 + l]; };
+    std::vector<double> coeffs(2 * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        const int ci = elmnum[i];
+        const double Ra = Rav[ci];
+        double temp = 0.0;
+        const int pn = parents[ci];
+        if (pn >= 0) {
+            temp = Ra / Rav[pn];
+            for (int j = 0; j < nkids[pn]; j++) {
+                const int k = kid(pn, j);
+                if (k != ci) temp += Ra / Rav[k];
+            }
+        }
+        coeffs[2 * i] = (1.0 + temp) / 2.0;
+        temp = 0.0;
+        for (int j = 0; j < nkids[ci]; j++) temp += Ra / Rav[kid(ci, j)];
+        coeffs[2 * i + 1] = (1.0 + temp) / 2.0;
+    }
+    std::vector<int>& f = out.funcs;
+    std::vector<double>& ra = out.ravals;
+    std::vector<double>& dg = out.diag;
+    f.clear();
+    ra.clear();
+    dg.assign(n, 0.0);
+    const double dt = cdt;
+    // a placeholder at funcs index m takes ravals index q, with its copies
+    auto resolve = [&](int m, int q, bool need_copy) -> bool {
+        f[m] = q;
+        bool found = false;
+        for (size_t l = (size_t)m; l < f.size(); l++)
+            if (f[l] == -m) {
+                f[l] = q;
+                found = true;
+            }
+        return found || !need_copy;
+    };
+    int skipdiag = 0, self = 0;
+    for (int i = 0; i < n; i++) {
+        const int ci = elmnum[i];
+        const double Cm = Cmv[ci];
+        dg[i] = 1.0 + dt / (Cm * Rmv[ci]);
+        const int pn = parents[ci];
+        const int nkid = nkids[ci];
+        for (int l = nkid + 1; l <= maxkids; l++) RP(i, l) = 0;
+        int hassiblefts = 0;
+        if (pn > -1) {
+            const int np = nkids[pn];
+            int j = 0;
+            for (; j < np; j++)
+                if (kid(pn, j) == ci) {
+                    self = j;
+                    break;
+                }
+            if (np && j == np) return bug(0);
+            if (np > 1) {
+                if (self > 0) {
+                    f.push_back(COPY_ARRAY);
+                    RP(i, 0) = (int)f.size();
+                    f.push_back(-1);
+                    const double temp = -dt / (Cm * Rav[pn] * coeffs[2 * i]);
+                    ra.push_back(temp);
+                    dg[i] -= temp;
+                    hassiblefts = 1;
+                }
+            } else {
+                RP(i, 0) = 0;
+            }
+            if (i == 0) {
+                RP(0, maxkids + 1) = 0;
+                for (int l = maxkids + np + 1; l <= 2 * maxkids; l++) RP(0, l) = 0;
+                continue;
+            }
+        } else {
+            RP(i, 0) = 0;
+        }
+        if (skipdiag) {
+            skipdiag = 0;
+        } else if (nkid || hassiblefts || pn == -1) {
+            if (i > 1) f.push_back(SET_DIAG);
+        } else if (i > 1) {
+            f.push_back(SKIP_DIAG);
+            skipdiag = 1;
+        }
+        for (int j = 0; j < nkid; j++) {
+            const int k = kid(ci, j);
+            const int row = hnum[k];
+            f.push_back(FORWARD_ELIM);
+            f.push_back(2 * row);
+            if (j == 0) {
+                const double temp = -dt / (Cm * Rav[k] * coeffs[2 * i + 1]);
+                ra.push_back(temp);
+                dg[i] -= temp;
+            } else {
+                const int m = RP(i, j + 1);
+                if (m <= 0 || f[m] != -1) return bug(1);
+                if (!resolve(m, (int)ra.size(), true)) return bug(-1);
+                ra.push_back(0.0);    // a working slot: COPY_ARRAY fills it each step
+            }
+            RP(i, j + 1) = 0;
+            if (j == 0) {
+                const int m = RP(row, 0);
+                if (m < -1) {
+                    ra.push_back(ra[-m]);
+                } else {
+                    const double temp = -dt / (prod(Cmv[k], Rav[ci]) * coeffs[2 * row]);
+                    ra.push_back(temp);
+                    dg[row] -= temp;
+                }
+            } else {
+                const int m = RP(row, 0);
+                if (m <= 0 || f[m] != -1) return bug(2);
+                resolve(m, (int)ra.size(), false);
+                ra.push_back(0.0);
+            }
+            RP(row, 0) = -(int)(ra.size() - 1);
+            if (j == 0) {
+                for (int l = 1; l < nkid; l++) {
+                    f.push_back(COPY_ARRAY);
+                    RP(i, l + 1) = (int)f.size();
+                    f.push_back(-1);
+                    const int k2 = kid(ci, l);
+                    const double temp = -dt / (Cm * Rav[k2] * coeffs[2 * i + 1]);
+                    ra.push_back(temp);
+                    dg[i] -= temp;
+                }
+            }
+            for (int l = j + 1; l < nkid; l++) {
+                f.push_back(SIBARRAY_ELIM);
+                int m = RP(row, maxkids + l + 1);
+                f.push_back(-m);
+                if (m == 0) return bug(3);
+                m = RP(i, l + 1);
+                if (m <= 0) return bug(4);
+                f.push_back(-m);
+            }
+        }
+        int nsib = 0;   // sibling off-diagonals of this row eliminated so far
+        if (pn > -1) {
+            const int np = nkids[pn];
+            RP(i, maxkids + 1) = 0;
+            RP(i, maxkids + 1 + self) = 0;
+            for (int l = maxkids + np + 1; l <= 2 * maxkids; l++) RP(i, l) = 0;
+            if (np < 2) continue;
+            for (int j = 0; j < np; j++) {
+                const int k = kid(pn, j);
+                const int row = hnum[k];
+                if (np > 2 && self > 0 && j > 0) {
+                    const double temp = -dt / (Cm * Rav[k] * coeffs[2 * i]);
+                    ra.push_back(temp);
+                    dg[i] -= temp;
+                    f.push_back(COPY_ARRAY);
+                    RP(i, maxkids + j + 1) = (int)f.size();
+                    f.push_back(-1);
+                }
+                if (row >= i) continue;
+                f.push_back(FORWARD_ELIM);
+                f.push_back(2 * row);
+                if (nsib == 0) {
+                    const double temp = -dt / (Cm * Rav[k] * coeffs[2 * i]);
+                    ra.push_back(temp);
+                    dg[i] -= temp;
+                } else {
+                    const int m = RP(i, maxkids + j + 1);
+                    if (m <= 0 || f[m] != -1) return bug(5);
+                    if (!resolve(m, (int)ra.size(), true)) return bug(-5);
+                    RP(i, maxkids + j + 1) = (int)ra.size();
+                    ra.push_back(0.0);
+                }
+                if (nsib == 0) {
+                    const double temp = -dt / (prod(Cmv[k], Rav[ci]) * coeffs[2 * row]);
+                    RP(row, maxkids + self + 1) = -(int)ra.size();
+                    ra.push_back(temp);
+                    dg[row] -= temp;
+                } else {
+                    const int m = RP(row, maxkids + j + 1);
+                    if (m <= 0 || f[m] != -1) return bug(6);
+                    if (!resolve(m, (int)ra.size(), true)) return bug(-6);
+                    ra.push_back(0.0);
+                    RP(row, maxkids + j + 1) = 0;
+                }
+                if (self > 0) {
+                    int m = RP(row, 0);
+                    if (m == -1) {
+                        f.push_back(FASTSIBARRAY_ELIM);
+                        RP(row, 0) = -(int)ra.size();
+                        const double temp = -dt / (prod(Cmv[k], Rav[pn]) * coeffs[2 * row]);
+                        ra.push_back(temp);
+                        dg[row] -= temp;
+                        m = RP(i, 0);
+                        if (m <= 0) return bug(-7);
+                        f.push_back(-m);
+                    } else {
+                        f.push_back(SIBARRAY_ELIM);
+                        if (m == -1 || m == 0 || (j > 0 && m < 0)) return bug(7);
+                        f.push_back(-m);
+                        m = RP(i, 0);
+                        if (m <= 0) return bug(-7);
+                        f.push_back(-m);
+                    }
+                    for (int l = j + 1; l < np; l++) {
+                        if (l == self) continue;
+                        f.push_back(SIBARRAY_ELIM);
+                        m = RP(row, maxkids + l + 1);
+                        if (m == -1 || m == 0 || (j > 0 && m < 0)) return bug(8);
+                        f.push_back(-m);
+                        m = RP(i, maxkids + l + 1);
+                        if (m <= 0) return bug(-8);
+                        f.push_back(-m);
+                    }
+                    nsib++;
+                }
+            }
+        } else {
+            for (int l = maxkids + 1; l <= 2 * maxkids; l++) RP(i, l) = 0;
+        }
+    }
+    for (int v : f)
+        if (v < 0) return bug(9);
+    f.push_back(FINISH);
+    // backward pass: rows from the second last down; the parent's
+    // off-diagonal first, then those to later siblings (hines_init.c:1020-1160)
+    for (int i = n - 2; i >= 0; i--) {
+        const int ci = elmnum[i];
+        const double Cm = Cmv[ci];
+        const int pn = parents[ci];
+        if (pn != -1) {
+            const int np = nkids[pn];
+            for (int j = 0; j < np; j++)
+                if (kid(pn, j) == ci) {
+                    self = j;
+                    break;
+                }
+            const int row = hnum[pn];
+            if (self > 0) {
+                f.push_back(SIBARRAY_ELIM);
+                if (RP(i, 0) >= 0) return bug(11);
+                f.push_back(-RP(i, 0));
+                f.push_back(2 * row);
+            } else {
+                f.push_back(BACKWARD_ELIM);
+                f.push_back(2 * row);
+                ra.push_back(-dt / (Cm * Rav[pn] * coeffs[2 * i]));
+            }
+            for (int j = 0; j < np; j++) {
+                const int k = kid(pn, j);
+                const int rk = hnum[k];
+                if (rk <= i) continue;
+                if (j > 0) {
+                    f.push_back(SIBARRAY_ELIM);
+                    if (RP(i, maxkids + j + 1) >= 0) return bug(12);
+                    f.push_back(-RP(i, maxkids + j + 1));
+                    f.push_back(2 * rk);
+                } else {
+                    f.push_back(BACKWARD_ELIM);
+                    f.push_back(2 * rk);
+                    ra.push_back(-dt / (Cm * Rav[k] * coeffs[2 * i]));
+                }
+            }
+        }
+        f.push_back(CALC_RESULTS);
+    }
+    f.push_back(FINISH);
+    return 0;
+}
 }  // namespace
 
 extern "C" int32_t hsm_build(const hsm_spec* s, hsm_out** out_p)
@@ -440,49 +751,57 @@ extern "C" int32_t hsm_build(const hsm_spec* s, hsm_out** out_p)
     }
     ops.push_back(LCOMPT_OP);
 
-    // ---- elimination program (hines_init.c:361-1273, asymmetric branch)
+    // ---- elimination program (hines_init.c:361-1273)
     std::vector<int> funcs;
     std::vector<double> ravals;
     std::vector<double> diagterms(n);
     int skipdiag = 0;
-    for (int i = 0; i < n; i++) {
-        int ci = elmnum[i];
-        double Cm = fCm[ci];
-        diagterms[i] = 1.0 + cdt / (Cm * fRm[ci]);
-        int parentno = parents[ci];
-        int nk = nkids[ci];
-        if (skipdiag) {
-            skipdiag = 0;
-        } else if (nk || parentno == -1) {
-            if (i > 1) funcs.push_back(SET_DIAG);
-        } else {
-            if (i > 1) { funcs.push_back(SKIP_DIAG); skipdiag = 1; }
+    if (s->symmetric) {
+        SymOut so;
+        if (sym_funcs(n, cdt, elmnum, hnum, parents, nkids, kstart, kids, fCm, fRm, fRa, ff, so) != 0) return -1;
+        funcs = so.funcs;
+        ravals = so.ravals;
+        diagterms = so.diag;
+    } else {
+        for (int i = 0; i < n; i++) {
+            int ci = elmnum[i];
+            double Cm = fCm[ci];
+            diagterms[i] = 1.0 + cdt / (Cm * fRm[ci]);
+            int parentno = parents[ci];
+            int nk = nkids[ci];
+            if (skipdiag) {
+                skipdiag = 0;
+            } else if (nk || parentno == -1) {
+                if (i > 1) funcs.push_back(SET_DIAG);
+            } else {
+                if (i > 1) { funcs.push_back(SKIP_DIAG); skipdiag = 1; }
+            }
+            for (int q = kstart[ci]; q < kstart[ci + 1]; q++) {
+                int k = kids[q];
+                int row = hnum[k];
+                funcs.push_back(FORWARD_ELIM);
+                funcs.push_back(2 * row);
+                double temp = -cdt / (Cm * fRa[k]);
+                ravals.push_back(temp);
+                diagterms[i] -= temp;
+                temp = -cdt / cm_ra(k);
+                ravals.push_back(temp);
+                diagterms[row] -= temp;
+            }
         }
-        for (int q = kstart[ci]; q < kstart[ci + 1]; q++) {
-            int k = kids[q];
-            int row = hnum[k];
-            funcs.push_back(FORWARD_ELIM);
-            funcs.push_back(2 * row);
-            double temp = -cdt / (Cm * fRa[k]);
-            ravals.push_back(temp);
-            diagterms[i] -= temp;
-            temp = -cdt / cm_ra(k);
-            ravals.push_back(temp);
-            diagterms[row] -= temp;
+        funcs.push_back(FINISH);
+        for (int i = n - 2; i >= 0; i--) {
+            int ci = elmnum[i];
+            int parentno = parents[ci];
+            if (parentno != -1) {
+                funcs.push_back(BACKWARD_ELIM);
+                funcs.push_back(2 * hnum[parentno]);
+                ravals.push_back(-cdt / (fCm[ci] * fRa[ci]));
+            }
+            funcs.push_back(CALC_RESULTS);
         }
+        funcs.push_back(FINISH);
     }
-    funcs.push_back(FINISH);
-    for (int i = n - 2; i >= 0; i--) {
-        int ci = elmnum[i];
-        int parentno = parents[ci];
-        if (parentno != -1) {
-            funcs.push_back(BACKWARD_ELIM);
-            funcs.push_back(2 * hnum[parentno]);
-            ravals.push_back(-cdt / (fCm[ci] * fRa[ci]));
-        }
-        funcs.push_back(CALC_RESULTS);
-    }
-    funcs.push_back(FINISH);
     // diagterms into the last chip slot of each compartment
     // (hines_init.c:1214-1229)
     for (int i = 0; i < n; i++) {

@@ -87,6 +87,11 @@ typedef struct hsm_spec {
     const int32_t *syn_comp;
     const int32_t *syn_type;
     const double  *syn_gmax;
+
+    /* 1: the compartments are symcompartments (hines_init.c:101-113): the
+       elimination program couples siblings through their junction */
+    int32_t symmetric;
+    int32_t pad3;
 } hsm_spec;
 
 /* Emitted hsolve arrays.  Row index = Hines number. */

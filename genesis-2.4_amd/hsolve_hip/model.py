@@ -49,6 +49,7 @@ class Spec(C.Structure):
         ("spike_thresh", _f64p), ("spike_refract", _f64p),
         ("nsyntypes", C.c_int32), ("nsyn", C.c_int32), ("syntypes", C.POINTER(SynType)),
         ("syn_comp", _i32p), ("syn_type", _i32p), ("syn_gmax", _f64p),
+        ("symmetric", C.c_int32), ("pad3", C.c_int32),
     ]
 
 
@@ -205,7 +206,7 @@ def hh_chantypes(erest: float, ena: float | None = None, ek: float | None = None
 def build(comps: dict, chantypes: list, chans: list, *, dt: float, method: int = BEULER,
           chanmode: int = 4, xdivs: int = 3000, xmin: float = -0.1, xmax: float = 0.05,
           spikes: list | None = None, syntypes: list | None = None,
-          syns: list | None = None, float_fields: bool = True) -> Model:
+          syns: list | None = None, float_fields: bool = True, symmetric: bool = False) -> Model:
     """Build a solver from element-level descriptions.
 
     comps: dict of equal-length sequences parent, Cm, Rm, Ra, Em, initVm, inject
@@ -214,6 +215,9 @@ def build(comps: dict, chantypes: list, chans: list, *, dt: float, method: int =
     chans: list of (comp, type, gbar) in creation order.
     spikes: list of (comp, thresh, abs_refract).
     syntypes: list of dict(Ek, tau1, tau2); syns: list of (comp, type, gmax).
+    symmetric: the compartments are symcompartments (siblings couple through
+    their junction; the elimination program then uses COPY_ARRAY /
+    SIBARRAY_ELIM / FASTSIBARRAY_ELIM).
     """
     spikes = spikes or []
     syntypes = syntypes or []
@@ -246,6 +250,7 @@ def build(comps: dict, chantypes: list, chans: list, *, dt: float, method: int =
     s = Spec()
     s.dt, s.method, s.chanmode, s.xdivs, s.xmin, s.xmax = dt, method, chanmode, xdivs, xmin, xmax
     s.float_fields = 1 if float_fields else 0
+    s.symmetric = 1 if symmetric else 0
     s.ncomp = n
     s.parent = _p(parent, C.c_int32)
     for k in ("Cm", "Rm", "Ra", "Em", "initVm", "inject"):
