@@ -134,6 +134,10 @@ struct hsh_handle {
     int32_t *d_I = nullptr, *d_par = nullptr, *d_pred = nullptr, *d_step = nullptr, *d_lvl = nullptr,
             *d_fent = nullptr, *d_bkp = nullptr;
     uint16_t* d_kidx = nullptr;
+    // funcs[] units (symmetric compartments): units, programs, mutable ravals
+    FuncsUnitDev* d_funits = nullptr;
+    int32_t* d_ffuncs = nullptr;
+    double* d_fwork = nullptr;
     DevModel dm{};
     JitModule jit;
     std::vector<KClass> classes;
@@ -333,7 +337,7 @@ void free_layout(hsh_handle* h)
     free_graphs(h);
     void* ptrs[] = {h->d_blocks, h->d_tiles, h->d_packs, h->d_S, h->d_V, h->d_R, h->d_ta, h->d_tcf,
                     h->d_tcb, h->d_kta, h->d_ktcf, h->d_btcb, h->d_I, h->d_par, h->d_pred, h->d_step,
-                    h->d_lvl, h->d_fent, h->d_bkp, h->d_kidx};
+                    h->d_lvl, h->d_fent, h->d_bkp, h->d_kidx, h->d_funits, h->d_ffuncs, h->d_fwork};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_blocks = nullptr;
@@ -342,6 +346,9 @@ void free_layout(hsh_handle* h)
     h->d_S = h->d_V = h->d_R = h->d_ta = h->d_tcf = h->d_tcb = h->d_kta = h->d_ktcf = h->d_btcb = nullptr;
     h->d_I = h->d_par = h->d_pred = h->d_step = h->d_lvl = h->d_fent = h->d_bkp = nullptr;
     h->d_kidx = nullptr;
+    h->d_funits = nullptr;
+    h->d_ffuncs = nullptr;
+    h->d_fwork = nullptr;
 }
 
 void free_events(hsh_handle* h)
@@ -408,7 +415,10 @@ void compute_classes(hsh_handle* h)
         if (rows <= 64) return (int)C_WIDE;
         return (int)C_PLAINLOOP;
     };
-    for (int t = 0; t < (int)P.tiles.size(); t++) {
+    // funcs[] units run their own interpreter, so a handle with them steps
+    // plain (two launches and the units' one)
+    if (!P.funits.empty()) h->plain_loop = true;
+    for (int t = 0; t < P.ntiles_regular; t++) {
         const int k = kind_of(P.tiles[t].rows);
         if (k == C_PLAINLOOP) h->plain_loop = true;
         if (!h->classes.empty() && h->classes.back().kind == k) {
@@ -840,6 +850,23 @@ hsh_status alloc_layout(hsh_handle* h, const double* vm, const double* results)
     HIPCHK(dalloc(&h->d_ktcf, P.ktcf.size()));
     HIPCHK(dalloc(&h->d_bkp, P.bkp.size()));
     HIPCHK(dalloc(&h->d_btcb, P.btcb.size()));
+    if (!P.funits.empty()) {
+        std::vector<FuncsUnitDev> fu(P.funits.size());
+        for (size_t u = 0; u < P.funits.size(); u++) {
+            fu[u].tile = P.funit_tile[u];
+            fu[u].lane = P.funit_lane[u];
+            fu[u].nn = P.funits[u].nn;
+            fu[u].pad = 0;
+            fu[u].f0 = P.funits[u].f0;
+            fu[u].q0 = P.funits[u].q0;
+        }
+        HIPCHK(dalloc(&h->d_funits, fu.size()));
+        HIPCHK(dalloc(&h->d_ffuncs, P.ffuncs.size()));
+        HIPCHK(dalloc(&h->d_fwork, P.fravals.size()));
+        HIPCHK(dput(h->d_funits, fu));
+        HIPCHK(dput(h->d_ffuncs, P.ffuncs));
+        HIPCHK(dput(h->d_fwork, P.fravals));
+    }
     // tree and predecessor arrays of the tile layout (row within the tile)
     std::vector<int32_t> par(npos), pred(npos);
     std::vector<double> ta(npos), tcf(npos), tcb(npos);
@@ -1020,8 +1047,15 @@ hsh_status plain_step(hsh_handle* h, bool channels_only)
     if (channels_only) {
         HIPCHK(launch_step_advance(h->dm, 1, h->stream));
     } else {
-        if (h->dm.ntiles > 0) HIPCHK(launch_elim(h->dm, h->plan.max_rows, h->stream));
+        if (h->plan.ntiles_regular > 0) {
+            DevModel dr = h->dm;
+            dr.ntiles = h->plan.ntiles_regular;
+            HIPCHK(launch_elim(dr, h->plan.max_rows, h->stream));
+        }
         if (h->dm.npacks > 0) HIPCHK(launch_forest_elim(h->dm, pack_lds_bytes(h, false), h->stream));
+        if (!h->plan.funits.empty())
+            HIPCHK(launch_funcs(h->dm, h->d_funits, (int)h->plan.funits.size(), h->d_ffuncs, h->d_fwork,
+                                h->stream));
     }
     if (h->timing) {
         HIPCHK(hipEventRecord(h->evs[h->nev_used + 2], h->stream));
@@ -1193,6 +1227,10 @@ constexpr int ADJ_MAXG = 3;     // gates per channel of the reverse program
 
 bool adj_supported(const Plan& P, std::string& why)
 {
+    if (!P.funits.empty()) {
+        why = "symmetric compartments (sibling elimination ops)";
+        return false;
+    }
     for (size_t t = 0; t < P.templates.size(); t++) {
         const std::vector<int32_t>& code = P.templates[t].code;
         int ng = 0;
@@ -2207,8 +2245,11 @@ hsh_status hsh_set_ravals(hsh_handle* h, const double* ravals)
             P.btcb[e] = (P.bkp[e] >> 16) > 0 ? P.tcb[r] : 0.0;
         }
     }
+    for (const Plan::FuncsUnit& u : P.funits)
+        std::copy(ravals + u.qg, ravals + u.qg + u.nq, P.fravals.begin() + u.q0);
     if (h->host_only) return HSH_OK;
     HIPCHK(hipStreamSynchronize(h->stream));
+    if (!P.funits.empty()) HIPCHK(dput(h->d_fwork, P.fravals));
     HIPCHK(dput(h->d_ta, ta));
     HIPCHK(dput(h->d_tcf, tcf));
     HIPCHK(dput(h->d_tcb, tcb));

@@ -66,7 +66,8 @@ struct TileMeta {
     int block0;   // first block (row 0)
     int rows;     // rows of every neuron in the tile
     int lanes;    // active lanes (neurons)
-    int pad;
+    int units;    // 1: each lane is a funcs[] unit (symmetric compartments), solved by
+                  // the funcs interpreter; the tile kernels never see these tiles
 };
 
 // A pack: whole neurons of any sizes whose compartments fill blocks

@@ -542,10 +542,97 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
     {
         // One solver's funcs[]/ravals[] (hines_solve.c:223-296) for rows
         // r0 .. r0+nn-1, its ravals at q0.. of the merged ravals.
+        // A program with sibling ops (symmetric compartments,
+        // hines_solve.c:180-184, :262-268, :289-291) runs as a unit; it is
+        // checked here so the device interpreter stays inside its arrays:
+        // every ravals index and row in range, the row cursor on the last row
+        // at the forward FINISH, one CALC_RESULTS per row below it.  Returns
+        // 2 for such a program (unit recorded), 0 if it holds no sibling op,
+        // -2 if it is not one the interpreter can run.
+        auto sibling_unit = [&](const int32_t* f, int64_t nf, const double* ra, int64_t nra, int nn, int r0,
+                                int64_t q0) -> int {
+            bool sib = false;
+            int64_t fi = 0, ri = 0;
+            int row = 1;
+            auto idx_ok = [&](int64_t v) { return v >= 0 && v < nra; };
+            auto row_ok = [&](int64_t v) { return v >= 0 && !(v & 1) && v / 2 < nn; };
+            for (;;) {
+                if (fi >= nf) return -2;
+                const int32_t op = f[fi++];
+                if (op == F_ELIM) {
+                    if (fi >= nf || !row_ok(f[fi]) || ri + 2 > nra) return -2;
+                    fi++;
+                    ri += 2;
+                } else if (op == F_SET_DIAG || op == F_SKIP_DIAG) {
+                    row += op == F_SET_DIAG ? 1 : 2;
+                    if (row >= nn) return -2;
+                } else if (op == F_FASTSIB || op == F_COPY) {
+                    if (fi >= nf || !idx_ok(f[fi]) || ri + 1 > nra) return -2;
+                    fi++;
+                    ri++;
+                    sib = true;
+                } else if (op == F_SIB) {
+                    if (fi + 1 >= nf || !idx_ok(f[fi]) || !idx_ok(f[fi + 1])) return -2;
+                    fi += 2;
+                    sib = true;
+                } else if (op == F_FINISH) {
+                    break;
+                } else {
+                    return -2;
+                }
+            }
+            if (row != nn - 1) return sib ? -2 : 0;
+            int vi = nn - 1;
+            for (;;) {
+                if (fi >= nf) return -2;
+                const int32_t op = f[fi++];
+                if (op == F_ELIM) {
+                    if (fi >= nf || !row_ok(f[fi]) || ri + 1 > nra) return -2;
+                    fi++;
+                    ri++;
+                } else if (op == F_CALC) {
+                    if (vi == 0) return -2;
+                    vi--;
+                } else if (op == F_SIB) {
+                    if (fi + 1 >= nf || !idx_ok(f[fi]) || !row_ok(f[fi + 1])) return -2;
+                    fi += 2;
+                    sib = true;
+                } else if (op == F_FINISH) {
+                    break;
+                } else {
+                    return -2;
+                }
+            }
+            if (!sib) return 0;
+            if (vi != 0) return -2;
+            Plan::FuncsUnit u;
+            u.r0 = r0;
+            u.nn = nn;
+            u.f0 = (int64_t)P.ffuncs.size();
+            u.nf = fi;
+            u.q0 = (int64_t)P.fravals.size();
+            u.nq = nra;
+            u.qg = q0;
+            P.ffuncs.insert(P.ffuncs.end(), f, f + fi);
+            P.fravals.insert(P.fravals.end(), ra, ra + nra);
+            P.funits.push_back(u);
+            // the unit's rows as one tree of the layout: a chain of
+            // placeholder parents ending at its last row
+            for (int r = 0; r + 1 < nn; r++) P.parent[r0 + r] = r0 + r + 1;
+            return 2;
+        };
         auto decode = [&](const int32_t* f, int64_t nf, const double* ra, int64_t nra, int nn, int r0,
                           int64_t q0) -> int {   // 1 ok, 0 refused, -1 malformed
             int64_t fi = 0, ri = 0;
             if (nn <= 1) return 1;
+            {
+                const int u = sibling_unit(f, nf, ra, nra, nn, r0, q0);
+                if (u == 2) return 1;
+                if (u == -2) {
+                    refuse(P, r0, F_SIB, -1, HSH_R_FUNCS);
+                    return 0;
+                }
+            }
             int cur = 1;
             for (;;) {
                 if (fi >= nf) { err = "funcs[] missing forward FINISH"; return -1; }
@@ -637,6 +724,10 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
                 P.tree_hi.push_back(r);
                 lo = r + 1;
             }
+            std::vector<int32_t> unit_at(n, -1);
+            for (size_t u = 0; u < P.funits.size(); u++) unit_at[P.funits[u].r0] = (int32_t)u;
+            P.tree_funit.assign(P.tree_lo.size(), -1);
+            for (size_t t = 0; t < P.tree_lo.size(); t++) P.tree_funit[t] = unit_at[P.tree_lo[t]];
         }
     }
     if (!P.refusals.empty()) return true;
@@ -698,9 +789,10 @@ bool build_layout(Plan& P, const LayoutOpts& o, std::string& err)
     // a pack's level schedule keeps positions in 15 bits, and its V/rhs/diag
     // and predecessor index (28 bytes a position) must fit the LDS
     constexpr int PACK_MAX_POS = 4096;
+    const auto is_unit = [&](int t) { return !P.tree_funit.empty() && P.tree_funit[t] >= 0; };
     auto packable = [&](int t) {
         const int R = rows_of(t);
-        if (R <= o.tile_max_rows) return false;
+        if (R <= o.tile_max_rows || is_unit(t)) return false;
         // worst case: every row of the tree alone in its program's block
         const int ntmpl = (int)P.templates.size();
         return (int64_t)R + (int64_t)ntmpl * (W - 1) <= PACK_MAX_POS;
@@ -716,12 +808,21 @@ bool build_layout(Plan& P, const LayoutOpts& o, std::string& err)
         std::vector<int32_t> sig;
         sig.push_back(rows_of(t));
         for (int r = P.tree_lo[t]; r <= P.tree_hi[t]; r++) sig.push_back(P.comp_tmpl[r]);
+        sig.push_back(is_unit(t) ? 1 : 0);     // units share tiles only with units
         groups[sig].push_back(t);
     }
-    // larger trees first: they run longest in the persistent kernel
+    // larger trees first: they run longest in the persistent kernel; the
+    // units' tiles after all others (the tile kernels take the first
+    // ntiles_regular tiles)
     std::vector<const std::pair<const std::vector<int32_t>, std::vector<int32_t>>*> order;
     for (auto& g : groups) order.push_back(&g);
-    std::stable_sort(order.begin(), order.end(), [](auto* a, auto* b) { return a->first[0] > b->first[0]; });
+    std::stable_sort(order.begin(), order.end(), [](auto* a, auto* b) {
+        if (a->first.back() != b->first.back()) return a->first.back() < b->first.back();
+        return a->first[0] > b->first[0];
+    });
+    P.funit_tile.assign(P.funits.size(), -1);
+    P.funit_lane.assign(P.funits.size(), -1);
+    P.ntiles_regular = 0;
 
     int64_t nS = 0, nI = 0;
     auto add_block = [&](int ti, int tile, int lanes) {
@@ -741,15 +842,21 @@ bool build_layout(Plan& P, const LayoutOpts& o, std::string& err)
         const std::vector<int32_t>& sig = g->first;
         const std::vector<int32_t>& trees = g->second;
         int R = sig[0];
-        P.max_rows = std::max(P.max_rows, R);
+        const bool units = sig.back() != 0;
+        if (!units) P.max_rows = std::max(P.max_rows, R);
         for (size_t t0 = 0; t0 < trees.size(); t0 += W) {
             TileMeta tm;
             tm.block0 = (int32_t)P.blocks.size();
             tm.rows = R;
             tm.lanes = (int32_t)std::min<size_t>(W, trees.size() - t0);
-            tm.pad = 0;
+            tm.units = units ? 1 : 0;
             int tile = (int)P.tiles.size();
             P.tiles.push_back(tm);
+            if (!units) P.ntiles_regular = (int)P.tiles.size();
+            for (int l = 0; units && l < tm.lanes; l++) {
+                P.funit_tile[P.tree_funit[trees[t0 + l]]] = tile;
+                P.funit_lane[P.tree_funit[trees[t0 + l]]] = l;
+            }
             for (int r = 0; r < R; r++) {
                 const int b = add_block(sig[1 + r], tile, tm.lanes);
                 for (int l = 0; l < tm.lanes; l++) {

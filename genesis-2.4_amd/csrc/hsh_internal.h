@@ -85,6 +85,23 @@ struct Plan {
     int64_t nS = 0, nI = 0;
     int max_rows = 0;                     // rows of the largest tiled tree
     int max_pack_pos = 0;                 // positions of the largest pack
+    // Solvers of symmetric compartments: their elimination program couples
+    // siblings (COPY_ARRAY, SIBARRAY_ELIM, FASTSIBARRAY_ELIM), so each runs
+    // as one unit, one lane interpreting its funcs[] over a mutable ravals
+    // copy.  Its rows form one tree of the layout (a chain of placeholder
+    // parents), placed in tiles after the other trees.
+    struct FuncsUnit {
+        int32_t r0, nn;                   // rows
+        int64_t f0, nf;                   // its program in ffuncs
+        int64_t q0, nq;                   // its ravals in fravals
+        int64_t qg;                       // and in the caller's (merged) ravals
+    };
+    std::vector<FuncsUnit> funits;
+    std::vector<int32_t> ffuncs;
+    std::vector<double> fravals;
+    std::vector<int32_t> tree_funit;      // per tree: its unit, -1
+    std::vector<int32_t> funit_tile, funit_lane;   // per unit: tile and lane (build_layout)
+    int ntiles_regular = 0;               // tiles before the units' tiles
     int64_t pack_lds = 0, pack_lds_stage = 0;   // LDS bytes of the pack kernels
 };
 
@@ -163,6 +180,15 @@ hipError_t launch_ev_post(const DevModel& d, const EvPost& p, EvCtl* ctl, hipStr
 // A message input applied to a device slot before a step (h_in_msgs,
 // hines_msgs.c:351-405): mode 0 sets S[pos] = value, mode 1 adds value.
 // Entries are sorted by position, in the caller's order within one.
+// A unit of hsh_kernels.hip's funcs[] interpreter: tile and lane of its rows
+// (row r at block tiles[tile].block0 + r), its program and ravals offsets.
+struct FuncsUnitDev {
+    int tile, lane, nn, pad;
+    long long f0, q0;
+};
+hipError_t launch_funcs(const DevModel& d, const FuncsUnitDev* u, int nu, const int32_t* ffuncs, double* fwork,
+                        hipStream_t s);
+
 struct InEntry {
     int pos;
     int mode;

@@ -285,6 +285,86 @@ __global__ __launch_bounds__(64) void k_elim_l(DevModel d, int M)
     }
 }
 
+// The elimination of a solver of symmetric compartments: one thread per
+// unit interprets its funcs[] as do_euler_hsolve / do_crank_hsolve do
+// (hines_solve.c:140-296), with results[2r] and results[2r+1] at rhs and
+// diag of row r's block, and a mutable copy of its ravals: COPY_ARRAY fills
+// working slots from the ravals stream, FASTSIBARRAY_ELIM and SIBARRAY_ELIM
+// update them as pivot rows are eliminated, FORWARD_ELIM / BACKWARD_ELIM
+// read the stream (working slots included) in order.  The host checked every
+// index and the row cursor's path (sibling_unit, hsh_ingest.cpp).
+__global__ __launch_bounds__(64) void k_funcs(DevModel d, const FuncsUnitDev* __restrict__ units, int nu,
+                                              const int32_t* __restrict__ ffuncs, double* __restrict__ fwork)
+{
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    const FuncsUnitDev U = units[u];
+    const int b0 = d.tiles[U.tile].block0;
+    const int lane = U.lane;
+    const int32_t* f = ffuncs + U.f0;
+    double* raval = fwork + U.q0;
+    if (lane == 0) d.step[U.tile] += 1;
+    auto RB = [&](int r) -> double& { return d.R[(long long)(b0 + r) * 2 * W + lane]; };
+    auto RD = [&](int r) -> double& { return d.R[(long long)(b0 + r) * 2 * W + W + lane]; };
+    auto VM = [&](int r) -> double& { return d.V[(long long)(b0 + r) * W + lane]; };
+    int fi = 0, ra = 0, row = 1;
+    double temp = 0.0, rv = RB(1), dv = RD(1);
+    for (;;) {
+        const int op = f[fi++];
+        if (op == 0) {                    // FORWARD_ELIM
+            const int k = f[fi++] >> 1;
+            temp = raval[ra++] / RD(k);
+            dv -= raval[ra++] * temp;
+            rv -= RB(k) * temp;
+        } else if (op == 1 || op == 2) {  // SET_DIAG, SKIP_DIAG
+            RB(row) = rv;
+            RD(row) = dv;
+            row += op;
+            rv = RB(row);
+            dv = RD(row);
+        } else if (op == 3) {             // FASTSIBARRAY_ELIM
+            raval[f[fi++]] -= raval[ra++] * temp;
+        } else if (op == 4) {             // COPY_ARRAY
+            raval[f[fi++]] = raval[ra++];
+        } else if (op == 5) {             // SIBARRAY_ELIM
+            raval[f[fi + 1]] -= raval[f[fi]] * temp;
+            fi += 2;
+        } else {                          // FINISH
+            break;
+        }
+    }
+    // the last row, then the back-substitution up the rows
+    int vi = U.nn - 1;
+    if (d.crank) {
+        rv = rv / dv;
+        VM(vi) = rv + rv - VM(vi);
+    } else {
+        rv = rv / dv;
+        VM(vi) = rv;
+    }
+    RB(row) = rv;
+    row--;
+    rv = RB(row);
+    for (;;) {
+        const int op = f[fi++];
+        if (op == 0) {                    // BACKWARD_ELIM
+            rv -= raval[ra++] * RB(f[fi++] >> 1);
+        } else if (op == 6) {             // CALC_RESULTS
+            rv = rv / RD(row);
+            vi--;
+            VM(vi) = d.crank ? rv + rv - VM(vi) : rv;
+            RB(row) = rv;
+            row--;
+            if (row >= 0) rv = RB(row);
+        } else if (op == 5) {             // SIBARRAY_ELIM
+            rv -= raval[f[fi]] * RB(f[fi + 1] >> 1);
+            fi += 2;
+        } else {                          // FINISH
+            break;
+        }
+    }
+}
+
 __global__ void k_step_advance(int32_t* step, int32_t n, int32_t by)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -425,6 +505,14 @@ hipError_t launch_forest_elim(const DevModel& d, int lds, hipStream_t s)
 {
     if (d.npacks <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_forest_elim, dim3(d.npacks), dim3(256), lds, s, d);
+    return hipGetLastError();
+}
+
+hipError_t launch_funcs(const DevModel& d, const FuncsUnitDev* u, int nu, const int32_t* ffuncs, double* fwork,
+                        hipStream_t s)
+{
+    if (nu <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_funcs, dim3((nu + 63) / 64), dim3(64), 0, s, d, u, nu, ffuncs, fwork);
     return hipGetLastError();
 }
 

@@ -41,7 +41,7 @@ F_HOST_ONLY = 2
 F_NO_JIT = 4
 
 REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
-           5: "funcs (sibling ops)", 6: "tree layout", 7: "calcmode", 8: "pending synaptic event",
+           5: "funcs (elimination program the device cannot run)", 6: "tree layout", 7: "calcmode", 8: "pending synaptic event",
            9: "shared pool"}
 
 

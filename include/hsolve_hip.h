@@ -138,8 +138,10 @@ typedef struct hsh_options {
 #define HSH_R_EK_LIVEIN      2   /* channel inherits a computed Ek from an earlier compartment */
 #define HSH_R_TABLE_LIVEIN   3   /* gate lookup without NEWVOLT in its compartment */
 #define HSH_R_REGISTER_LIVEIN 4  /* other register carried in from an earlier compartment */
-#define HSH_R_FUNCS          5   /* elimination program the device solver does not run
-                                    (sibling ops of symcompartments) */
+#define HSH_R_FUNCS          5   /* elimination program the device solver does not run: an
+                                    operand out of range, or a row cursor that leaves
+                                    the solver (sibling programs of symcompartments run
+                                    as units after the host has walked them) */
 #define HSH_R_TREE           6   /* tree rows are not contiguous / kid order not ascending */
 #define HSH_R_CALCMODE       7   /* calcmode 0 (no interpolation) */
 #define HSH_R_SYN_PENDING    8   /* SYN2_OP counter holds a pending event (>= 0) */

@@ -138,3 +138,41 @@ def with_mgblock(m: M.Model, which, A: float = 3.57, invB: float = 62.0, Mg: flo
         spike_op=remap(m.spike_op).astype(np.int32), syn_op=remap(m.syn_op).astype(np.int32),
         syn_chip=remap_chip(m.syn_chip).astype(np.int32))
     return out, remap
+
+
+def symmetric_model(n_trees: int, nmax: int, *, seed: int = 1, method: int = M.BEULER,
+                    dt: float = 20e-6) -> M.Model:
+    """Random trees of symmetric compartments (two kids per junction at most:
+    the reference's program builder stops at three), HH Na and K in every
+    root, K in the others, one solver."""
+    rng = np.random.default_rng(seed)
+    na, k = M.hh_chantypes(-0.065)
+    parent, Cm, Rm, Ra, Em, V0, inj, chans = [], [], [], [], [], [], [], []
+    for t in range(n_trees):
+        n = int(rng.integers(1, nmax + 1))
+        base = len(parent)
+        nk = [0]
+        for i in range(n):
+            if i == 0:
+                parent.append(-1)
+            else:
+                while True:
+                    p = int(rng.integers(0, i))
+                    if nk[p] < 2:
+                        break
+                nk[p] += 1
+                parent.append(base + p)
+            nk.append(0)
+            L, d = (30e-6, 30e-6) if i == 0 else (rng.uniform(10e-6, 100e-6), rng.uniform(0.5e-6, 4e-6))
+            area = np.pi * d * L
+            Cm.append(0.01 * area)
+            Rm.append(0.33 / area)
+            Ra.append(1.0 * L / (np.pi * (d / 2) ** 2))
+            Em.append(-0.065)
+            V0.append(-0.065)
+            inj.append(rng.uniform(0.2e-9, 1.0e-9) if i == 0 else 0.0)
+            if i == 0:
+                chans.append((base + i, 0, 1200.0 * area))
+            chans.append((base + i, 1, (360.0 if i == 0 else 50.0) * area))
+    comps = dict(parent=parent, Cm=Cm, Rm=Rm, Ra=Ra, Em=Em, initVm=V0, inject=inj)
+    return M.build(comps, [na, k], chans, dt=dt, method=method, symmetric=True)

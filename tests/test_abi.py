@@ -255,21 +255,62 @@ def test_group_offsets_refusals_and_compatibility():
     assert opidx == a4.ops.size - 1 + int(syn.syn_op[0])
 
 
-@pytest.mark.parametrize("op", [3, 4, 5], ids=["FASTSIB", "COPY", "SIBARRAY"])
-@pytest.mark.parametrize("where", ["forward", "backward"])
-def test_refuses_symcompartment_elimination_ops(op, where):
-    """The symcompartment elimination ops of funcs[] (FASTSIBARRAY_ELIM,
-    COPY_ARRAY, SIBARRAY_ELIM; hines_solve.c:262-268, :289-291) are refused
-    with their reason: the solver keeps its CPU path."""
-    m = random_tree_model(2, 4, seed=3)
+def test_symmetric_solver_is_a_unit():
+    """A solver of symmetric compartments (its funcs[] holds COPY_ARRAY,
+    SIBARRAY_ELIM, FASTSIBARRAY_ELIM) is accepted: its rows form one unit in
+    a tile of their own, after the other trees' tiles."""
+    from helpers import symmetric_model
+    m = symmetric_model(3, 30, seed=4)
+    with D.HipSolver(m, host_only=True) as h:
+        info = h.info()
+        assert info["refused"] == 0 and info["ntrees"] == 1 and info["npacks"] == 0
+    with D.HipSolver([random_tree_model(70, 16, seed=1), m], host_only=True) as h:
+        info = h.info()
+        assert info["refused"] == 0 and info["ntiles"] == 3     # two of 64 trees, the unit's
+
+
+def _sibling_ops(funcs):
+    """(index, opcode) of every op of an elimination program."""
+    f, out, i, fwd = list(funcs), [], 0, True
+    while i < len(f):
+        out.append((i, f[i]))
+        if f[i] == 7:
+            if not fwd:
+                break
+            fwd = False
+            i += 1
+        else:
+            i += {0: 2, 3: 2, 4: 2, 5: 3}.get(f[i], 1)
+    return out
+
+
+@pytest.mark.parametrize("damage", ["ravals-index", "row-index", "past-last-row", "missing-calc"])
+def test_refuses_malformed_sibling_programs(damage):
+    """The device interprets a sibling program only after the host has
+    walked it: every ravals index and row in range, the row cursor on the
+    last row at the forward FINISH and one CALC_RESULTS per row below it.
+    A program that fails is refused with its reason (the CPU path keeps the
+    solver)."""
+    from helpers import symmetric_model
+    m = symmetric_model(1, 30, seed=4)
     f = np.array(m.funcs, dtype=np.int32)
-    i = 0 if where == "forward" else int(np.nonzero(f == 7)[0][0]) + 1
-    f[i] = op
+    ops = _sibling_ops(f)
+    if damage == "ravals-index":
+        i = next(i for i, op in ops if op == 4)
+        f[i + 1] = m.ravals.size + 3
+    elif damage == "row-index":
+        i = next(i for i, op in ops if op == 0)
+        f[i + 1] = 2 * m.ncompts
+    elif damage == "past-last-row":
+        i = next(i for i, op in ops if op == 1)
+        f[i] = 2                     # SET_DIAG -> SKIP_DIAG: the cursor overshoots
+    else:
+        i = next(i for i, op in ops if op == 6)
+        f = np.delete(f, i)
     m.funcs = f
     with pytest.raises(D.Refused) as e:
         D.HipSolver(m, host_only=True)
-    refs = e.value.refusals
-    assert refs and all(r[3] == "funcs (sibling ops)" and r[1] == op for r in refs)
+    assert any(r[3] == "funcs (elimination program the device cannot run)" for r in e.value.refusals)
 
 
 def test_group_member_stablist_offsets():
