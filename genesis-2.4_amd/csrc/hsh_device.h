@@ -603,7 +603,10 @@ __device__ __forceinline__ void body_channel(const DevModel& d)
 // waves meet at a barrier where the next row belongs to another wave -- so
 // the tile needs only 24 bytes of LDS per compartment and every tile of the
 // workload is resident at once.
-template <int RPW, class RUN>
+// CHAN = false is the plain drop-in's elimination of one step: rhs and diag
+// come from results[] as the channel kernel left them, and the same owners
+// and barriers run the sweep (four waves per tile instead of one).
+template <int RPW, class RUN, bool CHAN = true>
 __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double* lds)
 {
     constexpr int NW = 4;
@@ -629,14 +632,14 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
     }
     // every row's loads first, then the LDS stores of V (a store right
     // after its row's loads would wait for them, one latency per row)
-    double v0[RPW];
+    double v0[RPW], rb0[RPW], rd0[RPW];
     int pp[RPW];
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
         const int r = wave * RPW + i;
         par[i] = -1;
         pp[i] = -1;
-        ta[i] = tcf[i] = tcb[i] = v0[i] = 0.0;
+        ta[i] = tcf[i] = tcb[i] = v0[i] = rb0[i] = rd0[i] = 0.0;
         if (r < R) {
             const long long q = (long long)(b0 + r) * W + lane;
             par[i] = d.par[q];
@@ -644,15 +647,26 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
             tcf[i] = d.tcf[q];
             tcb[i] = d.tcb[q];
             v0[i] = d.V[q];
-            pp[i] = d.pred[q];
+            if (CHAN) {
+                pp[i] = d.pred[q];
+            } else {
+                rb0[i] = d.R[(long long)(b0 + r) * 2 * W + lane];
+                rd0[i] = d.R[(long long)(b0 + r) * 2 * W + W + lane];
+            }
         }
     }
+    double droot_in = 0.0;     // CHAN = false: the last row's diagonal as results[] holds it
 #pragma unroll
     for (int i = 0; i < RPW; i++) {
         const int r = wave * RPW + i;
         pidx[i] = -1;
         if (r < R) {
             LV[r * W + lane] = v0[i];
+            if (!CHAN) {
+                LB[r * W + lane] = rb0[i];
+                LD[r * W + lane] = rd0[i];
+                if (r == R - 1) droot_in = rd0[i];
+            }
             if (pp[i] >= 0 && (pp[i] >> 6) >= b0 && (pp[i] >> 6) < b0 + R)
                 pidx[i] = ((pp[i] >> 6) - b0) * W + (pp[i] & (W - 1));
         }
@@ -664,9 +678,9 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
         // the last row's diagonal as the channel program leaves it: its
         // owner keeps it (other waves may fold kids into it right after the
         // barrier) and puts it back after the sweep
-        double droot0 = 0.0;
+        double droot0 = droot_in;
 #pragma unroll 1
-        for (int i = 0; i < RPW; i++) {
+        for (int i = 0; i < RPW && CHAN; i++) {
             const int r = wave * RPW + i;
             if (r < R) {
                 // i is wave-uniform: pick the record with selects (indexing
@@ -678,10 +692,12 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                 const double Vm = LV[r * W + lane];
                 const bool pok = pidx[i] >= 0;
                 const double Vp = pok ? LV[pidx[i]] : 0.0;
-                const CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, active, step0 + k);
-                LB[r * W + lane] = o.rhs;
-                LD[r * W + lane] = o.diag;
-                if (r == R - 1) droot0 = o.diag;
+                if constexpr (CHAN) {
+                    const CompOut o = RUN::run(d, bm, lane, Vm, pok, Vp, active, step0 + k);
+                    LB[r * W + lane] = o.rhs;
+                    LD[r * W + lane] = o.diag;
+                    if (r == R - 1) droot0 = o.diag;
+                }
             }
         }
         __syncthreads();
@@ -717,7 +733,8 @@ __device__ __forceinline__ void body_batched_r(const DevModel& d, int K, double*
                     double bv = LB[r * W + lane];
                     if (p >= 0) bv -= tcb[i] * xp;
                     const double x = bv / dr;
-                    LB[r * W + lane] = x;
+                    // a one-row solver keeps rhs in results[0] (hines_solve.c:236-238)
+                    if (CHAN || !d.single) LB[r * W + lane] = x;
                     if (d.crank) {
                         const double v = LV[r * W + lane];
                         LV[r * W + lane] = x + x - v;

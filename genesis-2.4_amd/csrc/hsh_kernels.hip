@@ -107,84 +107,21 @@ __global__ __launch_bounds__(256) void k_elim(DevModel d)
     if (is_last) Rr[(size_t)(R - 1) * 2 * W + W] = droot0;
 }
 
-// Plain drop-in elimination for trees of up to RMAX rows: one wave per tile,
-// the tile's rhs/diag staged in LDS (each lane touches only its own column),
-// tree coefficients loaded into registers before the sweep starts.
-template <int RMAX>
-__global__ __launch_bounds__(64) void k_elim_r(DevModel d)
+// Plain drop-in elimination for trees of up to 16 rows with the batched
+// kernel's row owners: four waves per tile, wave w the rows 4w..4w+3, their
+// coefficients in registers, rhs/diag/V of the tile in LDS.
+struct NoPrograms {
+    static __device__ __forceinline__ CompOut run(const DevModel&, const BlockMeta&, int, double, bool, double,
+                                                  bool, int)
+    {
+        return CompOut{0.0, 0.0};
+    }
+};
+
+__global__ __launch_bounds__(256) void k_elim_w(DevModel d)
 {
-    __shared__ double sB[RMAX * W], sD[RMAX * W];
-    const int t = blockIdx.x;
-    const int lane = threadIdx.x;
-    const TileMeta tm = d.tiles[t];
-    const int R = tm.rows, b0 = tm.block0;
-    if (lane == 0) d.step[t] += 1;
-    double* Rr = d.R + (size_t)b0 * 2 * W + lane;
-    const size_t p0 = (size_t)b0 * W + lane;
-    int par[RMAX];
-    double ta[RMAX], tcf[RMAX], tcb[RMAX], rb[RMAX], rd[RMAX];
-    // every load first, then the LDS stores: a store right after its row's
-    // loads would wait for them, one memory latency per row
-#pragma unroll
-    for (int r = 0; r < RMAX; r++) {
-        if (r < R) {
-            const size_t q = p0 + (size_t)r * W;
-            par[r] = d.par[q];
-            ta[r] = d.ta[q];
-            tcf[r] = d.tcf[q];
-            tcb[r] = d.tcb[q];
-            rb[r] = Rr[(size_t)r * 2 * W];
-            rd[r] = Rr[(size_t)r * 2 * W + W];
-        } else {
-            par[r] = -1;
-            ta[r] = tcf[r] = tcb[r] = rb[r] = rd[r] = 0.0;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < RMAX; r++) {
-        if (r < R) {
-            sB[r * W + lane] = rb[r];
-            sD[r * W + lane] = rd[r];
-        }
-    }
-    const bool is_last = (p0 + (size_t)(R - 1) * W) == (size_t)d.last_pos;
-    const double droot0 = sD[(R - 1) * W + lane];
-#pragma unroll
-    for (int r = 0; r < RMAX - 1; r++) {
-        if (r < R - 1) {
-            const int p = par[r];
-            if (p >= 0) {
-                const double tt = ta[r] / sD[r * W + lane];
-                sD[p * W + lane] -= tcf[r] * tt;
-                sB[p * W + lane] -= sB[r * W + lane] * tt;
-            }
-        }
-    }
-#pragma unroll
-    for (int r = RMAX - 1; r >= 0; r--) {
-        if (r < R) {
-            const size_t q = p0 + (size_t)r * W;
-            double bv = sB[r * W + lane];
-            const int p = par[r];
-            if (p >= 0) bv -= tcb[r] * sB[p * W + lane];
-            const double x = bv / sD[r * W + lane];
-            if (!d.single) sB[r * W + lane] = x;
-            if (d.crank) {
-                const double v = d.V[q];
-                d.V[q] = x + x - v;
-            } else {
-                d.V[q] = x;
-            }
-        }
-    }
-    if (is_last) sD[(R - 1) * W + lane] = droot0;
-#pragma unroll
-    for (int r = 0; r < RMAX; r++) {
-        if (r < R) {
-            Rr[(size_t)r * 2 * W] = sB[r * W + lane];
-            Rr[(size_t)r * 2 * W + W] = sD[r * W + lane];
-        }
-    }
+    extern __shared__ double lds[];
+    body_batched_r<4, NoPrograms, false>(d, 1, lds);
 }
 
 // Plain drop-in elimination for trees of 17..ELIM_LDS_ROWS rows: one wave
@@ -414,7 +351,7 @@ hipError_t launch_channel(const DevModel& d, const JitModule* jm, hipStream_t s)
 hipError_t launch_elim(const DevModel& d, int max_rows, hipStream_t s)
 {
     if (max_rows <= 16) {
-        hipLaunchKernelGGL(k_elim_r<16>, dim3(d.ntiles), dim3(64), 0, s, d);
+        hipLaunchKernelGGL(k_elim_w, dim3(d.ntiles), dim3(256), (unsigned)batched_lds_bytes(max_rows), s, d);
     } else if (max_rows <= ELIM_LDS_ROWS) {
         const unsigned lds = (unsigned)(2 * max_rows * W * sizeof(double));
         hipLaunchKernelGGL(k_elim_l, dim3(d.ntiles), dim3(64), lds, s, d, max_rows);
