@@ -101,7 +101,9 @@ struct hsh_handle {
             *d_post_sid = nullptr, *d_post_T = nullptr, *d_post_after = nullptr;
     std::vector<int32_t> out_off;      // per spike source: its out-connections in d_out_slot / d_out_sid
     double *d_in_wdt = nullptr, *d_lst_w = nullptr, *d_tle = nullptr, *d_post_wdt = nullptr;
-    SpikeSrc* d_spksrc = nullptr;
+    SpikeSrc* d_spksrc = nullptr;      // spike sources, sorted by V position
+    int32_t* d_src_boff = nullptr;     // per block: its first source in d_spksrc
+    int arr_room = 1;                  // arrivals one step's buffer holds (in-connections)
     size_t lst_total = 0;              // list nodes allocated
     int post_cap = 0;                  // posted events the device buffers hold
     struct Post {
@@ -356,10 +358,11 @@ void free_events(hsh_handle* h)
     void* ptrs[] = {h->d_evctl, h->d_syn_op, h->d_in_off, h->d_in_mid, h->d_in_T, h->d_lst_off, h->d_lst_cap,
                     h->d_lst_n, h->d_lst_t, h->d_ipos, h->d_narr, h->d_arr, h->d_out_slot, h->d_out_sid,
                     h->d_post_sid, h->d_post_T, h->d_post_after, h->d_in_wdt, h->d_lst_w, h->d_tle,
-                    h->d_post_wdt, h->d_spksrc};
+                    h->d_post_wdt, h->d_spksrc, h->d_src_boff};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     h->d_evctl = nullptr;
+    h->d_src_boff = nullptr;
     h->d_syn_op = h->d_in_off = h->d_in_mid = h->d_in_T = h->d_lst_off = h->d_lst_cap = nullptr;
     h->d_lst_n = h->d_lst_t = h->d_ipos = h->d_narr = h->d_arr = h->d_out_slot = h->d_out_sid = nullptr;
     h->d_post_sid = h->d_post_T = h->d_post_after = nullptr;
@@ -491,6 +494,8 @@ void set_devmodel(hsh_handle* h)
     d.ev = h->ev_on ? 1 : 0;
     d.narr = h->d_narr;
     d.arr = h->d_arr;
+    d.ev_nsyn = (int32_t)P.syn_ops.size();
+    d.ev_nconn = h->arr_room;
     d.in_off = h->d_in_off;
     d.in_mid = h->d_in_mid;
     d.in_T = h->d_in_T;
@@ -502,8 +507,15 @@ void set_devmodel(hsh_handle* h)
     d.lst_t = h->d_lst_t;
     d.lst_w = h->d_lst_w;
     d.tle = h->d_tle;
-    d.tnow = h->d_evctl ? &h->d_evctl->tnow : nullptr;
+    d.tnow = h->d_evctl ? h->d_evctl->tstep : nullptr;
     d.everr = h->d_evctl ? &h->d_evctl->err : nullptr;
+    d.src = h->d_spksrc;
+    d.src_boff = h->d_src_boff;
+    d.out_slot = h->d_out_slot;
+    d.out_sid = h->d_out_sid;
+    d.evctl = h->d_evctl;
+    d.nsrc = (int32_t)P.spike_ops.size();
+    d.look = 0;
     d.dt = h->m.dt;
     d.spk_cap = SPIKE_CAP;
     d.giv = P.giv ? h->d_giv : nullptr;
@@ -581,8 +593,20 @@ hsh_status ev_layout(hsh_handle* h)
             return fail(HSH_ERR_ARG, "hsh: a SPIKE_OP's values are not in the device layout");
     for (int32_t q : ipos)
         if (q < 0) return fail(HSH_ERR_ARG, "hsh: a SYN2_OP counter is not in the device layout");
+    // by position, so that each block's sources are one range (the step
+    // kernel's look-ahead spike pass takes its blocks' ranges)
+    std::stable_sort(src.begin(), src.end(), [](const SpikeSrc& a, const SpikeSrc& b) { return a.vpos < b.vpos; });
+    const size_t nb = h->plan.blocks.size();
+    std::vector<int32_t> boff(nb + 1, 0);
+    for (const SpikeSrc& q : src) boff[q.vpos / W + 1]++;
+    for (size_t b = 0; b < nb; b++) boff[b + 1] += boff[b];
+    if (h->d_src_boff) HIPCHK(hipFree(h->d_src_boff));
+    h->d_src_boff = nullptr;
+    HIPCHK(dalloc(&h->d_src_boff, nb + 1));
+    HIPCHK(dput(h->d_src_boff, boff));
     HIPCHK(dput(h->d_ipos, ipos));
     HIPCHK(dput(h->d_spksrc, src));
+    set_devmodel(h);
     return HSH_OK;
 }
 
@@ -728,8 +752,10 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
     HIPCHK(dalloc(&h->d_in_mid, ns));
     HIPCHK(dalloc(&h->d_out_slot, out_slot.size()));
     HIPCHK(dalloc(&h->d_out_sid, out_sid.size()));
-    HIPCHK(dalloc(&h->d_arr, std::max<size_t>(1, n)));
-    HIPCHK(dalloc(&h->d_narr, ns));
+    // arrivals: two buffers, by step parity
+    h->arr_room = (int)std::max<size_t>(1, n);
+    HIPCHK(dalloc(&h->d_arr, 2 * (size_t)h->arr_room));
+    HIPCHK(dalloc(&h->d_narr, 2 * std::max<size_t>(1, ns)));
     HIPCHK(dalloc(&h->d_in_T, in_T.size()));
     HIPCHK(dalloc(&h->d_in_wdt, in_wdt.size()));
     HIPCHK(dalloc(&h->d_lst_off, ns));
@@ -746,7 +772,7 @@ hsh_status ev_rebuild(hsh_handle* h, const std::vector<int32_t>& csrc, const std
     HIPCHK(dput(h->d_in_mid, in_mid));
     HIPCHK(dput(h->d_out_slot, out_slot));
     HIPCHK(dput(h->d_out_sid, out_sid));
-    HIPCHK(hipMemset(h->d_narr, 0, std::max<size_t>(1, ns) * sizeof(int32_t)));
+    HIPCHK(hipMemset(h->d_narr, 0, 2 * std::max<size_t>(1, ns) * sizeof(int32_t)));
     HIPCHK(dput(h->d_in_T, in_T));
     HIPCHK(dput(h->d_in_wdt, in_wdt));
     HIPCHK(dput(h->d_lst_off, lst_off));
@@ -818,8 +844,7 @@ hsh_status ev_flush(hsh_handle* h)
 hsh_status ev_spikes(hsh_handle* h)
 {
     if (!h->ev_on) return HSH_OK;
-    HIPCHK(launch_ev_spikes(h->dm, h->d_spksrc, (int)h->plan.spike_ops.size(), h->d_evctl, h->d_out_slot,
-                            h->d_out_sid, h->stream));
+    HIPCHK(launch_ev_spikes(h->dm, h->stream));
     return HSH_OK;
 }
 
@@ -972,13 +997,14 @@ DevModel tile_view(const DevModel& d, const KClass& c)
     return v;
 }
 
-hsh_status launch_class(hsh_handle* h, const KClass& c, int K, hipStream_t s)
+hsh_status launch_class(hsh_handle* h, const KClass& c, int K, hipStream_t s, bool look = false)
 {
     if (c.kind == C_PACKS) {
         HIPCHK(launch_forest(h->dm, &h->jit, K, h->pack_waves, h->pack_stage, pack_lds_bytes(h, h->pack_stage), s));
         return HSH_OK;
     }
-    const DevModel v = tile_view(h->dm, c);
+    DevModel v = tile_view(h->dm, c);
+    v.look = look ? 1 : 0;
     if (c.kind == C_ISO) {
         HIPCHK(launch_iso(v, &h->jit, K, s));
     } else {
@@ -988,12 +1014,26 @@ hsh_status launch_class(hsh_handle* h, const KClass& c, int K, hipStream_t s)
     return HSH_OK;
 }
 
-// One batched launch of K steps: the kernel of every class, in turn, on the
-// handle's stream (K = 1 after the spike pass when the lists are on).
-hsh_status batched_launch(hsh_handle* h, int K)
+// With the lists on and every tile one row (one kernel class, no packs), the
+// step kernel runs the next step's spike pass for its own blocks as soon as
+// it has solved them (body_iso): a run of n steps is one spike launch and n
+// step launches, the last of which leaves the next step's pass to the next
+// run (which may follow a change of the state, the inputs or the posted
+// events).  Other layouts launch the spike pass before every step.
+bool lookahead_ok(const hsh_handle* h)
 {
-    STCHK(ev_spikes(h));
-    for (const KClass& c : h->classes) STCHK(launch_class(h, c, K, h->stream));
+    return h->ev_on && h->classes.size() == 1 && h->classes[0].kind == C_ISO && h->plan.packs.empty() &&
+           h->plan.funits.empty() && h->d_src_boff != nullptr;
+}
+
+// One batched launch of K steps: the kernel of every class, in turn, on the
+// handle's stream (K = 1 when the lists are on; `first` and `last` place the
+// launch in a run of launches).
+hsh_status batched_launch(hsh_handle* h, int K, bool first, bool last)
+{
+    const bool look = lookahead_ok(h);
+    if (!look || first) STCHK(ev_spikes(h));
+    for (const KClass& c : h->classes) STCHK(launch_class(h, c, K, h->stream, look && !last));
     return HSH_OK;
 }
 
@@ -1010,7 +1050,7 @@ hsh_status run_batched(hsh_handle* h, int nsteps, int* launches)
             hipGraph_t g = nullptr;
             HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
             hsh_status st = HSH_OK;
-            for (int i = 0; i < M && st == HSH_OK; i++) st = batched_launch(h, K);
+            for (int i = 0; i < M && st == HSH_OK; i++) st = batched_launch(h, K, i == 0, i == M - 1);
             hipError_t ce = hipStreamEndCapture(h->stream, &g);
             if (st != HSH_OK) return st;
             if (ce != hipSuccess) return fail(HSH_ERR_DEVICE, std::string("HIP: graph capture failed: ") + hipGetErrorString(ce));
@@ -1024,7 +1064,7 @@ hsh_status run_batched(hsh_handle* h, int nsteps, int* launches)
         full -= M;
     }
     if (rem > 0) {
-        STCHK(batched_launch(h, rem));
+        STCHK(batched_launch(h, rem, true, true));
         (*launches)++;
     }
     return HSH_OK;

@@ -166,16 +166,18 @@ __device__ __forceinline__ double syn2_op(const DevModel& d, double* __restrict_
     if (d.ev && active) {
         const int sid = Ib[(is + 1) * W + lane];
         int cnt = Ib[is * W + lane];
-        const int na = d.narr[sid];
+        const int par = step & 1;
+        int* __restrict__ narr = d.narr + par * d.ev_nsyn;
+        const int na = narr[sid];
         if (na > 0 || cnt == 0) {
             const int j0 = d.in_off[sid], mid = d.in_mid[sid];
             const int off = d.lst_off[sid], cap = d.lst_cap[sid];
             int* __restrict__ lt = d.lst_t + off;
             double* __restrict__ lw = d.lst_w + off;
-            int* __restrict__ ar = d.arr + j0;
+            int* __restrict__ ar = d.arr + par * d.ev_nconn + j0;
             int n = d.lst_n[sid];
             double tle = d.tle[sid];
-            const double now = *d.tnow;
+            const double now = d.tnow[par];
             bool full = false;
             // the arrivals in ops[] order of their sources (slot order)
             for (int a = 1; a < na; a++) {
@@ -212,7 +214,7 @@ __device__ __forceinline__ double syn2_op(const DevModel& d, double* __restrict_
             cnt -= 1;
             for (; a < na; a++)
                 full |= !ev_insert(lt, lw, n, cap, cnt, tle, d.in_T[ar[a]], d.in_wdt[ar[a]], now, d.dt);
-            if (na > 0) d.narr[sid] = 0;
+            if (na > 0) narr[sid] = 0;
             d.lst_n[sid] = n;
             d.tle[sid] = tle;
             if (full) atomicOr(d.everr, 2);
@@ -1227,6 +1229,82 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
     if (tid == 0) d.pstep[pk] = step0 + K;
 }
 
+// The spike pass of step `step` for one source (hines_chip.c:266-276
+// SPIKE_OP, which reads only Vm -- the last step's solution -- its threshold
+// and its counter): counter - 1; when v > thresh with the counter spent,
+// record the spike, reload the counter and add each of the source's
+// connections (its in-connection slot) to the target synchan's arrivals of
+// that step (the buffers of step's parity).  Every lane of the wave calls it,
+// live or not: the wave scatters its spiking lanes' connections together, so
+// a wave's atomics go to consecutive arrivals.
+__device__ __forceinline__ void spike_src_pass(const DevModel& d, bool live, const SpikeSrc& s, double v, int step)
+{
+    bool fired = false;
+    if (live) {
+        int cnt = d.I[s.cpos] - 1;
+        if (v > d.S[s.spos] && cnt <= 0) {
+            const int q = atomicAdd(d.nspk, 1);
+            if (q < d.spk_cap) {
+                d.spk[2 * q] = s.opidx;
+                d.spk[2 * q + 1] = step;
+            }
+            cnt = d.I[s.rpos];
+            fired = true;
+        }
+        d.I[s.cpos] = cnt;
+    }
+    const int lane = threadIdx.x & (W - 1);
+    unsigned long long m = __ballot(fired);
+    while (m) {
+        const int l = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        // every lane scatters lane l's connections, into the buffers of
+        // lane l's step (a lane past the last source holds no step)
+        const int o0 = __shfl(s.out0, l), o1 = __shfl(s.out1, l);
+        const int par = __shfl(step, l) & 1;
+        int* __restrict__ narr = d.narr + par * d.ev_nsyn;
+        int* __restrict__ arr = d.arr + par * d.ev_nconn;
+        for (int o = o0 + lane; o < o1; o += W) {
+            const int sid = d.out_sid[o];
+            const int k = atomicAdd(narr + sid, 1);
+            arr[d.in_off[sid] + k] = d.out_slot[o];
+        }
+    }
+}
+
+// The clock of the spike pass of step `step`: SimulationTime() advances by
+// dt (sim/sim_step.c:112); the synchans of that step read tstep[step & 1].
+// One thread per step calls it.
+__device__ __forceinline__ void ev_clock(const DevModel& d, int step)
+{
+    EvCtl* c = d.evctl;
+    const double t = c->tnext;
+    c->tlast = c->tnow;
+    c->tnow = t;
+    c->tstep[step & 1] = t;
+    c->tnext = t + d.dt;
+}
+
+// The spike pass of step `step` for the sources of block b, run by the wave
+// that has just solved the block: V is its lane's new voltage, which is all
+// the next step's SPIKE_OPs read of the step, so the next step needs no
+// launch of its own for them.  Block 0's wave advances the clock.  The
+// arrivals go to the buffers of step's parity, which no synchan of the
+// running step reads.
+__device__ __forceinline__ void spike_lookahead(const DevModel& d, int b, int lane, double V, int step)
+{
+    if (b == 0 && lane == 0) ev_clock(d, step);
+    const int s0 = d.src_boff[b], s1 = d.src_boff[b + 1];
+    for (int i0 = s0; i0 < s1; i0 += W) {
+        const int i = i0 + lane;
+        const bool live = i < s1;
+        SpikeSrc q{};
+        if (live) q = d.src[i];
+        const double v = __shfl(V, q.vpos & (W - 1), W);
+        spike_src_pass(d, live, q, v, step);
+    }
+}
+
 // Isopotential cells (every tree one compartment): one lane per cell, the
 // voltage stays in a register for K steps.  V = rhs/diag is the reference's
 // single-row solve (hines_solve.c:236-238; CN :158-162).
@@ -1261,6 +1339,7 @@ __device__ __forceinline__ void body_iso(const DevModel& d, int K)
     d.R[(long long)b * 2 * W + lane] = d.single ? o.rhs : x;
     d.R[(long long)b * 2 * W + W + lane] = o.diag;
     if (lane == 0) d.step[t] = step0 + K;
+    if (d.look) spike_lookahead(d, b, lane, V, step0 + K);
 }
 
 }  // namespace hsh

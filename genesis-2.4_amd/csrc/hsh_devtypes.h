@@ -88,6 +88,26 @@ struct PackMeta {
     int last_lp;      // lp of the reference's last row (ncompts-1), -1 if elsewhere
 };
 
+struct EvCtl {
+    double tnext;         // SimulationTime() of the next step
+    double tnow;          // of the step being run (set by its spike pass)
+    double tlast;         // of the last step run
+    int err;              // EV_ERR_* bits
+    int npost;            // posted events the next post pass applies
+    double tstep[2];      // SimulationTime() of step s at [s & 1] (set by its spike pass)
+};
+
+// Where the spike pass finds a spike source's values in the layout.
+struct SpikeSrc {
+    int vpos;             // V position of its compartment
+    int spos;             // S position of its threshold
+    int cpos;             // I positions of its counter and reload
+    int rpos;
+    int opidx;            // ops index of the SPIKE_OP
+    int out0, out1;       // its out-connections: out_slot / out_sid [out0, out1)
+    int pad;
+};
+
 // Device-side argument bundle (passed by value to every kernel).
 struct DevModel {
     const int* prog;
@@ -126,8 +146,14 @@ struct DevModel {
     // newconn/synchan.c:161-358, from the spikes the spike pass of the step
     // decided before the step ran
     int ev;               // 1: lists on; SPIKE_OPs run in the spike pass, not in programs
-    int* narr;            // per synchan: EVENTs arriving this step (the spike pass counts, the synchan clears)
-    int* arr;             // their in-connection slots, at in_off[synchan] + 0..narr-1, unordered
+    // arrivals, double-buffered by step parity so that the spike pass of
+    // step s + 1 may run while the synchans of step s still read theirs
+    int* narr;            // EVENTs arriving at step s: narr[(s & 1) * ev_nsyn + synchan] (the spike pass
+                          // counts, the synchan clears)
+    int* arr;             // their in-connection slots, at (s & 1) * ev_nconn + in_off[synchan] + 0..narr-1,
+                          // unordered
+    int ev_nsyn;          // synchans
+    int ev_nconn;         // in-connections (room for the arrivals of one step)
     const int* in_off;    // per synchan: in-connections, by source ops index then connection order
     const int* in_mid;    // per synchan: first in-connection whose source comes after its SYN2_OP
     const int* in_T;      // steps to the hit, (int)(delay/dt + 0.9)
@@ -139,8 +165,16 @@ struct DevModel {
     int* lst_t;           // node time fields
     double* lst_w;        // node weight / dt
     double* tle;          // per synchan: time_last_event
-    const double* tnow;   // SimulationTime() of the step being run
+    const double* tnow;   // SimulationTime() of step s: tnow[s & 1] (EvCtl::tstep)
     int* everr;           // error bits (a full list)
+    // the spike pass (k_ev_spikes, or the step kernel's look-ahead)
+    const SpikeSrc* src;  // spike sources, by position
+    const int* src_boff;  // per block: its sources src[src_boff[b] .. src_boff[b+1])
+    const int* out_slot;  // out-connections of the sources (SpikeSrc::out0/out1):
+    const int* out_sid;   // the in-connection slot and the synchan
+    EvCtl* evctl;
+    int nsrc;
+    int look;             // 1: the step kernel runs the next step's spike pass for its blocks
     double dt;
     // concentration-indexed rows of tab
     double ymin;

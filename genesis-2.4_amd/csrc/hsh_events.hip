@@ -32,54 +32,24 @@ constexpr int MAX_DEVICES = 64;
 std::mutex scratch_mu;
 InEntry* scratch_ptr[MAX_DEVICES] = {};
 
-// SPIKE_OP (hines_chip.c:266-276) for every spike source, and the clock:
-// SimulationTime() of this step, advanced by dt (sim/sim_step.c:112).  A
-// spike adds each of its connections (the in-connection slot) to its target
+// The step's spike pass for every spike source (spike_src_pass), and the
+// clock of the step.  A spike adds each of its connections to its target
 // synchan's arrivals, which the synchan takes in slot order when its SYN2_OP
-// runs.
-__global__ __launch_bounds__(256) void k_ev_spikes(DevModel d, const SpikeSrc* __restrict__ src, int nsrc,
-                                                   EvCtl* ctl, const int* __restrict__ out_slot,
-                                                   const int* __restrict__ out_sid)
+// runs.  Every step counter holds the step about to run.
+__global__ __launch_bounds__(256) void k_ev_spikes(DevModel d)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        const double t = ctl->tnext;
-        ctl->tlast = ctl->tnow;
-        ctl->tnow = t;
-        ctl->tnext = t + d.dt;
-    }
-    // every lane of the wave takes part in the scatter below
-    const bool live = i < nsrc;
-    SpikeSrc s{};
-    bool fired = false;
+    if (i == 0) ev_clock(d, d.step[0]);
+    const bool live = i < d.nsrc;
+    SpikeSrc q{};
+    double v = 0.0;
+    int step = 0;
     if (live) {
-        s = src[i];
-        const int step = d.step[d.blocks[s.vpos / W].tile];
-        int cnt = d.I[s.cpos] - 1;
-        if (d.V[s.vpos] > d.S[s.spos] && cnt <= 0) {
-            const int q = atomicAdd(d.nspk, 1);
-            if (q < d.spk_cap) {
-                d.spk[2 * q] = s.opidx;
-                d.spk[2 * q + 1] = step;
-            }
-            cnt = d.I[s.rpos];
-            fired = true;
-        }
-        d.I[s.cpos] = cnt;
+        q = d.src[i];
+        v = d.V[q.vpos];
+        step = d.step[d.blocks[q.vpos / W].tile];
     }
-    // the wave scatters each spiking lane's connections together
-    const int lane = threadIdx.x & (W - 1);
-    unsigned long long m = __ballot(fired);
-    while (m) {
-        const int l = __ffsll((long long)m) - 1;
-        m &= m - 1;
-        const int o0 = __shfl(s.out0, l), o1 = __shfl(s.out1, l);
-        for (int o = o0 + lane; o < o1; o += W) {
-            const int sid = out_sid[o];
-            const int k = atomicAdd(d.narr + sid, 1);
-            d.arr[d.in_off[sid] + k] = out_slot[o];
-        }
-    }
+    spike_src_pass(d, live, q, v, step);
 }
 
 // Events posted from the host (hsh_post_events), one thread per synchan:
@@ -154,11 +124,10 @@ hipError_t launch_inputs(double* S, const InEntry* e, int n, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, const int* out_slot,
-                            const int* out_sid, hipStream_t s)
+hipError_t launch_ev_spikes(const DevModel& d, hipStream_t s)
 {
-    const int n = nsrc > 0 ? nsrc : 1;
-    hipLaunchKernelGGL(k_ev_spikes, dim3((n + 255) / 256), dim3(256), 0, s, d, src, nsrc, ctl, out_slot, out_sid);
+    const int n = d.nsrc > 0 ? d.nsrc : 1;
+    hipLaunchKernelGGL(k_ev_spikes, dim3((n + 255) / 256), dim3(256), 0, s, d);
     return hipGetLastError();
 }
 

@@ -145,25 +145,6 @@ void jit_free(JitModule& jm);
 // (syn2_op, hsh_device.h).
 constexpr int EV_ERR_FULL = 2;    // a pending list ran out of room
 
-struct EvCtl {
-    double tnext;         // SimulationTime() of the next step
-    double tnow;          // of the step being run (set by its spike pass)
-    double tlast;         // of the last step run
-    int err;              // EV_ERR_* bits
-    int npost;            // posted events the next post pass applies
-};
-
-// Where the spike pass finds a spike source's values in the layout.
-struct SpikeSrc {
-    int vpos;             // V position of its compartment
-    int spos;             // S position of its threshold
-    int cpos;             // I positions of its counter and reload
-    int rpos;
-    int opidx;            // ops index of the SPIKE_OP
-    int out0, out1;       // its out-connections: out_slot / out_sid [out0, out1)
-    int pad;
-};
-
 struct EvPost {
     const int* sid;       // sorted by synchan, then post order
     const int* T;
@@ -173,8 +154,8 @@ struct EvPost {
     int nsyn;
 };
 
-hipError_t launch_ev_spikes(const DevModel& d, const SpikeSrc* src, int nsrc, EvCtl* ctl, const int* out_slot,
-                            const int* out_sid, hipStream_t s);
+// The spike pass of the step every counter holds, over d.src (d.nsrc sources).
+hipError_t launch_ev_spikes(const DevModel& d, hipStream_t s);
 hipError_t launch_ev_post(const DevModel& d, const EvPost& p, EvCtl* ctl, hipStream_t s);
 
 // A message input applied to a device slot before a step (h_in_msgs,

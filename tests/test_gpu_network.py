@@ -205,3 +205,31 @@ def test_event_state_survives_autotune_and_put(gpu):
         o.step(100)
         g.step(100, D.BATCHED)
         check_same(o, g)
+
+
+def test_fused_and_per_step_spike_passes_at_4000_cells(gpu):
+    """4000 isopotential cells (63 one-row tiles, ~80 connections per
+    source, more than a wave scatters at once): the batched step kernel
+    runs the next step's spike pass for its own blocks (arrivals in the
+    buffers of the next step's parity), the plain loop launches the pass
+    before every step, whose last wave holds lanes past the last source.
+    Over call boundaries -- each batched call starts with a spike launch and
+    its last step leaves the next pass alone -- both equal the oracle: raster
+    identical, vm, chip, results and ops bit for bit."""
+    m, table0 = M.network(4000, seed=2, inject=(0.3e-9, 0.8e-9))
+    table = unequal_table(table0, 4)
+    o = Oracle(m, givals=False)
+    o.set_synapses(*table)
+    o.step(2000)
+    want_op, want_step = o.spikes()
+    assert want_op.size > 1000
+    for mode in (D.BATCHED, D.PLAIN):
+        print(f"[case] 4000-cell network mode {mode}", flush=True)
+        with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
+            g.set_synapses(*table)
+            for n in (1500, 1, 17, 482):
+                g.step(n, mode)
+            got_op, got_step = g.drain_spikes()
+            check_same(o, g)
+        assert np.array_equal(got_step, want_step)
+        assert np.array_equal(got_op, want_op)
