@@ -139,6 +139,23 @@ class Model:
     ydivs: int = 0
     ymin: float = 0.0
     invdy: float = 0.0
+    # diffusion systems of shells and buffers (h_init_conc_solve,
+    # hines_conc_init.c:59-291): per system its shells, rows per shell, fixed
+    # buffers, first concentration and first buffer-rate value; per
+    # compartment (Hines row) its first system (ncompts + 1 entries); the
+    # coupling values in the order the solve consumes them; buffer rates
+    nshells: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    nshrows: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    nshfix: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    dconc: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    fbufval: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    fdiffs: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    covals0: np.ndarray = field(default_factory=lambda: np.zeros(0))
+    bufvals: np.ndarray = field(default_factory=lambda: np.zeros(0))
+
+    @property
+    def ndiffs(self):
+        return int(self.nshells.size)
 
     @property
     def nchips(self):
@@ -655,3 +672,244 @@ def ca_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, features=CA_FEATU
         compchips=np.array(compchips, np.int32), conc=np.array(conc, np.float64),
         concops=np.array(concops, np.int32), concchip=np.array(concchip, np.float64),
         ydivs=ydivs, ymin=0.0, invdy=invdy, givals=np.zeros(0))
+
+
+# ---------------------------------------------------------------------------
+# Diffusion shells, buffers and pumps (the concentration solve)
+
+_CONC, _FCONC, _LCONC, _FLUXCONC = 10100, 10101, 10102, 10103
+_TAUPUMP, _CTAUPUMP, _MMPUMP, _HILLPUMP, _PHILLPUMP = 11000, 11001, 11010, 11011, 11012
+_BUFFER, _POOL, _ELMMPUMP = 12000, 12100, 500
+SHELL_FEATURES = ("fixbuf", "difbuf", "ctaupump", "taupump", "mmpump", "hillpump", "elmmpump", "pool",
+                  "kc", "leak")
+_FARADAY = 96485.3329
+
+
+def shell_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, shells: int = 4,
+                     features=("fixbuf", "difbuf", "ctaupump", "mmpump", "hillpump", "kc", "leak"),
+                     seed: int = 0, dt: float = 20e-6, method: int = CRANK, hill: int = 2,
+                     ydivs: int = 1000, ymax: float = 0.5, ca0: float = 5e-5,
+                     inject=(0.2e-9, 1.0e-9), **kw) -> Model:
+    """The population workload with a diffusion system of ``shells``
+    concentric Ca shells in every compartment, emitted as the reference's
+    SETUP writes it: the system arrays of h_init_conc_solve /
+    h_setup_conc_solve (hines_conc_init.c:59-411), the concentration program
+    of h_init_conc_chip (:413-805) and the values of the chip_put_* functions
+    (hines_data.c:189-400).
+
+    Per compartment (Hines row r) the children are, in order: the system's
+    rows shell by shell from the innermost (its buffers -- "fixbuf" a fixed
+    buffer, "difbuf" a diffusible one -- then the shell), the pumps on the
+    outermost shell ("ctaupump", "taupump" with a voltage-dependent rate,
+    "mmpump", "hillpump" with Hill coefficient ``hill``), and with "pool" a
+    concpool fed by a Hill pump from the outermost shell, holding a pool
+    buffer.  The outermost shell takes the Ca channel's current (FLUX_OP,
+    then FLUXCONC_OP with a "leak"); "kc" adds a K channel gated by that
+    shell (NEWCONC1_OP, IPOL1C_OP), "elmmpump" an electrogenic pump on it
+    (ELMMPUMP_OP in the channel program).  Shell m (0 innermost) of a
+    cylinder of diameter ~U(1, 4) um and length ~U(10, 100) um is
+    ``f * 2**(shells - 1 - m)`` of the radius thick, the shells outside the
+    innermost together half of it (the outermost thinnest), the innermost
+    the rest.  Concentrations in mM; buffers start at equilibrium.  Buffers
+    need Crank-Nicolson (the reference refuses them under backward Euler,
+    hines_conc_init.c:160-164).
+    """
+    feats = set(features)
+    bad = feats - set(SHELL_FEATURES)
+    if bad:
+        raise ValueError(f"unknown features {sorted(bad)}")
+    if shells < 2:
+        raise ValueError("a diffusion system needs at least two shells")
+    if ("fixbuf" in feats or "difbuf" in feats) and method != CRANK:
+        raise ValueError("buffers need Crank-Nicolson (hines_conc_init.c:160-164)")
+    base = population(n_neurons, sizes, probs, seed=seed, dt=dt, method=method, inject=inject, **kw)
+    rng = np.random.default_rng(seed + 104729)
+    crank = method == CRANK
+    nd = 1 if "fixbuf" in feats else 0
+    dw = 1 + nd + (1 if "difbuf" in feats else 0)
+    # chip_put_flux / _taupump / _mmpump / _hillpump: dt/2 for a system of
+    # shells alone under Crank-Nicolson (hines_data.c:224-227); the solve's
+    # own dt2 is dt/2 under Crank-Nicolson (hines_conc_init.c:312-315)
+    dtp = dt / 2.0 if (crank and dw == 1) else dt
+    dt2 = dt / 2.0 if crank else dt
+    xdivs, ncols0 = base.xdivs, base.ncols
+    rows = max(xdivs, ydivs) + 2
+    tab0 = base.tablist.reshape(xdivs + 2, ncols0)
+    tab = np.zeros((rows, ncols0 + 2))
+    tab[:xdivs + 2, :ncols0] = tab0
+    tab[xdivs + 2:, :ncols0] = tab0[xdivs + 1]
+    invdy = ydivs / ymax
+    C_r = np.arange(ydivs + 1) / invdy
+    col_z = ncols0
+    al = 500.0 * C_r / (C_r + 1e-3)
+    zB, zA = 1.0 + dt / 2.0 * (al + 100.0), dt * al
+    tab[:ydivs + 1, col_z], tab[ydivs + 1:, col_z] = zB, zB[-1]
+    tab[:ydivs + 1, col_z + 1], tab[ydivs + 1:, col_z + 1] = zA, zA[-1]
+    col_m = next(int(base.ops[i + 1]) for i in range(base.ops.size - 2)
+                 if base.ops[i] == _IPOL1V and base.ops[i + 2] == 3)
+    segs = _segments(base)
+    ops, chip, conc, concops, concchip = [], [], [], [], []
+    compops, compchips = [], []
+    nshells, nshrows, nshfix, dconc, fbufval, covals0, bufvals = [], [], [], [], [], [], []
+    shell_vol = []        # per system: the shells' volumes (m^3), innermost first
+    kf_f, kb_f, bt_f = 1.0e5, 100.0, 0.05          # fixed buffer: /mM/s, /s, mM
+    kf_d, kb_d, bt_d, D_b = 2.0e5, 300.0, 0.02, 2.0e-11   # diffusible buffer
+    D_ca = 6.0e-10
+    last = None           # the h_init_conc_chip index register: (conc, is_pool) or None
+    first = True
+
+    def switch(k, pool):
+        # (F)CONC_OP when the ops move to another concentration
+        # (hines_conc_init.c:642-687)
+        nonlocal last, first
+        if last is not None and last[0] == k:
+            return
+        if first:
+            code = _FCONC
+        elif pool and last is not None and last[1]:
+            code = _FCONC
+        else:
+            code = _CONC
+        concops.extend([code, k])
+        first = False
+        last = (k, pool)
+
+    reg = base_reg = 0.0
+    for r, (head, body, pre, post) in enumerate(segs):
+        compops.append(len(ops)); compchips.append(len(chip))
+        ops.append(head); chip += pre
+        for (o, opnd, vals) in body:
+            if o == _CHAN_EK:
+                base_reg = reg = vals[0]
+            elif o == _CHAN and reg != base_reg:
+                o, vals = _CHAN_EK, [base_reg] + vals
+                reg = base_reg
+            ops += [o] + opnd
+            chip += vals
+        # geometry of the shells (difshell.c:205-229, shape_mode SHELL)
+        dia = rng.uniform(1e-6, 4e-6)
+        ln = rng.uniform(10e-6, 100e-6)
+        rad = dia / 2.0
+        f = 0.5 / (2 ** (shells - 1) - 1)
+        th = [rad * f * 2 ** (shells - 1 - m) for m in range(shells)]
+        th[0] = rad - sum(th[1:])
+        r_out = np.empty(shells)
+        acc = rad
+        for m in range(shells - 1, -1, -1):
+            r_out[m] = acc
+            acc -= th[m]
+        r_in = r_out - np.array(th)
+        r_in[0] = 0.0
+        vol = np.pi * ln * (r_out ** 2 - r_in ** 2)
+        s_up = 2.0 * np.pi * r_out * ln
+        s_dn = 2.0 * np.pi * r_in * ln
+        shell_vol.append(vol)
+        k0 = len(conc)
+        bf_f = bt_f * kb_f / (kb_f + kf_f * ca0)
+        bf_d = bt_d * kb_d / (kb_d + kf_d * ca0)
+        for m in range(shells):
+            if nd:
+                conc.append(bf_f)
+            if dw - nd > 1:
+                conc.append(bf_d)
+            conc.append(ca0)
+        kout = k0 + (shells - 1) * dw + dw - 1        # the outermost shell
+        nshells.append(shells); nshrows.append(dw); nshfix.append(nd); dconc.append(k0)
+        fbufval.append(len(bufvals))
+        # h_setup_conc_solve (hines_conc_init.c:294-411)
+        def coupling(D, m, surf_dn, surf_up, th):
+            Diff = 2.0 * dt2 * D / vol[m]
+            out = []
+            if m < shells - 1:
+                if m:
+                    den = (2 * th[m - 1] + th[m]) if m == 1 else (th[m - 1] + th[m])
+                    out.append(Diff * surf_dn / den)
+                    out.append(Diff * surf_up / (th[m] + th[m + 1]))
+                else:
+                    out.append(Diff * surf_up / (2 * th[m] + th[m + 1]))
+            else:
+                out.append(Diff * surf_dn / (th[m - 1] + th[m]))
+            return out
+        for m in range(shells):
+            if nd:
+                btot = dt * kb_f * bt_f
+                if m == 0:
+                    bufvals.extend([dt2 * kf_f, dt2 * kb_f, btot])
+                covals0.append(btot)
+            if dw - nd > 1:
+                btot = dt * kb_d * bt_d
+                if m == 0:
+                    bufvals.extend([dt2 * kf_d, dt2 * kb_d, btot])
+                covals0.extend(coupling(D_b, m, s_dn[m], s_up[m], th))
+            covals0.extend(coupling(D_ca, m, s_dn[m], s_up[m], th))
+        # channel program: Ca channel into the outermost shell, optional
+        # electrogenic pump and Ca-gated K channel on it
+        area = float(pre[0]) * kw.get("RM", 0.33) / kw.get("Em", -0.065)
+        v0 = (base.vm[r] - base.xmin) * base.invdx
+        rr = min(max(int(v0), 0), xdivs)
+        fB = tab[rr, col_m] + (v0 - rr) * (tab[rr + 1, col_m] - tab[rr, col_m])
+        fA = tab[rr, col_m + 1] + (v0 - rr) * (tab[rr + 1, col_m + 1] - tab[rr, col_m + 1])
+        m0 = fA / (2.0 * (fB - 1.0))
+        g_ca = 2.0 * area * rng.uniform(0.5, 1.5)
+        ops += [_CHAN_EK, _IPOL1V, col_m, 2, _ADD_CURR, _CONC_VAL, kout, _FLUX]
+        chip += [0.12, g_ca, m0]
+        reg = 0.12
+        if "elmmpump" in feats:
+            vmax = 2e-16 * s_up[-1] / 1e-10
+            ops += [_ELMMPUMP]
+            chip += [1e-3, vmax * 2.0 * _FARADAY, dtp * vmax / vol[-1]]
+        if "kc" in feats:
+            ops += [_NEWCONC1, _CHAN_EK, _IPOL1C, col_z, 1, _ADD_CURR]
+            c = ca0 * invdy
+            z0 = (tab[0, col_z + 1] + c * (tab[1, col_z + 1] - tab[0, col_z + 1])) / (
+                2.0 * (tab[0, col_z] + c * (tab[1, col_z] - tab[0, col_z]) - 1.0))
+            chip += [-0.09, 10.0 * area, z0]
+            reg = -0.09
+        chip += post
+        # h_init_conc_chip: the outermost shell's flux and leak, its pumps
+        switch(kout, False)
+        leak = 1e-6 if "leak" in feats else 0.0
+        concops.append(_FLUXCONC)
+        concchip += [dtp * leak, dtp / (2.0 * _FARADAY * vol[-1])]
+        if "ctaupump" in feats:
+            T_C, Ceq = 2e-3, 4e-5
+            concops.append(_CTAUPUMP)
+            concchip += [Ceq * dtp / T_C, dtp / T_C]
+        if "taupump" in feats:
+            concops += [_TAUPUMP, r]
+            concchip += [3e-3, 1e-3, -0.04, 1.0 / 0.01, 5e-5]
+        if "mmpump" in feats:
+            concops.append(_MMPUMP)
+            concchip += [dtp * 2e-4 / (vol[-1] * 1e12), 2e-4]
+        if "hillpump" in feats:
+            concops += [_HILLPUMP, hill]
+            concchip += [3e-4 ** hill, dtp * 5e-4 / (vol[-1] * 1e12)]
+        if "pool" in feats:
+            kp = len(conc)
+            pool_vol = 0.05 * vol.sum()
+            kb_p, kf_p, bt_p, cp0 = 50.0, 5e4, 0.1, 1e-3
+            conc += [cp0, bt_p * kb_p / (kb_p + kf_p * cp0)]
+            switch(kp, True)           # the Hill pump into the pool
+            vmax = 3e-4 / 1e12
+            concops += [_PHILLPUMP, hill, kout]
+            concchip += [2e-4 ** hill, dtp * vmax / vol[-1], dt * vmax / pool_vol]
+            concops += [_BUFFER, kp + 1]
+            concchip += [dt * kb_p / 2.0, dt * kb_p * bt_p, dt * kf_p]
+            concops.append(_POOL)
+    ops.append(_LCOMPT)
+    concops.append(_LCONC if (last is not None and not last[1]) else _LCACONC)
+    import dataclasses
+    n = base.ncompts
+    extra = dict(base.extra)
+    extra.update(shell_vol=np.array(shell_vol), shell_params=dict(
+        fixbuf=(kf_f, kb_f, bt_f), difbuf=(kf_d, kb_d, bt_d, D_b), D_ca=D_ca, ca0=ca0, dt2=dt2, dtp=dtp))
+    return dataclasses.replace(
+        base, ops=np.array(ops, np.int32), chip=np.array(chip, np.float64), ncols=tab.shape[1],
+        tablist=np.ascontiguousarray(tab.reshape(-1)), compops=np.array(compops, np.int32),
+        compchips=np.array(compchips, np.int32), conc=np.array(conc, np.float64),
+        concops=np.array(concops, np.int32), concchip=np.array(concchip, np.float64),
+        ydivs=ydivs, ymin=0.0, invdy=invdy, givals=np.zeros(0), extra=extra,
+        nshells=np.array(nshells, np.int32), nshrows=np.array(nshrows, np.int32),
+        nshfix=np.array(nshfix, np.int32), dconc=np.array(dconc, np.int32),
+        fbufval=np.array(fbufval, np.int32), fdiffs=np.arange(n + 1, dtype=np.int32),
+        covals0=np.array(covals0, np.float64), bufvals=np.array(bufvals, np.float64))

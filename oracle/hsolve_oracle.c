@@ -6,8 +6,11 @@
  * shapes, the reference CPU code:
  *   - channel-opcode interpreter: genesis/src/hines/hines_chip.c:78-602
  *     (built as do_chip_hh4_update via hines_4chip.c:19-24);
- *   - Ca_concen pools: genesis/src/hines/hines_conc_chip.c:196-211
- *     (do_h4_conc_chip_update; CACONC_OP and LCACONC_OP only);
+ *   - the concentration program: genesis/src/hines/hines_conc_chip.c:59-222
+ *     (do_h4_conc_chip_update: shells' results and diagonals, pumps, pool
+ *     buffers, pools, Ca_concen);
+ *   - the diffusion systems' solve: genesis/src/hines/hines_conc.c:39-396
+ *     (do_h_conc_solve: tridiagonal for shells alone, banded with buffers);
  *   - Euler / Crank-Nicolson elimination: genesis/src/hines/hines_solve.c
  *     :223-296 and :140-218;
  *   - step phase order: genesis/src/hines/hines.c:222-270.
@@ -50,8 +53,20 @@
 #define NEWCONC1_OP 5110
 #define NEWCONC2_OP 5111
 #define DOMAIN_OP 5112
-#define CACONC_OP 10200   /* concops[] codes: hines_defs.h:214-215 */
+/* concops[] codes: hines_defs.h:209-222 */
+#define CONC_OP 10100
+#define FCONC_OP 10101
+#define LCONC_OP 10102
+#define FLUXCONC_OP 10103
+#define CACONC_OP 10200
 #define LCACONC_OP 10202
+#define TAUPUMP_OP 11000
+#define CTAUPUMP_OP 11001
+#define MMPUMP_OP 11010
+#define HILLPUMP_OP 11011
+#define PHILLPUMP_OP 11012
+#define BUFFER_OP 12000
+#define POOL_OP 12100
 /* funcs codes: hines_defs.h:119-128 */
 #define FORWARD_ELIM 0
 #define BACKWARD_ELIM 0
@@ -86,6 +101,8 @@ struct ohs {
     ohs_arrays a;
     int32_t *ops;
     double *chip, *vm, *results, *givals, *conc, *flux, *codiag, *itotal;
+    double *coresult;          /* per concentration: the concops result (hines_conc_chip.c:88) */
+    double *dscr_cv, *dscr_rs; /* the solve's elimination scratch (covals, cores) */
     double *results_base;
     int32_t bad_funcs;         /* a funcs[] program that left the row range (not run further) */
     double *ravals;
@@ -137,6 +154,19 @@ ohs *ohs_create(const ohs_arrays *a)
         h->conc = (double *)dupmem(a->conc, sizeof(double) * (a->nconcs + 1));
         h->flux = (double *)calloc(a->nconcs + 1, sizeof(double));
         h->codiag = (double *)calloc(a->nconcs + 1, sizeof(double));
+        h->coresult = (double *)calloc(a->nconcs + 1, sizeof(double));
+        /* h_init_conc_solve sets every diagonal to 1 (hines_conc_init.c:127-131) */
+        for (int32_t i = 0; i <= a->nconcs; i++) h->codiag[i] = 1.0;
+    }
+    if (a->ndiffs > 0) {
+        size_t rows = 1, cells = 1;
+        for (int32_t i = 0; i < a->ndiffs; i++) {
+            const size_t r = (size_t)a->nshells[i] * (size_t)a->nshrows[i];
+            if (r > rows) rows = r;
+            if (r * (size_t)a->nshrows[i] > cells) cells = r * (size_t)a->nshrows[i];
+        }
+        h->dscr_cv = (double *)calloc(cells + 1, sizeof(double));
+        h->dscr_rs = (double *)calloc(rows + 1, sizeof(double));
     }
     if (a->nelm_names > 0) h->itotal = (double *)calloc(a->nelm_names, sizeof(double));
     h->capspk = 1024;
@@ -151,6 +181,7 @@ void ohs_destroy(ohs *h)
     if (!h) return;
     free(h->ops); free(h->chip); free(h->vm); free(h->results_base); free(h->ravals);
     free(h->givals); free(h->conc); free(h->flux); free(h->codiag); free(h->itotal);
+    free(h->coresult); free(h->dscr_cv); free(h->dscr_rs);
     free(h->spike_op); free(h->spike_step); free(h->trace); free(h->rows);
     free(h->conn_off); free(h->conn_dst); free(h->conn_T); free(h->conn_wdt);
     if (h->pend)
@@ -853,25 +884,321 @@ void ohs_crank_solve(ohs *h)
     }
 }
 
-/* Ca_concen pools (hines_conc_chip.c:196-207, CHANMODE4: the flux is not
- * reset here; CONC_VAL_OP resets it in the chip pass).  The reference runs
- * this pass after the whole chip update and before the solve
- * (hines.c:238-240). */
+/* X^(hill-1) as the Hill pumps form it (hines_conc_chip.c:126-135) */
+static double hill_pow(int32_t hill, double X)
+{
+    if (hill == 2) return X;
+    if (hill == 3) return X * X;
+    if (hill == 4) return X * X * X;
+    return 1.0;
+}
+
+/* The concentration program (do_h4_conc_chip_update, hines_conc_chip.c:59-222,
+ * built with CHANMODE4: FLUXCONC_OP and CACONC_OP leave the flux alone, the
+ * chip pass's CONC_VAL_OP resets it).  Two registers run through it: the
+ * concentration the ops apply to and the sum of their terms, which CONC_OP
+ * (before moving on) and LCONC_OP store as that concentration's coresult.
+ * The reference runs this pass after the whole chip update and before the
+ * solves (hines.c:238-240).  TAUPUMP_OP's chanmode-4 copy of its rate into
+ * the solver's ovals[] is not kept. */
 int32_t ohs_conc_chip_update(ohs *h)
 {
     const int32_t *op = h->a.concops;
     const double *cc = h->a.concchip;
+    double *conc = h->conc, *flux = h->flux, *codiag = h->codiag, *cores = h->coresult;
     if (!op || h->a.nconcops <= 0) return 0;
+    int32_t at = 0;
+    double sum = 0.0;
     for (int32_t q = 0; q < h->a.nconcops;) {
         const int32_t code = op[q++];
-        if (code == LCACONC_OP) return 0;
-        if (code != CACONC_OP) return -1;
-        const int32_t k = op[q++];
-        const double base = *cc++;
-        const double itau = *cc++;
-        h->conc[k] = base + ((h->conc[k] - base) * (2.0 - itau) + (h->flux[k] * *cc++)) / itau;
+        switch (code) {
+        case CONC_OP:
+            cores[at] = sum;
+            /* fall through */
+        case FCONC_OP:
+            at = op[q++];
+            sum = 0.0;
+            break;
+        case FLUXCONC_OP:               /* dt leak, dt/(z F vol) */
+            sum += cc[0] + flux[at] * cc[1];
+            cc += 2;
+            break;
+        case CTAUPUMP_OP:               /* Ceq dt/T_C, dt/T_C */
+            sum += cc[0];
+            codiag[at] += cc[1];
+            cc += 2;
+            break;
+        case TAUPUMP_OP: {              /* T_C, T_A, T_V, 1/T_B, Ceq; operand: compartment */
+            const double itau = h->a.dt / (cc[0] + cc[1] * exp((h->vm[op[q++]] - cc[2]) * cc[3]));
+            sum += cc[4] * itau;
+            codiag[at] += itau;
+            cc += 5;
+            break;
+        }
+        case MMPUMP_OP:                 /* dt vmax/vol, Kd */
+            codiag[at] += cc[0] / (conc[at] + cc[1]);
+            cc += 2;
+            break;
+        case HILLPUMP_OP: {             /* Kd, dt vmax/vol; operand: Hill coefficient */
+            const double X = conc[at];
+            double c = hill_pow(op[q++], X);
+            c = c / (c * X + cc[0]);
+            codiag[at] += cc[1] * c;
+            cc += 2;
+            break;
+        }
+        case PHILLPUMP_OP: {            /* Kd, dt vmax/vol shell, dt vmax/vol pool; operands: Hill, shell */
+            const int32_t hill = op[q++];
+            const int32_t sh = op[q++];
+            const double X = conc[sh];
+            double c = hill_pow(hill, X);
+            c = c / (c * X + cc[0]);
+            codiag[sh] += cc[1] * c;
+            sum += cc[2] * c * X;
+            cc += 3;
+            break;
+        }
+        case BUFFER_OP: {               /* kb dt/2, dt kb Btot, dt kf; operand: the buffer */
+            const int32_t b = op[q++];
+            const double X = cc[0];
+            const double c = ((1.0 - X) * conc[b] + cc[1]) / (1.0 + X + cc[2] * conc[at]);
+            sum += c - conc[b];
+            conc[b] = c;
+            cc += 3;
+            break;
+        }
+        case POOL_OP:
+            conc[at] += sum;
+            break;
+        case CACONC_OP: {               /* base, 1 + dt/(2 tau), dt B; operand: the pool */
+            const int32_t k = op[q++];
+            const double base = cc[0];
+            const double itau = cc[1];
+            conc[k] = base + ((conc[k] - base) * (2.0 - itau) + (flux[k] * cc[2])) / itau;
+            cc += 3;
+            break;
+        }
+        case LCONC_OP:
+            cores[at] = sum;
+            return 0;
+        case LCACONC_OP:
+            return 0;
+        default:
+            return -1;
+        }
+    }
+    return -1;
+}
+
+/* Shells alone (nshrows 1): a tridiagonal system per step, eliminated from
+ * the innermost shell out and substituted back (hines_conc.c:322-393).  Row j
+ * reads  (codiag_j + Ddn_j + Dup_j) x_j - Ddn_j x_(j-1) - Dup_j x_(j+1)
+ *        = C_j + coresult_j
+ * with the coupling values of covals0 in order (shell 0: up; inner shells:
+ * down, up; the last: down).  Crank-Nicolson takes C <- 2x - C.  Every
+ * diagonal used is set back to 1.  Returns the covals0 cursor after the
+ * system. */
+static int32_t conc_tridiag(ohs *h, int32_t c0, int32_t ms, int32_t l)
+{
+    double *conc = h->conc, *codiag = h->codiag, *cores = h->coresult;
+    double *cv = h->dscr_cv, *rs = h->dscr_rs;
+    const double *cov0 = h->a.covals0;
+    double up = cov0[l++];
+    double dg = up + codiag[c0];
+    codiag[c0] = 1.0;
+    rs[0] = (conc[c0] + cores[c0]) / dg;
+    int32_t j;
+    for (j = 1; j < ms - 1; j++) {
+        const double e = cv[j] = -up / dg;
+        const double dn = cov0[l];
+        up = cov0[l + 1];
+        l += 2;
+        dg = codiag[c0 + j] + dn * (1.0 + e) + up;
+        codiag[c0 + j] = 1.0;
+        rs[j] = (conc[c0 + j] + cores[c0 + j] + dn * rs[j - 1]) / dg;
+    }
+    const double e = cv[j] = -up / dg;
+    const double dn = cov0[l++];
+    dg = codiag[c0 + j] + dn * (1.0 + e);
+    codiag[c0 + j] = 1.0;
+    double x = (conc[c0 + j] + cores[c0 + j] + dn * rs[j - 1]) / dg;
+    if (h->a.method == CRANK_INT) {
+        conc[c0 + j] = x + x - conc[c0 + j];
+        for (int32_t r = j - 1; r >= 0; r--) {
+            x = rs[r] - cv[r + 1] * x;
+            conc[c0 + r] = x + x - conc[c0 + r];
+        }
+    } else {
+        conc[c0 + j] = x;
+        for (int32_t r = j - 1; r >= 0; r--) {
+            x = rs[r] - cv[r + 1] * x;
+            conc[c0 + r] = x;
+        }
+    }
+    return l;
+}
+
+/* Shells with buffers (nshrows dw > 1): the banded Crank-Nicolson system of
+ * hines_conc.c:106-321, filled and forward-eliminated shell by shell from
+ * the innermost out, then substituted back.  Rows are numbered r = m*dw + n
+ * (shell m, row n: the nd fixed buffers, the diffusible buffers, the shell
+ * last); the scratch holds per row r the dw eliminated upper entries at
+ * cv[r*dw ...] and the eliminated right-hand side rs[r].  Buffer rate
+ * triples (dt/2 kf, dt/2 kb, dt kb Btot) come from bufvals, per-shell totals
+ * of fixed buffers and every coupling from covals0 in the order the rows use
+ * them.  The reference computes a diffusible buffer row's upper entry twice
+ * (hines_conc.c:228-232, then :250); only the second value is kept, so only
+ * it is computed here.  Returns the covals0 cursor after the system. */
+static int32_t conc_banded(ohs *h, int32_t c0, int32_t ms, int32_t dw, int32_t nd, int32_t nb0,
+                           int32_t l)
+{
+    double *conc = h->conc, *codiag = h->codiag, *cores = h->coresult;
+    double *cv = h->dscr_cv, *rs = h->dscr_rs;
+    const double *cov0 = h->a.covals0, *bv = h->a.bufvals;
+    const int32_t sw = dw * dw;
+    double co = conc[c0 + dw - 1];            /* the shell's concentration at t */
+    for (int32_t m = 0; m < ms; m++) {
+        const int32_t rS = m * dw + dw - 1;   /* the shell row */
+        const int32_t xS = rS * dw;           /* its scratch entries */
+        const int32_t xP = xS - sw;           /* the previous shell row's */
+        const int32_t rP = rS - dw;
+        const int last = (m == ms - 1);
+        double dgc = codiag[c0 + rS];
+        codiag[c0 + rS] = 1.0;
+        double rsc = cores[c0 + rS];
+        for (int32_t n = 0; n < nd; n++) {    /* fixed buffers */
+            const int32_t r = m * dw + n, nb = nb0 + 3 * n;
+            double dg = cv[xS + n] = bv[nb] * co + bv[nb + 1];
+            const double t = bv[nb] * conc[c0 + r];
+            dgc += t;
+            dg = 1.0 / (1.0 + dg);
+            cv[r * dw] = t * dg;
+            const double t2 = -bv[nb + 1] * conc[c0 + r] + cov0[l++];
+            rsc += t2;
+            rs[r] = (conc[c0 + r] + t2) * dg;
+        }
+        for (int32_t n = nd; n < dw - 1; n++) {   /* diffusible buffers */
+            const int32_t r = m * dw + n, nb = nb0 + 3 * n;
+            double dg = cv[xS + n] = bv[nb] * co + bv[nb + 1];
+            const double t = bv[nb] * conc[c0 + r];
+            dgc += t;
+            double t2 = -bv[nb + 1] * conc[c0 + r] + bv[nb + 2];
+            rsc += t2;
+            if (m) {
+                const double nw = cov0[l] * cv[(r - dw) * dw];
+                if (!last) {
+                    dg += 1.0 + cov0[l + 1] + cov0[l] * (1.0 + cv[(r - dw) * dw + 1]) - nw * cv[xP + n];
+                    cv[r * dw + 1] = -cov0[l + 1] / dg;
+                    t2 += cov0[l] * conc[c0 + r - dw] + (1.0 - cov0[l] - cov0[l + 1]) * conc[c0 + r] +
+                          cov0[l + 1] * conc[c0 + r + dw];
+                } else {
+                    dg += 1.0 + cov0[l] * (1.0 + cv[(r - dw) * dw + 1]) - nw * cv[xP + n];
+                    t2 += cov0[l] * conc[c0 + r - dw] + (1.0 - cov0[l]) * conc[c0 + r];
+                }
+                rs[r] = (t2 + cov0[l] * rs[r - dw] - nw * rs[rP]) / dg;
+                l += last ? 1 : 2;
+            } else {
+                dg += 1.0 + cov0[l];
+                cv[r * dw + 1] = -cov0[l] / dg;
+                rs[r] = (t2 + (1.0 - cov0[l]) * conc[c0 + r] + cov0[l] * conc[c0 + r + dw]) / dg;
+                l++;
+            }
+            cv[r * dw] = t / dg;
+        }
+        /* the shell row: diffusion to the neighbours */
+        if (m) {
+            if (!last) {
+                dgc += cov0[l + 1] + cov0[l] * (1.0 + cv[xP + dw - 1]);
+                rsc += cov0[l] * (conc[c0 + rS - dw] + rs[rP]) + (1.0 - cov0[l] - cov0[l + 1]) * co;
+                co = conc[c0 + rS + dw];
+                rsc += cov0[l + 1] * co;
+            } else {
+                dgc += cov0[l] * (1.0 + cv[xP + dw - 1]);
+                rsc += cov0[l] * (conc[c0 + rS - dw] + rs[rP]) + (1.0 - cov0[l]) * co;
+            }
+        } else {
+            dgc += cov0[l];
+            rsc += (1.0 - cov0[l]) * co;
+            co = conc[c0 + rS + dw];
+            rsc += cov0[l] * co;
+        }
+        /* eliminate the buffer entries of the shell row */
+        for (int32_t n = 0; n < nd; n++) {
+            const int32_t rb = m * dw + n;
+            dgc -= cv[xS + n] * cv[rb * dw];
+            rsc -= cv[xS + n] * rs[rb];
+        }
+        for (int32_t n = nd; n < dw - 1; n++) {
+            const int32_t rb = m * dw + n;
+            const double nw = m ? cv[xS + n] + cov0[l] * cv[xP + n] : cv[xS + n];
+            dgc -= nw * cv[rb * dw];
+            rsc -= nw * rs[rb];
+            if (!last) cv[xS + n] *= -cv[rb * dw + 1];
+        }
+        if (!last) {
+            for (int32_t n = nd; n < dw - 1; n++) cv[xS + n] /= dgc;
+            if (m) {
+                cv[xS + dw - 1] = -cov0[l + 1] / dgc;
+                l += 2;
+            } else {
+                cv[xS + dw - 1] = -cov0[l] / dgc;
+                l++;
+            }
+            rs[rS] = rsc / dgc;
+        } else {
+            co = conc[c0 + rS] = rsc / dgc;
+            l++;
+        }
+    }
+    /* back substitution, outermost shell first */
+    for (int32_t m = ms - 1; m >= 0; m--) {
+        const int32_t rS = m * dw + dw - 1;
+        const int32_t xS = rS * dw;
+        if (m < ms - 1) {
+            co = rs[rS] - cv[xS + dw - 1] * conc[c0 + rS + dw];
+            for (int32_t n = nd; n < dw - 1; n++) co -= cv[xS + n] * conc[c0 + rS + n + 1];
+            conc[c0 + rS] = co;
+            for (int32_t n = dw - 2; n >= nd; n--) {
+                const int32_t rb = m * dw + n;
+                conc[c0 + rb] = rs[rb] - cv[rb * dw] * co - cv[rb * dw + 1] * conc[c0 + rb + dw];
+            }
+        } else {
+            for (int32_t n = dw - 2; n >= nd; n--) {
+                const int32_t rb = m * dw + n;
+                conc[c0 + rb] = rs[rb] - cv[rb * dw] * co;
+            }
+        }
+        for (int32_t n = nd - 1; n >= 0; n--) {
+            const int32_t rb = m * dw + n;
+            conc[c0 + rb] = rs[rb] - cv[rb * dw] * co;
+        }
+    }
+    return l;
+}
+
+/* do_h_conc_solve (hines_conc.c:39-396): every diffusion system in order,
+ * one covals0 cursor running through them all.  Systems of fewer than two
+ * shells are not stepped: the reference's elimination of one shell reads the
+ * row after the system (its last-row step runs unconditionally). */
+int32_t ohs_conc_solve(ohs *h)
+{
+    int32_t l = 0;
+    for (int32_t i = 0; i < h->a.ndiffs; i++) {
+        const int32_t ms = h->a.nshells[i], dw = h->a.nshrows[i];
+        if (ms < 2 || dw < 1) return -1;
+        if (dw == 1)
+            l = conc_tridiag(h, h->a.dconc[i], ms, l);
+        else
+            l = conc_banded(h, h->a.dconc[i], ms, dw, h->a.nshfix[i], h->a.fbufval[i], l);
     }
     return 0;
+}
+
+void ohs_conc_regs(ohs *h, double *codiag, double *coresult)
+{
+    if (h->a.nconcs <= 0) return;
+    if (codiag) memcpy(codiag, h->codiag, sizeof(double) * h->a.nconcs);
+    if (coresult) memcpy(coresult, h->coresult, sizeof(double) * h->a.nconcs);
 }
 
 void ohs_get_conc(ohs *h, double *conc)
@@ -889,6 +1216,7 @@ int32_t ohs_step(ohs *h, int32_t nsteps)
     for (int32_t s = 0; s < nsteps; s++) {
         if (ohs_chip_update(h)) return -1;
         if (ohs_conc_chip_update(h)) return -1;
+        if (h->a.ndiffs > 0 && ohs_conc_solve(h)) return -1;
         if (h->a.method == CRANK_INT) ohs_crank_solve(h);
         else ohs_euler_solve(h);
         if (h->bad_funcs) return -1;           /* malformed funcs[]: stopped inside the rows */

@@ -49,9 +49,23 @@ typedef struct ohs_arrays {
     const double  *vm;
     const double  *givals;    /* may be NULL: then no givals are kept */
     const double  *conc;      /* nconcs or NULL */
-    const int32_t *concops;   /* nconcops or NULL: (CACONC_OP k)* LCACONC_OP */
-    const double  *concchip;  /* nconcchip or NULL: Ca_base, 1 + dt/(2 tau), dt B per pool */
+    const int32_t *concops;   /* nconcops or NULL: the concentration program
+                                 (hines_conc_init.c:426-805): CONC_OP / FCONC_OP
+                                 index, FLUXCONC_OP, the pumps, BUFFER_OP, POOL_OP,
+                                 CACONC_OP, ended by LCONC_OP or LCACONC_OP */
+    const double  *concchip;  /* nconcchip or NULL: the values those ops consume */
     int32_t nconcops, nconcchip;
+    /* Diffusion systems of shells and buffers (do_h_conc_solve,
+     * hines_conc.c:39-396; arrays of h_init_conc_solve,
+     * hines_conc_init.c:59-291).  System i has nshells[i] shells of
+     * nshrows[i] rows each (its buffers, the nshfix[i] fixed ones first, then
+     * the shell); its concentrations start at conc[dconc[i]], row r of shell m
+     * at dconc[i] + m*nshrows[i] + r; its buffer rates at bufvals[fbufval[i]]
+     * (3 per buffer).  covals0 holds every system's coupling and buffer-total
+     * values in the order the solve consumes them.  ndiffs = 0: none. */
+    int32_t ndiffs, ncovals0, nbufvals, pad2;
+    const int32_t *nshells, *nshrows, *nshfix, *dconc, *fbufval;
+    const double  *covals0, *bufvals;
 } ohs_arrays;
 
 ohs    *ohs_create(const ohs_arrays *a);
@@ -63,13 +77,20 @@ void    ohs_set_hooks(ohs *h, ohs_syn_fn syn, ohs_rand_fn rnd, void *ctx);
 int32_t ohs_chip_update(ohs *h);
 void    ohs_euler_solve(ohs *h);
 void    ohs_crank_solve(ohs *h);
-/* Ca_concen pass over concops[] (do_h4_conc_chip_update).  Returns 0 or -1
- * on a pool opcode other than CACONC_OP / LCACONC_OP. */
+/* The pass over concops[] (do_h4_conc_chip_update): shell results and
+ * diagonals, pumps, pool buffers, pools and Ca_concen.  Returns 0 or -1 on an
+ * undefined opcode. */
 int32_t ohs_conc_chip_update(ohs *h);
+/* The diffusion systems' implicit step (do_h_conc_solve).  Returns 0, or -1
+ * for a system of fewer than two shells. */
+int32_t ohs_conc_solve(ohs *h);
+/* codiag[] and coresult[] (nconcs each; either may be NULL): the per
+ * concentration diagonal and result the concops pass leaves for the solve */
+void    ohs_conc_regs(ohs *h, double *codiag, double *coresult);
 void    ohs_get_conc(ohs *h, double *conc);
 void    ohs_put_conc(ohs *h, const double *conc);
-/* nsteps full steps: chip update, the Ca_concen pass, then the elimination
- * selected by method. */
+/* nsteps full steps: chip update, the concops pass, the diffusion solve, then
+ * the elimination selected by method. */
 int32_t ohs_step(ohs *h, int32_t nsteps);
 
 /* state access (copies) */

@@ -26,6 +26,9 @@ class Arrays(C.Structure):
         ("yvals", _f64p), ("stablist", _f64p), ("funcs", _i32p), ("ravals", _f64p),
         ("vm", _f64p), ("givals", _f64p), ("conc", _f64p),
         ("concops", _i32p), ("concchip", _f64p), ("nconcops", C.c_int32), ("nconcchip", C.c_int32),
+        ("ndiffs", C.c_int32), ("ncovals0", C.c_int32), ("nbufvals", C.c_int32), ("pad2", C.c_int32),
+        ("nshells", _i32p), ("nshrows", _i32p), ("nshfix", _i32p), ("dconc", _i32p), ("fbufval", _i32p),
+        ("covals0", _f64p), ("bufvals", _f64p),
     ]
 
 
@@ -66,7 +69,10 @@ def oracle_lib():
         L.ohs_step.restype = C.c_int32
         L.ohs_conc_chip_update.argtypes = [C.c_void_p]
         L.ohs_conc_chip_update.restype = C.c_int32
+        L.ohs_conc_solve.argtypes = [C.c_void_p]
+        L.ohs_conc_solve.restype = C.c_int32
         L.ohs_get_conc.argtypes = [C.c_void_p, _f64p]
+        L.ohs_conc_regs.argtypes = [C.c_void_p, _f64p, _f64p]
         L.ohs_put_conc.argtypes = [C.c_void_p, _f64p]
         L.ohs_get.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _i32p, _f64p]
         L.ohs_put.argtypes = [C.c_void_p, _f64p, _f64p, _i32p]
@@ -167,6 +173,14 @@ class Oracle:
             k["concchip"] = np.ascontiguousarray(m.concchip, np.float64)
             a.concops, a.nconcops = _p(k["concops"], C.c_int32), k["concops"].size
             a.concchip, a.nconcchip = _p(k["concchip"], C.c_double), k["concchip"].size
+        if getattr(m, "nshells", None) is not None and np.size(m.nshells):
+            for f in ("nshells", "nshrows", "nshfix", "dconc", "fbufval"):
+                k[f] = np.ascontiguousarray(getattr(m, f), np.int32)
+                setattr(a, f, _p(k[f], C.c_int32))
+            for f in ("covals0", "bufvals"):
+                k[f] = np.ascontiguousarray(getattr(m, f), np.float64)
+                setattr(a, f, _p(k[f], C.c_double) if k[f].size else None)
+            a.ndiffs, a.ncovals0, a.nbufvals = k["nshells"].size, k["covals0"].size, k["bufvals"].size
         self._a = a
         self.h = L.ohs_create(C.byref(a))
         self._hooks = None
@@ -202,6 +216,14 @@ class Oracle:
     def crank_solve(self):
         oracle_lib().ohs_crank_solve(self.h)
 
+    def conc_chip_update(self):
+        if oracle_lib().ohs_conc_chip_update(self.h) != 0:
+            raise RuntimeError("oracle: undefined concentration operation")
+
+    def conc_solve(self):
+        if oracle_lib().ohs_conc_solve(self.h) != 0:
+            raise RuntimeError("oracle: a diffusion system of fewer than two shells")
+
     @property
     def vm(self):
         v = np.empty(self.m.ncompts)
@@ -224,6 +246,13 @@ class Oracle:
         if out.size:
             oracle_lib().ohs_get_conc(self.h, _p(out, C.c_double))
         return out
+
+    def conc_regs(self):
+        """(codiag, coresult) as the concops pass leaves them."""
+        d, r = np.empty(self._a.nconcs), np.empty(self._a.nconcs)
+        if d.size:
+            oracle_lib().ohs_conc_regs(self.h, _p(d, C.c_double), _p(r, C.c_double))
+        return d, r
 
     def put_conc(self, conc):
         a = np.ascontiguousarray(conc, np.float64)
