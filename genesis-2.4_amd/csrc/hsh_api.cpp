@@ -1458,6 +1458,10 @@ hsh_status check_model(const hsh_model* m)
         return fail(HSH_ERR_ARG, "hsh_create: inconsistent concentration arrays");
     if (m->ngivals < 0 || (m->ngivals > 0 && !m->givals))
         return fail(HSH_ERR_ARG, "hsh_create: ngivals > 0 but no givals");
+    if (m->ndiffs < 0 || m->ncovals0 < 0 || m->nbufvals < 0 ||
+        (m->ndiffs > 0 && (!m->nshells || !m->nshrows || !m->nshfix || !m->dconc || !m->fbufval ||
+                           !m->fdiffs || (m->ncovals0 > 0 && !m->covals0) || (m->nbufvals > 0 && !m->bufvals))))
+        return fail(HSH_ERR_ARG, "hsh_create: inconsistent diffusion arrays");
     return HSH_OK;
 }
 
@@ -1479,6 +1483,10 @@ hsh_status create_impl(const hsh_model* m, const hsh_options* opt, hsh_handle** 
     if (m->nconcs > 0) h->chip.insert(h->chip.end(), m->conc, m->conc + m->nconcs);
     if (m->nconcchip > 0) h->chip.insert(h->chip.end(), m->concchip, m->concchip + m->nconcchip);
     if (m->ngivals > 0 && m->givals) h->chip.insert(h->chip.end(), m->givals, m->givals + m->ngivals);
+    if (m->ndiffs > 0) {
+        if (m->ncovals0 > 0) h->chip.insert(h->chip.end(), m->covals0, m->covals0 + m->ncovals0);
+        if (m->nbufvals > 0) h->chip.insert(h->chip.end(), m->bufvals, m->bufvals + m->nbufvals);
+    }
     std::string err;
     if (!analyse(*m, h->plan, err, h->garr ? &h->garr->spec : nullptr)) {
         delete h;
@@ -1626,6 +1634,18 @@ hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_opt
     for (int32_t i = 0; i < n; i++) {
         if (!ms[i]) return fail(HSH_ERR_ARG, "hsh_create_group: null member");
         STCHK(check_model(ms[i]));
+        // members merge Ca_concen pools only: the concentration program of
+        // shells, buffers, pumps and pools is owned per compartment of one
+        // solver and is not offset across members
+        bool conc_prog = ms[i]->ndiffs > 0;
+        for (int32_t q = 0; q < ms[i]->nconcops && !conc_prog; q += 2) {
+            const int32_t c = ms[i]->concops[q];
+            if (c == LCACONC_OP) break;
+            if (c != CACONC_OP) conc_prog = true;
+        }
+        if (conc_prog)
+            return fail(HSH_ERR_ARG, "hsh_create_group: a member with diffusion shells, buffers, pumps or "
+                                     "concpools (create it as a solver of its own)");
     }
     const hsh_model& m0 = *ms[0];
     const bool giv = m0.ngivals > 0;
@@ -1748,6 +1768,9 @@ hsh_status hsh_create_group(const hsh_model* const* ms, int32_t n, const hsh_opt
     A.ops.push_back(LCOMPT_OP);
     if (!A.concops.empty()) A.concops.push_back(LCACONC_OP);
     hsh_model mm = m0;
+    mm.ndiffs = mm.ncovals0 = mm.nbufvals = 0;
+    mm.nshells = mm.nshrows = mm.nshfix = mm.dconc = mm.fbufval = mm.fdiffs = nullptr;
+    mm.covals0 = mm.bufvals = nullptr;
     if (mx) {
         mm.xdivs = mx->xdivs;
         mm.xmin = mx->xmin;
@@ -1948,6 +1971,9 @@ hsh_status input_entries(hsh_handle* h, int32_t n, const hsh_input* in, hsh_hand
         } else if (e.op == HSH_IN_FLUX) {
             if (e.index < 0 || e.index >= std::max(0, h->m.nconcs))
                 return fail(HSH_ERR_ARG, std::string(who) + ": pool index out of range");
+            if ((size_t)e.index < h->plan.flux_live.size() && h->plan.flux_live[e.index])
+                return fail(HSH_ERR_ARG, std::string(who) + ": FLUX input into a shell whose FLUXCONC_OP reads a "
+                                                            "flux no channel clears (not run on the device)");
         } else {
             return fail(HSH_ERR_ARG, std::string(who) + ": unknown input op " + std::to_string(e.op));
         }
@@ -1971,8 +1997,9 @@ hsh_status input_entries(hsh_handle* h, int32_t n, const hsh_input* in, hsh_hand
         }
         // HSH_IN_FLUX: CONC_VAL_OP clears the pool's flux register at the
         // start of every program that feeds it (hines_chip.c:205-212), and
-        // the device runs only pools such a program feeds: the value is
-        // overwritten before anything reads it
+        // the device runs only pools such a program feeds (a shell whose
+        // FLUXCONC_OP reads a flux nothing clears is refused above): the
+        // value is overwritten before anything reads it
     }
     std::sort(buf, buf + k, [](const hsh_handle::InTagged& a, const hsh_handle::InTagged& b) {
         return a.e.pos != b.e.pos ? a.e.pos < b.e.pos : a.seq < b.seq;

@@ -355,6 +355,246 @@ __device__ __forceinline__ double caconc(double C, double base, double itau, dou
 }
 
 // ---------------------------------------------------------------------------
+// A compartment's concentration program and diffusion systems
+// (hines_conc_chip.c:59-222, hines_conc.c:39-396), run at the end of the
+// program of the compartment that owns them.  They read and write the
+// compartment's slots in memory (a shell's concentration, diagonal and result
+// change within the step), with the same expressions in the same order as
+// the reference.
+
+// C^(hill-1) as the Hill pumps form it (hines_conc_chip.c:126-135)
+__device__ __forceinline__ double hill_pow(int hill, double X)
+{
+    if (hill == 2) return X;
+    if (hill == 3) return X * X;
+    if (hill == 4) return X * X * X;
+    return 1.0;
+}
+
+// One diffusion system.  Slots: concentrations cs0 + r (row r = shell*dw +
+// row in shell, shell 0 innermost), covals0 values cv0 + l in the order they
+// are used, buffer rates bv0 + 3*buffer, codiag dg0 + shell, coresult
+// cr0 + shell, scratch x0 (the eliminated upper entries, dw per row) and r0
+// (the eliminated right-hand sides).  Shells alone (dw 1): tridiagonal,
+// Crank-Nicolson as C <- 2x - C; with buffers: the banded Crank-Nicolson
+// system, whose diffusible buffer rows keep the reference's second value of
+// their upper entry (hines_conc.c:250).  The diagonals used are set back to 1.
+__device__ inline void diff_solve(double* __restrict__ Sb, int lane, int ms, int dw, int nd, int cs0, int cv0,
+                                  int bv0, int dg0, int cr0, int x0, int r0, int crank)
+{
+#define DS(c) Sb[(c) * W + lane]
+#define CN(r) DS(cs0 + (r))
+#define CV(i) DS(x0 + (i))
+#define RS(i) DS(r0 + (i))
+#define K0(i) DS(cv0 + (i))
+#define BV(i) DS(bv0 + (i))
+    int l = 0;
+    if (dw == 1) {
+        double up = K0(l++);
+        double dg = up + DS(dg0);
+        DS(dg0) = 1.0;
+        RS(0) = (CN(0) + DS(cr0)) / dg;
+        int j;
+        for (j = 1; j < ms - 1; j++) {
+            const double e = -up / dg;
+            CV(j) = e;
+            const double dn = K0(l);
+            up = K0(l + 1);
+            l += 2;
+            dg = DS(dg0 + j) + dn * (1.0 + e) + up;
+            DS(dg0 + j) = 1.0;
+            RS(j) = (CN(j) + DS(cr0 + j) + dn * RS(j - 1)) / dg;
+        }
+        const double e = -up / dg;
+        CV(j) = e;
+        const double dn = K0(l++);
+        dg = DS(dg0 + j) + dn * (1.0 + e);
+        DS(dg0 + j) = 1.0;
+        double x = (CN(j) + DS(cr0 + j) + dn * RS(j - 1)) / dg;
+        if (crank) {
+            CN(j) = x + x - CN(j);
+            for (int r = j - 1; r >= 0; r--) {
+                x = RS(r) - CV(r + 1) * x;
+                CN(r) = x + x - CN(r);
+            }
+        } else {
+            CN(j) = x;
+            for (int r = j - 1; r >= 0; r--) {
+                x = RS(r) - CV(r + 1) * x;
+                CN(r) = x;
+            }
+        }
+        return;
+    }
+    const int sw = dw * dw;
+    double co = CN(dw - 1);
+    for (int m = 0; m < ms; m++) {
+        const int rS = m * dw + dw - 1, xS = rS * dw, xP = xS - sw, rP = rS - dw;
+        const bool last = (m == ms - 1);
+        double dgc = DS(dg0 + m);
+        DS(dg0 + m) = 1.0;
+        double rsc = DS(cr0 + m);
+        for (int n = 0; n < nd; n++) {
+            const int r = m * dw + n, nb = 3 * n;
+            double dg = BV(nb) * co + BV(nb + 1);
+            CV(xS + n) = dg;
+            const double t = BV(nb) * CN(r);
+            dgc += t;
+            dg = 1.0 / (1.0 + dg);
+            CV(r * dw) = t * dg;
+            const double t2 = -BV(nb + 1) * CN(r) + K0(l++);
+            rsc += t2;
+            RS(r) = (CN(r) + t2) * dg;
+        }
+        for (int n = nd; n < dw - 1; n++) {
+            const int r = m * dw + n, nb = 3 * n;
+            double dg = BV(nb) * co + BV(nb + 1);
+            CV(xS + n) = dg;
+            const double t = BV(nb) * CN(r);
+            dgc += t;
+            double t2 = -BV(nb + 1) * CN(r) + BV(nb + 2);
+            rsc += t2;
+            if (m) {
+                const double nw = K0(l) * CV((r - dw) * dw);
+                if (!last) {
+                    dg += 1.0 + K0(l + 1) + K0(l) * (1.0 + CV((r - dw) * dw + 1)) - nw * CV(xP + n);
+                    CV(r * dw + 1) = -K0(l + 1) / dg;
+                    t2 += K0(l) * CN(r - dw) + (1.0 - K0(l) - K0(l + 1)) * CN(r) + K0(l + 1) * CN(r + dw);
+                } else {
+                    dg += 1.0 + K0(l) * (1.0 + CV((r - dw) * dw + 1)) - nw * CV(xP + n);
+                    t2 += K0(l) * CN(r - dw) + (1.0 - K0(l)) * CN(r);
+                }
+                RS(r) = (t2 + K0(l) * RS(r - dw) - nw * RS(rP)) / dg;
+                l += last ? 1 : 2;
+            } else {
+                dg += 1.0 + K0(l);
+                CV(r * dw + 1) = -K0(l) / dg;
+                RS(r) = (t2 + (1.0 - K0(l)) * CN(r) + K0(l) * CN(r + dw)) / dg;
+                l++;
+            }
+            CV(r * dw) = t / dg;
+        }
+        if (m) {
+            if (!last) {
+                dgc += K0(l + 1) + K0(l) * (1.0 + CV(xP + dw - 1));
+                rsc += K0(l) * (CN(rS - dw) + RS(rP)) + (1.0 - K0(l) - K0(l + 1)) * co;
+                co = CN(rS + dw);
+                rsc += K0(l + 1) * co;
+            } else {
+                dgc += K0(l) * (1.0 + CV(xP + dw - 1));
+                rsc += K0(l) * (CN(rS - dw) + RS(rP)) + (1.0 - K0(l)) * co;
+            }
+        } else {
+            dgc += K0(l);
+            rsc += (1.0 - K0(l)) * co;
+            co = CN(rS + dw);
+            rsc += K0(l) * co;
+        }
+        for (int n = 0; n < nd; n++) {
+            const int rb = m * dw + n;
+            dgc -= CV(xS + n) * CV(rb * dw);
+            rsc -= CV(xS + n) * RS(rb);
+        }
+        for (int n = nd; n < dw - 1; n++) {
+            const int rb = m * dw + n;
+            const double nw = m ? CV(xS + n) + K0(l) * CV(xP + n) : CV(xS + n);
+            dgc -= nw * CV(rb * dw);
+            rsc -= nw * RS(rb);
+            if (!last) CV(xS + n) *= -CV(rb * dw + 1);
+        }
+        if (!last) {
+            for (int n = nd; n < dw - 1; n++) CV(xS + n) /= dgc;
+            if (m) {
+                CV(xS + dw - 1) = -K0(l + 1) / dgc;
+                l += 2;
+            } else {
+                CV(xS + dw - 1) = -K0(l) / dgc;
+                l++;
+            }
+            RS(rS) = rsc / dgc;
+        } else {
+            co = rsc / dgc;
+            CN(rS) = co;
+            l++;
+        }
+    }
+    for (int m = ms - 1; m >= 0; m--) {
+        const int rS = m * dw + dw - 1, xS = rS * dw;
+        if (m < ms - 1) {
+            co = RS(rS) - CV(xS + dw - 1) * CN(rS + dw);
+            for (int n = nd; n < dw - 1; n++) co -= CV(xS + n) * CN(rS + n + 1);
+            CN(rS) = co;
+            for (int n = dw - 2; n >= nd; n--) {
+                const int rb = m * dw + n;
+                CN(rb) = RS(rb) - CV(rb * dw) * co - CV(rb * dw + 1) * CN(rb + dw);
+            }
+        } else {
+            for (int n = dw - 2; n >= nd; n--) {
+                const int rb = m * dw + n;
+                CN(rb) = RS(rb) - CV(rb * dw) * co;
+            }
+        }
+        for (int n = nd - 1; n >= 0; n--) {
+            const int rb = m * dw + n;
+            CN(rb) = RS(rb) - CV(rb * dw) * co;
+        }
+    }
+#undef DS
+#undef CN
+#undef CV
+#undef RS
+#undef K0
+#undef BV
+}
+
+// The concentration ops of hines_conc_chip.c on slots (a: first value slot)
+__device__ __forceinline__ void conc_fluxconc(double* __restrict__ Sb, int lane, double& csum, int a, double flux)
+{
+    csum += Sb[a * W + lane] + flux * Sb[(a + 1) * W + lane];
+}
+__device__ __forceinline__ void conc_ctaupump(double* __restrict__ Sb, int lane, double& csum, int a, int dg)
+{
+    csum += Sb[a * W + lane];
+    Sb[dg * W + lane] += Sb[(a + 1) * W + lane];
+}
+__device__ __forceinline__ void conc_taupump(double* __restrict__ Sb, int lane, double& csum, int a, int dg,
+                                             double dt, double Vm)
+{
+    const double itau = dt / (Sb[a * W + lane] + Sb[(a + 1) * W + lane] *
+                                                      exp((Vm - Sb[(a + 2) * W + lane]) * Sb[(a + 3) * W + lane]));
+    csum += Sb[(a + 4) * W + lane] * itau;
+    Sb[dg * W + lane] += itau;
+}
+__device__ __forceinline__ void conc_mmpump(double* __restrict__ Sb, int lane, int a, int cs, int dg)
+{
+    Sb[dg * W + lane] += Sb[a * W + lane] / (Sb[cs * W + lane] + Sb[(a + 1) * W + lane]);
+}
+__device__ __forceinline__ void conc_hillpump(double* __restrict__ Sb, int lane, int a, int cs, int dg, int hill)
+{
+    const double X = Sb[cs * W + lane];
+    double c = hill_pow(hill, X);
+    c = c / (c * X + Sb[a * W + lane]);
+    Sb[dg * W + lane] += Sb[(a + 1) * W + lane] * c;
+}
+__device__ __forceinline__ void conc_phillpump(double* __restrict__ Sb, int lane, double& csum, int a, int cs,
+                                               int dg, int hill)
+{
+    const double X = Sb[cs * W + lane];
+    double c = hill_pow(hill, X);
+    c = c / (c * X + Sb[a * W + lane]);
+    Sb[dg * W + lane] += Sb[(a + 1) * W + lane] * c;
+    csum += Sb[(a + 2) * W + lane] * c * X;
+}
+__device__ __forceinline__ void conc_buffer(double* __restrict__ Sb, int lane, double& csum, int a, int cb, int cp)
+{
+    const double X = Sb[a * W + lane];
+    const double B = Sb[cb * W + lane];
+    const double c = ((1.0 - X) * B + Sb[(a + 1) * W + lane]) / (1.0 + X + Sb[(a + 2) * W + lane] * Sb[cp * W + lane]);
+    csum += c - B;
+    Sb[cb * W + lane] = c;
+}
+
+// ---------------------------------------------------------------------------
 // Generic micro-op interpreter.  With REG the compartment's whole slot vector
 // is fetched with back-to-back coalesced loads before the program runs (one
 // memory latency per compartment instead of one per micro-op) and indexed
@@ -394,6 +634,7 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
     double Gk = 0.0, Ek = 0.0, Ik = 0.0;
     double vipol = 0.0, cipol = 0.0;
     Flux flux;
+    double csum = 0.0;
     int cfilo = 0;
     double ipol = 0.0, ghk = 0.0;
     const double* tabr = d.tab;
@@ -550,6 +791,53 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             const int p = code[pc], cs = code[pc + 1], ks = code[pc + 2];
             pc += 3;
             SPUT(cs, caconc(SGET(cs), SGET(ks), SGET(ks + 1), flux.get(p), SGET(ks + 2)));
+        } else if (EXT && op == U_ELMMPUMP) {
+            const int cs = code[pc], dg = code[pc + 1];
+            pc += 2;
+            const double C = Sb[cs * W + lane];
+            const double X = 1.0 / (C + SGET(c));
+            Ik = SGET(c + 1) * C * X;
+            ichan += Ik;
+            if (giv) {
+                d.giv[gi] = d.chanmode5 ? Ik / SGET(c + 1) : Ik;
+                gi += 1;
+            }
+            Im += Ik;
+            if (dg >= 0) Sb[dg * W + lane] += SGET(c + 2) * X;
+            c += 3;
+        } else if (EXT && op == U_CZERO) {
+            csum = 0.0;
+        } else if (EXT && op == U_CSTORE) {
+            Sb[code[pc++] * W + lane] = csum;
+        } else if (EXT && op == U_FLUXCONC) {
+            const int a = code[pc], p = code[pc + 1];
+            pc += 2;
+            conc_fluxconc(Sb, lane, csum, a, p >= 0 ? flux.get(p) : 0.0);
+        } else if (EXT && op == U_CTAUPUMP) {
+            conc_ctaupump(Sb, lane, csum, code[pc], code[pc + 1]);
+            pc += 2;
+        } else if (EXT && op == U_TAUPUMP) {
+            conc_taupump(Sb, lane, csum, code[pc], code[pc + 1], d.dt, Vm);
+            pc += 2;
+        } else if (EXT && op == U_MMPUMP) {
+            conc_mmpump(Sb, lane, code[pc], code[pc + 1], code[pc + 2]);
+            pc += 3;
+        } else if (EXT && op == U_HILLPUMP) {
+            conc_hillpump(Sb, lane, code[pc], code[pc + 1], code[pc + 2], code[pc + 3]);
+            pc += 4;
+        } else if (EXT && op == U_PHILLPUMP) {
+            conc_phillpump(Sb, lane, csum, code[pc], code[pc + 1], code[pc + 2], code[pc + 3]);
+            pc += 4;
+        } else if (EXT && op == U_BUFFER) {
+            conc_buffer(Sb, lane, csum, code[pc], code[pc + 1], code[pc + 2]);
+            pc += 3;
+        } else if (EXT && op == U_POOL) {
+            Sb[code[pc] * W + lane] += csum;
+            pc += 1;
+        } else if (EXT && op == U_DIFF) {
+            diff_solve(Sb, lane, code[pc], code[pc + 1], code[pc + 2], code[pc + 3], code[pc + 4], code[pc + 5],
+                       code[pc + 6], code[pc + 7], code[pc + 8], code[pc + 9], d.crank);
+            pc += 10;
         } else {  // U_END
             if (giv) d.giv[gi] = Im;
             const double tbyc = SGET(c);

@@ -45,6 +45,26 @@ enum UOp : int {
     U_SAMEGHK = 22,   // Gk = S[c++] * GHK slope
     U_GIV0 = 23,      // operands: Rm slot, islot of the compartment's givals base: leak current
     // U_TABCURR's "which" takes the same four values as U_IPOLG / U_TABGHK
+    U_ELMMPUMP = 24,  // operands: conc slot, codiag slot (-1: none); Kd = S[c], I scale = S[c+1],
+                      // diagonal scale = S[c+2]: electrogenic pump current and diagonal
+    // The compartment's concentration program (its part of concops[]) and
+    // its diffusion systems, after its channel ops.  csum is the program's
+    // running sum of the concentration it is on; codiag / coresult of a
+    // shell are slots (1 and 0 between steps); slot operands are absolute.
+    U_CZERO = 25,     // csum = 0
+    U_CSTORE = 26,    // operand: coresult slot; S = csum
+    U_FLUXCONC = 27,  // operands: first of 2 value slots, flux register (-1: none, 0)
+    U_CTAUPUMP = 28,  // operands: first of 2 value slots, codiag slot
+    U_TAUPUMP = 29,   // operands: first of 5 value slots, codiag slot (rate from Vm)
+    U_MMPUMP = 30,    // operands: first of 2 value slots, conc slot, codiag slot
+    U_HILLPUMP = 31,  // operands: first of 2 value slots, conc slot, codiag slot, Hill coefficient
+    U_PHILLPUMP = 32, // operands: first of 3 value slots, shell conc slot, its codiag slot, Hill
+    U_BUFFER = 33,    // operands: first of 3 value slots, buffer conc slot, pool conc slot
+    U_POOL = 34,      // operand: pool conc slot; S += csum
+    U_DIFF = 35,      // operands: shells, rows per shell, fixed buffers, first slots of: the
+                      // concentrations, covals0 values, buffer rates, codiag per shell,
+                      // coresult per shell, elimination scratch (shells*rows*rows), eliminated
+                      // right-hand sides (shells*rows)
 };
 constexpr int MAXPOOL = 4;
 constexpr int PROG_HEADER = 2;

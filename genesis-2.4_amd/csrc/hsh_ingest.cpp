@@ -63,35 +63,200 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
         return false;
     };
     // Concentrations.  Slot sources beyond chip[]: nchips + k is conc[k],
-    // nchips + nconcs + j is concchip[j].  Ca_concen pools (concops: CACONC_OP k
-    // ... LCACONC_OP, hines_conc_chip.c:196-211) each take 3 concchip values.
+    // nchips + nconcs + j is concchip[j], then givals, covals0 and bufvals.
+    // Ca_concen pools (CACONC_OP k, hines_conc_chip.c:196-211) each take 3
+    // concchip values and are updated at the end of the program that feeds
+    // them.  The rest of concops[] -- the shells' results, pumps, pool
+    // buffers and pools -- is cut into segments, one per run of ops on one
+    // concentration (CONC_OP / FCONC_OP k ...), each run by the compartment
+    // that owns k: the compartment of k's diffusion system (fdiffs), or for a
+    // concpool the owner of the shell its Hill pump drains.
     const int64_t nchips = m.nchips, nconcs = std::max(0, m.nconcs);
     std::vector<int64_t> pool_chip(nconcs, -1);     // k -> first concchip value, -1: not a Ca_concen pool
     std::vector<int> pool_owner(nconcs, -1);       // k -> compartment whose program uses it
-    {
-        int64_t q = 0, j = 0;
-        bool ok = true;
-        while (q < m.nconcops && ok) {
-            const int32_t co = m.concops[q++];
-            if (co == LCACONC_OP) break;
-            if (co == CACONC_OP && q < m.nconcops) {
-                const int32_t k = m.concops[q++];
-                if (k < 0 || k >= nconcs || j + 3 > m.nconcchip) { err = "malformed concops[]"; return false; }
-                pool_chip[k] = j;
-                j += 3;
-            } else {
-                // pools with shells, buffers or pumps need the conc solve
-                refuse(P, -1, co, (int)(q - 1), HSH_R_OPCODE);
-                ok = false;
+    std::vector<int> conc_owner(nconcs, -1);       // k -> compartment whose program updates it (shells, pools)
+    const int nsys = m.ndiffs > 0 ? m.ndiffs : 0;
+    std::vector<int> sys_owner(nsys, -1), conc_sys(nconcs, -1), conc_row(nconcs, -1);
+    std::vector<int64_t> sys_cov(nsys, 0), sys_ncov(nsys, 0);
+    std::vector<std::vector<int>> comp_sys(n);     // per compartment: the systems it owns
+    const int64_t gsrc0 = nchips + nconcs + std::max(0, m.nconcchip);
+    const bool giv_on = (m.givals != nullptr && m.ngivals > 0);
+    const int64_t csrc0 = gsrc0 + (giv_on ? m.ngivals : 0);    // covals0[0]
+    const int64_t bsrc0 = csrc0 + std::max(0, m.ncovals0);     // bufvals[0]
+    bool diff_bad = false;
+    if (nsys > 0) {
+        if (g) {
+            diff_bad = true;
+        } else {
+            for (int r = 0; r < n && !diff_bad; r++) {
+                const int32_t a = m.fdiffs[r], b = m.fdiffs[r + 1];
+                if (a < 0 || b < a || b > nsys) { diff_bad = true; break; }
+                for (int32_t i = a; i < b; i++) {
+                    if (sys_owner[i] >= 0) { diff_bad = true; break; }
+                    sys_owner[i] = r;
+                    comp_sys[r].push_back(i);
+                }
             }
         }
+        int64_t cur = 0;
+        for (int i = 0; i < nsys && !diff_bad; i++) {
+            const int ms = m.nshells[i], dw = m.nshrows[i], nd = m.nshfix[i];
+            const int64_t c0 = m.dconc[i];
+            if (sys_owner[i] < 0 || ms < 2 || dw < 1 || nd < 0 || nd > dw - 1 || (int64_t)ms * dw > 256 ||
+                c0 < 0 || c0 + (int64_t)ms * dw > nconcs ||
+                (dw > 1 && (m.fbufval[i] < 0 || m.fbufval[i] + 3LL * (dw - 1) > m.nbufvals))) {
+                diff_bad = true;
+                break;
+            }
+            int64_t cnt = 0;
+            for (int s = 0; s < ms; s++) {
+                const int km = (s == 0 || s == ms - 1) ? 1 : 2;
+                cnt += nd + (int64_t)(dw - 1 - nd) * km + km;
+            }
+            sys_cov[i] = cur;
+            sys_ncov[i] = cnt;
+            cur += cnt;
+            if (cur > m.ncovals0) { diff_bad = true; break; }
+            for (int r = 0; r < ms * dw; r++) {
+                if (conc_sys[c0 + r] >= 0) { diff_bad = true; break; }
+                conc_sys[c0 + r] = i;
+                conc_row[c0 + r] = r;
+                conc_owner[c0 + r] = sys_owner[i];
+            }
+        }
+        if (diff_bad) refuse(P, -1, 0, -1, HSH_R_DIFFUSION);
+    }
+    auto is_shell = [&](int64_t k) {
+        return conc_sys[k] >= 0 && conc_row[k] % m.nshrows[conc_sys[k]] == m.nshrows[conc_sys[k]] - 1;
+    };
+    struct COp {
+        int32_t code;
+        int32_t x, hill;      // operand concentration (shell, buffer) / Hill coefficient / Vm row
+        int64_t cc;           // first concchip value
+    };
+    struct CSeg {
+        int32_t at;           // the concentration the ops apply to
+        int owner;
+        bool store;           // its sum is stored as at's coresult
+        std::vector<COp> ops;
+    };
+    std::vector<CSeg> segs;
+    std::vector<std::vector<int>> comp_segs(n);
+    P.flux_live.assign(nconcs, 0);
+    {
+        int64_t q = 0, j = 0;
+        int cur = -1;
+        auto open_seg = [&](int32_t k) {
+            segs.push_back(CSeg{k, -1, false, {}});
+            cur = (int)segs.size() - 1;
+        };
+        auto need = [&](int64_t ops_n, int64_t chip_n) {
+            return q + ops_n <= m.nconcops && j + chip_n <= m.nconcchip;
+        };
+        auto conc_ok = [&](int64_t k) { return k >= 0 && k < nconcs; };
+        bool done = false;
+        while (q < m.nconcops && !done) {
+            const int32_t co = m.concops[q++];
+            switch (co) {
+            case CONC_OP:
+            case FCONC_OP: {
+                if (!need(1, 0) || !conc_ok(m.concops[q])) { err = "malformed concops[]"; return false; }
+                if (co == CONC_OP && cur >= 0) segs[cur].store = true;
+                open_seg(m.concops[q++]);
+                break;
+            }
+            case LCONC_OP:
+                if (cur >= 0) segs[cur].store = true;
+                done = true;
+                break;
+            case LCACONC_OP:
+                done = true;
+                break;
+            case CACONC_OP: {
+                if (!need(1, 3) || !conc_ok(m.concops[q])) { err = "malformed concops[]"; return false; }
+                pool_chip[m.concops[q++]] = j;
+                j += 3;
+                break;
+            }
+            case FLUXCONC_OP: case CTAUPUMP_OP: case TAUPUMP_OP: case MMPUMP_OP: case HILLPUMP_OP:
+            case PHILLPUMP_OP: case BUFFER_OP: case POOL_OP: {
+                const int nop = (co == TAUPUMP_OP || co == HILLPUMP_OP || co == BUFFER_OP) ? 1
+                              : (co == PHILLPUMP_OP ? 2 : 0);
+                const int nch = (co == TAUPUMP_OP) ? 5 : (co == PHILLPUMP_OP || co == BUFFER_OP) ? 3
+                              : (co == POOL_OP ? 0 : 2);
+                if (!need(nop, nch)) { err = "malformed concops[]"; return false; }
+                if (cur < 0) open_seg(0);          // the reference's index starts at 0
+                COp o{co, -1, 0, j};
+                if (co == TAUPUMP_OP) o.x = m.concops[q];
+                if (co == HILLPUMP_OP) o.hill = m.concops[q];
+                if (co == PHILLPUMP_OP) { o.hill = m.concops[q]; o.x = m.concops[q + 1]; }
+                if (co == BUFFER_OP) o.x = m.concops[q];
+                if ((co == PHILLPUMP_OP || co == BUFFER_OP) && !conc_ok(o.x)) {
+                    err = "malformed concops[]";
+                    return false;
+                }
+                q += nop;
+                j += nch;
+                segs[cur].ops.push_back(o);
+                break;
+            }
+            default:
+                refuse(P, -1, co, (int)(q - 1), HSH_R_OPCODE);
+                done = true;
+                break;
+            }
+        }
+        // owners: the diffusion system of the concentration, else (a concpool)
+        // the owner of the shell its Hill pump drains
+        for (size_t si = 0; si < segs.size() && !diff_bad; si++) {
+            CSeg& sg = segs[si];
+            if (sg.ops.empty() && (!sg.store || !is_shell(sg.at))) continue;
+            int own = conc_owner[sg.at];
+            if (own < 0)
+                for (const COp& o : sg.ops)
+                    if (o.code == PHILLPUMP_OP && conc_sys[o.x] >= 0) own = sys_owner[conc_sys[o.x]];
+            bool ok = own >= 0;
+            int32_t badop = sg.ops.empty() ? CONC_OP : sg.ops[0].code;
+            for (const COp& o : sg.ops) {
+                if (!ok) break;
+                if ((o.code == CTAUPUMP_OP || o.code == TAUPUMP_OP || o.code == MMPUMP_OP ||
+                     o.code == HILLPUMP_OP) && !is_shell(sg.at))
+                    ok = false;                                                  // pumps act on shells
+                if (o.code == TAUPUMP_OP && o.x != own) ok = false;              // Vm of another compartment
+                if (o.code == PHILLPUMP_OP && (!is_shell(o.x) || conc_owner[o.x] != own)) ok = false;
+                if (o.code == BUFFER_OP && (conc_sys[o.x] >= 0 || (conc_owner[o.x] >= 0 && conc_owner[o.x] != own)))
+                    ok = false;
+                if (!ok) badop = o.code;
+            }
+            if (ok && conc_sys[sg.at] < 0 && conc_owner[sg.at] >= 0 && conc_owner[sg.at] != own) ok = false;
+            if (!ok) {
+                refuse(P, own, badop, -1, HSH_R_DIFFUSION);
+                continue;
+            }
+            sg.owner = own;
+            // a concpool and its buffers are updated by this owner only
+            bool upd = false;
+            for (const COp& o : sg.ops) {
+                if (o.code == POOL_OP) upd = true;
+                if (o.code == BUFFER_OP) conc_owner[o.x] = own;
+            }
+            if (upd && conc_sys[sg.at] < 0) conc_owner[sg.at] = own;
+            comp_segs[own].push_back((int)si);
+        }
+    }
+    // shared synthetic initial values of the solve's per-step slots
+    int64_t synth_one = 0, synth_zero = 0;
+    if (nsys > 0 && !diff_bad) {
+        P.synth.push_back(1.0);
+        synth_one = -(int64_t)P.synth.size();
+        P.synth.push_back(0.0);
+        synth_zero = -(int64_t)P.synth.size();
     }
     int creg = -1;            // conc register (CONC_VAL_OP), carried across compartments
     // givals: written in op order from givals[1] (hines_chip.c:114); value-space
     // sources of its entries follow concchip[]
-    P.giv = (m.givals != nullptr && m.ngivals > 0);
+    P.giv = giv_on;
     P.chanmode5 = (m.chanmode == 5);
-    const int64_t gsrc0 = nchips + nconcs + std::max(0, m.nconcchip);
     int64_t gcur = 1;
     // member offsets of a group (all zero for one solver)
     int32_t ocol = 0, ostab = 0, oconc = 0;
@@ -150,6 +315,14 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
             fix.emplace_back(code.size(), extra_slot(src));
             code.push_back(-1);
         };
+        auto raw_slot = [&](int64_t src) {
+            extra.push_back(src);
+            return (int)extra.size() - 1;
+        };
+        auto push_fixed = [&](int idx) {
+            fix.emplace_back(code.size(), idx);
+            code.push_back(-1);
+        };
         // true when the conductance ops from ops index q set Ek from a table
         // (power-0 IPOL) before anything reads it
         auto ek_dead = [&](int64_t q) {
@@ -176,6 +349,47 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
             gcur += 2;
         }
         auto need_chip = [&](int k) { return c + k <= m.nchips; };
+        // The diffusion systems this compartment owns: their slots first, each
+        // group contiguous -- concentrations (state), covals0 values, buffer
+        // rates, then per step codiag and coresult per shell and the
+        // elimination scratch (values within the step only).
+        struct SysSlots {
+            int sys, conc, cov, buf, dg, cr, x, r;
+        };
+        std::vector<SysSlots> sys_slots;
+        int nscratch = 0;
+        if (!diff_bad && !g)
+            for (int i : comp_sys[comp]) {
+                const int ms = m.nshells[i], dw = m.nshrows[i], N = ms * dw;
+                SysSlots ss;
+                ss.sys = i;
+                ss.conc = (int)extra.size();
+                for (int r = 0; r < N; r++) raw_slot(nchips + m.dconc[i] + r);
+                ss.cov = (int)extra.size();
+                for (int64_t l = 0; l < sys_ncov[i]; l++) raw_slot(csrc0 + sys_cov[i] + l);
+                ss.buf = (int)extra.size();
+                if (dw > 1)
+                    for (int q = 0; q < 3 * (dw - 1); q++) raw_slot(bsrc0 + m.fbufval[i] + q);
+                ss.dg = (int)extra.size();
+                for (int q = 0; q < ms; q++) raw_slot(synth_one);
+                ss.cr = (int)extra.size();
+                for (int q = 0; q < ms; q++) raw_slot(synth_zero);
+                ss.x = (int)extra.size();
+                const int nx = dw == 1 ? ms : N * dw;
+                for (int q = 0; q < nx; q++) raw_slot(synth_zero);
+                ss.r = (int)extra.size();
+                for (int q = 0; q < N; q++) raw_slot(synth_zero);
+                nscratch += 2 * ms + nx + N;
+                ngates += N;
+                sys_slots.push_back(ss);
+            }
+        // the codiag / coresult slot of a shell of an owned system, -1
+        auto shell_slot = [&](int64_t k, bool diag) {
+            if (k < 0 || k >= nconcs || !is_shell(k) || conc_owner[k] != comp) return -1;
+            for (const SysSlots& ss : sys_slots)
+                if (ss.sys == conc_sys[k]) return (diag ? ss.dg : ss.cr) + conc_row[k] / m.nshrows[ss.sys];
+            return -1;
+        };
         while (p < m.nops && ops[p] > LCOMPT_OP) {
             int64_t opidx = p;
             int32_t op = ops[p++];
@@ -207,7 +421,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
                 creg = k;
                 creg_intra = true;
                 if (pool_owner[k] < 0) pool_owner[k] = comp;
-                if (pool_owner[k] != comp && pool_chip[k] >= 0) {
+                if ((pool_owner[k] != comp && pool_chip[k] >= 0) || (conc_owner[k] >= 0 && conc_owner[k] != comp)) {
                     // a pool several compartments feed: the device keeps a
                     // pool's flux in the program of the compartment that owns it
                     refuse(P, comp, op, (int)opidx, HSH_R_POOL);
@@ -242,7 +456,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
                 const int32_t k = ops[p++] + oconc;
                 if (k < 0 || k >= nconcs) return bad("NEWCONC2_OP pool index out of range");
                 if (m.ydivs <= 0) return bad("NEWCONC2_OP without a concentration table");
-                if (pool_chip[k] >= 0 && pool_owner[k] != comp) {
+                if ((pool_chip[k] >= 0 && pool_owner[k] != comp) || (conc_owner[k] >= 0 && conc_owner[k] != comp)) {
                     refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true;
                     break;
                 }
@@ -252,7 +466,25 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
                 crow_intra = true;
                 break;
             }
-            case ELMMPUMP_OP: c += 3; refuse(P, comp, op, (int)opidx, HSH_R_OPCODE); refused = true; break;
+            case ELMMPUMP_OP:
+                // electrogenic pump on the concentration of the last CONC_VAL_OP
+                // (hines_chip.c:278-302): a current, and a diagonal term for
+                // that shell's solve
+                if (!need_chip(3)) return bad("chip[] too short");
+                if (!creg_intra) {
+                    refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true;
+                } else if (conc_owner[creg] >= 0 && conc_owner[creg] != comp) {
+                    refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true;
+                } else {
+                    code.push_back(U_ELMMPUMP);
+                    push_extra(nchips + creg);
+                    const int dgs = shell_slot(creg, true);
+                    if (dgs >= 0) push_fixed(dgs); else code.push_back(-1);
+                    for (int i = 0; i < 3; i++) dsrc.push_back(c + i);
+                }
+                c += 3;
+                if (P.giv) gcur += 1;
+                break;
             case FLUX_OP:
             case SCALE_FLUX_OP:
                 if (!creg_intra) { refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true;
@@ -274,7 +506,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
                 if (creg_intra && creg + 1 >= nconcs) return bad("no outside concentration after the pool");
                 if (!creg_intra) {
                     refuse(P, comp, op, (int)opidx, HSH_R_REGISTER_LIVEIN); refused = true;
-                } else if (pool_chip[creg + 1] >= 0) {
+                } else if (pool_chip[creg + 1] >= 0 || (conc_owner[creg + 1] >= 0 && conc_owner[creg + 1] != comp)) {
                     // the outside concentration is a pool another program updates
                     refuse(P, comp, op, (int)opidx, HSH_R_POOL); refused = true;
                 } else {
@@ -470,6 +702,96 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
         // of its ops have read and fed them (the reference updates pools after
         // the whole chip update, hines.c:222-270; a pool owned by one
         // compartment sees no difference)
+        // its concentration program (hines_conc_chip.c) and diffusion solves
+        // (hines_conc.c), after its channel ops as the reference's passes come
+        // after the whole chip update: nothing another compartment reads
+        if (!refused) {
+            std::vector<int64_t> upd;           // pool / buffer concentrations it updates
+            auto state_conc = [&](int64_t k) {
+                const int sl = extra_slot(nchips + k);
+                if (conc_sys[k] < 0 && std::find(upd.begin(), upd.end(), k) == upd.end()) upd.push_back(k);
+                return sl;
+            };
+            for (int si : comp_segs[comp]) {
+                const CSeg& sg = segs[si];
+                code.push_back(U_CZERO);
+                const int dgx = shell_slot(sg.at, true);
+                auto vals = [&](int64_t j0, int cnt) {
+                    const int first = (int)extra.size();
+                    for (int q = 0; q < cnt; q++) raw_slot(nchips + nconcs + j0 + q);
+                    return first;
+                };
+                for (const COp& o : sg.ops) {
+                    switch (o.code) {
+                    case FLUXCONC_OP: {
+                        int pn = -1;
+                        for (size_t i = 0; i < pools.size(); i++)
+                            if (pools[i] == sg.at) pn = (int)i;
+                        code.push_back(U_FLUXCONC);
+                        push_fixed(vals(o.cc, 2));
+                        code.push_back(pn);
+                        if (pn < 0) P.flux_live[sg.at] = 1;
+                        break;
+                    }
+                    case CTAUPUMP_OP:
+                        code.push_back(U_CTAUPUMP);
+                        push_fixed(vals(o.cc, 2));
+                        push_fixed(dgx);
+                        break;
+                    case TAUPUMP_OP:
+                        code.push_back(U_TAUPUMP);
+                        push_fixed(vals(o.cc, 5));
+                        push_fixed(dgx);
+                        break;
+                    case MMPUMP_OP:
+                        code.push_back(U_MMPUMP);
+                        push_fixed(vals(o.cc, 2));
+                        push_extra(nchips + sg.at);
+                        push_fixed(dgx);
+                        break;
+                    case HILLPUMP_OP:
+                        code.push_back(U_HILLPUMP);
+                        push_fixed(vals(o.cc, 2));
+                        push_extra(nchips + sg.at);
+                        push_fixed(dgx);
+                        code.push_back(o.hill);
+                        break;
+                    case PHILLPUMP_OP:
+                        code.push_back(U_PHILLPUMP);
+                        push_fixed(vals(o.cc, 3));
+                        push_extra(nchips + o.x);
+                        push_fixed(shell_slot(o.x, true));
+                        code.push_back(o.hill);
+                        break;
+                    case BUFFER_OP:
+                        code.push_back(U_BUFFER);
+                        push_fixed(vals(o.cc, 3));
+                        push_fixed(state_conc(o.x));
+                        push_extra(nchips + sg.at);
+                        break;
+                    case POOL_OP:
+                        code.push_back(U_POOL);
+                        push_fixed(state_conc(sg.at));
+                        break;
+                    default:
+                        break;
+                    }
+                }
+                const int crs = shell_slot(sg.at, false);
+                if (sg.store && crs >= 0) {
+                    code.push_back(U_CSTORE);
+                    push_fixed(crs);
+                }
+            }
+            for (const SysSlots& ss : sys_slots) {
+                code.push_back(U_DIFF);
+                code.push_back(m.nshells[ss.sys]);
+                code.push_back(m.nshrows[ss.sys]);
+                code.push_back(m.nshfix[ss.sys]);
+                for (int b : {ss.conc, ss.cov, ss.buf, ss.dg, ss.cr, ss.x, ss.r}) push_fixed(b);
+            }
+            ngates += (int)upd.size();
+        }
         for (size_t pn = 0; pn < pools.size() && !refused; pn++) {
             const int k = pools[pn];
             if (pool_chip[k] < 0) continue;
@@ -498,6 +820,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
         t.nslots = (int)dsrc.size();
         t.nislots = (int)isrc.size();
         t.ngates = ngates;
+        t.nscratch = nscratch;
         std::vector<int32_t> key = code;
         key.push_back(t.nslots);
         key.push_back(t.nislots);
@@ -751,7 +1074,7 @@ bool analyse(const hsh_model& m, Plan& P, std::string& err, const GroupSpec* g)
     double bytes = 0;
     for (int r = 0; r < n; r++) {
         const Template& t = P.templates[P.comp_tmpl[r]];
-        bytes += 16.0 + 16.0 * t.ngates + 8.0 * (t.nslots - t.ngates) + 8.0 * t.nislots;
+        bytes += 16.0 + 16.0 * t.ngates + 8.0 * (t.nslots - t.ngates - t.nscratch) + 8.0 * t.nislots;
         if (P.parent[r] >= 0) bytes += 28.0;
     }
     P.bytes_per_step = bytes;

@@ -20,7 +20,9 @@ enum RefOp : int32_t {
     SCALE_FLUX_OP = 3301, STORE_ALLI_OP = 3501, STORE_ALLC_OP = 3502, IPOL1V_OP = 4001,
     IPOL1C_OP = 4002, IPOL2V_OP = 4003, IPOL2C_OP = 4004, SYN2_OP = 4100, SYN3_OP = 4101,
     MGBLOCK_OP = 4200, NEWVOLT_OP = 5100, NEWCONC1_OP = 5110, NEWCONC2_OP = 5111,
-    DOMAIN_OP = 5112, CACONC_OP = 10200, LCACONC_OP = 10202,
+    DOMAIN_OP = 5112, CONC_OP = 10100, FCONC_OP = 10101, LCONC_OP = 10102, FLUXCONC_OP = 10103,
+    CACONC_OP = 10200, LCACONC_OP = 10202, TAUPUMP_OP = 11000, CTAUPUMP_OP = 11001, MMPUMP_OP = 11010,
+    HILLPUMP_OP = 11011, PHILLPUMP_OP = 11012, BUFFER_OP = 12000, POOL_OP = 12100,
 };
 // Reference funcs codes (hines_defs.h:119-128).
 enum RefFunc : int32_t {
@@ -33,6 +35,8 @@ struct Template {
     int nslots = 0;
     int nislots = 0;
     int ngates = 0;
+    int nscratch = 0;     // slots that only carry values within a step (the
+                          // diffusion solve's diagonals, results and elimination)
 };
 
 // Layout choices (the autotuner sweeps them).
@@ -72,6 +76,10 @@ struct Plan {
     std::vector<int32_t> tprog;           // template -> offset in prog
     int max_tree_rows = 0;
     double bytes_per_step = 0;
+    // per concentration: a FLUXCONC_OP reads its flux with no CONC_VAL_OP of
+    // the owner's program clearing it first (the device takes that flux as
+    // its initial 0; a FLUX message input would change it)
+    std::vector<char> flux_live;
     // layout (build_layout)
     LayoutOpts layout;
     std::vector<BlockMeta> blocks;

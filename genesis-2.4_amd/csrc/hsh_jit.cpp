@@ -45,7 +45,7 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
         o += "    const double " + slot(c) + " = Sb[" + std::to_string(c) + " * W + lane];\n";
     o += "    double sumgchan = 0.0;\n    double ichan = s0 + s1;\n";
     o += "    double Gk = 0.0, Ek = 0.0, Ik = 0.0, vipol = 0.0, cipol = 0.0;\n";
-    o += "    double flux[MAXPOOL] = {0.0, 0.0, 0.0, 0.0};\n";
+    o += "    double flux[MAXPOOL] = {0.0, 0.0, 0.0, 0.0};\n    double csum = 0.0; (void)csum;\n";
     o += "    const double* vtab = d.tab;\n    const double* ctab = d.tab;\n    const double* tabr = d.tab;\n";
     o += "    int cfilo = 0;\n    double ipol = 0.0, ghk = 0.0;\n";
     o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)Ik; (void)vipol; (void)cipol;\n";
@@ -206,6 +206,70 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
             pc += 3;
             o += "    Sb[" + std::to_string(cs) + " * W + lane] = caconc(" + slot(cs) + ", " + slot(ks) + ", " +
                  slot(ks + 1) + ", flux[" + std::to_string(p) + "], " + slot(ks + 2) + ");\n";
+            break;
+        }
+        case U_ELMMPUMP: {
+            const int cs = code[pc], dg = code[pc + 1];
+            pc += 2;
+            const std::string C = "Sb[" + std::to_string(cs) + " * W + lane]";
+            o += "    { const double X = 1.0 / (" + C + " + " + slot(c) + "); Ik = " + slot(c + 1) + " * " + C +
+                 " * X; ichan += Ik;";
+            if (giv) o += " if (giv) { d.giv[gi] = d.chanmode5 ? Ik / " + slot(c + 1) + " : Ik; gi += 1; } Im += Ik;";
+            if (dg >= 0) o += " Sb[" + std::to_string(dg) + " * W + lane] += " + slot(c + 2) + " * X;";
+            o += " }\n";
+            c += 3;
+            break;
+        }
+        case U_CZERO:
+            o += "    csum = 0.0;\n";
+            break;
+        case U_CSTORE:
+            o += "    Sb[" + std::to_string(code[pc++]) + " * W + lane] = csum;\n";
+            break;
+        case U_FLUXCONC: {
+            const int a = code[pc], p = code[pc + 1];
+            pc += 2;
+            o += "    conc_fluxconc(Sb, lane, csum, " + std::to_string(a) + ", " +
+                 (p >= 0 ? "flux[" + std::to_string(p) + "]" : std::string("0.0")) + ");\n";
+            break;
+        }
+        case U_CTAUPUMP:
+            o += "    conc_ctaupump(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) +
+                 ");\n";
+            pc += 2;
+            break;
+        case U_TAUPUMP:
+            o += "    conc_taupump(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) +
+                 ", d.dt, Vm);\n";
+            pc += 2;
+            break;
+        case U_MMPUMP:
+            o += "    conc_mmpump(Sb, lane, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) + ", " +
+                 std::to_string(code[pc + 2]) + ");\n";
+            pc += 3;
+            break;
+        case U_HILLPUMP:
+        case U_PHILLPUMP: {
+            const std::string args = std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) + ", " +
+                                     std::to_string(code[pc + 2]) + ", " + std::to_string(code[pc + 3]);
+            pc += 4;
+            o += op == U_HILLPUMP ? "    conc_hillpump(Sb, lane, " + args + ");\n"
+                                  : "    conc_phillpump(Sb, lane, csum, " + args + ");\n";
+            break;
+        }
+        case U_BUFFER:
+            o += "    conc_buffer(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) +
+                 ", " + std::to_string(code[pc + 2]) + ");\n";
+            pc += 3;
+            break;
+        case U_POOL:
+            o += "    Sb[" + std::to_string(code[pc++]) + " * W + lane] += csum;\n";
+            break;
+        case U_DIFF: {
+            o += "    diff_solve(Sb, lane";
+            for (int i = 0; i < 10; i++) o += ", " + std::to_string(code[pc + i]);
+            o += ", d.crank);\n";
+            pc += 10;
             break;
         }
         default:   // U_END

@@ -42,7 +42,7 @@ F_NO_JIT = 4
 
 REASONS = {1: "opcode", 2: "Ek live-in", 3: "table live-in", 4: "register live-in",
            5: "funcs (elimination program the device cannot run)", 6: "tree layout", 7: "calcmode", 8: "pending synaptic event",
-           9: "shared pool"}
+           9: "shared pool", 10: "diffusion system or concentration op the device does not run"}
 
 
 class HshModel(C.Structure):
@@ -58,6 +58,9 @@ class HshModel(C.Structure):
         ("ymin", C.c_double), ("invdy", C.c_double),
         ("conc", _f64p), ("concops", _i32p), ("concchip", _f64p),
         ("ngivals", C.c_int32), ("givals", _f64p),
+        ("ndiffs", C.c_int32), ("ncovals0", C.c_int32), ("nbufvals", C.c_int32), ("pad_diff", C.c_int32),
+        ("nshells", _i32p), ("nshrows", _i32p), ("nshfix", _i32p), ("dconc", _i32p), ("fbufval", _i32p),
+        ("fdiffs", _i32p), ("covals0", _f64p), ("bufvals", _f64p),
     ]
 
 
@@ -219,6 +222,14 @@ def _hsh_model(model, givals: bool = False):
     if givals and np.size(model.givals):
         k["givals"] = np.ascontiguousarray(model.givals, np.float64)
         mm.ngivals, mm.givals = k["givals"].size, _p(k["givals"], C.c_double)
+    if np.size(getattr(model, "nshells", ())):
+        for f in ("nshells", "nshrows", "nshfix", "dconc", "fbufval", "fdiffs"):
+            k[f] = np.ascontiguousarray(getattr(model, f), np.int32)
+            setattr(mm, f, _p(k[f], C.c_int32))
+        for f in ("covals0", "bufvals"):
+            k[f] = np.ascontiguousarray(getattr(model, f), np.float64)
+            setattr(mm, f, _p(k[f], C.c_double) if k[f].size else None)
+        mm.ndiffs, mm.ncovals0, mm.nbufvals = k["nshells"].size, k["covals0"].size, k["bufvals"].size
     return mm, k
 
 

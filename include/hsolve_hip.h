@@ -91,14 +91,19 @@ typedef struct hsh_model {
     int32_t ydivs;           /* conc table rows are ydivs+1; a row is found from
                                 ymin/invdy (hines_chip.c:238-247), so yvals[] is
                                 not needed */
-    int32_t nconcops;        /* concops[] length including LCACONC_OP */
+    int32_t nconcops;        /* concops[] length including its LCONC_OP / LCACONC_OP */
     int32_t nconcchip;
     double  ymin;
     double  invdy;
-    const double  *conc;     /* [nconcs] pool concentrations (Cin, Cout pairs) */
-    const int32_t *concops;  /* [nconcops]: (CACONC_OP k)* LCACONC_OP; other pool
-                                ops (shells, buffers, pumps) are refused */
-    const double  *concchip; /* [nconcchip]: Ca_base, 1 + dt/(2 tau), dt B per pool */
+    const double  *conc;     /* [nconcs] concentrations: pools, shells, buffers */
+    const int32_t *concops;  /* [nconcops]: the concentration program of
+                                h_init_conc_chip (hines_conc_init.c:413-805) --
+                                CONC_OP / FCONC_OP k, FLUXCONC_OP, TAUPUMP_OP row,
+                                CTAUPUMP_OP, MMPUMP_OP, HILLPUMP_OP hill,
+                                PHILLPUMP_OP hill shell, BUFFER_OP buffer, POOL_OP,
+                                CACONC_OP k -- ended by LCONC_OP or LCACONC_OP */
+    const double  *concchip; /* [nconcchip]: the values those ops consume, in order
+                                (hines_data.c:162-400) */
     /* chanmode-4/5 output stream (hines_chip.c:164-165, :536-564, :598): when
      * given, the device writes it every step -- per compartment the leak
      * current, {Gk, Ik, Ek} of every current (chanmode 5: Gk and Ik per gmax)
@@ -106,6 +111,28 @@ typedef struct hsh_model {
      * taken from here.  NULL / 0: not produced (no cost). */
     int32_t ngivals;
     const double  *givals;   /* [ngivals] */
+    /* Diffusion systems of shells and buffers, the solve of do_h_conc_solve
+     * (hines_conc.c:39-396; arrays of h_init_conc_solve / h_setup_conc_solve,
+     * hines_conc_init.c:59-411).  System i: nshells[i] shells (at least two)
+     * of nshrows[i] rows -- its nshfix[i] fixed buffers, its diffusible
+     * buffers, the shell last -- with row r of shell m at conc[dconc[i] +
+     * m*nshrows[i] + r] (shell 0 innermost), buffer rates at
+     * bufvals[fbufval[i]] (three per buffer).  covals0 holds every system's
+     * couplings and fixed-buffer totals in the order the solve consumes them.
+     * fdiffs[ncompts+1] (hines_struct.h fdiffs): the systems of compartment
+     * (Hines row) r are fdiffs[r] .. fdiffs[r+1]-1.  ndiffs = 0: none. */
+    int32_t ndiffs;
+    int32_t ncovals0;
+    int32_t nbufvals;
+    int32_t pad_diff;
+    const int32_t *nshells;  /* [ndiffs] */
+    const int32_t *nshrows;  /* [ndiffs] */
+    const int32_t *nshfix;   /* [ndiffs] */
+    const int32_t *dconc;    /* [ndiffs] */
+    const int32_t *fbufval;  /* [ndiffs] */
+    const int32_t *fdiffs;   /* [ncompts+1] */
+    const double  *covals0;  /* [ncovals0] */
+    const double  *bufvals;  /* [nbufvals] */
 } hsh_model;
 
 typedef struct hsh_options {
@@ -148,6 +175,13 @@ typedef struct hsh_options {
 #define HSH_R_POOL           9   /* concentration pool fed or read by several compartments,
                                     updated with no compartment feeding it, or more
                                     than 4 pools in one compartment */
+#define HSH_R_DIFFUSION     10   /* a diffusion system or concentration op the device does
+                                    not run: fewer than two shells, arrays out of range,
+                                    a concentration program whose owner compartment
+                                    cannot be told (a concpool not fed by a Hill pump
+                                    from a shell), a TAUPUMP_OP reading another
+                                    compartment's Vm (a group with such solvers is
+                                    HSH_ERR_ARG at hsh_create_group) */
 
 typedef struct hsh_refusal {
     int32_t comp;            /* row (Hines number), -1 for whole-solver reasons */
@@ -162,7 +196,12 @@ typedef struct hsh_refusal {
 #define HSH_BATCHED  1       /* persistent fused kernel, K steps per launch, replayed
                                 from a captured hipGraph */
 #define HSH_CHANNELS 2       /* channel interpreter only: results[] on the device,
-                                vm untouched (reference return-0 protocol) */
+                                vm untouched (reference return-0 protocol).  A
+                                compartment's concentration updates (Ca_concen
+                                pools, its concentration ops and diffusion systems)
+                                run inside its program, so they advance here too:
+                                a caller keeping the CPU elimination skips its own
+                                do_h4_conc_chip_update / do_h_conc_solve */
 
 typedef struct hsh_step_out {
     int32_t vm_ready;        /* 1: vm[] is final for these steps (skip the CPU
@@ -257,7 +296,9 @@ hsh_status hsh_get_givals(struct hsh_handle *h, double *givals);
  *   HSH_IN_FLUX: flux[index] = value -- accepted and without effect: in
  *     chanmodes 4 and 5 CONC_VAL_OP clears the pool's flux at the start of
  *     every program that feeds it (hines_chip.c:205-212), and the device
- *     runs only pools such a program feeds.
+ *     runs only pools such a program feeds.  A FLUX input into a shell
+ *     whose FLUXCONC_OP reads a flux no CONC_VAL_OP clears is HSH_ERR_ARG
+ *     (the device takes that flux as its initial 0).
  * Indices are those of the handle's arrays (a group's merged arrays). */
 #define HSH_IN_INJECT     1     /* INJECT_OP (hines_defs.h:225) */
 #define HSH_IN_BASE       2     /* BASE_OP */

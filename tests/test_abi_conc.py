@@ -84,10 +84,12 @@ def test_refuses_updated_pool_nobody_feeds():
     assert [(r[0], r[1], r[3]) for r in refusals(m)] == [(-1, CACONC, "shared pool")]
 
 
-def test_refuses_other_pool_ops():
+def test_refuses_undefined_concentration_op():
+    """An opcode do_h4_conc_chip_update does not define (it errors out,
+    hines_conc_chip.c:215-219) refuses the solver with that opcode."""
     m = spliced(passive(1), lambda r: ([CHAN_EK, ADD_CURR, CONC_VAL, 0, FLUX], [0.1, 1e-9]),
-                conc=[1e-4], concops=[CONC_OP, 0, LCACONC], concchip=[5e-5, 1.01, 1e6])
-    assert [(r[0], r[1], r[3]) for r in refusals(m)] == [(-1, CONC_OP, "opcode")]
+                conc=[1e-4], concops=[CONC_OP, 0, 10150, LCACONC], concchip=[5e-5, 1.01, 1e6])
+    assert [(r[0], r[1], r[3]) for r in refusals(m)] == [(-1, 10150, "opcode")]
 
 
 def test_refuses_ghk_on_updated_outside_pool():

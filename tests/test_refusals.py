@@ -68,17 +68,19 @@ def test_refuses_syn3():
     assert [(r[1], r[2], r[3]) for r in refs] == [(SYN3, op, "opcode")]
 
 
-@pytest.mark.parametrize("prog", [([ELMMPUMP], [1e-3, 2e-3, 3e-3]), ([STORE_ALLI, 0], []), ([STORE_ALLC, 0], [])],
+@pytest.mark.parametrize("prog,why", [(([ELMMPUMP], [1e-3, 2e-3, 3e-3]), "register live-in"),
+                                      (([STORE_ALLI, 0], []), "opcode"), (([STORE_ALLC, 0], []), "opcode")],
                          ids=["ELMMPUMP", "STORE_ALLI", "STORE_ALLC"])
-def test_refuses_pump_and_storeall_ops(prog):
-    """ELMMPUMP_OP (hines_chip.c:278-302) and the STORE_ALLI/STORE_ALLC
-    storemode ops (:314-322) are refused with their opcode in the
-    compartment that holds them."""
+def test_refuses_pump_and_storeall_ops(prog, why):
+    """The STORE_ALLI/STORE_ALLC storemode ops (hines_chip.c:314-322) are
+    refused with their opcode in the compartment that holds them; an
+    ELMMPUMP_OP (:278-302) with no CONC_VAL_OP before it in its compartment
+    would read the concentration register of an earlier one."""
     m = spliced(passive(3), lambda r: prog if r == 1 else ([], []))
     refs = refusals(m)
     assert len(refs) == 1
-    comp, opcode, opidx, why = refs[0]
-    assert (opcode, why) == (prog[0][0], "opcode") and comp == 1
+    comp, opcode, opidx, got = refs[0]
+    assert (opcode, got) == (prog[0][0], why) and comp == 1
     assert int(m.ops[opidx]) == prog[0][0]
 
 
