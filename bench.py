@@ -74,6 +74,10 @@ def parse():
                    help="skip the calcium section (Ca pools, Ca-gated K, GHK, table currents)")
     p.add_argument("--ca-neurons", type=int, default=50000)
     p.add_argument("--ca-steps", type=int, default=2000)
+    p.add_argument("--no-shells", action="store_true",
+                   help="skip the shells section (diffusion shells, buffers, pumps, concpools)")
+    p.add_argument("--shell-neurons", type=int, default=50000)
+    p.add_argument("--shell-steps", type=int, default=2000)
     p.add_argument("--no-percall", action="store_true",
                    help="skip the one-step-per-call leg (keeps a profile of the headline to K-step launches)")
     p.add_argument("--no-adjoint", action="store_true",
@@ -84,7 +88,7 @@ def parse():
                    help="skip the isopotential section (1000 single-compartment HH cells, 200 steps)")
     p.add_argument("--iso-cells", type=int, default=1000)
     p.add_argument("--iso-steps", type=int, default=200)
-    p.add_argument("--only", choices=("isopotential", "heterogeneous", "calcium", "network", "adjoint"),
+    p.add_argument("--only", choices=("isopotential", "heterogeneous", "calcium", "shells", "network", "adjoint"),
                    default=None,
                    help="run one section alone and print its JSON (for profiling it)")
     p.add_argument("--traffic-probe", action="store_true",
@@ -428,6 +432,78 @@ def calcium_section(a, D, M, Oracle):
     return res
 
 
+SHELL_FEATURES = ("fixbuf", "difbuf", "ctaupump", "mmpump", "hillpump", "elmmpump", "pool", "kc", "leak")
+
+
+def shells_section(a, D, M, Oracle):
+    """The 16-compartment population with a diffusion system of 4 Ca shells
+    in every compartment (a fixed and a diffusible buffer per shell,
+    Crank-Nicolson), fed by a Ca channel, drained by constant-rate,
+    Michaelis-Menten and Hill pumps and an electrogenic pump, with a concpool
+    and its buffer filled by a Hill pump from the outermost shell and a
+    Ca-gated K channel on that shell.  Each compartment's concentration
+    program and solve run at the end of its own program.  Batched
+    (autotuned) and plain throughput; bit-exact parity against the oracle on
+    a bounded number of steps."""
+    t0 = time.perf_counter()
+    m = M.shell_population(a.shell_neurons, (16,), (1.0,), shells=4, features=SHELL_FEATURES, seed=a.seed,
+                           method=M.CRANK)
+    t_build = time.perf_counter() - t0
+    n = m.ncompts
+    print(f"[bench] shells: {a.shell_neurons} neurons, {n} compartments, {m.ndiffs} diffusion systems "
+          f"(built {t_build:.1f}s)", file=sys.stderr, flush=True)
+    res = {"workload": f"{a.shell_neurons} random Hines trees x 16 compartments, HH soma, Kdr dendrites, "
+                       "Crank-Nicolson; per compartment 4 Ca shells each with a fixed and a diffusible buffer, "
+                       "constant-rate / Michaelis-Menten / Hill / electrogenic pumps, a concpool with its "
+                       "buffer, a Ca channel and a Ca-gated K channel",
+           "compartments": n, "diffusion_systems": int(m.ndiffs), "steps": a.shell_steps,
+           "concentrations": int(m.conc.size)}
+    with D.HipSolver(m) as g:
+        g.step(64, D.BATCHED)          # the autotuner runs here
+        g.sync()
+        info = g.info()
+        g.set_timing(True)
+        t0 = time.perf_counter()
+        g.step(a.shell_steps, D.BATCHED)
+        g.sync()
+        wall = time.perf_counter() - t0
+        dev_ms = g.last_time(D.T_TOTAL)
+        g.set_timing(False)
+        bps = info["bytes_per_step"]
+        res["batched"] = {"value": a.shell_steps * n / wall, "unit": "compartment-steps/s",
+                          "ms_per_step": wall * 1e3 / a.shell_steps,
+                          "device_us_per_step": dev_ms * 1e3 / a.shell_steps, "batch_k": info["batch_k"],
+                          "tile_max_rows": info["tile_max_rows"], "kernel": batched_kernel_name(info),
+                          "bytes_per_compartment_step": bps / n,
+                          "roofline": {"bound": "hbm", "achieved": bps * a.shell_steps / (dev_ms * 1e-3) / 1e9,
+                                       "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                       "frac": bps * a.shell_steps / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}}
+    with D.HipSolver(m, flags=D.F_NO_AUTOTUNE) as g:
+        g.step(16, D.PLAIN)
+        g.sync()
+        t0 = time.perf_counter()
+        g.step(200, D.PLAIN)
+        g.sync()
+        wall = time.perf_counter() - t0
+        res["plain"] = {"value": 200 * n / wall, "unit": "compartment-steps/s", "ms_per_step": wall * 1e3 / 200}
+    if not a.no_cpu:
+        S = 40
+        o = Oracle(m, givals=False)
+        t0 = time.perf_counter()
+        o.step(S)
+        cpu = time.perf_counter() - t0
+        with D.HipSolver(m, batch_k=8, flags=D.F_NO_AUTOTUNE) as g:
+            g.step(S, D.BATCHED)
+            v, c = g.vm, g.conc()
+        ref, cref = o.vm, o.conc()
+        res["parity"] = {"steps": S, "max_abs_dv_V": float(np.max(np.abs(v - ref))),
+                         "max_rel_dconc": float(np.max(np.abs(c - cref) / np.abs(cref))),
+                         "bitexact": bool(np.array_equal(v, ref) and np.array_equal(c, cref))}
+        res["cpu_baseline"] = {"value": S * n / cpu, "unit": "compartment-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{S} steps of the shells population ({cpu:.1f} s)"}
+    return res
+
+
 def network_section(a, D, M, Oracle):
     """Event-driven network: isopotential HH cells, 80% excitatory, 2%
     random connectivity, dual-exponential synchans (exc tau 5 ms, Erev 0;
@@ -708,6 +784,8 @@ def main():
             r = hetero_section(a, D, M, _O)
         elif a.only == "calcium":
             r = calcium_section(a, D, M, _O)
+        elif a.only == "shells":
+            r = shells_section(a, D, M, _O)
         elif a.only == "network":
             r = network_section(a, D, M, _O)
         else:
@@ -908,6 +986,9 @@ def main():
     if not a.no_calcium:
         from oracle import Oracle as _O
         out["calcium"] = calcium_section(a, D, M, _O)
+    if not a.no_shells:
+        from oracle import Oracle as _O
+        out["shells"] = shells_section(a, D, M, _O)
     if not a.no_network:
         from oracle import Oracle as _O
         out["network"] = network_section(a, D, M, _O)

@@ -373,27 +373,32 @@ __device__ __forceinline__ double hill_pow(int hill, double X)
 
 // One diffusion system.  Slots: concentrations cs0 + r (row r = shell*dw +
 // row in shell, shell 0 innermost), covals0 values cv0 + l in the order they
-// are used, buffer rates bv0 + 3*buffer, codiag dg0 + shell, coresult
-// cr0 + shell, scratch x0 (the eliminated upper entries, dw per row) and r0
-// (the eliminated right-hand sides).  Shells alone (dw 1): tridiagonal,
-// Crank-Nicolson as C <- 2x - C; with buffers: the banded Crank-Nicolson
-// system, whose diffusible buffer rows keep the reference's second value of
-// their upper entry (hines_conc.c:250).  The diagonals used are set back to 1.
-__device__ inline void diff_solve(double* __restrict__ Sb, int lane, int ms, int dw, int nd, int cs0, int cv0,
-                                  int bv0, int dg0, int cr0, int x0, int r0, int crank)
+// are used, buffer rates bv0 + 3*buffer.  Per-step arrays, TS apart (W: slots
+// in memory, the interpreter; 1: a thread's own arrays, generated code): the
+// shells' codiag and coresult, the eliminated upper entries (dw per row) and
+// right-hand sides.  Shells alone (dw 1): tridiagonal, Crank-Nicolson as
+// C <- 2x - C; with buffers: the banded Crank-Nicolson system, whose
+// diffusible buffer rows keep the reference's second value of their upper
+// entry (hines_conc.c:250).  The diagonals used are set back to 1.
+template <int TS>
+__device__ __forceinline__ void diff_solve_t(double* __restrict__ Sb, int lane, int ms, int dw, int nd, int cs0, int cv0,
+                                    int bv0, double* __restrict__ dgp, double* __restrict__ crp,
+                                    double* __restrict__ xp, double* __restrict__ rp, int crank)
 {
 #define DS(c) Sb[(c) * W + lane]
 #define CN(r) DS(cs0 + (r))
-#define CV(i) DS(x0 + (i))
-#define RS(i) DS(r0 + (i))
+#define CV(i) xp[(i) * TS]
+#define RS(i) rp[(i) * TS]
 #define K0(i) DS(cv0 + (i))
 #define BV(i) DS(bv0 + (i))
+#define DG(m) dgp[(m) * TS]
+#define CR(m) crp[(m) * TS]
     int l = 0;
     if (dw == 1) {
         double up = K0(l++);
-        double dg = up + DS(dg0);
-        DS(dg0) = 1.0;
-        RS(0) = (CN(0) + DS(cr0)) / dg;
+        double dg = up + DG(0);
+        DG(0) = 1.0;
+        RS(0) = (CN(0) + CR(0)) / dg;
         int j;
         for (j = 1; j < ms - 1; j++) {
             const double e = -up / dg;
@@ -401,16 +406,16 @@ __device__ inline void diff_solve(double* __restrict__ Sb, int lane, int ms, int
             const double dn = K0(l);
             up = K0(l + 1);
             l += 2;
-            dg = DS(dg0 + j) + dn * (1.0 + e) + up;
-            DS(dg0 + j) = 1.0;
-            RS(j) = (CN(j) + DS(cr0 + j) + dn * RS(j - 1)) / dg;
+            dg = DG(j) + dn * (1.0 + e) + up;
+            DG(j) = 1.0;
+            RS(j) = (CN(j) + CR(j) + dn * RS(j - 1)) / dg;
         }
         const double e = -up / dg;
         CV(j) = e;
         const double dn = K0(l++);
-        dg = DS(dg0 + j) + dn * (1.0 + e);
-        DS(dg0 + j) = 1.0;
-        double x = (CN(j) + DS(cr0 + j) + dn * RS(j - 1)) / dg;
+        dg = DG(j) + dn * (1.0 + e);
+        DG(j) = 1.0;
+        double x = (CN(j) + CR(j) + dn * RS(j - 1)) / dg;
         if (crank) {
             CN(j) = x + x - CN(j);
             for (int r = j - 1; r >= 0; r--) {
@@ -431,9 +436,9 @@ __device__ inline void diff_solve(double* __restrict__ Sb, int lane, int ms, int
     for (int m = 0; m < ms; m++) {
         const int rS = m * dw + dw - 1, xS = rS * dw, xP = xS - sw, rP = rS - dw;
         const bool last = (m == ms - 1);
-        double dgc = DS(dg0 + m);
-        DS(dg0 + m) = 1.0;
-        double rsc = DS(cr0 + m);
+        double dgc = DG(m);
+        DG(m) = 1.0;
+        double rsc = CR(m);
         for (int n = 0; n < nd; n++) {
             const int r = m * dw + n, nb = 3 * n;
             double dg = BV(nb) * co + BV(nb + 1);
@@ -545,44 +550,47 @@ __device__ inline void diff_solve(double* __restrict__ Sb, int lane, int ms, int
 #undef RS
 #undef K0
 #undef BV
+#undef DG
+#undef CR
 }
 
-// The concentration ops of hines_conc_chip.c on slots (a: first value slot)
-__device__ __forceinline__ void conc_fluxconc(double* __restrict__ Sb, int lane, double& csum, int a, double flux)
+// The concentration ops of hines_conc_chip.c (a: first value slot; dg: the
+// shell's codiag, wherever it is kept)
+__device__ __forceinline__ void conc_fluxconc(const double* __restrict__ Sb, int lane, double& csum, int a,
+                                              double flux)
 {
     csum += Sb[a * W + lane] + flux * Sb[(a + 1) * W + lane];
 }
-__device__ __forceinline__ void conc_ctaupump(double* __restrict__ Sb, int lane, double& csum, int a, int dg)
+__device__ __forceinline__ void conc_ctaupump(const double* __restrict__ Sb, int lane, double& csum, int a, double& dg)
 {
     csum += Sb[a * W + lane];
-    Sb[dg * W + lane] += Sb[(a + 1) * W + lane];
+    dg += Sb[(a + 1) * W + lane];
 }
-__device__ __forceinline__ void conc_taupump(double* __restrict__ Sb, int lane, double& csum, int a, int dg,
+__device__ __forceinline__ void conc_taupump(const double* __restrict__ Sb, int lane, double& csum, int a, double& dg,
                                              double dt, double Vm)
 {
     const double itau = dt / (Sb[a * W + lane] + Sb[(a + 1) * W + lane] *
                                                       exp((Vm - Sb[(a + 2) * W + lane]) * Sb[(a + 3) * W + lane]));
     csum += Sb[(a + 4) * W + lane] * itau;
-    Sb[dg * W + lane] += itau;
+    dg += itau;
 }
-__device__ __forceinline__ void conc_mmpump(double* __restrict__ Sb, int lane, int a, int cs, int dg)
+__device__ __forceinline__ void conc_mmpump(const double* __restrict__ Sb, int lane, int a, double C, double& dg)
 {
-    Sb[dg * W + lane] += Sb[a * W + lane] / (Sb[cs * W + lane] + Sb[(a + 1) * W + lane]);
+    dg += Sb[a * W + lane] / (C + Sb[(a + 1) * W + lane]);
 }
-__device__ __forceinline__ void conc_hillpump(double* __restrict__ Sb, int lane, int a, int cs, int dg, int hill)
+__device__ __forceinline__ void conc_hillpump(const double* __restrict__ Sb, int lane, int a, double X, double& dg,
+                                              int hill)
 {
-    const double X = Sb[cs * W + lane];
     double c = hill_pow(hill, X);
     c = c / (c * X + Sb[a * W + lane]);
-    Sb[dg * W + lane] += Sb[(a + 1) * W + lane] * c;
+    dg += Sb[(a + 1) * W + lane] * c;
 }
-__device__ __forceinline__ void conc_phillpump(double* __restrict__ Sb, int lane, double& csum, int a, int cs,
-                                               int dg, int hill)
+__device__ __forceinline__ void conc_phillpump(const double* __restrict__ Sb, int lane, double& csum, int a, double X,
+                                               double& dg, int hill)
 {
-    const double X = Sb[cs * W + lane];
     double c = hill_pow(hill, X);
     c = c / (c * X + Sb[a * W + lane]);
-    Sb[dg * W + lane] += Sb[(a + 1) * W + lane] * c;
+    dg += Sb[(a + 1) * W + lane] * c;
     csum += Sb[(a + 2) * W + lane] * c * X;
 }
 __device__ __forceinline__ void conc_buffer(double* __restrict__ Sb, int lane, double& csum, int a, int cb, int cp)
@@ -814,19 +822,20 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             pc += 2;
             conc_fluxconc(Sb, lane, csum, a, p >= 0 ? flux.get(p) : 0.0);
         } else if (EXT && op == U_CTAUPUMP) {
-            conc_ctaupump(Sb, lane, csum, code[pc], code[pc + 1]);
+            conc_ctaupump(Sb, lane, csum, code[pc], Sb[code[pc + 1] * W + lane]);
             pc += 2;
         } else if (EXT && op == U_TAUPUMP) {
-            conc_taupump(Sb, lane, csum, code[pc], code[pc + 1], d.dt, Vm);
+            conc_taupump(Sb, lane, csum, code[pc], Sb[code[pc + 1] * W + lane], d.dt, Vm);
             pc += 2;
         } else if (EXT && op == U_MMPUMP) {
-            conc_mmpump(Sb, lane, code[pc], code[pc + 1], code[pc + 2]);
+            conc_mmpump(Sb, lane, code[pc], Sb[code[pc + 1] * W + lane], Sb[code[pc + 2] * W + lane]);
             pc += 3;
         } else if (EXT && op == U_HILLPUMP) {
-            conc_hillpump(Sb, lane, code[pc], code[pc + 1], code[pc + 2], code[pc + 3]);
+            conc_hillpump(Sb, lane, code[pc], Sb[code[pc + 1] * W + lane], Sb[code[pc + 2] * W + lane], code[pc + 3]);
             pc += 4;
         } else if (EXT && op == U_PHILLPUMP) {
-            conc_phillpump(Sb, lane, csum, code[pc], code[pc + 1], code[pc + 2], code[pc + 3]);
+            conc_phillpump(Sb, lane, csum, code[pc], Sb[code[pc + 1] * W + lane], Sb[code[pc + 2] * W + lane],
+                           code[pc + 3]);
             pc += 4;
         } else if (EXT && op == U_BUFFER) {
             conc_buffer(Sb, lane, csum, code[pc], code[pc + 1], code[pc + 2]);
@@ -835,8 +844,9 @@ __device__ __forceinline__ CompOut run_prog(const DevModel& d, const int* __rest
             Sb[code[pc] * W + lane] += csum;
             pc += 1;
         } else if (EXT && op == U_DIFF) {
-            diff_solve(Sb, lane, code[pc], code[pc + 1], code[pc + 2], code[pc + 3], code[pc + 4], code[pc + 5],
-                       code[pc + 6], code[pc + 7], code[pc + 8], code[pc + 9], d.crank);
+            diff_solve_t<W>(Sb, lane, code[pc], code[pc + 1], code[pc + 2], code[pc + 3], code[pc + 4],
+                            code[pc + 5], Sb + code[pc + 6] * W + lane, Sb + code[pc + 7] * W + lane,
+                            Sb + code[pc + 8] * W + lane, Sb + code[pc + 9] * W + lane, d.crank);
             pc += 10;
         } else {  // U_END
             if (giv) d.giv[gi] = Im;

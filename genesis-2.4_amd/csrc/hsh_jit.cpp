@@ -46,6 +46,50 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
     o += "    double sumgchan = 0.0;\n    double ichan = s0 + s1;\n";
     o += "    double Gk = 0.0, Ek = 0.0, Ik = 0.0, vipol = 0.0, cipol = 0.0;\n";
     o += "    double flux[MAXPOOL] = {0.0, 0.0, 0.0, 0.0};\n    double csum = 0.0; (void)csum;\n";
+    // The diffusion systems' per-step arrays (codiag and coresult per shell,
+    // elimination scratch) live in the thread's own arrays here: codiag is 1
+    // and coresult 0 between steps, so they start from those values.
+    std::map<int, std::string> tmap;      // slot -> its element of those arrays
+    {
+        int k = 0;
+        for (size_t q = 0; q < t.code.size();) {
+            const int op = t.code[q++];
+            int skip = 0;
+            switch (op) {
+            case U_END: q = t.code.size(); break;
+            case U_IPOL1V: case U_SYN2: case U_SYN2MG: case U_GIV0: case U_TABGHK: case U_TABCURR: case U_ELMMPUMP:
+            case U_FLUXCONC: case U_CTAUPUMP: case U_TAUPUMP: case U_NERNST: case U_GHK: skip = 2; break;
+            case U_SPIKE: case U_CONCVAL: case U_NEWCONC1: case U_NEWCONC2: case U_FLUX: case U_SCALEFLUX:
+            case U_CSTORE: case U_POOL: skip = 1; break;
+            case U_IPOLG: case U_POOLCA: case U_MMPUMP: case U_BUFFER: skip = 3; break;
+            case U_HILLPUMP: case U_PHILLPUMP: skip = 4; break;
+            case U_DIFF: {
+                const int ms = t.code[q], dw = t.code[q + 1], N = ms * dw, nx = dw == 1 ? ms : N * dw;
+                const int dg = t.code[q + 6], cr = t.code[q + 7], x = t.code[q + 8], r = t.code[q + 9];
+                const std::string K = std::to_string(k);
+                o += "    double T" + K + "d[" + std::to_string(ms) + "], T" + K + "c[" + std::to_string(ms) +
+                     "], T" + K + "x[" + std::to_string(nx) + "], T" + K + "r[" + std::to_string(N) + "];\n";
+                o += "    for (int i = 0; i < " + std::to_string(ms) + "; i++) { T" + K + "d[i] = 1.0; T" + K +
+                     "c[i] = 0.0; }\n";
+                for (int i = 0; i < ms; i++) {
+                    tmap[dg + i] = "T" + K + "d[" + std::to_string(i) + "]";
+                    tmap[cr + i] = "T" + K + "c[" + std::to_string(i) + "]";
+                }
+                for (int i = 0; i < nx; i++) tmap[x + i] = "T" + K + "x[" + std::to_string(i) + "]";
+                for (int i = 0; i < N; i++) tmap[r + i] = "T" + K + "r[" + std::to_string(i) + "]";
+                k++;
+                skip = 10;
+                break;
+            }
+            default: break;
+            }
+            q += skip;
+        }
+    }
+    auto tref = [&](int slot) {
+        auto it = tmap.find(slot);
+        return it != tmap.end() ? it->second : "Sb[" + std::to_string(slot) + " * W + lane]";
+    };
     o += "    const double* vtab = d.tab;\n    const double* ctab = d.tab;\n    const double* tabr = d.tab;\n";
     o += "    int cfilo = 0;\n    double ipol = 0.0, ghk = 0.0;\n";
     o += "    (void)Ib; (void)active; (void)step; (void)Gk; (void)Ek; (void)Ik; (void)vipol; (void)cipol;\n";
@@ -215,7 +259,7 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
             o += "    { const double X = 1.0 / (" + C + " + " + slot(c) + "); Ik = " + slot(c + 1) + " * " + C +
                  " * X; ichan += Ik;";
             if (giv) o += " if (giv) { d.giv[gi] = d.chanmode5 ? Ik / " + slot(c + 1) + " : Ik; gi += 1; } Im += Ik;";
-            if (dg >= 0) o += " Sb[" + std::to_string(dg) + " * W + lane] += " + slot(c + 2) + " * X;";
+            if (dg >= 0) o += " " + tref(dg) + " += " + slot(c + 2) + " * X;";
             o += " }\n";
             c += 3;
             break;
@@ -224,7 +268,7 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
             o += "    csum = 0.0;\n";
             break;
         case U_CSTORE:
-            o += "    Sb[" + std::to_string(code[pc++]) + " * W + lane] = csum;\n";
+            o += "    " + tref(code[pc++]) + " = csum;\n";
             break;
         case U_FLUXCONC: {
             const int a = code[pc], p = code[pc + 1];
@@ -234,24 +278,23 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
             break;
         }
         case U_CTAUPUMP:
-            o += "    conc_ctaupump(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) +
-                 ");\n";
+            o += "    conc_ctaupump(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + tref(code[pc + 1]) + ");\n";
             pc += 2;
             break;
         case U_TAUPUMP:
-            o += "    conc_taupump(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) +
+            o += "    conc_taupump(Sb, lane, csum, " + std::to_string(code[pc]) + ", " + tref(code[pc + 1]) +
                  ", d.dt, Vm);\n";
             pc += 2;
             break;
         case U_MMPUMP:
-            o += "    conc_mmpump(Sb, lane, " + std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) + ", " +
-                 std::to_string(code[pc + 2]) + ");\n";
+            o += "    conc_mmpump(Sb, lane, " + std::to_string(code[pc]) + ", " + tref(code[pc + 1]) + ", " +
+                 tref(code[pc + 2]) + ");\n";
             pc += 3;
             break;
         case U_HILLPUMP:
         case U_PHILLPUMP: {
-            const std::string args = std::to_string(code[pc]) + ", " + std::to_string(code[pc + 1]) + ", " +
-                                     std::to_string(code[pc + 2]) + ", " + std::to_string(code[pc + 3]);
+            const std::string args = std::to_string(code[pc]) + ", " + tref(code[pc + 1]) + ", " +
+                                     tref(code[pc + 2]) + ", " + std::to_string(code[pc + 3]);
             pc += 4;
             o += op == U_HILLPUMP ? "    conc_hillpump(Sb, lane, " + args + ");\n"
                                   : "    conc_phillpump(Sb, lane, csum, " + args + ");\n";
@@ -266,9 +309,12 @@ std::string gen_template(const Template& t, int id, int ncols, bool giv)
             o += "    Sb[" + std::to_string(code[pc++]) + " * W + lane] += csum;\n";
             break;
         case U_DIFF: {
-            o += "    diff_solve(Sb, lane";
-            for (int i = 0; i < 10; i++) o += ", " + std::to_string(code[pc + i]);
-            o += ", d.crank);\n";
+            const std::string dg = tref(code[pc + 6]), cr = tref(code[pc + 7]), x = tref(code[pc + 8]),
+                              r = tref(code[pc + 9]);
+            auto arr = [](const std::string& e) { return e.substr(0, e.find('[')); };
+            o += "    diff_solve_t<1>(Sb, lane";
+            for (int i = 0; i < 6; i++) o += ", " + std::to_string(code[pc + i]);
+            o += ", " + arr(dg) + ", " + arr(cr) + ", " + arr(x) + ", " + arr(r) + ", d.crank);\n";
             pc += 10;
             break;
         }

@@ -140,7 +140,8 @@ static void fill_model(Hsolve *hsolve, HipState *st, hsh_model *m)
     m->funcs = hsolve->funcs;
     m->ravals = hsolve->ravals;
     m->vm = hsolve->vm;
-    /* Ca_concen pools run on the device, in the program that feeds them */
+    /* pools, shells, buffers and pumps run on the device, in the program of
+     * the compartment that owns them */
     m->nconcs = hsolve->nconcs;
     m->conc = hsolve->conc;
     m->nconcops = hsolve->concops ? hsolve->nconcops : 0;
@@ -156,6 +157,33 @@ static void fill_model(Hsolve *hsolve, HipState *st, hsh_model *m)
     m->nstab = st->nstab;
     m->tablist = st->tab;
     m->xvals = st->xv;
+    /* diffusion systems (h_init_conc_solve, hines_conc_init.c:59-291) */
+    if (hsolve->ndiffs > 0) {
+        m->ndiffs = hsolve->ndiffs;
+        m->nshells = hsolve->nshells;
+        m->nshrows = hsolve->nshrows;
+        m->nshfix = hsolve->nshfix;
+        m->dconc = hsolve->dconc;
+        m->fbufval = hsolve->fbufval;
+        m->fdiffs = hsolve->fdiffs;
+        m->ncovals0 = hsolve->fcoval[hsolve->ndiffs];
+        m->covals0 = hsolve->covals0;
+        m->nbufvals = hsolve->fbufval[hsolve->ndiffs];
+        m->bufvals = hsolve->bufvals;
+    }
+}
+
+/* Whether the solver's concops[] holds more than Ca_concen pools (shells,
+ * pumps, buffers, concpools): such solvers are not merged into a group. */
+static int conc_program(Hsolve *hsolve)
+{
+    int i;
+    if (hsolve->ndiffs > 0) return 1;
+    for (i = 0; hsolve->concops && i < hsolve->nconcops; i += 2) {
+        if (hsolve->concops[i] == LCACONC_OP) break;
+        if (hsolve->concops[i] != CACONC_OP) return 1;
+    }
+    return 0;
 }
 
 /* the ininfo list as device inputs (hines_msgs.c:329-348, :351-405) */
@@ -269,10 +297,9 @@ static int hip_init(Hsolve *hsolve)
     HipState *st;
     int rc, i;
 
-    if (hsolve->ydivs > hsolve->xdivs || hsolve->ndiffs) {
-        /* concentration rows beyond the composite table, or a conc solve
-         * (difshells): CPU path */
-        fprintf(stderr, "HIP: %s: concentration tables or shells the device path does not run; "
+    if (hsolve->ydivs > hsolve->xdivs) {
+        /* concentration rows beyond the composite table: CPU path */
+        fprintf(stderr, "HIP: %s: concentration tables the device path does not run; "
                 "CPU path for this solver\n", Pathname(hsolve));
         hsolve->accel_state = HIP_REFUSED;
         return -1;
@@ -343,7 +370,7 @@ static int hip_init(Hsolve *hsolve)
 
 static int group_eligible(HipState *st)
 {
-    return st->h && !st->grp && st->hsolve->ncompts == 1;
+    return st->h && !st->grp && st->hsolve->ncompts == 1 && !conc_program(st->hsolve);
 }
 
 /* Merge the one-cell solvers that ran the first step into one handle, in

@@ -726,7 +726,8 @@ def shell_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, shells: int = 
     rng = np.random.default_rng(seed + 104729)
     crank = method == CRANK
     nd = 1 if "fixbuf" in feats else 0
-    dw = 1 + nd + (1 if "difbuf" in feats else 0)
+    ndif = 1 if "difbuf" in feats else 0
+    dw = 1 + nd + ndif
     # chip_put_flux / _taupump / _mmpump / _hillpump: dt/2 for a system of
     # shells alone under Crank-Nicolson (hines_data.c:224-227); the solve's
     # own dt2 is dt/2 under Crank-Nicolson (hines_conc_init.c:312-315)
@@ -747,169 +748,200 @@ def shell_population(n_neurons: int, sizes=(1,), probs=(1.0,), *, shells: int = 
     tab[:ydivs + 1, col_z + 1], tab[ydivs + 1:, col_z + 1] = zA, zA[-1]
     col_m = next(int(base.ops[i + 1]) for i in range(base.ops.size - 2)
                  if base.ops[i] == _IPOL1V and base.ops[i + 2] == 3)
-    segs = _segments(base)
-    ops, chip, conc, concops, concchip = [], [], [], [], []
-    compops, compchips = [], []
-    nshells, nshrows, nshfix, dconc, fbufval, covals0, bufvals = [], [], [], [], [], [], []
-    shell_vol = []        # per system: the shells' volumes (m^3), innermost first
+    n = base.ncompts
     kf_f, kb_f, bt_f = 1.0e5, 100.0, 0.05          # fixed buffer: /mM/s, /s, mM
     kf_d, kb_d, bt_d, D_b = 2.0e5, 300.0, 0.02, 2.0e-11   # diffusible buffer
     D_ca = 6.0e-10
-    last = None           # the h_init_conc_chip index register: (conc, is_pool) or None
-    first = True
 
-    def switch(k, pool):
-        # (F)CONC_OP when the ops move to another concentration
-        # (hines_conc_init.c:642-687)
-        nonlocal last, first
-        if last is not None and last[0] == k:
-            return
-        if first:
-            code = _FCONC
-        elif pool and last is not None and last[1]:
-            code = _FCONC
-        else:
-            code = _CONC
-        concops.extend([code, k])
-        first = False
-        last = (k, pool)
+    # --- per compartment geometry (difshell.c:205-229, shape_mode SHELL)
+    dia = rng.uniform(1e-6, 4e-6, n)
+    ln = rng.uniform(10e-6, 100e-6, n)
+    g_scale = rng.uniform(0.5, 1.5, n)
+    rad = dia / 2.0
+    f = 0.5 / (2 ** (shells - 1) - 1)
+    th = rad[:, None] * f * 2.0 ** (shells - 1 - np.arange(shells))[None, :]
+    th[:, 0] = rad - th[:, 1:].sum(axis=1)
+    r_out = np.empty((n, shells))
+    acc = rad.copy()
+    for m_ in range(shells - 1, -1, -1):
+        r_out[:, m_] = acc
+        acc = acc - th[:, m_]
+    r_in = r_out - th
+    r_in[:, 0] = 0.0
+    vol = np.pi * ln[:, None] * (r_out ** 2 - r_in ** 2)
+    s_up = 2.0 * np.pi * r_out * ln[:, None]
+    s_dn = 2.0 * np.pi * r_in * ln[:, None]
 
-    reg = base_reg = 0.0
-    for r, (head, body, pre, post) in enumerate(segs):
-        compops.append(len(ops)); compchips.append(len(chip))
-        ops.append(head); chip += pre
-        for (o, opnd, vals) in body:
-            if o == _CHAN_EK:
-                base_reg = reg = vals[0]
-            elif o == _CHAN and reg != base_reg:
-                o, vals = _CHAN_EK, [base_reg] + vals
-                reg = base_reg
-            ops += [o] + opnd
-            chip += vals
-        # geometry of the shells (difshell.c:205-229, shape_mode SHELL)
-        dia = rng.uniform(1e-6, 4e-6)
-        ln = rng.uniform(10e-6, 100e-6)
-        rad = dia / 2.0
-        f = 0.5 / (2 ** (shells - 1) - 1)
-        th = [rad * f * 2 ** (shells - 1 - m) for m in range(shells)]
-        th[0] = rad - sum(th[1:])
-        r_out = np.empty(shells)
-        acc = rad
-        for m in range(shells - 1, -1, -1):
-            r_out[m] = acc
-            acc -= th[m]
-        r_in = r_out - np.array(th)
-        r_in[0] = 0.0
-        vol = np.pi * ln * (r_out ** 2 - r_in ** 2)
-        s_up = 2.0 * np.pi * r_out * ln
-        s_dn = 2.0 * np.pi * r_in * ln
-        shell_vol.append(vol)
-        k0 = len(conc)
-        bf_f = bt_f * kb_f / (kb_f + kf_f * ca0)
-        bf_d = bt_d * kb_d / (kb_d + kf_d * ca0)
-        for m in range(shells):
-            if nd:
-                conc.append(bf_f)
-            if dw - nd > 1:
-                conc.append(bf_d)
-            conc.append(ca0)
-        kout = k0 + (shells - 1) * dw + dw - 1        # the outermost shell
-        nshells.append(shells); nshrows.append(dw); nshfix.append(nd); dconc.append(k0)
-        fbufval.append(len(bufvals))
-        # h_setup_conc_solve (hines_conc_init.c:294-411)
-        def coupling(D, m, surf_dn, surf_up, th):
-            Diff = 2.0 * dt2 * D / vol[m]
-            out = []
-            if m < shells - 1:
-                if m:
-                    den = (2 * th[m - 1] + th[m]) if m == 1 else (th[m - 1] + th[m])
-                    out.append(Diff * surf_dn / den)
-                    out.append(Diff * surf_up / (th[m] + th[m + 1]))
-                else:
-                    out.append(Diff * surf_up / (2 * th[m] + th[m + 1]))
-            else:
-                out.append(Diff * surf_dn / (th[m - 1] + th[m]))
-            return out
-        for m in range(shells):
-            if nd:
-                btot = dt * kb_f * bt_f
-                if m == 0:
-                    bufvals.extend([dt2 * kf_f, dt2 * kb_f, btot])
-                covals0.append(btot)
-            if dw - nd > 1:
-                btot = dt * kb_d * bt_d
-                if m == 0:
-                    bufvals.extend([dt2 * kf_d, dt2 * kb_d, btot])
-                covals0.extend(coupling(D_b, m, s_dn[m], s_up[m], th))
-            covals0.extend(coupling(D_ca, m, s_dn[m], s_up[m], th))
-        # channel program: Ca channel into the outermost shell, optional
-        # electrogenic pump and Ca-gated K channel on it
-        area = float(pre[0]) * kw.get("RM", 0.33) / kw.get("Em", -0.065)
-        v0 = (base.vm[r] - base.xmin) * base.invdx
-        rr = min(max(int(v0), 0), xdivs)
-        fB = tab[rr, col_m] + (v0 - rr) * (tab[rr + 1, col_m] - tab[rr, col_m])
-        fA = tab[rr, col_m + 1] + (v0 - rr) * (tab[rr + 1, col_m + 1] - tab[rr, col_m + 1])
-        m0 = fA / (2.0 * (fB - 1.0))
-        g_ca = 2.0 * area * rng.uniform(0.5, 1.5)
-        ops += [_CHAN_EK, _IPOL1V, col_m, 2, _ADD_CURR, _CONC_VAL, kout, _FLUX]
-        chip += [0.12, g_ca, m0]
-        reg = 0.12
-        if "elmmpump" in feats:
-            vmax = 2e-16 * s_up[-1] / 1e-10
-            ops += [_ELMMPUMP]
-            chip += [1e-3, vmax * 2.0 * _FARADAY, dtp * vmax / vol[-1]]
-        if "kc" in feats:
-            ops += [_NEWCONC1, _CHAN_EK, _IPOL1C, col_z, 1, _ADD_CURR]
-            c = ca0 * invdy
-            z0 = (tab[0, col_z + 1] + c * (tab[1, col_z + 1] - tab[0, col_z + 1])) / (
-                2.0 * (tab[0, col_z] + c * (tab[1, col_z] - tab[0, col_z]) - 1.0))
-            chip += [-0.09, 10.0 * area, z0]
-            reg = -0.09
-        chip += post
-        # h_init_conc_chip: the outermost shell's flux and leak, its pumps
-        switch(kout, False)
-        leak = 1e-6 if "leak" in feats else 0.0
-        concops.append(_FLUXCONC)
-        concchip += [dtp * leak, dtp / (2.0 * _FARADAY * vol[-1])]
-        if "ctaupump" in feats:
-            T_C, Ceq = 2e-3, 4e-5
-            concops.append(_CTAUPUMP)
-            concchip += [Ceq * dtp / T_C, dtp / T_C]
-        if "taupump" in feats:
-            concops += [_TAUPUMP, r]
-            concchip += [3e-3, 1e-3, -0.04, 1.0 / 0.01, 5e-5]
-        if "mmpump" in feats:
-            concops.append(_MMPUMP)
-            concchip += [dtp * 2e-4 / (vol[-1] * 1e12), 2e-4]
-        if "hillpump" in feats:
-            concops += [_HILLPUMP, hill]
-            concchip += [3e-4 ** hill, dtp * 5e-4 / (vol[-1] * 1e12)]
-        if "pool" in feats:
-            kp = len(conc)
-            pool_vol = 0.05 * vol.sum()
-            kb_p, kf_p, bt_p, cp0 = 50.0, 5e4, 0.1, 1e-3
-            conc += [cp0, bt_p * kb_p / (kb_p + kf_p * cp0)]
-            switch(kp, True)           # the Hill pump into the pool
-            vmax = 3e-4 / 1e12
-            concops += [_PHILLPUMP, hill, kout]
-            concchip += [2e-4 ** hill, dtp * vmax / vol[-1], dt * vmax / pool_vol]
-            concops += [_BUFFER, kp + 1]
-            concchip += [dt * kb_p / 2.0, dt * kb_p * bt_p, dt * kf_p]
-            concops.append(_POOL)
-    ops.append(_LCOMPT)
-    concops.append(_LCONC if (last is not None and not last[1]) else _LCACONC)
+    # --- h_setup_conc_solve (hines_conc_init.c:294-411): covals0 per system
+    def coupling(D, m_):
+        Diff = 2.0 * dt2 * D / vol[:, m_]
+        if m_ < shells - 1:
+            up = Diff * s_up[:, m_] / ((2 * th[:, m_] if m_ == 0 else th[:, m_]) + th[:, m_ + 1])
+            if m_ == 0:
+                return [up]
+            den = (2 * th[:, m_ - 1] + th[:, m_]) if m_ == 1 else (th[:, m_ - 1] + th[:, m_])
+            return [Diff * s_dn[:, m_] / den, up]
+        return [Diff * s_dn[:, m_] / (th[:, m_ - 1] + th[:, m_])]
+    cols = []
+    for m_ in range(shells):
+        if nd:
+            cols.append(np.full(n, dt * kb_f * bt_f))
+        if ndif:
+            cols.extend(coupling(D_b, m_))
+        cols.extend(coupling(D_ca, m_))
+    covals0 = np.stack(cols, axis=1).reshape(-1)
+    buf1 = ([dt2 * kf_f, dt2 * kb_f, dt * kb_f * bt_f] if nd else []) + \
+           ([dt2 * kf_d, dt2 * kb_d, dt * kb_d * bt_d] if ndif else [])
+    bufvals = np.tile(np.array(buf1, np.float64), n)
+
+    # --- concentrations: per compartment its system (buffers, then the shell,
+    # shell by shell), then with "pool" the concpool and its buffer
+    bf_f = bt_f * kb_f / (kb_f + kf_f * ca0)
+    bf_d = bt_d * kb_d / (kb_d + kf_d * ca0)
+    row = ([bf_f] if nd else []) + ([bf_d] if ndif else []) + [ca0]
+    kb_p, kf_p, bt_p, cp0 = 50.0, 5e4, 0.1, 1e-3
+    per = row * shells + ([cp0, bt_p * kb_p / (kb_p + kf_p * cp0)] if "pool" in feats else [])
+    npc = len(per)
+    conc = np.tile(np.array(per, np.float64), n)
+    k0 = np.arange(n) * npc
+    kout = k0 + (shells - 1) * dw + dw - 1            # the outermost shell
+    kp = k0 + shells * dw                             # the concpool
+
+    # --- the concentration program (h_init_conc_chip, hines_conc_init.c:
+    # 413-805): per compartment the outermost shell's flux and leak, its
+    # pumps, then the Hill pump into the concpool, the pool buffer and the
+    # pool; the first switch is FCONC_OP, later ones CONC_OP (after a pool a
+    # shell's switch still stores, hines_conc_init.c:642-687)
+    leak = 1e-6 if "leak" in feats else 0.0
+    v_out = vol[:, -1]
+    ops_cols, chip_cols = [], []
+    def oc(v): ops_cols.append(np.broadcast_to(np.asarray(v, np.int64), (n,)))
+    def cc(v): chip_cols.append(np.broadcast_to(np.asarray(v, np.float64), (n,)))
+    sw = np.full(n, _CONC, np.int64)
+    sw[0] = _FCONC
+    oc(sw); oc(kout); oc(_FLUXCONC); cc(dtp * leak); cc(dtp / (2.0 * _FARADAY * v_out))
+    if "ctaupump" in feats:
+        T_C, Ceq = 2e-3, 4e-5
+        oc(_CTAUPUMP); cc(Ceq * dtp / T_C); cc(dtp / T_C)
+    if "taupump" in feats:
+        oc(_TAUPUMP); oc(np.arange(n))
+        for v in (3e-3, 1e-3, -0.04, 1.0 / 0.01, 5e-5):
+            cc(v)
+    if "mmpump" in feats:
+        oc(_MMPUMP); cc(dtp * 2e-4 / (v_out * 1e12)); cc(2e-4)
+    if "hillpump" in feats:
+        oc(_HILLPUMP); oc(hill); cc(3e-4 ** hill); cc(dtp * 5e-4 / (v_out * 1e12))
+    if "pool" in feats:
+        pool_vol = 0.05 * vol.sum(axis=1)
+        vmax = 3e-4 / 1e12
+        oc(_CONC); oc(kp); oc(_PHILLPUMP); oc(hill); oc(kout)
+        cc(2e-4 ** hill); cc(dtp * vmax / v_out); cc(dt * vmax / pool_vol)
+        oc(_BUFFER); oc(kp + 1)
+        cc(dt * kb_p / 2.0); cc(dt * kb_p * bt_p); cc(dt * kf_p)
+        oc(_POOL)
+    concops = np.concatenate([np.stack(ops_cols, axis=1).reshape(-1),
+                              [_LCACONC if "pool" in feats else _LCONC]]).astype(np.int32)
+    concchip = np.stack(chip_cols, axis=1).reshape(-1)
+
+    # --- channel programs: the base compartment's ops, then the Ca channel
+    # into the outermost shell, optional electrogenic pump and Ca-gated K
+    # channel on it.  The Ek register carries across compartments, and the
+    # channels added here change it, so a base CHAN_OP that inherited Ek
+    # becomes a CHAN_EK_OP with the base's value, as SETUP writes it
+    # (hines_chip_init.c:1029-1049).
+    bops, bchip = base.ops, base.chip
+    cops = np.append(base.compops.astype(np.int64), bops.size - 1)      # LCOMPT_OP ends the last
+    cchp = np.append(base.compchips.astype(np.int64), bchip.size)
+    sig_info = {}
+    first_chan = np.full(n, -1, np.int64)      # ops index of a CHAN_OP to rewrite
+    first_chan_c = np.full(n, -1, np.int64)    # chip index of its gmax
+    last_ek_c = np.full(n, -1, np.int64)       # chip index of the compartment's last CHAN_EK Ek
+    for r in range(n):
+        a, b = cops[r], cops[r + 1]
+        key = bops[a:b].tobytes()
+        info = sig_info.get(key)
+        if info is None:
+            p, c = 1, 2
+            fc = fcc = lek = -1
+            seen = False
+            seg = bops[a:b]
+            while p < seg.size:
+                o = int(seg[p]); p += 1
+                if o == _CHAN_EK:
+                    lek = c
+                    seen = True
+                    c += 2
+                elif o == _CHAN:
+                    if not seen:
+                        fc, fcc = p - 1, c
+                    seen = True
+                    c += 1
+                elif o == _IPOL1V:
+                    c += 1 if int(seg[p + 1]) != 0 else 0
+                    p += 2
+                elif o not in (_NEWVOLT, _ADD_CURR):
+                    raise ValueError(f"shell_population: base op {o} not in the supported subset")
+            info = (fc, fcc, lek)
+            sig_info[key] = info
+        fc, fcc, lek = info
+        if fc >= 0:
+            first_chan[r], first_chan_c[r] = a + fc, cchp[r] + fcc
+        if lek >= 0:
+            last_ek_c[r] = cchp[r] + lek
+    # the register each compartment starts with in the base model
+    idx = np.where(last_ek_c >= 0, np.arange(n), -1)
+    idx = np.maximum.accumulate(idx)
+    reg_after = np.where(idx >= 0, bchip[np.where(idx >= 0, last_ek_c[np.maximum(idx, 0)], 0)], 0.0)
+    reg_before = np.concatenate([[0.0], reg_after[:-1]])
+    rw = first_chan >= 0
+    area = bchip[cchp[:-1]] * kw.get("RM", 0.33) / kw.get("Em", -0.065)
+    v0 = (base.vm - base.xmin) * base.invdx
+    rr = np.clip(v0.astype(np.int64), 0, xdivs)
+    fr = v0 - rr
+    fB = tab[rr, col_m] + fr * (tab[rr + 1, col_m] - tab[rr, col_m])
+    fA = tab[rr, col_m + 1] + fr * (tab[rr + 1, col_m + 1] - tab[rr, col_m + 1])
+    m0 = fA / (2.0 * (fB - 1.0))
+    g_ca = 2.0 * area * g_scale
+    eo, ec = [], []
+    def eop(v): eo.append(np.broadcast_to(np.asarray(v, np.int64), (n,)))
+    def ech(v): ec.append(np.broadcast_to(np.asarray(v, np.float64), (n,)))
+    for v in (_CHAN_EK, _IPOL1V, col_m, 2, _ADD_CURR, _CONC_VAL):
+        eop(v)
+    eop(kout); eop(_FLUX)
+    ech(0.12); ech(g_ca); ech(m0)
+    if "elmmpump" in feats:
+        vm_ = 2e-16 * s_up[:, -1] / 1e-10
+        eop(_ELMMPUMP)
+        ech(1e-3); ech(vm_ * 2.0 * _FARADAY); ech(dtp * vm_ / v_out)
+    if "kc" in feats:
+        for v in (_NEWCONC1, _CHAN_EK, _IPOL1C, col_z, 1, _ADD_CURR):
+            eop(v)
+        c = ca0 * invdy
+        z0 = (tab[0, col_z + 1] + c * (tab[1, col_z + 1] - tab[0, col_z + 1])) / (
+            2.0 * (tab[0, col_z] + c * (tab[1, col_z] - tab[0, col_z]) - 1.0))
+        ech(-0.09); ech(10.0 * area); ech(z0)
+    E_o, E_c = len(eo), len(ec)
+    ops2 = bops.astype(np.int64).copy()
+    ops2[first_chan[rw]] = _CHAN_EK
+    ops2 = np.insert(ops2, np.repeat(cops[1:], E_o), np.stack(eo, axis=1).reshape(-1)).astype(np.int32)
+    pos = np.concatenate([first_chan_c[rw], np.repeat(cchp[1:] - 2, E_c)])
+    vals = np.concatenate([reg_before[rw], np.stack(ec, axis=1).reshape(-1)])
+    order = np.argsort(pos, kind="stable")
+    chip2 = np.insert(bchip, pos[order], vals[order])
+    nrw = np.concatenate([[0], np.cumsum(rw)[:-1]])
+    compops = (cops[:-1] + np.arange(n) * E_o).astype(np.int32)
+    compchips = (cchp[:-1] + np.arange(n) * E_c + nrw).astype(np.int32)
+
     import dataclasses
-    n = base.ncompts
     extra = dict(base.extra)
-    extra.update(shell_vol=np.array(shell_vol), shell_params=dict(
+    extra.update(shell_vol=vol, shell_params=dict(
         fixbuf=(kf_f, kb_f, bt_f), difbuf=(kf_d, kb_d, bt_d, D_b), D_ca=D_ca, ca0=ca0, dt2=dt2, dtp=dtp))
     return dataclasses.replace(
-        base, ops=np.array(ops, np.int32), chip=np.array(chip, np.float64), ncols=tab.shape[1],
-        tablist=np.ascontiguousarray(tab.reshape(-1)), compops=np.array(compops, np.int32),
-        compchips=np.array(compchips, np.int32), conc=np.array(conc, np.float64),
-        concops=np.array(concops, np.int32), concchip=np.array(concchip, np.float64),
+        base, ops=ops2, chip=chip2, ncols=tab.shape[1],
+        tablist=np.ascontiguousarray(tab.reshape(-1)), compops=compops, compchips=compchips,
+        conc=conc, concops=concops, concchip=concchip,
         ydivs=ydivs, ymin=0.0, invdy=invdy, givals=np.zeros(0), extra=extra,
-        nshells=np.array(nshells, np.int32), nshrows=np.array(nshrows, np.int32),
-        nshfix=np.array(nshfix, np.int32), dconc=np.array(dconc, np.int32),
-        fbufval=np.array(fbufval, np.int32), fdiffs=np.arange(n + 1, dtype=np.int32),
-        covals0=np.array(covals0, np.float64), bufvals=np.array(bufvals, np.float64))
+        nshells=np.full(n, shells, np.int32), nshrows=np.full(n, dw, np.int32),
+        nshfix=np.full(n, nd, np.int32), dconc=k0.astype(np.int32),
+        fbufval=(np.arange(n) * len(buf1)).astype(np.int32), fdiffs=np.arange(n + 1, dtype=np.int32),
+        covals0=covals0, bufvals=bufvals)

@@ -37,7 +37,7 @@ def test_shell_generated_source_compiles_for_gfx950():
     L = C.CDLL("/opt/rocm/lib/libhiprtc.so")
     m = M.shell_population(12, (1, 4), (0.5, 0.5), shells=4, features=ALL, method=M.CRANK, seed=1)
     src = D.jit_source(m)
-    for frag in ("diff_solve(Sb, lane, 4, 3, 1,", "conc_fluxconc(", "conc_ctaupump(", "conc_taupump(",
+    for frag in ("diff_solve_t<1>(Sb, lane, 4, 3, 1,", "conc_fluxconc(", "conc_ctaupump(", "conc_taupump(",
                  "conc_mmpump(", "conc_hillpump(", "conc_phillpump(", "conc_buffer(", "+= csum;", "= csum;"):
         assert frag in src, frag
     prog = C.c_void_p()
