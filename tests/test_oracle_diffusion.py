@@ -387,3 +387,33 @@ def test_single_shell_systems_are_not_stepped():
     o = Oracle(m, givals=False)
     with pytest.raises(RuntimeError):
         o.conc_solve()
+
+
+def test_electrogenic_pump_closed_form():
+    """ELMMPUMP_OP (hines_chip.c:278-302) after the CONC_VAL_OP of a shell:
+    the current vmax*val*F * C/(C + Kd) joins the compartment's ichan (so
+    results[2i] grows by it times tbyc) and dt*vmax/vol/(C + Kd) joins the
+    shell's codiag, C the shell's concentration before the step."""
+    base = M.shell_population(8, (1,), (1.0,), shells=3, features=("ctaupump",), method=M.BEULER, seed=4)
+    m = M.shell_population(8, (1,), (1.0,), shells=3, features=("ctaupump", "elmmpump"), method=M.BEULER, seed=4)
+    o0, o1 = Oracle(base, givals=False), Oracle(m, givals=False)
+    o0.chip_update()
+    o1.chip_update()
+    r0, r1 = o0.state()["results"], o1.state()["results"]
+    d1, _ = o1.conc_regs()
+    ELM = 500
+    for r in range(m.ncompts):
+        a, b = int(m.compops[r]), int(m.compops[r + 1]) if r + 1 < m.ncompts else m.ops.size - 1
+        q = a + int(np.flatnonzero(m.ops[a:b] == ELM)[0])
+        # the chip values ELMMPUMP consumes: the pump's three, found after the
+        # Ca channel's (Ek, gmax, m) in the compartment's chip run
+        c0 = int(m.compchips[r])
+        cend = int(m.compchips[r + 1]) if r + 1 < m.ncompts else m.chip.size
+        Kd, I0, B0 = m.chip[cend - 5], m.chip[cend - 4], m.chip[cend - 3]
+        k = int(m.ops[q - 2])                       # CONC_VAL_OP operand: the outermost shell
+        C = m.conc[k]
+        Ik = I0 * C * (1.0 / (C + Kd))
+        tbyc = m.chip[cend - 2]
+        assert r1[2 * r] - r0[2 * r] == pytest.approx(Ik * tbyc, rel=1e-9, abs=1e-18)
+        assert d1[k] - 1.0 == pytest.approx(B0 / (C + Kd), rel=1e-12)
+        assert r1[2 * r + 1] == r0[2 * r + 1]      # no conductance
