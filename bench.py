@@ -91,6 +91,8 @@ def parse():
     p.add_argument("--only", choices=("isopotential", "heterogeneous", "calcium", "shells", "network", "adjoint"),
                    default=None,
                    help="run one section alone and print its JSON (for profiling it)")
+    p.add_argument("--probe-workload", choices=("population", "shells"), default="population",
+                   help="workload of --traffic-probe")
     p.add_argument("--traffic-probe", action="store_true",
                    help="(child of the traffic measurement) copy calibration, then a few batched "
                         "launches, and exit")
@@ -153,12 +155,13 @@ def read_counter(db, counter, kernel):
     return sum(kern) / len(kern), sum(elem) / len(elem), sum(blit) / len(blit)
 
 
-def measure_traffic(a, K, info, kernel):
+def measure_traffic(a, K, info, kernel, workload="population"):
     """HBM bytes per batched launch: two child runs of this script under
-    ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE).  Each counter is
-    scaled by what it reads for an elementwise copy of CAL_MB MiB in the
-    same run (true bytes / counter), so the unit of the counter does not
-    matter."""
+    ``rocprofv3 --pmc`` (FETCH_SIZE, then WRITE_SIZE) on the same workload
+    ("population": the headline trees, "shells": the shells section's).  Each
+    counter is scaled by what it reads for an elementwise copy of CAL_MB MiB
+    in the same run (true bytes / counter), so the unit of the counter does
+    not matter."""
     exe = shutil.which("rocprofv3")
     if exe is None:
         return None, "rocprofv3 not found"
@@ -173,7 +176,8 @@ def measure_traffic(a, K, info, kernel):
             cmd = [exe, "--pmc", counter, "-d", d, "-o", "p", "--", sys.executable, os.path.abspath(__file__),
                    "--traffic-probe", "--neurons", str(a.neurons), "--ncomp", str(a.ncomp),
                    "--seed", str(a.seed), "--batch-k", str(K), "--flags", str(a.flags | 1),
-                   "--probe-layout", lay]
+                   "--probe-layout", lay, "--probe-workload", workload,
+                   "--shell-neurons", str(a.shell_neurons)]
             print(f"[bench] traffic pass {counter}: {' '.join(cmd)}", file=sys.stderr, flush=True)
             try:
                 p = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
@@ -470,6 +474,15 @@ def shells_section(a, D, M, Oracle):
         dev_ms = g.last_time(D.T_TOTAL)
         g.set_timing(False)
         bps = info["bytes_per_step"]
+        K = info["batch_k"]
+        if not a.no_traffic:
+            tr, why = measure_traffic(a, K, info, batched_kernel_name(info), workload="shells")
+            if tr is not None:
+                res["traffic"] = {"bytes_per_compartment_step": tr["bytes_per_launch"] / (n * K),
+                                  "traffic_over_algorithmic": tr["bytes_per_launch"] / (bps * K),
+                                  "detail": tr}
+            else:
+                res["traffic"] = {"error": why}
         res["batched"] = {"value": a.shell_steps * n / wall, "unit": "compartment-steps/s",
                           "ms_per_step": wall * 1e3 / a.shell_steps,
                           "device_us_per_step": dev_ms * 1e3 / a.shell_steps, "batch_k": info["batch_k"],
@@ -767,7 +780,11 @@ def main():
         return
     if a.traffic_probe:
         copy_calibration(torch, CAL_MB)
-        m = M.population(a.neurons, (a.ncomp,), (1.0,), seed=a.seed)
+        if a.probe_workload == "shells":
+            m = M.shell_population(a.shell_neurons, (16,), (1.0,), shells=4, features=SHELL_FEATURES,
+                                   seed=a.seed, method=M.CRANK)
+        else:
+            m = M.population(a.neurons, (a.ncomp,), (1.0,), seed=a.seed)
         tmr, cap, wv, st = (int(x) for x in a.probe_layout.split(","))
         with D.HipSolver(m, batch_k=a.batch_k, flags=a.flags, tile_max_rows=tmr, pack_cap=cap, pack_waves=wv,
                          pack_stage=st) as g:
