@@ -1123,9 +1123,13 @@ hsh_status time_batched(hsh_handle* h, int steps, double* ms_per_step)
 // (waves, schedule in LDS) and K, each swept on this model with the others
 // held, fastest kept.  Options the caller fixed are not swept.  Only layouts
 // that one kernel runs are candidates (all trees in tiles of one class, or
-// all in packs); a layout that mixes classes runs only as configured.  The
-// state is saved first and restored after (spikes raised while tuning are
-// dropped).
+// all in packs); a layout that mixes classes runs only as configured, unless
+// the library is built with HSH_AUTOTUNE_MIXED=1 (the host-sanitizer build
+// of tests/hostsan sweeps them).  The state is saved first and restored
+// after (spikes raised while tuning are dropped).
+#ifndef HSH_AUTOTUNE_MIXED
+#define HSH_AUTOTUNE_MIXED 0
+#endif
 hsh_status autotune(hsh_handle* h)
 {
     const auto t0 = std::chrono::steady_clock::now();
@@ -1197,7 +1201,7 @@ hsh_status autotune(hsh_handle* h)
             // pack size matters only when some tree went to a pack
             const bool any_pack = !h->plan.packs.empty();
             if (h->plain_loop) break;        // layouts with too-large tiles are not batched
-            if (h->classes.size() != 1) break;
+            if (!HSH_AUTOTUNE_MIXED && h->classes.size() != 1) break;
             for (int wv : (any_pack ? waves_c : std::vector<int>{waves_c[0]})) {
                 for (int st : (any_pack ? stage_c : std::vector<int>{0})) {
                     if (st && !stage_fits(h)) continue;
