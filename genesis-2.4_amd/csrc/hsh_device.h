@@ -1451,6 +1451,105 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
                 wave_lds_sync();
         }
     };
+    // Levels [l0, l1) of at most one wave's width, run by wave 0 (one entry
+    // per lane at most), when there are fewer than 64 of them: the levels'
+    // bounds are held one per lane and read with readlane, and each lane's
+    // entry of the next level -- its schedule indices -- is read while the
+    // running level waits for its own reads, into the other of two records
+    // (so nothing carried to the next level is copied, which would wait).
+    // A level then waits for one round of LDS reads (kids' values or the
+    // parent's solution, with the coefficients and the entry's own values)
+    // instead of four (bounds, entry, kid indices, kids' values).
+    struct FwdI {
+        int p, kb, nk, c0, c1, c2;
+        bool v;
+    };
+    auto fwd_narrow = [&](int l0, int l1) {
+        const int nl = l1 - l0;
+        const int bnd = lane <= nl ? lvl[l0 + lane] : 0;
+        auto fidx = [&](int j, FwdI& x) {
+            const int e = __builtin_amdgcn_readlane(bnd, j) + lane;
+            x.v = e < __builtin_amdgcn_readlane(bnd, j + 1);
+            if (x.v) {
+                const int f0 = FE[2 * e];
+                x.kb = FE[2 * e + 1];
+                x.p = f0 & 0xffff;
+                x.nk = f0 >> 16;
+                x.c0 = KI[x.kb];
+                x.c1 = KI[x.nk > 1 ? x.kb + 1 : x.kb];
+                x.c2 = KI[x.nk > 2 ? x.kb + 2 : x.kb];
+            }
+        };
+        auto level = [&](const FwdI& a, int jn, FwdI& b) {
+            FwdE x;
+            FwdK k;
+            if (a.v) {
+                x.p = a.p;
+                x.kb = a.kb;
+                x.nk = a.nk;
+                x.c[0] = a.c0;
+                x.c[1] = a.c1;
+                x.c[2] = a.c2;
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const int j = q < x.nk ? x.kb + q : x.kb;
+                    x.ta[q] = KTA[j];
+                    x.tf[q] = KTF[j];
+                }
+                x.dp = LD[x.p];
+                x.bp = LB[x.p];
+                fwd_kids(x, k);
+            }
+            if (jn < nl) fidx(jn, b);
+            if (a.v) fwd_run(x, k);
+            wave_lds_sync();
+        };
+        FwdI A, B;
+        fidx(0, A);
+        for (int j = 0; j < nl; j += 2) {
+            level(A, j + 1, B);
+            if (j + 1 < nl) level(B, j + 2, A);
+        }
+    };
+    struct BwdI {
+        int c, p, e;
+        bool v;
+    };
+    auto bwd_narrow = [&](const int* lv, int l0, int l1) {
+        const int nl = l1 - l0;
+        const int bnd = lane <= nl ? lv[l0 + lane] : 0;
+        auto bidx = [&](int j, BwdI& x) {
+            x.e = __builtin_amdgcn_readlane(bnd, j) + lane;
+            x.v = x.e < __builtin_amdgcn_readlane(bnd, j + 1);
+            if (x.v) {
+                const int kp = BKP[x.e];
+                x.c = kp & 0xffff;
+                x.p = (kp >> 16) - 1;
+            }
+        };
+        auto level = [&](const BwdI& a, int jn, BwdI& b) {
+            BwdE x;
+            double xp = 0.0;
+            if (a.v) {
+                x.c = a.c;
+                x.p = a.p;
+                x.tcb = BTC[a.e];
+                x.bc = LB[x.c];
+                x.dc = LD[x.c];
+                x.v = d.crank ? LV[x.c] : 0.0;
+                xp = bwd_par(x);
+            }
+            if (jn < nl) bidx(jn, b);
+            if (a.v) bwd_run(x, xp);
+            wave_lds_sync();
+        };
+        BwdI A, B;
+        bidx(0, A);
+        for (int j = 0; j < nl; j += 2) {
+            level(A, j + 1, B);
+            if (j + 1 < nl) level(B, j + 2, A);
+        }
+    };
     // Levels of at most one wave's width at the root end of the forward
     // sweep and at either end of the backward sweep run in wave 0 alone:
     // one wave's LDS accesses complete in order, so those levels need no
@@ -1499,14 +1598,19 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
         fwd_levels(0, fwide, true);
         if (fwide < pm.nfl || bhead > 0) {
             if (wave == 0) {
-                fwd_levels(fwide, pm.nfl, false);
-                bwd_levels(blv, 0, bhead, false);
+                if (pm.nfl - fwide < W) fwd_narrow(fwide, pm.nfl);
+                else fwd_levels(fwide, pm.nfl, false);
+                if (bhead < W) bwd_narrow(blv, 0, bhead);
+                else bwd_levels(blv, 0, bhead, false);
             }
             __syncthreads();
         }
         bwd_levels(blv, bhead, btail, true);
         if (btail < pm.nbl) {
-            if (wave == 0) bwd_levels(blv, btail, pm.nbl, false);
+            if (wave == 0) {
+                if (pm.nbl - btail < W) bwd_narrow(blv, btail, pm.nbl);
+                else bwd_levels(blv, btail, pm.nbl, false);
+            }
             __syncthreads();
         }
         // the adjoint's forward run keeps the observed voltages of each step
