@@ -236,8 +236,7 @@ def batched_kernel_name(info):
     """Name of the kernel a single-class batched launch runs."""
     jit = bool(info["jit"])
     if info["ntiles"] == 0:
-        st = 1 if info["pack_stage"] == 2 else 0
-        return f"hsh_jit_forest{st}" if jit else "k_forest"
+        return f"hsh_jit_forest{max(0, info['pack_stage'] - 1)}" if jit else "k_forest"
     r = info["max_rows"]
     if r == 1:
         return "hsh_jit_iso" if jit else "k_iso"
@@ -280,7 +279,7 @@ def hetero_section(a, D, M, Oracle):
     packs = info["tile_max_rows"] == 0
     res["layout"] = {"autotuned": bool(info["autotuned"]), "autotune_s": info["autotune_s"],
                      "tile_max_rows": info["tile_max_rows"], "pack_cap": info["pack_cap"],
-                     "pack_waves": info["pack_waves"], "schedule_in_lds": info["pack_stage"] == 2,
+                     "pack_waves": info["pack_waves"], "schedule_in_lds": {2: "all", 3: "indices"}.get(info["pack_stage"], "none"),
                      "packs": info["npacks"], "tiles": info["ntiles"], "batch_k": K,
                      "kernel": batched_kernel_name(info), "lane_fill": info["lane_fill"]}
     res["batched"] = {"value": a.hetero_steps * n / wall, "unit": "compartment-steps/s",

@@ -146,7 +146,7 @@ struct hsh_handle {
     bool plain_loop = false;           // some tile too large for the batched kernels
     int batch_k = 32, batch_waves = 4;
     int pack_waves = 4;
-    bool pack_stage = false;
+    int pack_stage = 0;       // pack schedule: 0 in memory, 1 all in LDS, 2 its indices in LDS
     bool tuned = false;
     double autotune_s = 0.0;
     // graphs keyed by (K, launches)
@@ -981,12 +981,14 @@ hsh_status relayout(hsh_handle* h, const LayoutOpts& o)
     return alloc_layout(h, vm.data(), res.data());
 }
 
-int pack_lds_bytes(const hsh_handle* h, bool stage)
+int64_t pack_lds_of(const Plan& P, int stage)
 {
-    return (int)(stage ? h->plan.pack_lds_stage : h->plan.pack_lds);
+    return stage == 1 ? P.pack_lds_stage : stage == 2 ? P.pack_lds_stage_idx : P.pack_lds;
 }
 
-bool stage_fits(const hsh_handle* h) { return h->plan.pack_lds_stage <= MAX_LDS; }
+int pack_lds_bytes(const hsh_handle* h, int stage) { return (int)pack_lds_of(h->plan, stage); }
+
+bool stage_fits(const hsh_handle* h, int stage) { return pack_lds_of(h->plan, stage) <= MAX_LDS; }
 
 DevModel tile_view(const DevModel& d, const KClass& c)
 {
@@ -1184,13 +1186,13 @@ hsh_status autotune(hsh_handle* h)
         }
     }
     waves_c = o.pack_waves > 0 ? std::vector<int>{o.pack_waves} : std::vector<int>{4, 8};
-    stage_c = o.pack_stage > 0 ? std::vector<int>{o.pack_stage == 2} : std::vector<int>{0, 1};
+    stage_c = o.pack_stage > 0 ? std::vector<int>{o.pack_stage - 1} : std::vector<int>{0, 1, 2};
 
     struct Best {
         double ms = 1e300;
         LayoutOpts lo;
         int waves = 4;
-        bool stage = false;
+        int stage = 0;
     } best;
     for (int tmr : tmr_c) {
         for (size_t ci = 0; ci < cap_c.size(); ci++) {
@@ -1204,10 +1206,10 @@ hsh_status autotune(hsh_handle* h)
             if (!HSH_AUTOTUNE_MIXED && h->classes.size() != 1) break;
             for (int wv : (any_pack ? waves_c : std::vector<int>{waves_c[0]})) {
                 for (int st : (any_pack ? stage_c : std::vector<int>{0})) {
-                    if (st && !stage_fits(h)) continue;
+                    if (st && !stage_fits(h, st)) continue;
                     if (h->plan.pack_lds > MAX_LDS) continue;
                     h->pack_waves = wv;
-                    h->pack_stage = st != 0;
+                    h->pack_stage = st;
                     free_graphs(h);
                     double ms = 0;
                     STCHK(time_batched(h, trial, &ms));
@@ -1218,7 +1220,7 @@ hsh_status autotune(hsh_handle* h)
                         best.ms = ms;
                         best.lo = lo;
                         best.waves = wv;
-                        best.stage = st != 0;
+                        best.stage = st;
                     }
                 }
             }
@@ -1473,7 +1475,7 @@ hsh_status create_impl(const hsh_model* m, const hsh_options* opt, hsh_handle** 
                        std::vector<hsh_member> members)
 {
     STCHK(check_model(m));
-    if (opt && (opt->pack_waves < 0 || opt->pack_waves > 8 || opt->pack_stage < 0 || opt->pack_stage > 2 ||
+    if (opt && (opt->pack_waves < 0 || opt->pack_waves > 8 || opt->pack_stage < 0 || opt->pack_stage > 3 ||
                 opt->pack_cap < 0 || opt->batch_k < 0 || opt->batch_waves < 0))
         return fail(HSH_ERR_ARG, "hsh_create: option out of range");
     hsh_handle* h = new hsh_handle();
@@ -1518,8 +1520,8 @@ hsh_status create_impl(const hsh_model* m, const hsh_options* opt, hsh_handle** 
     }
     if (o.batch_k > 0) h->batch_k = o.batch_k;
     if (o.pack_waves > 0) h->pack_waves = o.pack_waves;
-    if (o.pack_stage > 0) h->pack_stage = (o.pack_stage == 2);
-    if (h->plan.pack_lds_stage > MAX_LDS) h->pack_stage = false;
+    if (o.pack_stage > 0) h->pack_stage = o.pack_stage - 1;
+    if (!stage_fits(h, h->pack_stage)) h->pack_stage = 0;
     if (h->plan.pack_lds > MAX_LDS) {
         delete h;
         return fail(HSH_ERR_ARG, "hsh_create: a pack needs more LDS than a workgroup has");
@@ -2125,7 +2127,7 @@ hsh_status hsh_info_get(hsh_handle* h, hsh_info* info)
     o.tile_max_rows = P.layout.tile_max_rows;
     o.pack_cap = P.layout.pack_cap;
     o.pack_waves = h->pack_waves;
-    o.pack_stage = h->pack_stage ? 2 : 1;
+    o.pack_stage = h->pack_stage + 1;
     o.max_tree_rows = P.max_tree_rows;
     o.autotuned = h->tuned ? 1 : 0;
     o.autotune_s = h->autotune_s;

@@ -1197,10 +1197,13 @@ __device__ __forceinline__ void wave_lds_sync()
 // roundings; backward, roots solve and children follow their parents
 // (:275-295) -- with a barrier between levels wider than one wave; the
 // narrower levels run in wave 0 alone.  CHAN = false is the plain drop-in's
-// elimination (rhs/diag read from results, one step).  STAGE copies the
-// pack's level schedule into LDS once per launch; without it each level's
-// entries are loaded from global memory one level ahead.
-template <class RUN, bool CHAN, bool STAGE>
+// elimination (rhs/diag read from results, one step).  STAGE = 1 copies the
+// pack's level schedule into LDS once per launch; STAGE = 2 copies only its
+// indices (level bounds, entries, kid and backward rows: 2-8 bytes per row
+// instead of ~36, so more packs fit a CU at once) and leaves the
+// coefficients in global memory; without STAGE = 1 each level's entries are
+// loaded one level ahead.
+template <class RUN, bool CHAN, int STAGE>
 __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* lds)
 {
     const int pk = blockIdx.x;
@@ -1225,7 +1228,21 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
     const double* KTF = d.ktcf + pm.ke0;
     const int* BKP = d.bkp + pm.be0;
     const double* BTC = d.btcb + pm.be0;
-    if (STAGE) {
+    if (STAGE == 2) {
+        int* sLVL = (int*)tail;
+        int* sFE = sLVL + (pm.nfl + pm.nbl + 2);
+        int* sBKP = sFE + 2 * pm.nfe;
+        unsigned short* sKI = (unsigned short*)(sBKP + pm.nbe);
+        for (int i = tid; i < pm.nke; i += T) sKI[i] = KI[i];
+        for (int i = tid; i < pm.nbe; i += T) sBKP[i] = BKP[i];
+        for (int i = tid; i < 2 * pm.nfe; i += T) sFE[i] = FE[i];
+        for (int i = tid; i < pm.nfl + pm.nbl + 2; i += T) sLVL[i] = lvl[i];
+        lvl = sLVL;
+        FE = sFE;
+        BKP = sBKP;
+        KI = sKI;
+    }
+    if (STAGE == 1) {
         // doubles first (8-byte aligned), then ints, then shorts
         double* sKTA = (double*)tail;
         double* sKTF = sKTA + pm.nke;
@@ -1373,7 +1390,7 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
     // Levels [l0, l1) of a sweep (level l: entries lv[l] .. lv[l+1]-1).
     // ALL: every thread, a workgroup barrier after each level.  Otherwise
     // wave 0 alone, whose LDS accesses complete in order, so its levels need
-    // no barrier.  With the schedule in global memory (no STAGE) each
+    // no barrier.  With the coefficients in global memory (STAGE != 1) each
     // thread's first entry of the next level is loaded before the current
     // level runs, taking the schedule's memory latency off the level chain;
     // with the schedule in LDS that would only queue the next level's LDS
@@ -1383,14 +1400,14 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
         const int t0 = all ? tid : lane, st = all ? T : W;
         FwdE nx;
         bool vn = false;
-        if (!STAGE && l0 < l1) {
+        if (STAGE != 1 && l0 < l1) {
             const int e = lvl[l0] + t0;
             vn = e < lvl[l0 + 1];
             if (vn) fwd_load(e, nx);
         }
         for (int l = l0; l < l1; l++) {
             int e1 = lvl[l] + t0;
-            if (!STAGE) {
+            if (STAGE != 1) {
                 const FwdE cu = nx;
                 const bool vc = vn;
                 if (l + 1 < l1) {
@@ -1422,14 +1439,14 @@ __device__ __forceinline__ void body_forest(const DevModel& d, int K, double* ld
         const int t0 = all ? tid : lane, st = all ? T : W;
         BwdE nx;
         bool vn = false;
-        if (!STAGE && l0 < l1) {
+        if (STAGE != 1 && l0 < l1) {
             const int e = lv[l0] + t0;
             vn = e < lv[l0 + 1];
             if (vn) bwd_load(e, nx);
         }
         for (int l = l0; l < l1; l++) {
             int e1 = lv[l] + t0;
-            if (!STAGE) {
+            if (STAGE != 1) {
                 const BwdE cu = nx;
                 const bool vc = vn;
                 if (l + 1 < l1) {

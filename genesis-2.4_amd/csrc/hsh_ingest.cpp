@@ -1106,6 +1106,7 @@ bool build_layout(Plan& P, const LayoutOpts& o, std::string& err)
     P.max_pack_pos = 0;
     P.pack_lds = 0;
     P.pack_lds_stage = 0;
+    P.pack_lds_stage_idx = 0;
     P.comp_pos.assign(n, -1);
     const int ntrees = (int)P.tree_lo.size();
     auto rows_of = [&](int t) { return P.tree_hi[t] - P.tree_lo[t] + 1; };
@@ -1304,6 +1305,8 @@ bool build_layout(Plan& P, const LayoutOpts& o, std::string& err)
                               (int64_t)(H + D + 3) * 4 + 16;
         P.pack_lds = std::max<int64_t>(P.pack_lds, lds);
         P.pack_lds_stage = std::max<int64_t>(P.pack_lds_stage, lds + stage);
+        const int64_t stage_idx = (int64_t)ke * 2 + (int64_t)be * 4 + (int64_t)fe * 8 + (int64_t)(H + D + 3) * 4 + 16;
+        P.pack_lds_stage_idx = std::max<int64_t>(P.pack_lds_stage_idx, lds + stage_idx);
         i0 = i1;
     }
     for (auto& pm : P.packs) {

@@ -110,7 +110,8 @@ struct Plan {
     std::vector<int32_t> tree_funit;      // per tree: its unit, -1
     std::vector<int32_t> funit_tile, funit_lane;   // per unit: tile and lane (build_layout)
     int ntiles_regular = 0;               // tiles before the units' tiles
-    int64_t pack_lds = 0, pack_lds_stage = 0;   // LDS bytes of the pack kernels
+    // LDS bytes of the pack kernels: schedule in memory, all of it in LDS, its indices in LDS
+    int64_t pack_lds = 0, pack_lds_stage = 0, pack_lds_stage_idx = 0;
 };
 
 // Several solvers in one handle (hsh_create_group): the analysed model is
@@ -137,7 +138,7 @@ bool build_layout(Plan& plan, const LayoutOpts& o, std::string& err);
 struct JitModule {
     hipModule_t mod = nullptr;
     hipFunction_t channel = nullptr, batched_r[3] = {nullptr, nullptr, nullptr}, batched = nullptr,
-                  iso = nullptr, forest[2] = {nullptr, nullptr}, adj_forest = nullptr;
+                  iso = nullptr, forest[3] = {nullptr, nullptr, nullptr}, adj_forest = nullptr;
     bool ok = false;
     double compile_s = 0.0;
 };
@@ -197,8 +198,9 @@ hipError_t launch_batched(const DevModel& d, const JitModule* jm, int ksteps, in
                           hipStream_t s);
 hipError_t launch_iso(const DevModel& d, const JitModule* jm, int ksteps, hipStream_t s);
 // packs: batched (K steps, channel programs + elimination), waves per pack,
-// stage = schedule copied to LDS; lds = bytes of dynamic LDS
-hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, bool stage, int lds,
+// stage = schedule copied to LDS (0 none, 1 all of it, 2 its indices); lds =
+// bytes of dynamic LDS
+hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, int stage, int lds,
                          hipStream_t s);
 // packs, plain drop-in: elimination only (rhs/diag from results)
 hipError_t launch_forest_elim(const DevModel& d, int lds, hipStream_t s);

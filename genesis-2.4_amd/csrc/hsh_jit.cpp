@@ -503,10 +503,10 @@ std::string jit_source(const Plan& P, int ncols)
            "{ hsh::body_iso<hsh::JitRun>(d, K); }\n";
     src += "extern \"C\" __global__ __launch_bounds__(256) void hsh_jit_adj_forest(hsh::AdjPack a)\n"
            "{ extern __shared__ double lds[]; hsh::body_adj_forest<hsh::JitAdj>(a, lds); }\n";
-    for (int st : {0, 1})
+    for (int st : {0, 1, 2})
         src += "extern \"C\" __global__ __launch_bounds__(512) void hsh_jit_forest" + std::to_string(st) +
                "(hsh::DevModel d, int K)\n{ extern __shared__ double lds[]; hsh::body_forest<hsh::JitRun, true, " +
-               (st ? "true" : "false") + ">(d, K, lds); }\n";
+               std::to_string(st) + ">(d, K, lds); }\n";
     return src;
 }
 
@@ -562,10 +562,11 @@ bool jit_build(const Plan& P, int ncols, JitModule& jm, std::string& err)
     }
     const char* names[] = {"hsh_jit_channel", "hsh_jit_batched_r1", "hsh_jit_batched_r2",
                            "hsh_jit_batched_r4", "hsh_jit_batched", "hsh_jit_iso",
-                           "hsh_jit_forest0", "hsh_jit_forest1", "hsh_jit_adj_forest"};
+                           "hsh_jit_forest0", "hsh_jit_forest1", "hsh_jit_forest2", "hsh_jit_adj_forest"};
     hipFunction_t* slots[] = {&jm.channel, &jm.batched_r[0], &jm.batched_r[1], &jm.batched_r[2],
-                              &jm.batched, &jm.iso, &jm.forest[0], &jm.forest[1], &jm.adj_forest};
-    for (int i = 0; i < 9; i++) {
+                              &jm.batched, &jm.iso, &jm.forest[0], &jm.forest[1], &jm.forest[2],
+                              &jm.adj_forest};
+    for (int i = 0; i < 10; i++) {
         if (hipModuleGetFunction(slots[i], jm.mod, names[i]) != hipSuccess) {
             err = std::string("hipModuleGetFunction failed: ") + names[i];
             jit_free(jm);

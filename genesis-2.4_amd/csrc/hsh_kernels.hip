@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_iso(DevModel d, int K)
     body_iso<InterpRun<REG, EXT>>(d, K);
 }
 
-template <bool REG, bool STAGE, bool EXT>
+template <bool REG, int STAGE, bool EXT>
 __global__ __launch_bounds__(512) void k_forest(DevModel d, int K)
 {
     extern __shared__ double lds[];
@@ -56,7 +56,7 @@ __global__ __launch_bounds__(512) void k_forest(DevModel d, int K)
 __global__ __launch_bounds__(256) void k_forest_elim(DevModel d)
 {
     extern __shared__ double lds[];
-    body_forest<InterpRun<false, false>, false, false>(d, 1, lds);
+    body_forest<InterpRun<false, false>, false, 0>(d, 1, lds);
 }
 
 // Plain drop-in, part (b): one lane per neuron, one wave per tile.  Forward
@@ -413,7 +413,7 @@ hipError_t launch_iso(const DevModel& d, const JitModule* jm, int ksteps, hipStr
     return hipGetLastError();
 }
 
-template <bool STAGE>
+template <int STAGE>
 void launch_forest_v(const DevModel& d, Variant v, int ksteps, unsigned bx, int lds, hipStream_t s)
 {
     const dim3 grid(d.npacks), block(bx);
@@ -424,17 +424,20 @@ void launch_forest_v(const DevModel& d, Variant v, int ksteps, unsigned bx, int 
     }
 }
 
-hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, bool stage, int lds,
+hipError_t launch_forest(const DevModel& d, const JitModule* jm, int ksteps, int waves, int stage, int lds,
                          hipStream_t s)
 {
     if (d.npacks <= 0) return hipSuccess;
+    if (stage < 0 || stage > 2) return hipErrorInvalidValue;
     waves = waves < 1 ? 1 : (waves > 8 ? 8 : waves);
     const unsigned bx = (unsigned)(waves * W);
-    if (jm && jm->ok) return module_launch(jm->forest[stage ? 1 : 0], (unsigned)d.npacks, bx, (unsigned)lds, s, d, &ksteps);
-    if (stage)
-        launch_forest_v<true>(d, variant_of(d), ksteps, bx, lds, s);
+    if (jm && jm->ok) return module_launch(jm->forest[stage], (unsigned)d.npacks, bx, (unsigned)lds, s, d, &ksteps);
+    if (stage == 2)
+        launch_forest_v<2>(d, variant_of(d), ksteps, bx, lds, s);
+    else if (stage == 1)
+        launch_forest_v<1>(d, variant_of(d), ksteps, bx, lds, s);
     else
-        launch_forest_v<false>(d, variant_of(d), ksteps, bx, lds, s);
+        launch_forest_v<0>(d, variant_of(d), ksteps, bx, lds, s);
     return hipGetLastError();
 }
 

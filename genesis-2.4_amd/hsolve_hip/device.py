@@ -262,7 +262,8 @@ class HipSolver:
                  pack_waves: int = 0, pack_stage: int = 0, givals: bool = False):
         """Options as in hsh_options: 0 = autotuned (or the default under
         F_NO_AUTOTUNE); ``tile_max_rows < 0`` puts every tree in packs;
-        ``pack_stage`` 1 = schedule read from memory, 2 = staged in LDS;
+        ``pack_stage`` 1 = schedule read from memory, 2 = staged in LDS, 3 = its
+        indices staged in LDS (coefficients read from memory);
         ``givals``: pass the model's givals so the device writes that stream."""
         L = lib()
         if host_only:

@@ -147,7 +147,7 @@ typedef struct hsh_options {
     int32_t tile_max_rows;   /* 0 = autotuned; < 0 = every tree in packs */
     int32_t pack_cap;        /* compartments per pack; 0 = autotuned */
     int32_t pack_waves;      /* waves per pack workgroup (1..8); 0 = autotuned */
-    int32_t pack_stage;      /* pack schedule in LDS: 0 = autotuned, 1 = no, 2 = yes */
+    int32_t pack_stage;      /* pack schedule in LDS: 0 = autotuned, 1 = no, 2 = all of it, 3 = its indices */
 } hsh_options;
 
 #define HSH_F_NO_AUTOTUNE 1  /* no autotuning: options as given, defaults for the rest
@@ -228,7 +228,7 @@ typedef struct hsh_info {
     int32_t tile_max_rows;   /* layout in use (see hsh_options) */
     int32_t pack_cap;
     int32_t pack_waves;
-    int32_t pack_stage;      /* 1 = no, 2 = yes */
+    int32_t pack_stage;      /* 1 = no, 2 = all of it, 3 = its indices */
     int32_t max_tree_rows;   /* largest neuron */
     int32_t autotuned;       /* 1 once the autotuner has run */
     double  autotune_s;      /* time it took */

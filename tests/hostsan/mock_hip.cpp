@@ -317,9 +317,10 @@ hipError_t launch_batched(const DevModel& d, const JitModule*, int, int, int, hi
     return hipSuccess;
 }
 hipError_t launch_iso(const DevModel& d, const JitModule*, int, hipStream_t s) { record(d, s, "iso"); return hipSuccess; }
-hipError_t launch_forest(const DevModel& d, const JitModule*, int, int, bool, int lds, hipStream_t s)
+hipError_t launch_forest(const DevModel& d, const JitModule*, int, int, int stage, int lds, hipStream_t s)
 {
     if (lds > 160 * 1024) die("forest LDS over 160 KiB", "lds", nullptr, (size_t)lds);
+    if (stage < 0 || stage > 2) die("forest staging mode out of range", "stage", nullptr, (size_t)stage);
     record(d, s, "forest");
     return hipSuccess;
 }

@@ -86,7 +86,7 @@ def test_plan_packs_fill_blocks():
 
 def test_rejects_bad_options():
     m = random_tree_model(3, 4)
-    for kw in (dict(pack_waves=9), dict(pack_stage=3), dict(pack_cap=-1)):
+    for kw in (dict(pack_waves=9), dict(pack_stage=4), dict(pack_stage=-1), dict(pack_cap=-1)):
         with pytest.raises(D.HshError):
             D.HipSolver(m, host_only=True, **kw)
 

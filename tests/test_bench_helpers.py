@@ -66,6 +66,8 @@ def test_read_counter_needs_both_calibrations(bench, tmp_path):
     (dict(jit=1, ntiles=10, max_rows=1, pack_stage=1), "hsh_jit_iso"),
     (dict(jit=1, ntiles=10, max_rows=40, pack_stage=1), "hsh_jit_batched"),
     (dict(jit=1, ntiles=0, max_rows=0, pack_stage=2), "hsh_jit_forest1"),
+    (dict(jit=1, ntiles=0, max_rows=0, pack_stage=3), "hsh_jit_forest2"),
+    (dict(jit=1, ntiles=0, max_rows=0, pack_stage=1), "hsh_jit_forest0"),
     (dict(jit=0, ntiles=10, max_rows=16, pack_stage=1), "k_batched_r<4>"),
     (dict(jit=0, ntiles=0, max_rows=0, pack_stage=1), "k_forest"),
 ])

@@ -117,10 +117,12 @@ def test_heterogeneous_population(gpu, mode, method):
 
 
 @pytest.mark.parametrize("tmr,cap,waves,stage", [(-1, 512, 4, 1), (-1, 1024, 8, 2), (16, 2048, 4, 2),
-                                                 (64, 1024, 8, 1), (16, 100, 2, 1)])
+                                                 (64, 1024, 8, 1), (16, 100, 2, 1), (-1, 1024, 8, 3),
+                                                 (-1, 256, 4, 3), (16, 2048, 2, 3)])
 def test_pack_variants(gpu, tmr, cap, waves, stage):
     """Pack layouts and workgroup shapes: every tree in packs or only the
-    large ones, pack capacity, waves per pack, schedule staged in LDS."""
+    large ones, pack capacity, waves per pack, schedule read from memory,
+    staged in LDS, or its indices staged in LDS."""
     m = c3_model(300, seed=6)
     os_, gs = run_pair(m, 40, D.BATCHED, batch_k=16, flags=D.F_NO_AUTOTUNE, tile_max_rows=tmr,
                        pack_cap=cap, pack_waves=waves, pack_stage=stage)
