@@ -54,7 +54,7 @@ def hetero():
 def layouts():
     m = c3_model(200, seed=6)
     for tmr in (16, 64, -1):
-        for cap, waves, stage in ((512, 4, 1), (1024, 8, 2), (2048, 4, 2)):
+        for cap, waves, stage in ((512, 4, 1), (1024, 8, 2), (2048, 4, 2), (2048, 8, 3), (256, 2, 3)):
             with D.HipSolver(m, batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=tmr, pack_cap=cap,
                              pack_waves=waves, pack_stage=stage) as g:
                 g.step(20, D.BATCHED)
