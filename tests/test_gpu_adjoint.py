@@ -30,6 +30,8 @@ def solver_for(m, rows, layout="tiles", cap=256):
         return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=cap)
     if layout == "packs-interp":
         return D.HipSolver(m, flags=D.F_NO_AUTOTUNE | D.F_NO_JIT, tile_max_rows=-1, pack_cap=cap)
+    if layout == "packs-idx":
+        return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=cap, pack_waves=8, pack_stage=3)
     return D.HipSolver(m, flags=D.F_NO_AUTOTUNE, tile_max_rows=rows)
 
 
@@ -75,7 +77,8 @@ def test_adjoint_dot_product_with_tangent_linear(gpu, method, sizes, rows):
 def test_pack_adjoint_equals_tile_adjoint(gpu, method, sizes, cap):
     """The pack layout's adjoint (one recording launch and one reverse launch
     per checkpoint interval, level-scheduled sweeps; generated reverse
-    programs, and the interpreter) gives the tile layout's trace and
+    programs, and the interpreter; the recording launch with the schedule
+    read from memory and with its indices in LDS) gives the tile layout's trace and
     gradients bit for bit, and leaves the forward run's state."""
     pm = F.ParamMap(9, sizes, (1.0 / len(sizes),) * len(sizes), seed=7, method=method)
     m, obs, T = pm.base, pm.soma, 160
@@ -83,7 +86,7 @@ def test_pack_adjoint_equals_tile_adjoint(gpu, method, sizes, cap):
     print(f"[case] packs vs tiles method={method} sizes={sizes} cap={cap}", flush=True)
     w = np.random.default_rng(3).standard_normal((T, obs.size))
     out = {}
-    for layout in ("tiles", "packs", "packs-interp"):
+    for layout in ("tiles", "packs", "packs-interp", "packs-idx"):
         with solver_for(m, rows, layout, cap) as s:
             tr = s.adj_forward(T, 50, obs)
             after_fwd = s.state()
@@ -93,7 +96,7 @@ def test_pack_adjoint_equals_tile_adjoint(gpu, method, sizes, cap):
                 assert np.array_equal(after_fwd[k], after_bwd[k]), (layout, k)
             out[layout] = (tr, g)
     assert np.array_equal(out["tiles"][0], oracle_trace(m, T, obs))
-    for layout in ("packs", "packs-interp"):
+    for layout in ("packs", "packs-interp", "packs-idx"):
         assert np.array_equal(out["tiles"][0], out[layout][0]), layout
         for a, b in zip(out["tiles"][1], out[layout][1]):
             assert np.array_equal(a, b), layout

@@ -59,8 +59,10 @@ LAYOUTS = [
     (D.BATCHED, dict(batch_k=16, flags=D.F_NO_AUTOTUNE)),
     (D.BATCHED, dict(batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=64)),
     (D.BATCHED, dict(batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=512, pack_stage=2)),
+    (D.BATCHED, dict(batch_k=8, flags=D.F_NO_AUTOTUNE, tile_max_rows=-1, pack_cap=1024, pack_waves=8,
+                     pack_stage=3)),
 ]
-IDS = ["plain", "tiles16", "tiles64", "packs"]
+IDS = ["plain", "tiles16", "tiles64", "packs", "packs-idx"]
 
 
 @pytest.mark.parametrize("mode,kw", LAYOUTS, ids=IDS)

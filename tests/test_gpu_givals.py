@@ -50,8 +50,9 @@ def test_givals_hh_cells(gpu, mode, chanmode):
     assert np.any(gg != m.givals)                # written
 
 
-@pytest.mark.parametrize("kw", [dict(tile_max_rows=16), dict(tile_max_rows=-1, pack_cap=512)],
-                         ids=["tiles", "packs"])
+@pytest.mark.parametrize("kw", [dict(tile_max_rows=16), dict(tile_max_rows=-1, pack_cap=512),
+                                dict(tile_max_rows=-1, pack_cap=512, pack_stage=3)],
+                         ids=["tiles", "packs", "packs-idx"])
 def test_givals_trees_batched(gpu, kw):
     m = random_tree_model(150, 16, seed=3)
     os_, gs, gg = run(m, 100, D.BATCHED, batch_k=16, flags=D.F_NO_AUTOTUNE, **kw)
